@@ -1,0 +1,36 @@
+#!/usr/bin/env bash
+# Build the reference harness (test infrastructure only) into oracle/_ref/.
+#
+# Compiles oracle/ref_harness.cpp against the taichi_collisional reference
+# headers where they lie (/root/reference/taichi_with_jerks), with the jerk
+# build's own feature flags (taichi_with_jerks/Makefile:1-9, minus DEBUG and
+# the HDF5 I/O, which the harness does not include).  glibc >= 2.27 dropped
+# the SVID macro HUGE the reference uses; the historical value is passed in.  One library per
+# expansion order; two ISA variants so the GPU hosts (AMD EPYC) can load one:
+#   libtcref_p<P>_v4.so  x86-64-v4 (AVX-512), libtcref_p<P>_v3.so  x86-64-v3 (AVX2)
+# Nothing is built when the reference is absent (e.g. on the GPU hosts, which
+# only use the libraries built here).
+set -euo pipefail
+REF=${TC_REFERENCE_DIR:-/root/reference/taichi_with_jerks}
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="$HERE/_ref"
+ORDERS=${TC_REF_ORDERS:-"4 8 10 12"}
+if [ ! -d "$REF" ]; then
+  echo "build_ref: reference not present at $REF; skipping"
+  exit 0
+fi
+mkdir -p "$OUT"
+FLAGS="-O3 -fopenmp -fcx-limited-range -funroll-loops -Wno-format -w -fPIC -shared \
+  -DFMM -DSIMD_P2P -DDOUBLE_P2P -DSIMD_M2L -DMINIBALL -DHUGE=3.40282347e+38F"
+pids=()
+for P in $ORDERS; do
+  for V in v4 v3; do
+    tgt="$OUT/libtcref_p${P}_${V}.so"
+    if [ -f "$tgt" ] && [ "$tgt" -nt "$HERE/ref_harness.cpp" ]; then continue; fi
+    g++ -march=x86-64-${V} $FLAGS -DEXPANSION=$P -I"$REF" "$HERE/ref_harness.cpp" -o "$tgt.tmp" \
+      && mv "$tgt.tmp" "$tgt" &
+    pids+=($!)
+  done
+done
+for p in "${pids[@]}"; do wait "$p"; done
+echo "build_ref: libraries in $OUT"

@@ -1,0 +1,215 @@
+"""ctypes access to the oracle (liboracle.so) and the compiled reference
+(_ref/libtcref_p<P>_<isa>.so).  TEST INFRASTRUCTURE ONLY -- imported by
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, never by the
+product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("p", C.c_int), ("ncrit", C.c_int), ("theta", C.c_double), ("mac", C.c_int),
+                ("force_accuracy", C.c_double), ("quirk", C.c_int),
+                ("shift_frac", C.c_double * 3), ("direct_below", C.c_int),
+                ("nthreads", C.c_int)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("n_m2l", C.c_longlong), ("n_p2p_cells", C.c_longlong),
+                ("n_p2p_pairs", C.c_longlong), ("ncells", C.c_int), ("nleaves", C.c_int),
+                ("depth", C.c_int)]
+
+
+def _opt(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcParams)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_eval.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
+                               _dp, _dp, _dp, _dp, C.POINTER(OrcStats)]
+        L.orc_cells.argtypes = [C.c_void_p, C.c_int] + [_ip] * 5 + [_dp] * 4 + [_ip]
+        L.orc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_lists.restype = C.c_longlong
+        L.orc_lists.argtypes = [C.c_void_p, C.c_int, C.c_longlong, _ip, _ip]
+        L.orc_direct.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_int, C.c_int,
+                                 _dp, _dp, _dp]
+        L.orc_k_p2m.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_m2l.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_m2m.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_l2l.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_l2p.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_p2p.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_minball.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_harmonics.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_swap.argtypes = [C.c_int, _dp]
+        L.orc_shift_from_seed.argtypes = [C.c_long, C.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def shift_from_seed(seed, nskip=0):
+    u = np.zeros(3)
+    lib().orc_shift_from_seed(seed, nskip, u)
+    return u
+
+
+class Oracle:
+    """One FMM configuration of the CPU restatement."""
+
+    def __init__(self, p, ncrit=64, theta=0.5, mac=0, force_accuracy=2e-8, quirk=1,
+                 shift_frac=(0.0, 0.0, 0.0), direct_below=0, nthreads=0):
+        prm = OrcParams(p, ncrit, theta, mac, force_accuracy, quirk,
+                        (C.c_double * 3)(*shift_frac), direct_below, nthreads)
+        self.p = p
+        self.h = lib().orc_create(C.byref(prm))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def eval(self, x, v, m, issink=None, issource=None):
+        n = m.size
+        pot, acc, jerk, acco = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
+        st = OrcStats()
+        s = _opt(issink, np.uint8)
+        so = _opt(issource, np.uint8)
+        rc = lib().orc_eval(self.h, n, np.ascontiguousarray(x).reshape(-1),
+                            np.ascontiguousarray(v).reshape(-1), np.ascontiguousarray(m),
+                            None if s is None else s.ctypes.data,
+                            None if so is None else so.ctypes.data,
+                            pot, acc, jerk, acco, C.byref(st))
+        assert rc == 0
+        self._n = n
+        self.stats = {k: getattr(st, k) for k, _ in OrcStats._fields_}
+        return pot, acc.reshape(n, 3), jerk.reshape(n, 3), acco
+
+    def cells(self):
+        n = -lib().orc_cells(self.h, 0, *([np.zeros(1, np.int32)] * 5),
+                             *([np.zeros(1)] * 4), np.zeros(1, np.int32))
+        nb = self._n
+        out = dict(level=np.zeros(n, np.int32), nchild=np.zeros(n, np.int32),
+                   child=np.zeros(n, np.int32), body=np.zeros(n, np.int32),
+                   nbody=np.zeros(n, np.int32), Xg=np.zeros(3 * n), Rg=np.zeros(n),
+                   X=np.zeros(3 * n), R=np.zeros(n), order=np.zeros(nb, np.int32))
+        lib().orc_cells(self.h, n, out["level"], out["nchild"], out["child"], out["body"],
+                        out["nbody"], out["Xg"], out["Rg"], out["X"], out["R"], out["order"])
+        out["Xg"] = out["Xg"].reshape(n, 3)
+        out["X"] = out["X"].reshape(n, 3)
+        return out
+
+    def coefs(self, ncells):
+        nt = (self.p + 1) ** 2
+        M, Md, L, Ld = (np.zeros(ncells * nt) for _ in range(4))
+        lib().orc_coefs(self.h, M, Md, L, Ld)
+        return [a.reshape(ncells, nt) for a in (M, Md, L, Ld)]
+
+    def lists(self, which):
+        n = -lib().orc_lists(self.h, which, 0, np.zeros(1, np.int32), np.zeros(1, np.int32))
+        t, s = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        lib().orc_lists(self.h, which, n, t, s)
+        return t[:n], s[:n]
+
+
+def direct(x, v, m, targets=None, long_double=False, nthreads=0):
+    """All-pairs potential/acceleration/jerk for the given targets (default all)."""
+    n = m.size
+    tg = np.arange(n, dtype=np.int32) if targets is None else np.ascontiguousarray(targets, np.int32)
+    k = tg.size
+    pot, acc, jerk = np.zeros(k), np.zeros(3 * k), np.zeros(3 * k)
+    lib().orc_direct(n, np.ascontiguousarray(x).reshape(-1), np.ascontiguousarray(v).reshape(-1),
+                     np.ascontiguousarray(m), k, tg, int(long_double), nthreads, pot, acc, jerk)
+    return pot, acc.reshape(k, 3), jerk.reshape(k, 3)
+
+
+# ----------------------------------------------------------- reference ----
+def _cpu_has_avx512():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return all(f in flags for f in ("avx512f", "avx512dq", "avx512bw", "avx512vl"))
+
+
+_refs = {}
+
+
+def ref_available(p):
+    d = os.path.join(HERE, "_ref")
+    return any(os.path.exists(os.path.join(d, "libtcref_p%d_%s.so" % (p, v))) for v in ("v4", "v3"))
+
+
+def reference(p):
+    """ctypes handle on the reference harness built for expansion order p."""
+    if p in _refs:
+        return _refs[p]
+    d = os.path.join(HERE, "_ref")
+    isa = "v4" if _cpu_has_avx512() else "v3"
+    path = os.path.join(d, "libtcref_p%d_%s.so" % (p, isa))
+    if not os.path.exists(path):
+        raise RuntimeError("reference harness for p=%d not built (%s)" % (p, path))
+    L = C.CDLL(path)
+    L.ref_eval.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_long, C.c_int,
+                           C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                           np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"), _dp]
+    L.ref_cells.argtypes = [C.c_int] + [_ip] * 4 + [_dp, _dp, _ip]
+    L.ref_k_p2m.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.ref_k_m2l.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.ref_k_m2m.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp]
+    L.ref_k_l2l.argtypes = [_dp] * 6
+    L.ref_k_l2p.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.ref_k_p2p.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.ref_layout.argtypes = [np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]
+    L.isa = isa
+    L.path = path
+    _refs[p] = L
+    return L
+
+
+def ref_eval(p, x, v, m, seed, ncrit=64, theta=0.5, mode=0, nthreads=0):
+    """Run the compiled reference once; returns pot, acc, jerk, acc_old, counts, shift."""
+    L = reference(p)
+    n = m.size
+    pot, acc, jerk, acco = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n), np.zeros(n)
+    counts = np.zeros(3, np.int64)
+    shift = np.zeros(3)
+    rc = L.ref_eval(n, np.ascontiguousarray(x).reshape(-1), np.ascontiguousarray(v).reshape(-1),
+                    np.ascontiguousarray(m), None, None, seed, ncrit, theta, mode, nthreads,
+                    pot, acc, jerk, acco, counts, shift)
+    assert rc == 0
+    return pot, acc.reshape(n, 3), jerk.reshape(n, 3), acco, counts, shift
+
+
+def ref_cells(p, ncells_max, nbodies):
+    L = reference(p)
+    nc = ncells_max
+    out = dict(nchild=np.zeros(nc, np.int32), child=np.zeros(nc, np.int32),
+               body=np.zeros(nc, np.int32), nbody=np.zeros(nc, np.int32), X=np.zeros(3 * nc),
+               R=np.zeros(nc), order=np.zeros(nbodies, np.int32))
+    n = L.ref_cells(nc, out["nchild"], out["child"], out["body"], out["nbody"], out["X"],
+                    out["R"], out["order"])
+    if n < 0:
+        return ref_cells(p, -n, nbodies)
+    for k in ("nchild", "child", "body", "nbody", "R"):
+        out[k] = out[k][:n]
+    out["X"] = out["X"][:3 * n].reshape(n, 3)
+    return out
