@@ -1,0 +1,134 @@
+/*
+ * tcfmm -- MI355X (gfx950) fast-multipole force, potential and jerk evaluation
+ * for the taichi_collisional N-body code.  C ABI: plain pointers and sizes,
+ * int status codes (0 = success), no exceptions across the boundary.
+ *
+ * Reference interfaces replaced (paths relative to the reference's
+ * taichi_with_jerks/ directory):
+ *   tc_eval_bodies  <- void get_force_and_potential(Bodies&, bool)   integrator.h:308-432
+ *   tc_eval_soa     <- the same, on device-resident structure-of-arrays bodies
+ *   tc_config       <- globals P/EXPANSION, ncrit, theta, force_accuracy  exafmm.h:99,113-116
+ *   tc_set_seed     <- srand48(seed) before buildTree's drand48() root shift
+ *                      (taichi.cxx:32, build_tree.h:256-267)
+ *   tc_body         <- struct Body (exafmm.h:22-46), byte-for-byte
+ *   tc_direct_soa   <- direct(bodies, bodies) / directPass  traverse_eager.h:215-304
+ *
+ * Semantics kept from the reference: outputs are accelerations (no target
+ * mass factor, G = 1), potential is +sum m/r, jerk is da/dt; bodies with
+ * issink == false receive nothing, bodies with issource == false exert
+ * nothing; coincident bodies (r == 0) do not interact; below
+ * config.direct_below bodies the evaluation is an all-pairs sum
+ * (integrator.h:311).
+ */
+#ifndef TCFMM_H
+#define TCFMM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+#define TC_BOOL bool
+extern "C" {
+#else
+#define TC_BOOL _Bool
+#endif
+
+/* Body record of the reference (tj exafmm.h:22-46), 152 bytes on x86-64. */
+typedef struct tc_body {
+  double X[3];             /* position            @0   */
+  double q;                /* mass                @24  */
+  double F[3];             /* acceleration (out)  @32  */
+  double p;                /* potential (out)     @56  */
+  double V[3];             /* velocity            @64  */
+  double acc_old;          /* |a| estimate        @88  */
+  unsigned int Morton[3];  /* tree keys (scratch) @96  */
+  double timestep;         /*                     @112 */
+  double J[3];             /* jerk (out)          @120 */
+  int index;               /* caller's index      @144 */
+  TC_BOOL issink;            /*                     @148 */
+  TC_BOOL issource;          /*                     @149 */
+} tc_body;
+
+enum { TC_MAC_THETA = 0, TC_MAC_ERROR = 1 };
+
+typedef struct tc_config {
+  int p;                 /* expansion order (EXPANSION); compiled: 2,4,...,16       */
+  int ncrit;             /* max bodies per leaf (ncrit, exafmm.h:115)               */
+  double theta;          /* opening angle of the theta-MAC (exafmm.h:116)          */
+  int mac;               /* TC_MAC_THETA (horizontalPass) or TC_MAC_ERROR           */
+  double force_accuracy; /* error-MAC target (exafmm.h:99, 2e-8)                    */
+  int direct_below;      /* N below this -> all-pairs sum (integrator.h:311: 8000); 0 = never */
+  int quirk_factorial;   /* 1: drop M_p^{+-p} in M2L as factorial_table[2p]==0 does
+                            (kernel_fast_lean.h:129,1179); 0: exact truncated sum  */
+} tc_config;
+
+typedef struct tc_stats {
+  long long n_m2l;       /* accepted (target, source) cell pairs                   */
+  long long n_p2p_cells; /* leaf pairs evaluated directly                          */
+  long long n_p2p_pairs; /* body pairs in those leaf pairs (incl. self pairs)      */
+  int ncells, nleaves, depth, nlevels;
+  double ms_tree, ms_upward, ms_traverse, ms_lists, ms_m2l, ms_p2p, ms_downward, ms_total;
+} tc_stats;
+
+typedef struct tc_context tc_context;
+
+/* Create a context on the current HIP device.  stream: a hipStream_t (NULL =
+ * the context creates its own).  Returns 0, or a negative code. */
+int tc_create(const tc_config *cfg, void *stream, tc_context **out);
+int tc_destroy(tc_context *ctx);
+
+/* Root-shift control.  The reference shifts the root centre by
+ * (drand48()-0.5)*boxsize/25 per axis on every tree build.  tc_set_seed(s)
+ * reproduces srand48(s) followed by the reference's drand48() draws (three
+ * per evaluation); tc_set_shift fixes the three (drand48()-0.5) values for
+ * every following evaluation until the next tc_set_seed. */
+int tc_set_seed(tc_context *ctx, long seed);
+int tc_set_shift(tc_context *ctx, const double frac[3]);
+/* The three values used by the most recent evaluation. */
+int tc_last_shift(const tc_context *ctx, double frac[3]);
+
+/* Drop-in for get_force_and_potential on the reference's host Body array:
+ * F, J, p are overwritten (the reference driver zeroes them before the
+ * call, integrator.h:350-366), acc_old is written under TC_MAC_ERROR, the
+ * array is not reordered.  Copies host<->device inside. */
+int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n);
+
+/* Device-resident evaluation.  x, v: [n][3] doubles; m: [n]; flags: [n]
+ * bytes, bit0 = sink, bit1 = source (NULL = all both); pot: [n]; acc, jerk:
+ * [n][3] (any output may be NULL).  All pointers are device pointers; work
+ * is queued on the context's stream and the call returns after it finishes. */
+int tc_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                const unsigned char *flags, double *pot, double *acc, double *jerk);
+
+/* FP64 all-pairs sum for nt targets (indices into the n bodies), device
+ * pointers; used to measure the FMM error. */
+int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                  const unsigned char *flags, int nt, const int *targets, double *pot,
+                  double *acc, double *jerk);
+
+int tc_get_stats(const tc_context *ctx, tc_stats *out);
+const char *tc_last_error(const tc_context *ctx);
+
+/* Expansion orders this build was compiled for: writes up to maxn values,
+ * returns the count. */
+int tc_supported_orders(int *orders, int maxn);
+
+#ifdef __cplusplus
+}
+#endif
+
+#if defined(__cplusplus)
+static_assert(sizeof(tc_body) == 152, "tc_body must match the reference Body (152 bytes)");
+static_assert(__builtin_offsetof(tc_body, q) == 24 && __builtin_offsetof(tc_body, F) == 32 && __builtin_offsetof(tc_body, p) == 56,
+              "tc_body layout");
+static_assert(__builtin_offsetof(tc_body, V) == 64 && __builtin_offsetof(tc_body, acc_old) == 88 &&
+                  __builtin_offsetof(tc_body, Morton) == 96 && __builtin_offsetof(tc_body, timestep) == 112,
+              "tc_body layout");
+static_assert(__builtin_offsetof(tc_body, J) == 120 && __builtin_offsetof(tc_body, index) == 144 &&
+                  __builtin_offsetof(tc_body, issink) == 148 && __builtin_offsetof(tc_body, issource) == 149,
+              "tc_body layout");
+#else
+_Static_assert(sizeof(tc_body) == 152, "tc_body must match the reference Body (152 bytes)");
+_Static_assert(__builtin_offsetof(tc_body, J) == 120 && __builtin_offsetof(tc_body, issink) == 148, "tc_body layout");
+#endif
+
+#endif /* TCFMM_H */

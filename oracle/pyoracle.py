@@ -53,6 +53,7 @@ def lib():
         L.orc_direct.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_int, C.c_int,
                                  _dp, _dp, _dp]
         L.orc_k_p2m.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_k_p2m_at.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_k_m2l.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_k_m2m.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_k_l2l.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]

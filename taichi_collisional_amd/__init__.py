@@ -1,0 +1,178 @@
+"""taichi_collisional_amd -- MI355X (gfx950) FMM force + potential + jerk.
+
+Python side of the C ABI in include/tcfmm.h (libtcfmm.so, built in-tree by
+``python -m taichi_collisional_amd.build``).  The interface mirrors the
+reference's force path (taichi_with_jerks/integrator.h:308-432):
+
+    fmm = Fmm(p=12, ncrit=64, theta=0.4)          # P/EXPANSION, ncrit, theta
+    fmm.set_seed(1)                               # srand48(seed) of taichi.cxx:32
+    fmm.get_force_and_potential(bodies)           # host Body records (body_dtype)
+    pot, acc, jerk = fmm.eval(x, v, m)            # device tensors (torch, cuda)
+
+The product path is the HIP library only; if it is missing the import of
+this module's ``lib()`` raises, there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtcfmm.so")
+
+TC_MAC_THETA = 0
+TC_MAC_ERROR = 1
+
+# struct Body of the reference (taichi_with_jerks/exafmm.h:22-46), 152 bytes.
+body_dtype = np.dtype({
+    "names": ["X", "q", "F", "p", "V", "acc_old", "Morton", "timestep", "J", "index",
+              "issink", "issource"],
+    "formats": [("<f8", 3), "<f8", ("<f8", 3), "<f8", ("<f8", 3), "<f8", ("<u4", 3), "<f8",
+                ("<f8", 3), "<i4", "u1", "u1"],
+    "offsets": [0, 24, 32, 56, 64, 88, 96, 112, 120, 144, 148, 149],
+    "itemsize": 152,
+})
+
+
+class TcConfig(C.Structure):
+    _fields_ = [("p", C.c_int), ("ncrit", C.c_int), ("theta", C.c_double), ("mac", C.c_int),
+                ("force_accuracy", C.c_double), ("direct_below", C.c_int),
+                ("quirk_factorial", C.c_int)]
+
+
+class TcStats(C.Structure):
+    _fields_ = [("n_m2l", C.c_longlong), ("n_p2p_cells", C.c_longlong),
+                ("n_p2p_pairs", C.c_longlong), ("ncells", C.c_int), ("nleaves", C.c_int),
+                ("depth", C.c_int), ("nlevels", C.c_int), ("ms_tree", C.c_double),
+                ("ms_upward", C.c_double), ("ms_traverse", C.c_double),
+                ("ms_lists", C.c_double), ("ms_m2l", C.c_double), ("ms_p2p", C.c_double),
+                ("ms_downward", C.c_double), ("ms_total", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtcfmm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtcfmm.so is not built (%s); run "
+                               "`python -m taichi_collisional_amd.build`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.tc_create.argtypes = [C.POINTER(TcConfig), vp, C.POINTER(vp)]
+        L.tc_destroy.argtypes = [vp]
+        L.tc_set_seed.argtypes = [vp, C.c_long]
+        L.tc_set_shift.argtypes = [vp, C.POINTER(C.c_double)]
+        L.tc_last_shift.argtypes = [vp, C.POINTER(C.c_double)]
+        L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
+        L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.tc_get_stats.argtypes = [vp, C.POINTER(TcStats)]
+        L.tc_last_error.argtypes = [vp]
+        L.tc_last_error.restype = C.c_char_p
+        L.tc_supported_orders.argtypes = [C.POINTER(C.c_int), C.c_int]
+        _lib = L
+    return _lib
+
+
+EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
+                    "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_stats",
+                    "tc_last_error", "tc_supported_orders"]
+
+
+def supported_orders():
+    buf = (C.c_int * 64)()
+    n = lib().tc_supported_orders(buf, 64)
+    return list(buf[:n])
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Fmm:
+    """One FMM configuration on the current GPU (see include/tcfmm.h)."""
+
+    def __init__(self, p=12, ncrit=64, theta=0.4, mac=TC_MAC_THETA, force_accuracy=2e-8,
+                 direct_below=8000, quirk_factorial=True, stream=None):
+        cfg = TcConfig(p, ncrit, theta, mac, force_accuracy, direct_below, int(quirk_factorial))
+        h = C.c_void_p()
+        s = None if stream is None else C.c_void_p(int(stream))
+        rc = lib().tc_create(C.byref(cfg), s, C.byref(h))
+        if rc != 0:
+            raise ValueError("tc_create failed (%d): unsupported configuration %r; compiled "
+                             "orders %s" % (rc, dict(p=p, ncrit=ncrit, theta=theta),
+                                            supported_orders()))
+        self.h = h
+        self.p, self.ncrit, self.theta = p, ncrit, theta
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc,
+                                                       lib().tc_last_error(self.h).decode()))
+
+    def set_seed(self, seed):
+        self._check(lib().tc_set_seed(self.h, int(seed)), "tc_set_seed")
+
+    def set_shift(self, frac):
+        a = (C.c_double * 3)(*[float(f) for f in frac])
+        self._check(lib().tc_set_shift(self.h, a), "tc_set_shift")
+
+    def last_shift(self):
+        a = (C.c_double * 3)()
+        lib().tc_last_shift(self.h, a)
+        return list(a)
+
+    def get_force_and_potential(self, bodies):
+        """Drop-in for the reference call on a host array of body_dtype records."""
+        if bodies.dtype != body_dtype or not bodies.flags.c_contiguous:
+            raise TypeError("bodies must be a C-contiguous array of body_dtype")
+        self._check(lib().tc_eval_bodies(self.h, bodies.ctypes.data, bodies.size),
+                    "tc_eval_bodies")
+        return bodies
+
+    def eval(self, x, v, m, flags=None, pot=None, acc=None, jerk=None):
+        """Device tensors: x, v (n,3) float64, m (n,) float64, flags (n,) uint8."""
+        import torch
+        n = m.shape[0]
+        dev = x.device
+        for t in (x, v, m):
+            if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
+                raise TypeError("x, v, m must be contiguous float64 cuda tensors")
+        if pot is None:
+            pot = torch.empty(n, dtype=torch.float64, device=dev)
+        if acc is None:
+            acc = torch.empty(n, 3, dtype=torch.float64, device=dev)
+        if jerk is None:
+            jerk = torch.empty(n, 3, dtype=torch.float64, device=dev)
+        self._check(lib().tc_eval_soa(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(flags),
+                                      _ptr(pot), _ptr(acc), _ptr(jerk)), "tc_eval_soa")
+        return pot, acc, jerk
+
+    def direct(self, x, v, m, targets, flags=None):
+        """FP64 all-pairs potential/acceleration/jerk for the given target indices."""
+        import torch
+        nt = targets.shape[0]
+        dev = x.device
+        tg = targets.to(device=dev, dtype=torch.int32).contiguous()
+        pot = torch.empty(nt, dtype=torch.float64, device=dev)
+        acc = torch.empty(nt, 3, dtype=torch.float64, device=dev)
+        jerk = torch.empty(nt, 3, dtype=torch.float64, device=dev)
+        self._check(lib().tc_direct_soa(self.h, m.shape[0], _ptr(x), _ptr(v), _ptr(m),
+                                        _ptr(flags), nt, _ptr(tg), _ptr(pot), _ptr(acc),
+                                        _ptr(jerk)), "tc_direct_soa")
+        return pot, acc, jerk
+
+    def stats(self):
+        st = TcStats()
+        lib().tc_get_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in TcStats._fields_}

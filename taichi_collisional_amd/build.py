@@ -1,0 +1,68 @@
+"""Build libtcfmm.so (HIP, gfx950) in-tree.
+
+    python -m taichi_collisional_amd.build [--jobs N]
+
+Each csrc/*.hip translation unit is compiled with hipcc for gfx950 into
+build/*.o, then linked into taichi_collisional_amd/libtcfmm.so.  Packed-FP32
+instructions are disabled for all device code.  Always recompiles every unit
+(the driver's build check expects that); object files go to
+taichi_collisional_amd/build/ (git-ignored).
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "build")
+LIB = os.path.join(HERE, "libtcfmm.so")
+ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+CFLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-command-line-argument",
+]
+UNITS = ["tc_m2l.hip", "tc_tree.hip", "tc_upward.hip", "tc_p2p.hip", "tc_downward.hip",
+         "tc_engine.hip"]
+
+
+def _compile(unit):
+    src = os.path.join(CSRC, unit)
+    obj = os.path.join(BUILD, unit.replace(".hip", ".o"))
+    t = time.time()
+    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (unit, " ".join(cmd), r.stderr[-6000:]))
+    return unit, time.time() - t
+
+
+def build(jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for unit, dt in ex.map(_compile, UNITS):
+            if verbose:
+                print("  hipcc %-18s %6.1fs" % (unit, dt), flush=True)
+    objs = [os.path.join(BUILD, u.replace(".hip", ".o")) for u in UNITS]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("  linked", LIB, flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    j = None
+    if "--jobs" in sys.argv:
+        j = int(sys.argv[sys.argv.index("--jobs") + 1])
+    build(j)

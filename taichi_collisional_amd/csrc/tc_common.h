@@ -1,0 +1,59 @@
+// Shared device helpers for the tcfmm HIP kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tcfmm {
+
+constexpr int kWave = 64;
+
+__host__ __device__ constexpr int nterm(int p) { return (p + 1) * (p + 1); }
+__host__ __device__ constexpr int ixnm(int n, int m) { return n * n + n + m; }
+
+// Body flag bits (packed from Body::issink / Body::issource, exafmm.h:44-45).
+constexpr unsigned char kSink = 1, kSource = 2;
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cplx cscale(double s, cplx a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+
+// Wave-wide sum of a double, result in every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Wave-wide argmax of (e, idx): larger e wins, ties -> smaller idx.
+__device__ __forceinline__ void wave_argmax(double &e, int &idx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double e2 = __shfl_xor(e, o, kWave);
+    int i2 = __shfl_xor(idx, o, kWave);
+    if (e2 > e || (e2 == e && i2 < idx)) {
+      e = e2;
+      idx = i2;
+    }
+  }
+}
+
+// Exclusive prefix sum over the wave of a small int; total returned in *tot.
+__device__ __forceinline__ int wave_excl_scan(int v, int *tot) {
+  int lane = threadIdx.x & (kWave - 1);
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  *tot = __shfl(x, kWave - 1, kWave);
+  return x - v;
+}
+
+}  // namespace tcfmm

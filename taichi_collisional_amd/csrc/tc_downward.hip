@@ -1,0 +1,206 @@
+// Downward pass on the device (taichi_with_jerks/traverse_eager.h:272-292):
+// L2L from each internal cell into its children, one launch per level from the
+// root down, then L2P at every leaf, then the results are scattered back to
+// the caller's body order.
+//
+//   L2L (kernel_fast_lean.h:1001-1058): for each child,
+//       L_child[n,m] += sum_{k<=p-n} sum_{|l|<=k} L[n+k, m+l] conj Y_k^l(X - X_child)
+//       one wave per parent cell, lanes over output terms, harmonics in LDS.
+//   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
+//       dY/dt with velocity -v_b; only n <= 1 of the local field is formed:
+//       p += Re Phi_0^0, a -= (Re Phi_1^1, Im Phi_1^1, Re Phi_1^0),
+//       j -= the same of Phidot = L conj dY + Ldot conj Y.
+#include "tc_common.h"
+#include "tc_harm.h"
+#include "tc_kernels.h"
+
+namespace tcfmm {
+
+template <int P>
+__global__ __launch_bounds__(64) void k_l2l(CellsDev c, int ls) {
+  constexpr int NT = nterm(P);
+  constexpr int NO = tri(P, P) + 1;
+  constexpr int RO = (NO + kWave - 1) / kWave;
+  __shared__ cplx G[NT], cl[NT], cld[NT];
+  const int lane = threadIdx.x;
+  const int ci = ls + blockIdx.x;
+  const int nch = c.nchild[ci];
+  if (nch == 0) return;
+  const int ch0 = c.child[ci];
+  const double *Lp = c.L + (size_t)ci * NT;
+  const double *Ldp = c.Ld + (size_t)ci * NT;
+  for (int t = lane; t < NT; t += kWave) {
+    int n = 0;
+    while ((n + 1) * (n + 1) <= t) n++;
+    int m = t - n * n - n;
+    cl[t] = packed_to_c(Lp, n, m);
+    cld[t] = packed_to_c(Ldp, n, m);
+  }
+  int on[RO], om[RO];
+#pragma unroll
+  for (int r = 0; r < RO; r++) {
+    int o = lane + kWave * r, n = 0;
+    while (tri(n + 1, 0) <= o) n++;
+    on[r] = n;
+    om[r] = o - tri(n, 0);
+  }
+  const double X0 = c.X[3 * ci], X1 = c.X[3 * ci + 1], X2 = c.X[3 * ci + 2];
+  for (int k = 0; k < nch; k++) {
+    const int ch = ch0 + k;
+    __syncthreads();
+    harm_table_lds<P>(X0 - c.X[3 * ch], X1 - c.X[3 * ch + 1], X2 - c.X[3 * ch + 2], G);
+    __syncthreads();
+    double *Lc = c.L + (size_t)ch * NT;
+    double *Ldc = c.Ld + (size_t)ch * NT;
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+      int o = lane + kWave * r;
+      if (o >= NO) continue;
+      const int n = on[r], m = om[r];
+      cplx a = {0, 0}, ad = {0, 0};
+      for (int kk = 0; kk <= P - n; kk++)
+        for (int l = -kk; l <= kk; l++) {
+          cplx g = cconj(G[ixnm(kk, l)]);
+          int sidx = ixnm(n + kk, m + l);
+          a = cadd(a, cmul(cl[sidx], g));
+          ad = cadd(ad, cmul(cld[sidx], g));
+        }
+      const int s = n * n + n;
+      Lc[s + m] += a.re;
+      Ldc[s + m] += ad.re;
+      if (m > 0) {
+        Lc[s - m] += a.im;
+        Ldc[s - m] += ad.im;
+      }
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev r,
+                                            const int *__restrict__ leaf_ids) {
+  constexpr int NT = nterm(P);
+  constexpr int NY = tri(P, P) + 1;
+  __shared__ cplx cl[NT], cld[NT];
+  const int lane = threadIdx.x;
+  const int ci = leaf_ids[blockIdx.x];
+  if (!c.has_sink[ci]) return;
+  const int b0 = c.body[ci], nb = c.nbody[ci];
+  const double *Lp = c.L + (size_t)ci * NT;
+  const double *Ldp = c.Ld + (size_t)ci * NT;
+  for (int t = lane; t < NT; t += kWave) {
+    int n = 0;
+    while ((n + 1) * (n + 1) <= t) n++;
+    int m = t - n * n - n;
+    cl[t] = packed_to_c(Lp, n, m);
+    cld[t] = packed_to_c(Ldp, n, m);
+  }
+  __syncthreads();
+  const double X0 = c.X[3 * ci], X1 = c.X[3 * ci + 1], X2 = c.X[3 * ci + 2];
+  for (int base = 0; base < nb; base += kWave) {
+    const int i = b0 + base + lane;
+    if (base + lane >= nb) continue;
+    if (!(b.flags[i] & kSink)) continue;
+    const double vx = -b.vx[i], vy = -b.vy[i], vz = -b.vz[i];
+    cplx y[NY];
+    harm_all<P>(X0 - b.x[i], X1 - b.y[i], X2 - b.z[i], y);
+    cplx p00 = {0, 0}, p10 = {0, 0}, p11 = {0, 0};
+    cplx d10 = {0, 0}, d11 = {0, 0};
+#pragma unroll
+    for (int k = 0; k <= P; k++) {
+#pragma unroll
+      for (int l = -k; l <= k; l++) {
+        const int al = l < 0 ? -l : l;
+        cplx g = harm_at(y, k, l);
+        cplx gd = harm_dot(y, k, al, vx, vy, vz);
+        if (l < 0) {
+          gd = cconj(gd);
+          if (al & 1) gd = {-gd.re, -gd.im};
+        }
+        g = cconj(g);
+        gd = cconj(gd);
+        p00 = cadd(p00, cmul(cl[ixnm(k, l)], g));
+        if (k + 1 <= P) {
+          p10 = cadd(p10, cmul(cl[ixnm(k + 1, l)], g));
+          p11 = cadd(p11, cmul(cl[ixnm(k + 1, l + 1)], g));
+          d10 = cadd(d10, cadd(cmul(cl[ixnm(k + 1, l)], gd), cmul(cld[ixnm(k + 1, l)], g)));
+          d11 = cadd(d11,
+                     cadd(cmul(cl[ixnm(k + 1, l + 1)], gd), cmul(cld[ixnm(k + 1, l + 1)], g)));
+        }
+      }
+    }
+    r.pot[i] += p00.re;
+    r.ax[i] -= p11.re;
+    r.ay[i] -= p11.im;
+    r.az[i] -= p10.re;
+    r.jx[i] -= d11.re;
+    r.jy[i] -= d11.im;
+    r.jz[i] -= d10.re;
+  }
+}
+
+__global__ void k_scatter_results(BodiesDev b, ResultsDev r, int n, double *__restrict__ pot,
+                                  double *__restrict__ acc, double *__restrict__ jerk) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int i = b.order[k];
+  if (pot) pot[i] = r.pot[k];
+  if (acc) {
+    acc[3 * (size_t)i] = r.ax[k];
+    acc[3 * (size_t)i + 1] = r.ay[k];
+    acc[3 * (size_t)i + 2] = r.az[k];
+  }
+  if (jerk) {
+    jerk[3 * (size_t)i] = r.jx[k];
+    jerk[3 * (size_t)i + 1] = r.jy[k];
+    jerk[3 * (size_t)i + 2] = r.jz[k];
+  }
+}
+
+int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
+                           double *jerk3, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_scatter_results, dim3((n + 255) / 256), dim3(256), 0, s, b, r, n, pot,
+                     acc3, jerk3);
+  return 0;
+}
+
+template <int P>
+static void l2l_go(CellsDev c, int ls, int le, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2l<P>, dim3(le - ls), dim3(64), 0, s, c, ls);
+}
+template <int P>
+static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2p<P>, dim3(n), dim3(64), 0, s, c, b, r, ids);
+}
+
+int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s) {
+  if (le <= ls) return 0;
+  switch (p) {
+#define TC_CASE(P)              \
+  case P:                       \
+    l2l_go<P>(c, ls, le, s);    \
+    return 0;
+    TC_FOR_EACH_P(TC_CASE)
+#undef TC_CASE
+    default:
+      return -1;
+  }
+}
+
+int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids, int nleaves,
+               hipStream_t s) {
+  if (nleaves <= 0) return 0;
+  switch (p) {
+#define TC_CASE(P)                               \
+  case P:                                        \
+    l2p_go<P>(c, b, r, leaf_ids, nleaves, s);    \
+    return 0;
+    TC_FOR_EACH_P(TC_CASE)
+#undef TC_CASE
+    default:
+      return -1;
+  }
+}
+
+}  // namespace tcfmm

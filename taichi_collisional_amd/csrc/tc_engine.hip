@@ -1,0 +1,702 @@
+// Host engine and C ABI of tcfmm (include/tcfmm.h).
+//
+// One evaluation = get_force_and_potential (taichi_with_jerks/integrator.h:
+// 308-432) on device-resident bodies: bounds -> keys -> sort -> level-by-level
+// octree -> upward pass (ball geometry, P2M, M2M per level) -> breadth-first
+// dual traversal into M2L / P2P lists -> M2L -> P2P -> L2L per level -> L2P ->
+// results back in the caller's order.  All kernels run on the context's
+// stream; the host only reads level sizes, list sizes and the leaf set
+// between phases.  Buffers grow on demand and are kept between calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tcfmm.h"
+#include "tc_common.h"
+#include "tc_kernels.h"
+
+using namespace tcfmm;
+
+namespace {
+
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  // grow to at least b bytes; keep `keep` bytes of old content
+  hipError_t ensure(size_t b, size_t keep = 0, hipStream_t s = nullptr) {
+    if (b <= bytes) return hipSuccess;
+    size_t nb = std::max(b, bytes + bytes / 2);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, nb);
+    if (e != hipSuccess) return e;
+    if (keep && p) {
+      e = hipMemcpyAsync(q, p, std::min(keep, bytes), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+      hipStreamSynchronize(s);
+    }
+    if (p) hipFree(p);
+    p = q;
+    bytes = nb;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+struct Lcg48 {  // srand48 / drand48: X' = (0x5DEECE66D X + 0xB) mod 2^48
+  uint64_t X = 0x1234ABCD330EULL;
+  void seed(long s) { X = (((uint64_t)(uint32_t)s) << 16) | 0x330E; }
+  double next() {
+    X = (0x5DEECE66DULL * X + 0xB) & ((1ULL << 48) - 1);
+    return std::ldexp((double)X, -48);
+  }
+};
+
+}  // namespace
+
+struct tc_context {
+  tc_config cfg;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  tc_stats st{};
+  Lcg48 rng;
+  bool fixed_shift = false;
+  double shift[3] = {0, 0, 0};
+  // bodies
+  int cap_n = 0;
+  DBuf bx, by, bz, bvx, bvy, bvz, bq, bflags, border, bacc;
+  DBuf rpot, rax, ray, raz, rjx, rjy, rjz;
+  DBuf key_hi, key_a, key_b, val_a, val_b, sort_tmp, part, tp, dshift;
+  // cells
+  int cap_c = 0;
+  DBuf clevel, cnchild, cchild, cbody, cnbody, cXg, cRg, cX, cR, cmin, csink;
+  DBuf cM, cMd, cL, cLd;
+  DBuf split_offs, split_nch, split_scan, scan_tmp;
+  std::vector<int> level_start;
+  DBuf leaf_ids;
+  std::vector<int> h_leaves;
+  int nleaves = 0;
+  // traversal / lists
+  DBuf front_a, front_b, m2l, p2p, tcnt;
+  int cap_front = 0, cap_m2l = 0, cap_p2p = 0;
+  DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
+  DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
+  // direct
+  DBuf dtg, dpot, dacc, djerk, dx3, dv3, dm, dfl;
+  hipEvent_t ev[9];
+};
+
+#define TC_CHECK(expr)                                                   \
+  do {                                                                   \
+    hipError_t _e = (expr);                                              \
+    if (_e != hipSuccess) {                                              \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);      \
+      return -3;                                                         \
+    }                                                                    \
+  } while (0)
+
+static BodiesDev bodies_of(tc_context *ctx) {
+  return {ctx->bx.as<double>(),   ctx->by.as<double>(),  ctx->bz.as<double>(),
+          ctx->bvx.as<double>(),  ctx->bvy.as<double>(), ctx->bvz.as<double>(),
+          ctx->bq.as<double>(),   ctx->bflags.as<unsigned char>(), ctx->border.as<int>(),
+          ctx->bacc.as<double>()};
+}
+
+static CellsDev cells_of(tc_context *ctx) {
+  return {ctx->clevel.as<int>(), ctx->cnchild.as<int>(), ctx->cchild.as<int>(),
+          ctx->cbody.as<int>(),  ctx->cnbody.as<int>(),  ctx->cXg.as<double>(),
+          ctx->cRg.as<double>(), ctx->cX.as<double>(),   ctx->cR.as<double>(),
+          ctx->cmin.as<double>(), ctx->csink.as<unsigned char>(), ctx->cM.as<double>(),
+          ctx->cMd.as<double>(), ctx->cL.as<double>(),   ctx->cLd.as<double>()};
+}
+
+static ResultsDev results_of(tc_context *ctx) {
+  return {ctx->rpot.as<double>(), ctx->rax.as<double>(), ctx->ray.as<double>(),
+          ctx->raz.as<double>(),  ctx->rjx.as<double>(), ctx->rjy.as<double>(),
+          ctx->rjz.as<double>()};
+}
+
+static int ensure_bodies(tc_context *ctx, int n) {
+  if (n <= ctx->cap_n) return 0;
+  size_t d = sizeof(double) * (size_t)n;
+  for (DBuf *b : {&ctx->bx, &ctx->by, &ctx->bz, &ctx->bvx, &ctx->bvy, &ctx->bvz, &ctx->bq,
+                  &ctx->bacc, &ctx->rpot, &ctx->rax, &ctx->ray, &ctx->raz, &ctx->rjx, &ctx->rjy,
+                  &ctx->rjz})
+    TC_CHECK(b->ensure(d));
+  TC_CHECK(ctx->bflags.ensure(n));
+  TC_CHECK(ctx->border.ensure(sizeof(int) * (size_t)n));
+  TC_CHECK(ctx->key_hi.ensure(8 * (size_t)n));
+  TC_CHECK(ctx->key_a.ensure(8 * (size_t)n));
+  TC_CHECK(ctx->key_b.ensure(8 * (size_t)n));
+  TC_CHECK(ctx->val_a.ensure(4 * (size_t)n));
+  TC_CHECK(ctx->val_b.ensure(4 * (size_t)n));
+  TC_CHECK(ctx->sort_tmp.ensure(sort_temp_bytes(n) + 256));
+  TC_CHECK(ctx->part.ensure(sizeof(double) * 6 * 1024));
+  TC_CHECK(ctx->tp.ensure(sizeof(TreeParams)));
+  TC_CHECK(ctx->dshift.ensure(sizeof(double) * 3));
+  ctx->cap_n = n;
+  return 0;
+}
+
+static int ensure_cells(tc_context *ctx, int nc, int keep) {
+  if (nc <= ctx->cap_c) return 0;
+  int cap = std::max(nc, ctx->cap_c + ctx->cap_c / 2);
+  size_t NT = (size_t)nterm(ctx->cfg.p);
+  size_t ki = sizeof(int) * (size_t)keep, kd = sizeof(double) * (size_t)keep;
+  hipStream_t s = ctx->stream;
+  TC_CHECK(ctx->clevel.ensure(sizeof(int) * cap, ki, s));
+  TC_CHECK(ctx->cnchild.ensure(sizeof(int) * cap, ki, s));
+  TC_CHECK(ctx->cchild.ensure(sizeof(int) * cap, ki, s));
+  TC_CHECK(ctx->cbody.ensure(sizeof(int) * cap, ki, s));
+  TC_CHECK(ctx->cnbody.ensure(sizeof(int) * cap, ki, s));
+  TC_CHECK(ctx->cXg.ensure(sizeof(double) * 3 * cap, 3 * kd, s));
+  TC_CHECK(ctx->cRg.ensure(sizeof(double) * cap, kd, s));
+  TC_CHECK(ctx->cX.ensure(sizeof(double) * 3 * cap, 3 * kd, s));
+  TC_CHECK(ctx->cR.ensure(sizeof(double) * cap, kd, s));
+  TC_CHECK(ctx->cmin.ensure(sizeof(double) * cap, kd, s));
+  TC_CHECK(ctx->csink.ensure(cap, keep, s));
+  for (DBuf *b : {&ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd})
+    TC_CHECK(b->ensure(sizeof(double) * NT * cap));
+  for (DBuf *b : {&ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur, &ctx->m2l_coff, &ctx->p2p_cnt,
+                  &ctx->p2p_off, &ctx->p2p_cur})
+    TC_CHECK(b->ensure(sizeof(int) * (cap + 1)));
+  TC_CHECK(ctx->split_offs.ensure(sizeof(int) * 8 * cap));
+  TC_CHECK(ctx->split_nch.ensure(sizeof(int) * cap));
+  TC_CHECK(ctx->split_scan.ensure(sizeof(int) * cap));
+  TC_CHECK(ctx->scan_tmp.ensure(scan_temp_bytes(cap + 1) + 256));
+  TC_CHECK(ctx->leaf_ids.ensure(sizeof(int) * cap));
+  ctx->cap_c = cap;
+  return 0;
+}
+
+static int build_tree(tc_context *ctx, int n) {
+  hipStream_t s = ctx->stream;
+  BodiesDev b = bodies_of(ctx);
+  int ncrit = ctx->cfg.ncrit;
+  ctx->level_start.assign(1, 0);
+  int ls = 0, le = 1;
+  TC_CHECK(hipStreamSynchronize(s));
+  if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64), 0)) return e;
+  launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s);
+  int depth = 0;
+  while (le > ls) {
+    ctx->level_start.push_back(le);
+    int nl = le - ls;
+    // Reserve for the worst case (8 children per cell) before splitting, so no
+    // buffer the split writes is reallocated before the children are made.
+    if (int e = ensure_cells(ctx, le + 8 * nl, le)) return e;
+    CellsDev c = cells_of(ctx);
+    launch_split(c, b, ls, le, ncrit, ctx->split_offs.as<int>(), ctx->split_nch.as<int>(), s);
+    if (launch_scan(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, ctx->scan_tmp.p,
+                    ctx->scan_tmp.bytes, s))
+      return -4;
+    int last[2];
+    TC_CHECK(hipMemcpyAsync(&last[0], ctx->split_scan.as<int>() + nl - 1, sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipMemcpyAsync(&last[1], ctx->split_nch.as<int>() + nl - 1, sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    int total = last[0] + last[1];
+    if (total == 0) break;
+    launch_children(cells_of(ctx), ls, le, ctx->split_offs.as<int>(), ctx->split_scan.as<int>(),
+                    ctx->tp.as<TreeParams>(), s);
+    ls = le;
+    le = le + total;
+    depth++;
+  }
+  // level_start holds the start of every level and, last, the total cell count
+  int nc = ctx->level_start.back();
+  ctx->st.ncells = nc;
+  ctx->st.depth = depth;
+  ctx->st.nlevels = (int)ctx->level_start.size() - 1;
+  // leaf list (host side, one copy of the child counts)
+  std::vector<int> nch(nc);
+  std::vector<int> &leaves = ctx->h_leaves;  // outlives the asynchronous upload
+  leaves.clear();
+  TC_CHECK(hipMemcpyAsync(nch.data(), ctx->cnchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamSynchronize(s));
+  leaves.reserve(nc);
+  for (int i = 0; i < nc; i++)
+    if (nch[i] == 0) leaves.push_back(i);
+  ctx->nleaves = (int)leaves.size();
+  ctx->st.nleaves = ctx->nleaves;
+  TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves.data(), sizeof(int) * leaves.size(),
+                          hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+static int upward(tc_context *ctx) {
+  hipStream_t s = ctx->stream;
+  int p = ctx->cfg.p;
+  int nc = ctx->st.ncells;
+  size_t NT = (size_t)nterm(p);
+  TC_CHECK(hipMemsetAsync(ctx->cM.p, 0, sizeof(double) * NT * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->cMd.p, 0, sizeof(double) * NT * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->cL.p, 0, sizeof(double) * NT * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->cLd.p, 0, sizeof(double) * NT * nc, s));
+  CellsDev c = cells_of(ctx);
+  BodiesDev b = bodies_of(ctx);
+  if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s)) {
+    ctx->err = "expansion order not compiled";
+    return -2;
+  }
+  int L = (int)ctx->level_start.size() - 1;
+  for (int l = L - 1; l >= 0; l--)
+    launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s);
+  return 0;
+}
+
+static int grow_list(tc_context *ctx, DBuf &buf, int &cap, int need, int keep) {
+  int nc = std::max(need, cap + cap / 2);
+  TC_CHECK(buf.ensure(sizeof(int2) * (size_t)nc, sizeof(int2) * (size_t)keep, ctx->stream));
+  cap = nc;
+  return 0;
+}
+
+static int traverse(tc_context *ctx, int n) {
+  hipStream_t s = ctx->stream;
+  if (ctx->cap_front == 0) {
+    if (int e = grow_list(ctx, ctx->front_a, ctx->cap_front, std::max(1 << 16, 4 * n), 0)) return e;
+    int cf = 0;
+    if (int e = grow_list(ctx, ctx->front_b, cf, ctx->cap_front, 0)) return e;
+    if (int e = grow_list(ctx, ctx->m2l, ctx->cap_m2l, std::max(1 << 16, 24 * n), 0)) return e;
+    if (int e = grow_list(ctx, ctx->p2p, ctx->cap_p2p, std::max(1 << 16, 8 * n), 0)) return e;
+    TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 4));
+  }
+  int2 root = make_int2(0, 0);
+  TC_CHECK(hipMemcpyAsync(ctx->front_a.p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
+  int nin = 1, nm2l = 0, np2p = 0;
+  while (nin > 0) {
+    int cnt[3] = {0, nm2l, np2p};
+    TC_CHECK(hipMemcpyAsync(ctx->tcnt.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+    TravArgs a;
+    a.fin = ctx->front_a.as<int2>();
+    a.nin = nin;
+    a.fout = ctx->front_b.as<int2>();
+    a.cnt = ctx->tcnt.as<int>();
+    a.cap_front = ctx->cap_front;
+    a.cap_m2l = ctx->cap_m2l;
+    a.cap_p2p = ctx->cap_p2p;
+    a.m2l = ctx->m2l.as<int2>();
+    a.p2p = ctx->p2p.as<int2>();
+    a.theta = ctx->cfg.theta;
+    launch_traverse(cells_of(ctx), a, s);
+    int out[3];
+    TC_CHECK(hipMemcpyAsync(out, ctx->tcnt.p, sizeof(out), hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    if (out[0] > ctx->cap_front || out[1] > ctx->cap_m2l || out[2] > ctx->cap_p2p) {
+      // grow and redo this step from the same frontier
+      if (out[0] > ctx->cap_front) {
+        int cf = ctx->cap_front;
+        if (int e = grow_list(ctx, ctx->front_b, cf, out[0], 0)) return e;
+        int ca = ctx->cap_front;
+        if (int e = grow_list(ctx, ctx->front_a, ca, out[0], nin)) return e;
+        ctx->cap_front = std::min(ca, cf);
+      }
+      if (out[1] > ctx->cap_m2l)
+        if (int e = grow_list(ctx, ctx->m2l, ctx->cap_m2l, out[1], nm2l)) return e;
+      if (out[2] > ctx->cap_p2p)
+        if (int e = grow_list(ctx, ctx->p2p, ctx->cap_p2p, out[2], np2p)) return e;
+      continue;
+    }
+    nm2l = out[1];
+    np2p = out[2];
+    nin = out[0];
+    std::swap(ctx->front_a, ctx->front_b);
+  }
+  ctx->st.n_m2l = nm2l;
+  ctx->st.n_p2p_cells = np2p;
+  return 0;
+}
+
+// group (target, source) pairs by target: count, scan, scatter
+static int build_csr(tc_context *ctx, const int2 *pairs, int npairs, DBuf &cnt, DBuf &off,
+                     DBuf &cur, DBuf &src) {
+  hipStream_t s = ctx->stream;
+  int nc = ctx->st.ncells;
+  TC_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * (nc + 1), s));
+  TC_CHECK(hipMemsetAsync(cur.p, 0, sizeof(int) * (nc + 1), s));
+  TC_CHECK(src.ensure(sizeof(int) * (size_t)std::max(npairs, 1)));
+  launch_count_targets(pairs, npairs, cnt.as<int>(), s);
+  if (launch_scan(cnt.as<int>(), off.as<int>(), nc + 1, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s))
+    return -4;
+  launch_scatter_csr(pairs, npairs, off.as<int>(), cur.as<int>(), src.as<int>(), s);
+  return 0;
+}
+
+__global__ void k_chunk_counts(const int *__restrict__ cnt, int n, int *__restrict__ out) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i <= n) out[i] = i < n ? (cnt[i] + 63) / 64 : 0;
+}
+
+static int build_lists(tc_context *ctx, int &nchunks) {
+  hipStream_t s = ctx->stream;
+  int nc = ctx->st.ncells;
+  if (int e = build_csr(ctx, ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->m2l_cnt, ctx->m2l_off,
+                        ctx->m2l_cur, ctx->m2l_src))
+    return e;
+  if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
+                        ctx->p2p_off, ctx->p2p_cur, ctx->p2p_src))
+    return e;
+  // chunks of <= 64 sources per target for M2L
+  hipLaunchKernelGGL(k_chunk_counts, dim3((nc + 1 + 255) / 256), dim3(256), 0, s,
+                     ctx->m2l_cnt.as<int>(), nc, ctx->m2l_cur.as<int>());
+  if (launch_scan(ctx->m2l_cur.as<int>(), ctx->m2l_coff.as<int>(), nc + 1, ctx->scan_tmp.p,
+                  ctx->scan_tmp.bytes, s))
+    return -4;
+  TC_CHECK(hipMemcpyAsync(&nchunks, ctx->m2l_coff.as<int>() + nc, sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(ctx->m2l_chunks.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
+  launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
+                     ctx->m2l_chunks.as<int4>(), s);
+  return 0;
+}
+
+__global__ void k_p2p_pairs(const int2 *__restrict__ pairs, int n, const int *__restrict__ nbody,
+                            unsigned long long *__restrict__ out) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  unsigned long long v = 0;
+  if (k < n) v = (unsigned long long)nbody[pairs[k].x] * nbody[pairs[k].y];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                       const unsigned char *flags, double *pot, double *acc, double *jerk) {
+  hipStream_t s = ctx->stream;
+  const tc_config &cfg = ctx->cfg;
+  std::memset(&ctx->st, 0, sizeof(ctx->st));
+  if (n <= 0) {
+    ctx->err = "n must be positive";
+    return -1;
+  }
+  if (int e = ensure_bodies(ctx, n)) return e;
+  // root-shift values: three drand48() per evaluation, as getBounds draws them
+  double u[3];
+  if (ctx->fixed_shift) {
+    std::memcpy(u, ctx->shift, sizeof(u));
+  } else {
+    for (int d = 0; d < 3; d++) u[d] = ctx->rng.next() - 0.5;
+  }
+  std::memcpy(ctx->shift, u, sizeof(u));
+  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, u, sizeof(u), hipMemcpyHostToDevice, s));
+  TC_CHECK(hipEventRecord(ctx->ev[0], s));
+
+  if (cfg.direct_below > 0 && n < cfg.direct_below) {
+    // all-pairs sum in the caller's order (direct(), traverse_eager.h:295-304)
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    BodiesDev b = bodies_of(ctx);
+    TC_CHECK(ctx->dtg.ensure(sizeof(int) * n));
+    TC_CHECK(hipMemcpyAsync(ctx->dtg.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, b, s);
+    TC_CHECK(ctx->dpot.ensure(sizeof(double) * n));
+    TC_CHECK(ctx->dacc.ensure(sizeof(double) * 3 * n));
+    TC_CHECK(ctx->djerk.ensure(sizeof(double) * 3 * n));
+    launch_direct(b, n, ctx->dtg.as<int>(), n, ctx->dpot.as<double>(), ctx->dacc.as<double>(),
+                  ctx->djerk.as<double>(), s);
+    // non-sinks receive nothing
+    std::vector<unsigned char> fl(n, 3);
+    if (flags) TC_CHECK(hipMemcpyAsync(fl.data(), flags, n, hipMemcpyDeviceToHost, s));
+    std::vector<double> hp(n), ha(3 * n), hj(3 * n);
+    TC_CHECK(hipMemcpyAsync(hp.data(), ctx->dpot.p, 8 * n, hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipMemcpyAsync(ha.data(), ctx->dacc.p, 24 * n, hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipMemcpyAsync(hj.data(), ctx->djerk.p, 24 * n, hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++)
+      if (!(fl[i] & kSink)) {
+        hp[i] = 0;
+        for (int d = 0; d < 3; d++) ha[3 * i + d] = hj[3 * i + d] = 0;
+      }
+    if (pot) TC_CHECK(hipMemcpyAsync(pot, hp.data(), 8 * n, hipMemcpyHostToDevice, s));
+    if (acc) TC_CHECK(hipMemcpyAsync(acc, ha.data(), 24 * n, hipMemcpyHostToDevice, s));
+    if (jerk) TC_CHECK(hipMemcpyAsync(jerk, hj.data(), 24 * n, hipMemcpyHostToDevice, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    return 0;
+  }
+  if (cfg.mac != TC_MAC_THETA) {
+    ctx->err = "error-controlled MAC is not available in this build";
+    return -2;
+  }
+
+  // ---- tree
+  launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
+                ctx->dshift.as<double>(), s);
+  launch_morton(x, n, ctx->tp.as<TreeParams>(), ctx->key_hi.as<uint64_t>(),
+                ctx->key_a.as<uint64_t>(), ctx->val_a.as<int>(), s);
+  if (launch_sort(ctx->key_a.as<uint64_t>(), ctx->key_b.as<uint64_t>(), ctx->val_a.as<int>(),
+                  ctx->val_b.as<int>(), ctx->key_hi.as<uint64_t>(), n, ctx->sort_tmp.p,
+                  ctx->sort_tmp.bytes, s)) {
+    ctx->err = "radix sort failed";
+    return -4;
+  }
+  launch_gather(x, v, m, flags, ctx->val_a.as<int>(), n, bodies_of(ctx), s);
+  if (int e = build_tree(ctx, n)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[1], s));
+  // ---- upward
+  if (int e = upward(ctx)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[2], s));
+  // ---- traversal and lists
+  if (int e = traverse(ctx, n)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[3], s));
+  int nchunks = 0;
+  if (int e = build_lists(ctx, nchunks)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[4], s));
+  // ---- far field
+  M2LArgs ma;
+  ma.chunks = ctx->m2l_chunks.as<int4>();
+  ma.nchunks = nchunks;
+  ma.src = ctx->m2l_src.as<int>();
+  ma.cX = ctx->cX.as<double>();
+  ma.has_sink = ctx->csink.as<unsigned char>();
+  ma.M = ctx->cM.as<double>();
+  ma.Md = ctx->cMd.as<double>();
+  ma.L = ctx->cL.as<double>();
+  ma.Ld = ctx->cLd.as<double>();
+  ma.quirk = cfg.quirk_factorial;
+  if (launch_m2l(cfg.p, ma, s)) {
+    ctx->err = "expansion order not compiled";
+    return -2;
+  }
+  TC_CHECK(hipEventRecord(ctx->ev[5], s));
+  // ---- near field
+  ResultsDev r = results_of(ctx);
+  for (double *q : {r.pot, r.ax, r.ay, r.az, r.jx, r.jy, r.jz})
+    TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
+  launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->leaf_ids.as<int>(), ctx->nleaves,
+             ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), s);
+  TC_CHECK(hipEventRecord(ctx->ev[6], s));
+  // ---- downward
+  int L = (int)ctx->level_start.size() - 1;
+  for (int l = 0; l < L; l++)
+    launch_l2l(cfg.p, cells_of(ctx), ctx->level_start[l], ctx->level_start[l + 1], s);
+  launch_l2p(cfg.p, cells_of(ctx), bodies_of(ctx), r, ctx->leaf_ids.as<int>(), ctx->nleaves, s);
+  launch_scatter_results(bodies_of(ctx), r, n, pot, acc, jerk, s);
+  TC_CHECK(hipEventRecord(ctx->ev[7], s));
+  // body pairs in the P2P list (statistics)
+  TC_CHECK(ctx->dtg.ensure(sizeof(unsigned long long)));
+  TC_CHECK(hipMemsetAsync(ctx->dtg.p, 0, sizeof(unsigned long long), s));
+  if (ctx->st.n_p2p_cells > 0)
+    hipLaunchKernelGGL(k_p2p_pairs, dim3((int)((ctx->st.n_p2p_cells + 255) / 256)), dim3(256), 0,
+                       s, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->cnbody.as<int>(),
+                       ctx->dtg.as<unsigned long long>());
+  unsigned long long pairs = 0;
+  TC_CHECK(hipMemcpyAsync(&pairs, ctx->dtg.p, sizeof(pairs), hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamSynchronize(s));
+  ctx->st.n_p2p_pairs = (long long)pairs;
+  float ms[8];
+  for (int k = 0; k < 7; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
+  hipEventElapsedTime(&ms[7], ctx->ev[0], ctx->ev[7]);
+  ctx->st.ms_tree = ms[0];
+  ctx->st.ms_upward = ms[1];
+  ctx->st.ms_traverse = ms[2];
+  ctx->st.ms_lists = ms[3];
+  ctx->st.ms_m2l = ms[4];
+  ctx->st.ms_p2p = ms[5];
+  ctx->st.ms_downward = ms[6];
+  ctx->st.ms_total = ms[7];
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
+    return -3;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ C ABI ---
+extern "C" {
+
+int tc_supported_orders(int *orders, int maxn) {
+  int all[] = {
+#define TC_LIST(P) P,
+      TC_FOR_EACH_P(TC_LIST)
+#undef TC_LIST
+  };
+  int n = (int)(sizeof(all) / sizeof(all[0]));
+  for (int i = 0; i < n && i < maxn; i++) orders[i] = all[i];
+  return n;
+}
+
+static bool order_supported(int p) {
+  int o[64];
+  int n = tc_supported_orders(o, 64);
+  for (int i = 0; i < n; i++)
+    if (o[i] == p) return true;
+  return false;
+}
+
+int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
+  if (!cfg || !out) return -1;
+  if (!order_supported(cfg->p) || cfg->ncrit < 1 || !(cfg->theta > 0)) return -1;
+  tc_context *ctx = new tc_context();
+  ctx->cfg = *cfg;
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+      delete ctx;
+      return -3;
+    }
+    ctx->own_stream = true;
+  }
+  for (auto &e : ctx->ev) hipEventCreate(&e);
+  *out = ctx;
+  return 0;
+}
+
+int tc_destroy(tc_context *ctx) {
+  if (!ctx) return -1;
+  hipStreamSynchronize(ctx->stream);
+  DBuf *all[] = {&ctx->bx, &ctx->by, &ctx->bz, &ctx->bvx, &ctx->bvy, &ctx->bvz, &ctx->bq,
+                 &ctx->bflags, &ctx->border, &ctx->bacc, &ctx->rpot, &ctx->rax, &ctx->ray,
+                 &ctx->raz, &ctx->rjx, &ctx->rjy, &ctx->rjz, &ctx->key_hi, &ctx->key_a,
+                 &ctx->key_b, &ctx->val_a, &ctx->val_b, &ctx->sort_tmp, &ctx->part, &ctx->tp,
+                 &ctx->dshift, &ctx->clevel, &ctx->cnchild, &ctx->cchild, &ctx->cbody,
+                 &ctx->cnbody, &ctx->cXg, &ctx->cRg, &ctx->cX, &ctx->cR, &ctx->cmin, &ctx->csink,
+                 &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
+                 &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
+                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
+                 &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
+                 &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl};
+  for (DBuf *b : all) b->release();
+  for (auto &e : ctx->ev) hipEventDestroy(e);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int tc_set_seed(tc_context *ctx, long seed) {
+  if (!ctx) return -1;
+  ctx->rng.seed(seed);
+  ctx->fixed_shift = false;
+  return 0;
+}
+
+int tc_set_shift(tc_context *ctx, const double frac[3]) {
+  if (!ctx || !frac) return -1;
+  for (int d = 0; d < 3; d++) {
+    if (!(frac[d] >= -0.5 && frac[d] < 0.5)) return -1;
+    ctx->shift[d] = frac[d];
+  }
+  ctx->fixed_shift = true;
+  return 0;
+}
+
+int tc_last_shift(const tc_context *ctx, double frac[3]) {
+  if (!ctx || !frac) return -1;
+  for (int d = 0; d < 3; d++) frac[d] = ctx->shift[d];
+  return 0;
+}
+
+int tc_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                const unsigned char *flags, double *pot, double *acc, double *jerk) {
+  if (!ctx) return -1;
+  if (!x || !v || !m) {
+    ctx->err = "x, v and m are required";
+    return -1;
+  }
+  return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+}
+
+int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
+  if (!ctx) return -1;
+  if (!bodies || n <= 0) {
+    ctx->err = "bodies must be non-empty";
+    return -1;
+  }
+  hipStream_t s = ctx->stream;
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), m(n);
+  std::vector<unsigned char> fl(n);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      x[3 * i + d] = bodies[i].X[d];
+      v[3 * i + d] = bodies[i].V[d];
+    }
+    m[i] = bodies[i].q;
+    fl[i] = (bodies[i].issink ? kSink : 0) | (bodies[i].issource ? kSource : 0);
+  }
+  TC_CHECK(ctx->dx3.ensure(24 * (size_t)n));
+  TC_CHECK(ctx->dv3.ensure(24 * (size_t)n));
+  TC_CHECK(ctx->dm.ensure(8 * (size_t)n));
+  TC_CHECK(ctx->dfl.ensure(n));
+  TC_CHECK(ctx->dpot.ensure(8 * (size_t)n));
+  TC_CHECK(ctx->dacc.ensure(24 * (size_t)n));
+  TC_CHECK(ctx->djerk.ensure(24 * (size_t)n));
+  TC_CHECK(hipMemcpyAsync(ctx->dx3.p, x.data(), 24 * (size_t)n, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dv3.p, v.data(), 24 * (size_t)n, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dm.p, m.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dfl.p, fl.data(), n, hipMemcpyHostToDevice, s));
+  // the all-pairs branch writes into dpot/dacc/djerk itself; use separate outputs
+  DBuf op, oa, oj;
+  TC_CHECK(op.ensure(8 * (size_t)n));
+  TC_CHECK(oa.ensure(24 * (size_t)n));
+  TC_CHECK(oj.ensure(24 * (size_t)n));
+  int e = eval_device(ctx, n, ctx->dx3.as<double>(), ctx->dv3.as<double>(), ctx->dm.as<double>(),
+                      ctx->dfl.as<unsigned char>(), op.as<double>(), oa.as<double>(),
+                      oj.as<double>());
+  if (e == 0) {
+    std::vector<double> hp(n), ha(3 * (size_t)n), hj(3 * (size_t)n);
+    hipMemcpyAsync(hp.data(), op.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(ha.data(), oa.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(hj.data(), oj.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    for (int i = 0; i < n; i++) {
+      bodies[i].p = hp[i];
+      for (int d = 0; d < 3; d++) {
+        bodies[i].F[d] = ha[3 * i + d];
+        bodies[i].J[d] = hj[3 * i + d];
+      }
+    }
+  }
+  op.release();
+  oa.release();
+  oj.release();
+  return e;
+}
+
+int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                  const unsigned char *flags, int nt, const int *targets, double *pot,
+                  double *acc, double *jerk) {
+  if (!ctx) return -1;
+  if (n <= 0 || nt < 0 || !x || !v || !m || (nt > 0 && !targets)) {
+    ctx->err = "bad arguments";
+    return -1;
+  }
+  if (int e = ensure_bodies(ctx, n)) return e;
+  hipStream_t s = ctx->stream;
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  TC_CHECK(ctx->dtg.ensure(sizeof(int) * (size_t)n));
+  TC_CHECK(hipMemcpyAsync(ctx->dtg.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, bodies_of(ctx), s);
+  launch_direct(bodies_of(ctx), n, targets, nt, pot, acc, jerk, s);
+  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(hipGetLastError());
+  return 0;
+}
+
+int tc_get_stats(const tc_context *ctx, tc_stats *out) {
+  if (!ctx || !out) return -1;
+  *out = ctx->st;
+  return 0;
+}
+
+const char *tc_last_error(const tc_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+}  // extern "C"

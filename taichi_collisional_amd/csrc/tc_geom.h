@@ -1,0 +1,165 @@
+// Smallest enclosing ball, computed by one wave (device side of the cell
+// geometry of P2M / M2M, taichi_with_jerks/kernel_fast_lean.h:733-763 and
+// :875-935, which use the Miniball of miniball.hpp).
+//
+// Algorithm (same as the CPU oracle, so both produce the same ball up to
+// rounding): start from point 0; repeatedly find the point of largest excess
+// |q-c|^2 - r^2 (lanes scan strided subsets, then a wave argmax with ties to
+// the lowest index); if it lies outside, the new ball is the smallest ball of
+// (support set + q) with q on its boundary, found by testing the circumball of
+// q with every subset of the support (<= 15 candidates, each of <= 4 points).
+// The radius grows strictly, so the loop ends; the final ball encloses every
+// point and is the minimum ball of its own support, hence of all points.
+// Every lane runs the small solves on identical data, so the result is
+// uniform across the wave.  Contraction is off to match the CPU oracle.
+#pragma once
+#include "tc_common.h"
+
+namespace tcfmm {
+
+struct DBall {
+  double c[3];
+  double r2;
+};
+
+__device__ inline bool circumball_dev(const double (*p)[3], int k, DBall &b) {
+#pragma clang fp contract(off)
+  if (k == 1) {
+    for (int d = 0; d < 3; d++) b.c[d] = p[0][d];
+    b.r2 = 0;
+    return true;
+  }
+  double v[3][3];
+  for (int i = 1; i < k; i++)
+    for (int d = 0; d < 3; d++) v[i - 1][d] = p[i][d] - p[0][d];
+  int q = k - 1;
+  double A[3][3], rhs[3];
+  for (int i = 0; i < q; i++) {
+    for (int j = 0; j < q; j++) A[i][j] = v[i][0] * v[j][0] + v[i][1] * v[j][1] + v[i][2] * v[j][2];
+    rhs[i] = 0.5 * A[i][i];
+  }
+  double lam[3] = {0, 0, 0};
+  if (q == 1) {
+    if (!(A[0][0] > 0)) return false;
+    lam[0] = 0.5;
+  } else if (q == 2) {
+    double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    if (!(fabs(det) > 1e-13 * A[0][0] * A[1][1])) return false;
+    lam[0] = (rhs[0] * A[1][1] - A[0][1] * rhs[1]) / det;
+    lam[1] = (A[0][0] * rhs[1] - rhs[0] * A[1][0]) / det;
+  } else {
+    double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    if (!(fabs(det) > 1e-13 * A[0][0] * A[1][1] * A[2][2])) return false;
+    double inv[3][3];
+    inv[0][0] = c00;
+    inv[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    inv[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    inv[1][0] = c01;
+    inv[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    inv[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    inv[2][0] = c02;
+    inv[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    inv[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    for (int i = 0; i < 3; i++)
+      lam[i] = (inv[i][0] * rhs[0] + inv[i][1] * rhs[1] + inv[i][2] * rhs[2]) / det;
+  }
+  for (int d = 0; d < 3; d++) {
+    double c = p[0][d];
+    for (int i = 0; i < q; i++) c += lam[i] * v[i][d];
+    b.c[d] = c;
+  }
+  double r2 = 0;
+  for (int i = 0; i < k; i++) {
+    double dd = 0;
+    for (int d = 0; d < 3; d++) dd += (p[i][d] - b.c[d]) * (p[i][d] - b.c[d]);
+    r2 = fmax(r2, dd);
+  }
+  b.r2 = r2;
+  return true;
+}
+
+constexpr double kBallTol = 1e-13;
+
+template <class Get>
+__device__ DBall wave_minball(int n, Get get) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (kWave - 1);
+  DBall b;
+  double sup[4][3];
+  int ns = 1;
+  get(0, sup[0]);
+  for (int d = 0; d < 3; d++) b.c[d] = sup[0][d];
+  b.r2 = 0;
+  for (int iter = 0; iter < 10000; iter++) {
+    double best = -1;
+    int bidx = 0x7fffffff;
+    for (int i = lane; i < n; i += kWave) {
+      double q[3];
+      get(i, q);
+      double e = 0;
+      for (int d = 0; d < 3; d++) e += (q[d] - b.c[d]) * (q[d] - b.c[d]);
+      e -= b.r2;
+      if (e > best) {
+        best = e;
+        bidx = i;
+      }
+    }
+    wave_argmax(best, bidx);
+    if (best <= kBallTol * b.r2 || best <= 0) break;
+    double bq[3];
+    get(bidx, bq);
+    DBall bestb;
+    bestb.r2 = -1;
+    int bestmask = 0;
+    for (int mask = 0; mask < (1 << ns); mask++) {
+      int cnt = __popc(mask);
+      if (cnt > 3) continue;
+      double pts[4][3];
+      int k = 0;
+      for (int d = 0; d < 3; d++) pts[k][d] = bq[d];
+      k++;
+      for (int s = 0; s < ns; s++)
+        if (mask & (1 << s)) {
+          for (int d = 0; d < 3; d++) pts[k][d] = sup[s][d];
+          k++;
+        }
+      DBall cb;
+      if (!circumball_dev(pts, k, cb)) continue;
+      if (bestb.r2 >= 0 && cb.r2 >= bestb.r2) continue;
+      bool ok = true;
+      for (int s = 0; s < ns && ok; s++) {
+        if (mask & (1 << s)) continue;
+        double dd = 0;
+        for (int d = 0; d < 3; d++) dd += (sup[s][d] - cb.c[d]) * (sup[s][d] - cb.c[d]);
+        if (dd > cb.r2 * (1 + 1e-12)) ok = false;
+      }
+      if (!ok) continue;
+      bestb = cb;
+      bestmask = mask;
+    }
+    if (bestb.r2 < 0) break;
+    double nsup[4][3];
+    int k = 0;
+    for (int d = 0; d < 3; d++) nsup[k][d] = bq[d];
+    k++;
+    for (int s = 0; s < ns; s++)
+      if (bestmask & (1 << s)) {
+        for (int d = 0; d < 3; d++) nsup[k][d] = sup[s][d];
+        k++;
+      }
+    for (int s = 0; s < 4; s++)
+      for (int d = 0; d < 3; d++) sup[s][d] = nsup[s][d];
+    ns = k;
+    if (bestb.r2 <= b.r2) {
+      b = bestb;
+      break;
+    }
+    b = bestb;
+  }
+  return b;
+}
+
+}  // namespace tcfmm

@@ -1,0 +1,106 @@
+// Solid harmonics on the device (make_Gnm / make_Gnmdot,
+// taichi_with_jerks/kernel_fast_lean.h:237-304).
+//
+// Y_n^m = r^n P_n^m(cos t) e^{i m phi} / (n+m)!  (no Condon-Shortley phase):
+//   Y_0^0 = 1,  Y_n^n = Y_{n-1}^{n-1} (x + i y)/(2n),  Y_n^{n-1} = z Y_{n-1}^{n-1},
+//   Y_n^m = [(2n-1) z Y_{n-1}^m - r^2 Y_{n-2}^m] / (n^2 - m^2),
+//   Y_n^{-m} = (-1)^m conj(Y_n^m).
+// Time derivative for a point moving with velocity v:
+//   dY_n^m/dt = eta' Y_{n-1}^{m+1} + z' Y_{n-1}^m + xi' Y_{n-1}^{m-1},
+//   eta' = (-v_x + i v_y)/2, z' = v_z, xi' = (v_x + i v_y)/2; Y_0^{-1} is taken
+//   as 0 (the reference reads the slot before its array there).
+#pragma once
+#include "tc_common.h"
+
+namespace tcfmm {
+
+__host__ __device__ constexpr int tri(int n, int m) { return n * (n + 1) / 2 + m; }
+
+// All Y_n^m, m >= 0, n <= P into y[tri(n,m)] (per lane, fully unrolled).
+template <int P>
+__device__ __forceinline__ void harm_all(double dx, double dy, double dz, cplx *y) {
+  double r2 = dx * dx + dy * dy + dz * dz;
+  y[0] = {1.0, 0.0};
+#pragma unroll
+  for (int n = 1; n <= P; n++) {
+    y[tri(n, n)] = cmul(y[tri(n - 1, n - 1)], {dx / (2 * n), dy / (2 * n)});
+#pragma unroll
+    for (int m = 0; m < n - 1; m++) {
+      double q = (double)(n * n - m * m);
+      double a = (2 * n - 1.0) / q * dz, b = r2 / q;
+      y[tri(n, m)] = {a * y[tri(n - 1, m)].re - b * y[tri(n - 2, m)].re,
+                      a * y[tri(n - 1, m)].im - b * y[tri(n - 2, m)].im};
+    }
+    y[tri(n, n - 1)] = cscale(dz, y[tri(n - 1, n - 1)]);
+  }
+}
+
+// Y_k^l for any l (|l| <= k) from the m >= 0 table.
+__device__ __forceinline__ cplx harm_at(const cplx *y, int k, int l) {
+  if (l >= 0) return y[tri(k, l)];
+  cplx c = cconj(y[tri(k, -l)]);
+  return (l & 1) ? cplx{-c.re, -c.im} : c;
+}
+
+// dY_n^m/dt for m >= 0 (n >= 0) from the table of Y (orders <= n-1 needed).
+__device__ __forceinline__ cplx harm_dot(const cplx *y, int n, int m, double vx, double vy,
+                                         double vz) {
+  cplx r = {0.0, 0.0};
+  if (n == 0) return r;
+  const cplx eta = {-vx * 0.5, vy * 0.5}, xi = {vx * 0.5, vy * 0.5};
+  if (m <= n - 2) r = cadd(r, cmul(y[tri(n - 1, m + 1)], eta));
+  if (m <= n - 1) r = cadd(r, cscale(vz, y[tri(n - 1, m)]));
+  if (m >= 1)
+    r = cadd(r, cmul(y[tri(n - 1, m - 1)], xi));
+  else if (n >= 2)
+    r = cadd(r, cmul(harm_at(y, n - 1, -1), xi));
+  return r;
+}
+
+// Cooperative table of Y_n^m for all n <= P, -n <= m <= n into LDS
+// (real-packed index order, complex values): lane m computes column m.
+template <int P>
+__device__ __forceinline__ void harm_table_lds(double dx, double dy, double dz, cplx *G) {
+  const int m = threadIdx.x;
+  if (m <= P) {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    cplx y = {1.0, 0.0};
+    for (int k = 1; k <= m; k++) y = cmul(y, {dx / (2 * k), dy / (2 * k)});
+    cplx y2 = y, y1 = y;  // Y_{n-2}^m, Y_{n-1}^m
+    auto put = [&](int n, cplx v) {
+      G[ixnm(n, m)] = v;
+      if (m > 0) {
+        cplx c = cconj(v);
+        G[ixnm(n, -m)] = (m & 1) ? cplx{-c.re, -c.im} : c;
+      }
+    };
+    put(m, y);
+    if (m + 1 <= P) {
+      y1 = cscale(dz, y);
+      put(m + 1, y1);
+      y2 = y;
+      for (int n = m + 2; n <= P; n++) {
+        double q = (double)(n * n - m * m);
+        double a = (2 * n - 1.0) / q * dz, b = r2 / q;
+        cplx yn = {a * y1.re - b * y2.re, a * y1.im - b * y2.im};
+        put(n, yn);
+        y2 = y1;
+        y1 = yn;
+      }
+    }
+  }
+}
+
+// Complex form of a real-packed expansion (real_2_complex,
+// kernel_fast_lean.h:164-179): c_n^m = (a[s+m], a[s-m]), c_n^-m = (-1)^m conj.
+__device__ __forceinline__ cplx packed_to_c(const double *a, int n, int m) {
+  int s = n * n + n;
+  int am = m < 0 ? -m : m;
+  if (am == 0) return {a[s], 0.0};
+  cplx c = {a[s + am], a[s - am]};
+  if (m > 0) return c;
+  c = cconj(c);
+  return (am & 1) ? cplx{-c.re, -c.im} : c;
+}
+
+}  // namespace tcfmm

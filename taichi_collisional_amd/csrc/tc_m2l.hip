@@ -1,0 +1,281 @@
+// M2L: rotation-based multipole-to-local translation, FP64, gfx950.
+//
+// Follows M2L_rotate (taichi_with_jerks/kernel_fast_lean.h:1130-1371): per
+// (target Ci, source Cj) pair the source moments are scaled to homogeneous
+// form, rotated about z by alpha_z, x<->z swapped, rotated by alpha_x, swapped
+// back (so the pair axis lies along z), translated along z, and rotated back
+// with the same four steps in reverse; the result is added to Ci's local
+// expansion.  The same operator maps Mdot to Ldot.
+//
+// GPU form.  The swap S_n is the same matrix for every pair, so it is applied
+// as a product of constant sparse blocks against each lane's coefficients:
+// one lane = one (target, source) pair, the integer swap entries
+// (k * 2^-(n-1)) are folded into the instruction stream as literal operands of
+// v_fmac_f64, and only the per-pair z-rotations and translation factors live in
+// registers.  Work that the translation cannot see is skipped: after the
+// second forward swap only components |m| <= min(n, p-n) are formed, and the
+// first backward swap only reads those.  The quirk of the reference's
+// factorial table (factorial_table[2p] == 0, kernel_fast_lean.h:129) removes
+// M_p^{+-p} from the sum; with quirk=1 those inputs are zeroed.
+//
+// Work decomposition: a block = one wave = one chunk of <= 64 sources of a
+// single target cell (the chunk list is grouped by target); blockIdx.y picks
+// the array (0: M -> L, 1: Mdot -> Ldot).  Each order's 2n+1 contributions are
+// summed across the wave and staged in LDS; the chunk adds its sum into the
+// target's L with one atomic per coefficient.
+#include "tc_common.h"
+#include "tc_kernels.h"
+#include "swap_tables.h"
+
+namespace tcfmm {
+
+// Scaling (n+|m|)!(n-|m|)! and its inverse per real-packed term, loaded with
+// scalar loads (uniform across the wave).  Filled by m2l_init_tables().
+__constant__ double c_scale[kMaxNT];
+__constant__ double c_scale_inv[kMaxNT];
+__constant__ double c_fact[2 * kMaxP + 2];
+
+template <int N>
+__device__ __forceinline__ void rot_fwd(double *v, const double *cr, const double *ci) {
+#pragma unroll
+  for (int m = 1; m <= N; m++) {
+    double c = cr[m], s = ci[m];
+    double im = v[N - m], re = v[N + m];
+    v[N - m] = c * im + s * re;
+    v[N + m] = c * re - s * im;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void rot_bwd(double *v, const double *cr, const double *ci) {
+#pragma unroll
+  for (int m = 1; m <= N; m++) {
+    double c = cr[m], s = ci[m];
+    double im = v[N - m], re = v[N + m];
+    v[N - m] = c * im - s * re;
+    v[N + m] = c * re + s * im;
+  }
+}
+
+// out rows |r-N| <= QR from input columns |c-N| <= QC (everything else 0).
+// S_n only couples components of one parity class (Re/Im part and parity of
+// m); rows are produced class by class so each class's inputs die early.
+template <int N, int QR, int QC>
+__device__ __forceinline__ void swap_block(const double *in, double *out) {
+#pragma unroll
+  for (int cls = 0; cls < 4; cls++) {
+#pragma unroll
+    for (int r = 0; r < 2 * N + 1; r++) {
+      const int m = r - N;
+      const int rc = (m >= 0 ? 0 : 2) + ((m < 0 ? -m : m) & 1);
+      if (rc != cls) continue;
+      double acc = 0.0;
+      if (m <= QR && -m <= QR) {
+#pragma unroll
+        for (int e = kSwapRowPtr[N * N + r]; e < kSwapRowPtr[N * N + r + 1]; e++) {
+          const int c = kSwapCol[e];
+          if (c - N <= QC && N - c <= QC) acc += kSwapVal[e] * in[c];
+        }
+      }
+      out[r] = acc;
+    }
+  }
+}
+
+template <int P>
+struct M2LRegs {
+  static constexpr int q(int n) { return n < P - n ? n : P - n; }
+  // offset of order n in the packed rotated-multipole store (|m| <= q(n))
+  static constexpr int off(int n) {
+    int o = 0;
+    for (int k = 0; k < n; k++) o += 2 * q(k) + 1;
+    return o;
+  }
+  static constexpr int total = off(P + 1);
+};
+
+template <int P, int N>
+__device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src, bool quirk,
+                                                  const double *ezr, const double *ezi,
+                                                  const double *exr, const double *exi,
+                                                  double *mp) {
+  constexpr int K = 2 * N + 1;
+  constexpr int Q = M2LRegs<P>::q(N);
+  constexpr int O = M2LRegs<P>::off(N);
+  double v[K], w[K];
+#pragma unroll
+  for (int r = 0; r < K; r++) v[r] = src[N * N + r] * c_scale[N * N + r];
+  if (N == P && quirk) {
+    v[0] = 0.0;
+    v[2 * N] = 0.0;
+  }
+  rot_fwd<N>(v, ezr, ezi);
+  swap_block<N, N, N>(v, w);
+  rot_fwd<N>(w, exr, exi);
+  swap_block<N, Q, N>(w, v);
+#pragma unroll
+  for (int m = -Q; m <= Q; m++) mp[O + Q + m] = v[N + m] * c_scale_inv[N * N + N + m];
+}
+
+template <int P, int N>
+__device__ __forceinline__ void m2l_backward_order(const double *mp, const double *f,
+                                                   const double *ezr, const double *ezi,
+                                                   const double *exr, const double *exi,
+                                                   bool active, double *acc_lds, int lane) {
+  constexpr int K = 2 * N + 1;
+  constexpr int Q = M2LRegs<P>::q(N);
+  double u[K], w[K];
+#pragma unroll
+  for (int r = 0; r < K; r++) u[r] = 0.0;
+  // translation along z (kernel_fast_lean.h:1253-1295):
+  // L_n^{+-m} += (-1)^m sum_{k=m}^{p-n} M_k^{+-m} (n+k)! r^-(n+k+1)
+#pragma unroll
+  for (int k = 0; k <= P - N; k++) {
+    const int qk = M2LRegs<P>::q(k);
+    const int ok = M2LRegs<P>::off(k);
+    u[N] += mp[ok + qk] * f[N + k];
+  }
+#pragma unroll
+  for (int m = 1; m <= Q; m++) {
+#pragma unroll
+    for (int k = m; k <= P - N; k++) {
+      const int qk = M2LRegs<P>::q(k);
+      const int ok = M2LRegs<P>::off(k);
+      double fk = (m & 1) ? -f[N + k] : f[N + k];
+      u[N - m] += mp[ok + qk - m] * fk;
+      u[N + m] += mp[ok + qk + m] * fk;
+    }
+  }
+  swap_block<N, N, Q>(u, w);
+  rot_bwd<N>(w, exr, exi);
+  swap_block<N, N, N>(w, u);
+  rot_bwd<N>(u, ezr, ezi);
+#pragma unroll
+  for (int r = 0; r < K; r++) {
+    double s = wave_sum(active ? u[r] : 0.0);
+    if (lane == 0) acc_lds[N * N + r] = s;
+  }
+}
+
+// Orders run from p down to 0: the large orders (biggest working set) run
+// while few rotated moments are live.
+template <int P, int... Ns>
+__device__ __forceinline__ void m2l_forward_all(const double *src, bool quirk, const double *ezr,
+                                                const double *ezi, const double *exr,
+                                                const double *exi, double *mp,
+                                                IntSeq<Ns...>) {
+  (m2l_forward_order<P, P - Ns>(src, quirk, ezr, ezi, exr, exi, mp), ...);
+}
+
+template <int P, int... Ns>
+__device__ __forceinline__ void m2l_backward_all(const double *mp, const double *f,
+                                                 const double *ezr, const double *ezi,
+                                                 const double *exr, const double *exi,
+                                                 bool active, double *acc, int lane,
+                                                 IntSeq<Ns...>) {
+  (m2l_backward_order<P, Ns>(mp, f, ezr, ezi, exr, exi, active, acc, lane), ...);
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
+                                            const int *__restrict__ src_list,
+                                            const double *__restrict__ cX,
+                                            const unsigned char *__restrict__ has_sink,
+                                            const double *__restrict__ M,
+                                            const double *__restrict__ Md,
+                                            double *__restrict__ L, double *__restrict__ Ld,
+                                            int quirk) {
+  constexpr int NT = nterm(P);
+  __shared__ double acc[NT];
+  const int lane = threadIdx.x;
+  const int4 ch = chunks[blockIdx.x];
+  const int tgt = ch.x, start = ch.y, cnt = ch.z;
+  if (!has_sink[tgt]) return;
+  const double *Msrc = blockIdx.y ? Md : M;
+  double *Lout = blockIdx.y ? Ld : L;
+  const bool active = lane < cnt;
+  const int j = src_list[start + (active ? lane : 0)];
+
+  double dx = cX[3 * tgt + 0] - cX[3 * j + 0];
+  double dy = cX[3 * tgt + 1] - cX[3 * j + 1];
+  double dz = cX[3 * tgt + 2] - cX[3 * j + 2];
+  double rxy2 = dx * dx + dy * dy;
+  double rxy = sqrt(rxy2);
+  double r = sqrt(rxy2 + dz * dz);
+  double ri = 1.0 / r;
+  double ezr[P + 1], ezi[P + 1], exr[P + 1], exi[P + 1], f[P + 1];
+  ezr[0] = 1.0, ezi[0] = 0.0, exr[0] = 1.0, exi[0] = 0.0;
+  ezr[1] = 1.0, ezi[1] = 0.0;
+  exr[1] = dz * ri;
+  exi[1] = -rxy * ri;
+  if (rxy > 0) {
+    double q = 1.0 / rxy;
+    ezr[1] = dy * q;
+    ezi[1] = dx * q;
+  }
+#pragma unroll
+  for (int m = 1; m < P; m++) {
+    ezr[m + 1] = ezr[m] * ezr[1] - ezi[m] * ezi[1];
+    ezi[m + 1] = ezr[m] * ezi[1] + ezi[m] * ezr[1];
+    exr[m + 1] = exr[m] * exr[1] - exi[m] * exi[1];
+    exi[m + 1] = exr[m] * exi[1] + exi[m] * exr[1];
+  }
+  double rp = ri;
+#pragma unroll
+  for (int k = 0; k <= P; k++) {
+    f[k] = c_fact[k] * rp;
+    rp *= ri;
+  }
+  double mp[M2LRegs<P>::total];
+  const double *src = Msrc + (size_t)j * NT;
+  m2l_forward_all<P>(src, quirk != 0, ezr, ezi, exr, exi, mp, MakeIntSeq<P + 1>{});
+  m2l_backward_all<P>(mp, f, ezr, ezi, exr, exi, active, acc, lane, MakeIntSeq<P + 1>{});
+  __syncthreads();
+  double *dst = Lout + (size_t)tgt * NT;
+  for (int t = lane; t < NT; t += kWave) atomicAdd(dst + t, acc[t]);
+}
+
+void m2l_init_tables(int p, int quirk) {
+  static int done_p = -1, done_q = -1;
+  if (done_p == p && done_q == quirk) return;
+  double fact[2 * kMaxP + 2];
+  fact[0] = 1.0;
+  for (int i = 1; i < 2 * kMaxP + 2; i++) fact[i] = fact[i - 1] * i;
+  double sc[kMaxNT], si[kMaxNT];
+  for (int n = 0; n <= p; n++)
+    for (int m = -n; m <= n; m++) {
+      int a = m < 0 ? -m : m;
+      double s = fact[n + a] * fact[n - a];
+      sc[ixnm(n, m)] = s;
+      si[ixnm(n, m)] = 1.0 / s;
+    }
+  hipMemcpyToSymbol(HIP_SYMBOL(c_scale), sc, sizeof(double) * nterm(p));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_scale_inv), si, sizeof(double) * nterm(p));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_fact), fact, sizeof(fact));
+  done_p = p;
+  done_q = quirk;
+}
+
+template <int P>
+static void launch(const M2LArgs &a, hipStream_t s) {
+  if (a.nchunks <= 0) return;
+  dim3 grid(a.nchunks, 2);
+  hipLaunchKernelGGL(k_m2l<P>, grid, dim3(64), 0, s, a.chunks, a.src, a.cX, a.has_sink, a.M,
+                     a.Md, a.L, a.Ld, a.quirk);
+}
+
+int launch_m2l(int p, const M2LArgs &a, hipStream_t s) {
+  m2l_init_tables(p, a.quirk);
+  switch (p) {
+#define TC_CASE(P) \
+  case P:          \
+    launch<P>(a, s); \
+    return 0;
+    TC_FOR_EACH_P(TC_CASE)
+#undef TC_CASE
+    default:
+      return -1;
+  }
+}
+
+}  // namespace tcfmm
