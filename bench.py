@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark: particles/s per FMM force+jerk evaluation on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--p P] [--theta T]
+                    [--ncrit C] [--no-cpu] [--gpus 1]
+
+Workload (default): Plummer sphere N = 2^20 (Henon units, seed 1), expansion
+order p = 12, opening angle theta = 0.4, ncrit = 64, FP64 throughout; a
+"step" is one full evaluation (tree build through L2P) of potential,
+acceleration and jerk for every body, with positions, velocities and masses
+already resident in HBM.  Prints one JSON line.
+
+The CPU baseline is the reference implementation itself (compiled from its
+sources into oracle/_ref/), run in a child process on the host cores before
+the GPU is touched, on the same workload (one evaluation, theta-MAC passes).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/s per FMM force+jerk eval at N=1M, rel force err <=1e-7 vs FP64 direct"
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(n, p, theta, ncrit, seed, threads, timeout):
+    """Time the reference (oracle/_ref) on the host in a child process."""
+    code = r"""
+import sys, time, json, os
+sys.path.insert(0, %r); sys.path.insert(0, os.path.join(%r, "oracle"))
+import numpy as np, pyoracle as po
+from taichi_collisional_amd import ics
+n, p, theta, ncrit, seed = %d, %d, %r, %d, %d
+x, v, m = ics.plummer(n, seed=seed)
+if not po.ref_available(p):
+    print(json.dumps({"error": "reference harness for p=%%d not built" %% p})); sys.exit(0)
+L = po.reference(p)
+t = time.time()
+pot, acc, jerk, acco, counts, shift = po.ref_eval(p, x, v, m, seed, ncrit=ncrit, theta=theta, mode=0)
+dt = time.time() - t
+print(json.dumps({"seconds": dt, "isa": L.isa, "counts": [int(c) for c in counts]}))
+""" % (ROOT, ROOT, n, p, theta, ncrit, seed)
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               OMP_WAIT_POLICY="active", OMP_STACKSIZE="100M")
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=timeout, env=env)
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # noqa: BLE001 - reported in the JSON line
+        return {"value": None, "unit": "particles/s", "cores": threads, "kind": "reference",
+                "sample": "failed: %s" % str(e)[:200]}
+    if "seconds" not in out:
+        return {"value": None, "unit": "particles/s", "cores": threads, "kind": "reference",
+                "sample": out.get("error", "unavailable")}
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": n / out["seconds"], "unit": "particles/s", "cores": threads,
+            "kind": "reference",
+            "sample": "one full evaluation of the same workload (theta-MAC passes, N=%d, p=%d, "
+                      "theta=%g, ncrit=%d), %.2f s wall, %s build, OpenMP %d threads, %s"
+                      % (n, p, theta, ncrit, out["seconds"], out["isa"], threads, cpu_model),
+            "seconds": out["seconds"], "m2l_pairs": out["counts"][0]}
+
+
+def load_traffic(path):
+    try:
+        d = json.load(open(path))
+        return d.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--p", type=int, default=12)
+    ap.add_argument("--theta", type=float, default=0.4)
+    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-timeout", type=float, default=150.0)
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "m2l_traffic.json"))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.n, args.p, args.theta, args.ncrit, args.seed, args.cpu_threads,
+                           args.cpu_timeout)
+
+    import numpy as np
+    import torch
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import flops, ics
+
+    x, v, m = ics.plummer(args.n, seed=args.seed)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    dv = torch.from_numpy(v).to(dev)
+    dm = torch.from_numpy(m).to(dev)
+    fmm = tca.Fmm(p=args.p, ncrit=args.ncrit, theta=args.theta, direct_below=0)
+    fmm.set_seed(args.seed)
+    pot = torch.empty(args.n, dtype=torch.float64, device=dev)
+    acc = torch.empty(args.n, 3, dtype=torch.float64, device=dev)
+    jerk = torch.empty(args.n, 3, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+        stats.append(fmm.stats())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = args.n / (ms * 1e-3)
+
+    st = stats[-1]
+    n_edges = st["ncells"] - 1
+    fl = flops.eval_flops(args.p, args.n, st["n_p2p_pairs"], st["n_m2l"], n_edges)
+    m2l_ms = sum(s["ms_m2l"] for s in stats) / len(stats)
+    m2l_tflops = fl["M2L"] / (m2l_ms * 1e-3) / 1e12
+    phases = {k[3:]: round(sum(s[k] for s in stats) / len(stats), 3)
+              for k in ("ms_tree", "ms_upward", "ms_traverse", "ms_lists", "ms_m2l", "ms_p2p",
+                        "ms_downward", "ms_total")}
+
+    # accuracy vs the FP64 direct sum on a seeded 4096-body subset (not timed)
+    rng = np.random.default_rng(12345)
+    tg = torch.from_numpy(rng.choice(args.n, min(4096, args.n), replace=False).astype(np.int32))
+    dp, da, dj = fmm.direct(dx, dv, dm, tg.to(dev))
+    idx = tg.long().to(dev)
+    ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
+    ej = torch.linalg.norm(jerk[idx] - dj, dim=1) / torch.linalg.norm(dj, dim=1)
+    err = {"force_rms": float(torch.sqrt(torch.mean(ea ** 2))), "force_max": float(ea.max()),
+           "jerk_rms": float(torch.sqrt(torch.mean(ej ** 2))), "jerk_max": float(ej.max()),
+           "subset": int(tg.numel()), "gate_force_rms_1e-7": bool(torch.sqrt(torch.mean(ea ** 2)) <= 1e-7)}
+
+    traffic = load_traffic(args.traffic_file)
+    line = {
+        "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Plummer N=%d seed=%d (Henon units), p=%d, theta=%g, ncrit=%d, "
+                               "FP64 force+potential+jerk" % (args.n, args.seed, args.p,
+                                                             args.theta, args.ncrit),
+                   "n": args.n, "p": args.p, "theta": args.theta, "ncrit": args.ncrit,
+                   "mac": "theta", "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": m2l_tflops, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": m2l_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": traffic, "kernel": "k_m2l (M2L, FP64)",
+                     "flops_per_launch": fl["M2L"], "ms_per_launch": m2l_ms,
+                     "note": "FP64 peak 78.6 TFLOP/s is the same for vector and matrix units"},
+        "cpu_baseline": cpu,
+        "error_vs_direct": err,
+        "phases_ms": phases,
+        "counts": {"m2l_pairs": st["n_m2l"], "p2p_leaf_pairs": st["n_p2p_cells"],
+                   "p2p_body_pairs": st["n_p2p_pairs"], "cells": st["ncells"],
+                   "leaves": st["nleaves"], "depth": st["depth"]},
+        "algorithmic_tflops_per_step": fl["total"] / 1e12,
+        "achieved_tflops_whole_step": fl["total"] / (ms * 1e-3) / 1e12,
+    }
+    if cpu and cpu.get("value"):
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

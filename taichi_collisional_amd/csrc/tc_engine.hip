@@ -89,10 +89,13 @@ struct tc_context {
   std::vector<int> level_start;
   DBuf leaf_ids;
   std::vector<int> h_leaves;
+  std::vector<int4> h_groups;
+  DBuf p2p_groups, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists
-  DBuf front_a, front_b, m2l, p2p, tcnt;
-  int cap_front = 0, cap_m2l = 0, cap_p2p = 0;
+  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan;
+  DBuf gtg_a, gtg_b, gsrc_a, gsort_tmp;
+  int *h_pin = nullptr;  // small pinned host buffer for per-step counts
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
   // direct
@@ -224,17 +227,32 @@ static int build_tree(tc_context *ctx, int n) {
   ctx->st.depth = depth;
   ctx->st.nlevels = (int)ctx->level_start.size() - 1;
   // leaf list (host side, one copy of the child counts)
-  std::vector<int> nch(nc);
-  std::vector<int> &leaves = ctx->h_leaves;  // outlives the asynchronous upload
+  std::vector<int> nch(nc), nbd(nc);
+  std::vector<int> &leaves = ctx->h_leaves;  // outlive the asynchronous uploads
+  std::vector<int4> &groups = ctx->h_groups;
   leaves.clear();
+  groups.clear();
   TC_CHECK(hipMemcpyAsync(nch.data(), ctx->cnchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(nbd.data(), ctx->cnbody.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
   TC_CHECK(hipStreamSynchronize(s));
   leaves.reserve(nc);
   for (int i = 0; i < nc; i++)
-    if (nch[i] == 0) leaves.push_back(i);
+    if (nch[i] == 0) {
+      leaves.push_back(i);
+      // near-field target groups of <= 32 bodies, balanced within the leaf
+      int nb = nbd[i], ng = (nb + 31) / 32, o = 0;
+      for (int g = 0; g < ng; g++) {
+        int k = (nb - o) / (ng - g);
+        groups.push_back(make_int4(i, o, k, 0));
+        o += k;
+      }
+    }
   ctx->nleaves = (int)leaves.size();
   ctx->st.nleaves = ctx->nleaves;
   TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves.data(), sizeof(int) * leaves.size(),
+                          hipMemcpyHostToDevice, s));
+  TC_CHECK(ctx->p2p_groups.ensure(sizeof(int4) * std::max<size_t>(groups.size(), 1)));
+  TC_CHECK(hipMemcpyAsync(ctx->p2p_groups.p, groups.data(), sizeof(int4) * groups.size(),
                           hipMemcpyHostToDevice, s));
   return 0;
 }
@@ -260,62 +278,63 @@ static int upward(tc_context *ctx) {
   return 0;
 }
 
-static int grow_list(tc_context *ctx, DBuf &buf, int &cap, int need, int keep) {
-  int nc = std::max(need, cap + cap / 2);
-  TC_CHECK(buf.ensure(sizeof(int2) * (size_t)nc, sizeof(int2) * (size_t)keep, ctx->stream));
-  cap = nc;
-  return 0;
+__global__ void k_trav_totals(const int *__restrict__ blk, int nblk, int stride,
+                              int *__restrict__ out) {
+  // blk holds three count arrays then their three exclusive scans
+  if (threadIdx.x < 3)
+    out[threadIdx.x] = blk[(3 + threadIdx.x) * stride + nblk - 1] +
+                       blk[threadIdx.x * stride + nblk - 1];
 }
 
+// Breadth-first dual traversal from (root, root), one launch pair per step.
 static int traverse(tc_context *ctx, int n) {
   hipStream_t s = ctx->stream;
-  if (ctx->cap_front == 0) {
-    if (int e = grow_list(ctx, ctx->front_a, ctx->cap_front, std::max(1 << 16, 4 * n), 0)) return e;
-    int cf = 0;
-    if (int e = grow_list(ctx, ctx->front_b, cf, ctx->cap_front, 0)) return e;
-    if (int e = grow_list(ctx, ctx->m2l, ctx->cap_m2l, std::max(1 << 16, 24 * n), 0)) return e;
-    if (int e = grow_list(ctx, ctx->p2p, ctx->cap_p2p, std::max(1 << 16, 8 * n), 0)) return e;
-    TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 4));
-  }
+  int *h = ctx->h_pin;
+  TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 8));
+  TC_CHECK(ctx->front_a.ensure(sizeof(int2) * 4096));
+  TC_CHECK(ctx->m2l.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 24 * n)));
+  TC_CHECK(ctx->p2p.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 8 * n)));
   int2 root = make_int2(0, 0);
   TC_CHECK(hipMemcpyAsync(ctx->front_a.p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
   int nin = 1, nm2l = 0, np2p = 0;
   while (nin > 0) {
-    int cnt[3] = {0, nm2l, np2p};
-    TC_CHECK(hipMemcpyAsync(ctx->tcnt.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+    const int nblk = (nin + 255) / 256, stride = nblk + 1;
+    TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * (size_t)stride));
+    TC_CHECK(ctx->tscan.ensure(scan_temp_bytes(nblk) + 256));
+    int *blk = ctx->tblk.as<int>();
+    CellsDev c = cells_of(ctx);
+    launch_trav_count(c, ctx->front_a.as<int2>(), nin, ctx->cfg.theta, blk, blk + stride,
+                      blk + 2 * stride, s);
+    for (int k = 0; k < 3; k++)
+      if (launch_scan(blk + k * stride, blk + (3 + k) * stride, nblk, ctx->tscan.p,
+                      ctx->tscan.bytes, s))
+        return -4;
+    hipLaunchKernelGGL(k_trav_totals, dim3(1), dim3(64), 0, s, blk, nblk, stride,
+                       ctx->tcnt.as<int>() + 4);
+    TC_CHECK(hipMemcpyAsync(h + 4, ctx->tcnt.as<int>() + 4, 3 * sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    const int tm = h[4], tp = h[5], ts = h[6];
+    // capacity for this step's output before anything is written
+    TC_CHECK(ctx->m2l.ensure(sizeof(int2) * ((size_t)nm2l + tm), sizeof(int2) * (size_t)nm2l, s));
+    TC_CHECK(ctx->p2p.ensure(sizeof(int2) * ((size_t)np2p + tp), sizeof(int2) * (size_t)np2p, s));
+    TC_CHECK(ctx->front_b.ensure(sizeof(int2) * (size_t)std::max(ts, 1)));
+    h[0] = 0;
+    h[1] = nm2l;
+    h[2] = np2p;
+    TC_CHECK(hipMemcpyAsync(ctx->tcnt.p, h, 3 * sizeof(int), hipMemcpyHostToDevice, s));
     TravArgs a;
     a.fin = ctx->front_a.as<int2>();
     a.nin = nin;
     a.fout = ctx->front_b.as<int2>();
     a.cnt = ctx->tcnt.as<int>();
-    a.cap_front = ctx->cap_front;
-    a.cap_m2l = ctx->cap_m2l;
-    a.cap_p2p = ctx->cap_p2p;
     a.m2l = ctx->m2l.as<int2>();
     a.p2p = ctx->p2p.as<int2>();
     a.theta = ctx->cfg.theta;
-    launch_traverse(cells_of(ctx), a, s);
-    int out[3];
-    TC_CHECK(hipMemcpyAsync(out, ctx->tcnt.p, sizeof(out), hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipStreamSynchronize(s));
-    if (out[0] > ctx->cap_front || out[1] > ctx->cap_m2l || out[2] > ctx->cap_p2p) {
-      // grow and redo this step from the same frontier
-      if (out[0] > ctx->cap_front) {
-        int cf = ctx->cap_front;
-        if (int e = grow_list(ctx, ctx->front_b, cf, out[0], 0)) return e;
-        int ca = ctx->cap_front;
-        if (int e = grow_list(ctx, ctx->front_a, ca, out[0], nin)) return e;
-        ctx->cap_front = std::min(ca, cf);
-      }
-      if (out[1] > ctx->cap_m2l)
-        if (int e = grow_list(ctx, ctx->m2l, ctx->cap_m2l, out[1], nm2l)) return e;
-      if (out[2] > ctx->cap_p2p)
-        if (int e = grow_list(ctx, ctx->p2p, ctx->cap_p2p, out[2], np2p)) return e;
-      continue;
-    }
-    nm2l = out[1];
-    np2p = out[2];
-    nin = out[0];
+    launch_trav_write(cells_of(ctx), a, blk + 3 * stride, blk + 4 * stride, blk + 5 * stride, s);
+    nm2l += tm;
+    np2p += tp;
+    nin = ts;
     std::swap(ctx->front_a, ctx->front_b);
   }
   ctx->st.n_m2l = nm2l;
@@ -323,18 +342,25 @@ static int traverse(tc_context *ctx, int n) {
   return 0;
 }
 
-// group (target, source) pairs by target: count, scan, scatter
+// group (target, source) pairs by target: stable radix sort + run bounds
 static int build_csr(tc_context *ctx, const int2 *pairs, int npairs, DBuf &cnt, DBuf &off,
-                     DBuf &cur, DBuf &src) {
+                     DBuf &src) {
   hipStream_t s = ctx->stream;
   int nc = ctx->st.ncells;
   TC_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * (nc + 1), s));
-  TC_CHECK(hipMemsetAsync(cur.p, 0, sizeof(int) * (nc + 1), s));
-  TC_CHECK(src.ensure(sizeof(int) * (size_t)std::max(npairs, 1)));
-  launch_count_targets(pairs, npairs, cnt.as<int>(), s);
-  if (launch_scan(cnt.as<int>(), off.as<int>(), nc + 1, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s))
+  TC_CHECK(hipMemsetAsync(off.p, 0, sizeof(int) * (nc + 1), s));
+  size_t m = (size_t)std::max(npairs, 1);
+  TC_CHECK(src.ensure(sizeof(int) * m));
+  TC_CHECK(ctx->gtg_a.ensure(sizeof(int) * m));
+  TC_CHECK(ctx->gtg_b.ensure(sizeof(int) * m));
+  TC_CHECK(ctx->gsrc_a.ensure(sizeof(int) * m));
+  TC_CHECK(ctx->gsort_tmp.ensure(pair_sort_temp_bytes(npairs) + 256));
+  int bits = 1;
+  while ((1 << bits) < nc) bits++;
+  if (launch_group_pairs(pairs, npairs, bits, nc, ctx->gtg_a.as<int>(), ctx->gtg_b.as<int>(),
+                         ctx->gsrc_a.as<int>(), src.as<int>(), off.as<int>(), cnt.as<int>(),
+                         ctx->gsort_tmp.p, ctx->gsort_tmp.bytes, s))
     return -4;
-  launch_scatter_csr(pairs, npairs, off.as<int>(), cur.as<int>(), src.as<int>(), s);
   return 0;
 }
 
@@ -347,10 +373,10 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   hipStream_t s = ctx->stream;
   int nc = ctx->st.ncells;
   if (int e = build_csr(ctx, ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->m2l_cnt, ctx->m2l_off,
-                        ctx->m2l_cur, ctx->m2l_src))
+                        ctx->m2l_src))
     return e;
   if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
-                        ctx->p2p_off, ctx->p2p_cur, ctx->p2p_src))
+                        ctx->p2p_off, ctx->p2p_src))
     return e;
   // chunks of <= 64 sources per target for M2L
   hipLaunchKernelGGL(k_chunk_counts, dim3((nc + 1 + 255) / 256), dim3(256), 0, s,
@@ -365,15 +391,6 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
                      ctx->m2l_chunks.as<int4>(), s);
   return 0;
-}
-
-__global__ void k_p2p_pairs(const int2 *__restrict__ pairs, int n, const int *__restrict__ nbody,
-                            unsigned long long *__restrict__ out) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  unsigned long long v = 0;
-  if (k < n) v = (unsigned long long)nbody[pairs[k].x] * nbody[pairs[k].y];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
 
 static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
@@ -478,8 +495,14 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   ResultsDev r = results_of(ctx);
   for (double *q : {r.pot, r.ax, r.ay, r.az, r.jx, r.jy, r.jz})
     TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
-  launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->leaf_ids.as<int>(), ctx->nleaves,
-             ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), s);
+  TC_CHECK(ctx->p2p_ranges.ensure(sizeof(int2) * (size_t)std::max<long long>(ctx->st.n_p2p_cells, 1)));
+  TC_CHECK(ctx->npairs.ensure(sizeof(unsigned long long)));
+  TC_CHECK(hipMemsetAsync(ctx->npairs.p, 0, sizeof(unsigned long long), s));
+  launch_p2p_ranges(cells_of(ctx), ctx->p2p_src.as<int>(), (int)ctx->st.n_p2p_cells,
+                    ctx->p2p_ranges.as<int2>(), s);
+  launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_groups.as<int4>(),
+             (int)ctx->h_groups.size(), ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
+             ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s);
   TC_CHECK(hipEventRecord(ctx->ev[6], s));
   // ---- downward
   int L = (int)ctx->level_start.size() - 1;
@@ -488,15 +511,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   launch_l2p(cfg.p, cells_of(ctx), bodies_of(ctx), r, ctx->leaf_ids.as<int>(), ctx->nleaves, s);
   launch_scatter_results(bodies_of(ctx), r, n, pot, acc, jerk, s);
   TC_CHECK(hipEventRecord(ctx->ev[7], s));
-  // body pairs in the P2P list (statistics)
-  TC_CHECK(ctx->dtg.ensure(sizeof(unsigned long long)));
-  TC_CHECK(hipMemsetAsync(ctx->dtg.p, 0, sizeof(unsigned long long), s));
-  if (ctx->st.n_p2p_cells > 0)
-    hipLaunchKernelGGL(k_p2p_pairs, dim3((int)((ctx->st.n_p2p_cells + 255) / 256)), dim3(256), 0,
-                       s, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->cnbody.as<int>(),
-                       ctx->dtg.as<unsigned long long>());
+  // body pairs evaluated by P2P (statistics, counted inside the P2P kernel)
   unsigned long long pairs = 0;
-  TC_CHECK(hipMemcpyAsync(&pairs, ctx->dtg.p, sizeof(pairs), hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(&pairs, ctx->npairs.p, sizeof(pairs), hipMemcpyDeviceToHost, s));
   TC_CHECK(hipStreamSynchronize(s));
   ctx->st.n_p2p_pairs = (long long)pairs;
   float ms[8];
@@ -555,6 +572,10 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     ctx->own_stream = true;
   }
   for (auto &e : ctx->ev) hipEventCreate(&e);
+  if (hipHostMalloc((void **)&ctx->h_pin, 64 * sizeof(int)) != hipSuccess) {
+    tc_destroy(ctx);
+    return -3;
+  }
   *out = ctx;
   return 0;
 }
@@ -570,12 +591,15 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->cnbody, &ctx->cXg, &ctx->cRg, &ctx->cX, &ctx->cR, &ctx->cmin, &ctx->csink,
                  &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
                  &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
-                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
+                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan,
+                 &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
-                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl};
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->p2p_groups,
+                 &ctx->p2p_ranges, &ctx->npairs};
   for (DBuf *b : all) b->release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
+  if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;

@@ -74,15 +74,18 @@ struct TravArgs {
   const int2 *fin;
   int nin;
   int2 *fout;
-  int *cnt;  // [0] frontier out, [1] m2l, [2] p2p
-  int cap_front, cap_m2l, cap_p2p;
+  int *cnt;  // [1] M2L pairs so far, [2] P2P pairs so far (device copy)
   int2 *m2l, *p2p;
   double theta;
 };
-int launch_traverse(CellsDev c, const TravArgs &a, hipStream_t s);
-int launch_count_targets(const int2 *pairs, int n, int *count, hipStream_t s);
-int launch_scatter_csr(const int2 *pairs, int n, const int *offs, int *cursor, int *src,
-                       hipStream_t s);
+int launch_trav_count(CellsDev c, const int2 *fin, int nin, double theta, int *bm2l, int *bp2p,
+                      int *bsplit, hipStream_t s);
+int launch_trav_write(CellsDev c, const TravArgs &a, const int *om2l, const int *op2p,
+                      const int *osplit, hipStream_t s);
+size_t pair_sort_temp_bytes(int n);
+int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *tg_a, int *tg_b,
+                       int *src_a, int *src, int *start, int *count, void *temp,
+                       size_t temp_bytes, hipStream_t s);
 int launch_make_chunks(const int *offs, const int *count, const int *chunk_offs, int ncells,
                        int4 *chunks, hipStream_t s);
 
@@ -104,8 +107,10 @@ struct M2LArgs {
 int launch_m2l(int p, const M2LArgs &a, hipStream_t s);
 
 // ---- P2P (tc_p2p.hip)
-int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids, int nleaves,
-               const int *offs, const int *count, const int *src, hipStream_t s);
+int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ngroups,
+               const int *offs, const int *count, const int2 *ranges,
+               unsigned long long *npairs, hipStream_t s);
+int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s);
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s);
 

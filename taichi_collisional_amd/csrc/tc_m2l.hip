@@ -21,8 +21,8 @@
 // Work decomposition: a block = one wave = one chunk of <= 64 sources of a
 // single target cell (the chunk list is grouped by target); blockIdx.y picks
 // the array (0: M -> L, 1: Mdot -> Ldot).  Each order's 2n+1 contributions are
-// summed across the wave and staged in LDS; the chunk adds its sum into the
-// target's L with one atomic per coefficient.
+// summed across the wave through an LDS transpose; the chunk adds its sum
+// into the target's L with one atomic per coefficient.
 #include "tc_common.h"
 #include "tc_kernels.h"
 #include "swap_tables.h"
@@ -35,25 +35,28 @@ __constant__ double c_scale[kMaxNT];
 __constant__ double c_scale_inv[kMaxNT];
 __constant__ double c_fact[2 * kMaxP + 2];
 
-template <int N>
-__device__ __forceinline__ void rot_fwd(double *v, const double *cr, const double *ci) {
+// z-rotation of one order by e^{i m alpha} (forward) or e^{-i m alpha}
+// (backward), m = 1..N.  The powers come from the recurrence
+// e^{i(m+1)a} = e^{ima} e^{ia} (kernel_fast_lean.h:1163-1169), recomputed per
+// order so no power table has to stay live in registers.
+template <int N, bool kBack>
+__device__ __forceinline__ void rotate(double *v, double c1, double s1) {
+  double c = c1, s = s1;
 #pragma unroll
   for (int m = 1; m <= N; m++) {
-    double c = cr[m], s = ci[m];
     double im = v[N - m], re = v[N + m];
-    v[N - m] = c * im + s * re;
-    v[N + m] = c * re - s * im;
-  }
-}
-
-template <int N>
-__device__ __forceinline__ void rot_bwd(double *v, const double *cr, const double *ci) {
-#pragma unroll
-  for (int m = 1; m <= N; m++) {
-    double c = cr[m], s = ci[m];
-    double im = v[N - m], re = v[N + m];
-    v[N - m] = c * im - s * re;
-    v[N + m] = c * re + s * im;
+    if (kBack) {
+      v[N - m] = c * im - s * re;
+      v[N + m] = c * re + s * im;
+    } else {
+      v[N - m] = c * im + s * re;
+      v[N + m] = c * re - s * im;
+    }
+    if (m < N) {
+      double cn = c * c1 - s * s1;
+      s = c * s1 + s * c1;
+      c = cn;
+    }
   }
 }
 
@@ -94,36 +97,58 @@ struct M2LRegs {
   static constexpr int total = off(P + 1);
 };
 
+template <int N>
+__device__ __forceinline__ void load_order(const double *__restrict__ src, double *v) {
+#pragma unroll
+  for (int r = 0; r < 2 * N + 1; r++) v[r] = src[N * N + r];
+}
+
+// Forward half for order N of the source moments (already loaded into vin):
+// scale, rotate by alpha_z, swap, rotate by alpha_x, swap (rows |m| <=
+// min(N, p-N) only), unscale into the rotated-moment store.  The loads of
+// order N-1 are issued first and fenced with scheduling barriers, so one
+// order's loads are in flight while this order computes and the compiler
+// cannot hoist every order's loads to the top of the kernel.
 template <int P, int N>
 __device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src, bool quirk,
-                                                  const double *ezr, const double *ezi,
-                                                  const double *exr, const double *exi,
-                                                  double *mp) {
+                                                  double ezr, double ezi, double exr,
+                                                  double exi, double *vin, double *mp) {
   constexpr int K = 2 * N + 1;
   constexpr int Q = M2LRegs<P>::q(N);
   constexpr int O = M2LRegs<P>::off(N);
+  constexpr int KN = N > 0 ? 2 * N - 1 : 1;
+  double nxt[KN];
+  if constexpr (N > 0) {
+    __builtin_amdgcn_sched_barrier(0);
+    load_order<N - 1>(src, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   double v[K], w[K];
 #pragma unroll
-  for (int r = 0; r < K; r++) v[r] = src[N * N + r] * c_scale[N * N + r];
+  for (int r = 0; r < K; r++) v[r] = vin[r] * c_scale[N * N + r];
   if (N == P && quirk) {
     v[0] = 0.0;
     v[2 * N] = 0.0;
   }
-  rot_fwd<N>(v, ezr, ezi);
+  rotate<N, false>(v, ezr, ezi);
   swap_block<N, N, N>(v, w);
-  rot_fwd<N>(w, exr, exi);
+  rotate<N, false>(w, exr, exi);
   swap_block<N, Q, N>(w, v);
 #pragma unroll
   for (int m = -Q; m <= Q; m++) mp[O + Q + m] = v[N + m] * c_scale_inv[N * N + N + m];
+  if constexpr (N > 0) m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, nxt, mp);
 }
 
 template <int P, int N>
 __device__ __forceinline__ void m2l_backward_order(const double *mp, const double *f,
-                                                   const double *ezr, const double *ezi,
-                                                   const double *exr, const double *exi,
-                                                   bool active, double *acc_lds, int lane) {
+                                                   double ezr, double ezi, double exr,
+                                                   double exi, bool active, double *acc_lds,
+                                                   double *red, int lane) {
   constexpr int K = 2 * N + 1;
   constexpr int Q = M2LRegs<P>::q(N);
+  // one order at a time: keeps the scheduler from interleaving the
+  // register-only work of several orders (which multiplies live state)
+  __builtin_amdgcn_sched_barrier(0);
   double u[K], w[K];
 #pragma unroll
   for (int r = 0; r < K; r++) u[r] = 0.0;
@@ -147,33 +172,42 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
     }
   }
   swap_block<N, N, Q>(u, w);
-  rot_bwd<N>(w, exr, exi);
+  rotate<N, true>(w, exr, exi);
   swap_block<N, N, N>(w, u);
-  rot_bwd<N>(u, ezr, ezi);
+  rotate<N, true>(u, ezr, ezi);
+  // Sum the K contributions over the wave: each lane writes its values to
+  // LDS rows, then groups of 8 lanes each sum one row (8 values per lane, a
+  // 3-step butterfly inside the group).
 #pragma unroll
-  for (int r = 0; r < K; r++) {
-    double s = wave_sum(active ? u[r] : 0.0);
-    if (lane == 0) acc_lds[N * N + r] = s;
+  for (int r = 0; r < K; r++) red[r * kWave + lane] = u[r];
+  __syncthreads();
+  const int seg = lane & 7, grp = lane >> 3;
+#pragma unroll
+  for (int g = 0; g < (K + 7) / 8; g++) {
+    const int r = g * 8 + grp;
+    double sum = 0.0;
+    if (r < K) {
+      const double2 *row = reinterpret_cast<const double2 *>(red + r * kWave + seg * 8);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        double2 t = row[q];
+        sum += t.x + t.y;
+      }
+    }
+    sum += __shfl_xor(sum, 1, kWave);
+    sum += __shfl_xor(sum, 2, kWave);
+    sum += __shfl_xor(sum, 4, kWave);
+    if (seg == 0 && r < K) acc_lds[N * N + r] = sum;
   }
-}
-
-// Orders run from p down to 0: the large orders (biggest working set) run
-// while few rotated moments are live.
-template <int P, int... Ns>
-__device__ __forceinline__ void m2l_forward_all(const double *src, bool quirk, const double *ezr,
-                                                const double *ezi, const double *exr,
-                                                const double *exi, double *mp,
-                                                IntSeq<Ns...>) {
-  (m2l_forward_order<P, P - Ns>(src, quirk, ezr, ezi, exr, exi, mp), ...);
+  __syncthreads();
 }
 
 template <int P, int... Ns>
-__device__ __forceinline__ void m2l_backward_all(const double *mp, const double *f,
-                                                 const double *ezr, const double *ezi,
-                                                 const double *exr, const double *exi,
-                                                 bool active, double *acc, int lane,
-                                                 IntSeq<Ns...>) {
-  (m2l_backward_order<P, Ns>(mp, f, ezr, ezi, exr, exi, active, acc, lane), ...);
+__device__ __forceinline__ void m2l_backward_all(const double *mp, const double *f, double ezr,
+                                                 double ezi, double exr, double exi,
+                                                 bool active, double *acc, double *red,
+                                                 int lane, IntSeq<Ns...>) {
+  (m2l_backward_order<P, Ns>(mp, f, ezr, ezi, exr, exi, active, acc, red, lane), ...);
 }
 
 template <int P>
@@ -187,6 +221,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
                                             int quirk) {
   constexpr int NT = nterm(P);
   __shared__ double acc[NT];
+  __shared__ __attribute__((aligned(16))) double red[(2 * P + 1) * kWave];
   const int lane = threadIdx.x;
   const int4 ch = chunks[blockIdx.x];
   const int tgt = ch.x, start = ch.y, cnt = ch.z;
@@ -203,34 +238,37 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
   double rxy = sqrt(rxy2);
   double r = sqrt(rxy2 + dz * dz);
   double ri = 1.0 / r;
-  double ezr[P + 1], ezi[P + 1], exr[P + 1], exi[P + 1], f[P + 1];
-  ezr[0] = 1.0, ezi[0] = 0.0, exr[0] = 1.0, exi[0] = 0.0;
-  ezr[1] = 1.0, ezi[1] = 0.0;
-  exr[1] = dz * ri;
-  exi[1] = -rxy * ri;
+  // e^{i alpha_z} = (dY + i dX)/r_xy (1 on the z axis), e^{i alpha_x} = (dZ - i r_xy)/r
+  double ezr = 1.0, ezi = 0.0;
+  const double exr = dz * ri, exi = -rxy * ri;
   if (rxy > 0) {
     double q = 1.0 / rxy;
-    ezr[1] = dy * q;
-    ezi[1] = dx * q;
+    ezr = dy * q;
+    ezi = dx * q;
   }
+  double mp[M2LRegs<P>::total];
+  const double *src = Msrc + (size_t)j * NT;
+  // Orders run from p down to 0: the large orders (biggest working set) run
+  // while few rotated moments are live.
+  double v0[2 * P + 1];
+  load_order<P>(src, v0);
+  m2l_forward_order<P, P>(src, quirk != 0, ezr, ezi, exr, exi, v0, mp);
+  // Fence the rotated moments: without it, IR-level code motion interleaves
+  // the forward and backward halves and roughly doubles the live registers
+  // (p = 12: 512 registers plus scratch spills instead of ~310, no spills).
 #pragma unroll
-  for (int m = 1; m < P; m++) {
-    ezr[m + 1] = ezr[m] * ezr[1] - ezi[m] * ezi[1];
-    ezi[m + 1] = ezr[m] * ezi[1] + ezi[m] * ezr[1];
-    exr[m + 1] = exr[m] * exr[1] - exi[m] * exi[1];
-    exi[m + 1] = exr[m] * exi[1] + exi[m] * exr[1];
-  }
-  double rp = ri;
+  for (int t = 0; t < M2LRegs<P>::total; t++) asm volatile("" : "+v"(mp[t]));
+  __builtin_amdgcn_sched_barrier(0);
+  // translation factors (n+k)! r^-(n+k+1), n+k <= p; zero on idle lanes,
+  // which makes their whole contribution zero
+  double f[P + 1];
+  double rp = active ? ri : 0.0;
 #pragma unroll
   for (int k = 0; k <= P; k++) {
     f[k] = c_fact[k] * rp;
     rp *= ri;
   }
-  double mp[M2LRegs<P>::total];
-  const double *src = Msrc + (size_t)j * NT;
-  m2l_forward_all<P>(src, quirk != 0, ezr, ezi, exr, exi, mp, MakeIntSeq<P + 1>{});
-  m2l_backward_all<P>(mp, f, ezr, ezi, exr, exi, active, acc, lane, MakeIntSeq<P + 1>{});
-  __syncthreads();
+  m2l_backward_all<P>(mp, f, ezr, ezi, exr, exi, active, acc, red, lane, MakeIntSeq<P + 1>{});
   double *dst = Lout + (size_t)tgt * NT;
   for (int t = lane; t < NT; t += kWave) atomicAdd(dst + t, acc[t]);
 }

@@ -6,11 +6,11 @@
 //   pot += m/r,  a += m dx / r^3,  j += m dv / r^3 - 3 m (dx.dv) dx / r^5,
 // pairs with r^2 == 0 contribute nothing (kernel_fast_lean.h:583-584).
 //
-// P2P kernel: one wave per target leaf.  With n_t target bodies the wave is
-// split into G = floor(64/n_t) slices of n_t lanes; lane (slice s, target t)
-// walks every G-th body of each source leaf in the target's list, and the
-// slices are summed through LDS at the end.  Source bodies are read from the
-// sorted SoA arrays (each source leaf is a contiguous range).
+// P2P kernel: one wave per target group (<= 32 bodies of a leaf).  With n_t
+// targets the wave is split into G = floor(64/n_t) slices of n_t lanes; the
+// source leaves of the leaf's list are copied tile by tile into LDS, lane
+// (slice s, target t) walks every G-th tile body, and the slices are summed
+// through LDS at the end.
 #include "tc_common.h"
 #include "tc_kernels.h"
 
@@ -44,114 +44,197 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
   a.jz += dz * f;
 }
 
+constexpr int kTile = 256;
+
+// One block (one wave) per target group: <= 32 consecutive bodies of one leaf.
+// The leaf's P2P list is a list of source body ranges (one per source leaf);
+// up to 64 ranges are fetched at once (one per lane), a wave prefix sum over
+// their lengths places as many whole ranges as fit into an LDS tile of kTile
+// bodies (position+mass, velocity as double4), and every lane then copies tile
+// slots in parallel.  Lane (slice s, target t) takes every G-th tile body,
+// G = floor(64 / group size).  A single range longer than the tile (only when
+// ncrit > kTile) is streamed through the tile in pieces.
 __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev r,
-                                            const int *__restrict__ leaf_ids,
+                                            const int4 *__restrict__ groups,
                                             const int *__restrict__ offs,
                                             const int *__restrict__ count,
-                                            const int *__restrict__ src) {
+                                            const int2 *__restrict__ ranges,
+                                            unsigned long long *__restrict__ npairs) {
+  __shared__ double4 sp[kTile], sv[kTile];
+  __shared__ int rstart[kWave], rexcl[kWave];
   __shared__ double red[7][kWave];
   const int lane = threadIdx.x;
-  const int ci = leaf_ids[blockIdx.x];
+  const int4 gw = groups[blockIdx.x];
+  const int ci = gw.x;
   if (!c.has_sink[ci]) return;
-  const int b0 = c.body[ci], nb = c.nbody[ci];
+  const int nt = gw.z;
+  const int G = kWave / nt;
+  const int s = lane / nt, t = lane - s * nt;
+  const bool act = s < G;
+  const int ib = c.body[ci] + gw.y + t;
+  const double xi = b.x[ib], yi = b.y[ib], zi = b.z[ib];
+  const double vxi = b.vx[ib], vyi = b.vy[ib], vzi = b.vz[ib];
+  Acc7 a = {0, 0, 0, 0, 0, 0, 0};
   const int l0 = offs[ci], ln = count[ci];
-  for (int tb = 0; tb < nb; tb += kWave) {
-    const int nt = nb - tb < kWave ? nb - tb : kWave;
-    const int G = kWave / nt;
-    const int s = lane / nt, t = lane - s * nt;
-    const bool act = s < G;
-    const int ib = b0 + tb + t;
-    double xi = b.x[ib], yi = b.y[ib], zi = b.z[ib];
-    double vxi = b.vx[ib], vyi = b.vy[ib], vzi = b.vz[ib];
-    Acc7 a = {0, 0, 0, 0, 0, 0, 0};
-    if (act) {
-      for (int e = 0; e < ln; e++) {
-        const int j = src[l0 + e];
-        const int jb0 = c.body[j], jn = c.nbody[j];
-        for (int k = s; k < jn; k += G) {
-          const int jb = jb0 + k;
-          if (!(b.flags[jb] & kSource)) continue;
-          pair_fp64(xi, yi, zi, vxi, vyi, vzi, b.x[jb], b.y[jb], b.z[jb], b.vx[jb], b.vy[jb],
-                    b.vz[jb], b.q[jb], a);
+  long long nsrc = 0;
+  int e = 0, off = 0;
+  while (e < ln) {
+    const int2 rg = (e + lane < ln) ? ranges[l0 + e + lane] : make_int2(0, 0);
+    int tot;
+    const int excl = wave_excl_scan(rg.y, &tot);
+    const bool fits = (e + lane < ln) && (excl + rg.y <= kTile);
+    const unsigned long long fm = __ballot(fits);
+    const int nfit = __popcll(fm);  // ranges 0..nfit-1 fit (prefix sums grow)
+    int fill;
+    if (nfit > 0) {
+      rstart[lane] = rg.x;
+      rexcl[lane] = excl;
+      fill = __shfl(excl + rg.y, nfit - 1, kWave);
+      __syncthreads();
+      for (int q = lane; q < fill; q += kWave) {
+        int lo = 0, hi = nfit - 1;  // last range with rexcl <= q
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (rexcl[mid] <= q)
+            lo = mid;
+          else
+            hi = mid - 1;
         }
+        const int bq = rstart[lo] + q - rexcl[lo];
+        const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
+        sp[q] = make_double4(b.x[bq], b.y[bq], b.z[bq], mq);
+        sv[q] = make_double4(b.vx[bq], b.vy[bq], b.vz[bq], 0.0);
+      }
+      e += nfit;
+    } else {
+      // first range longer than the tile: stream it piece by piece
+      const int base = __shfl(rg.x, 0, kWave), len = __shfl(rg.y, 0, kWave);
+      fill = min(kTile, len - off);
+      for (int q = lane; q < fill; q += kWave) {
+        const int bq = base + off + q;
+        const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
+        sp[q] = make_double4(b.x[bq], b.y[bq], b.z[bq], mq);
+        sv[q] = make_double4(b.vx[bq], b.vy[bq], b.vz[bq], 0.0);
+      }
+      off += fill;
+      if (off >= len) {
+        off = 0;
+        e += 1;
+      }
+    }
+    nsrc += fill;
+    __syncthreads();
+    if (act) {
+#pragma unroll 2
+      for (int k = s; k < fill; k += G) {
+        const double4 P = sp[k], V = sv[k];
+        pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
       }
     }
     __syncthreads();
-    red[0][lane] = a.p;
-    red[1][lane] = a.ax;
-    red[2][lane] = a.ay;
-    red[3][lane] = a.az;
-    red[4][lane] = a.jx;
-    red[5][lane] = a.jy;
-    red[6][lane] = a.jz;
-    __syncthreads();
-    if (s == 0 && (b.flags[ib] & kSink)) {
-      double o[7];
-      for (int q = 0; q < 7; q++) {
-        double v = 0;
-        for (int g = 0; g < G; g++) v += red[q][g * nt + t];
-        o[q] = v;
-      }
-      r.pot[ib] += o[0];
-      r.ax[ib] += o[1];
-      r.ay[ib] += o[2];
-      r.az[ib] += o[3];
-      r.jx[ib] += o[4];
-      r.jy[ib] += o[5];
-      r.jz[ib] += o[6];
+  }
+  if (lane == 0) atomicAdd(npairs, (unsigned long long)(nsrc * nt));
+  red[0][lane] = a.p;
+  red[1][lane] = a.ax;
+  red[2][lane] = a.ay;
+  red[3][lane] = a.az;
+  red[4][lane] = a.jx;
+  red[5][lane] = a.jy;
+  red[6][lane] = a.jz;
+  __syncthreads();
+  if (s == 0 && (b.flags[ib] & kSink)) {
+    double o[7];
+    for (int q = 0; q < 7; q++) {
+      double v = 0;
+      for (int g = 0; g < G; g++) v += red[q][g * nt + t];
+      o[q] = v;
     }
+    r.pot[ib] += o[0];
+    r.ax[ib] += o[1];
+    r.ay[ib] += o[2];
+    r.az[ib] += o[3];
+    r.jx[ib] += o[4];
+    r.jy[ib] += o[5];
+    r.jz[ib] += o[6];
   }
 }
 
-int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids, int nleaves,
-               const int *offs, const int *count, const int *src, hipStream_t s) {
-  if (nleaves <= 0) return 0;
-  hipLaunchKernelGGL(k_p2p, dim3(nleaves), dim3(64), 0, s, c, b, r, leaf_ids, offs, count, src);
+// Source body range (first body, count) of every P2P list entry.
+__global__ void k_p2p_ranges(CellsDev c, const int *__restrict__ src, int n,
+                             int2 *__restrict__ ranges) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int j = src[k];
+  ranges[k] = make_int2(c.body[j], c.nbody[j]);
+}
+
+int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_p2p_ranges, dim3((n + 255) / 256), dim3(256), 0, s, c, src, n, ranges);
   return 0;
 }
 
-// All-pairs sum for a list of targets (sorted-order indices), FP64.
+int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ngroups,
+               const int *offs, const int *count, const int2 *ranges,
+               unsigned long long *npairs, hipStream_t s) {
+  if (ngroups <= 0) return 0;
+  hipLaunchKernelGGL(k_p2p, dim3(ngroups), dim3(64), 0, s, c, b, r, groups, offs, count, ranges,
+                     npairs);
+  return 0;
+}
+
+// All-pairs sum for a list of targets (sorted-order indices), FP64.  The
+// sources are split into gridDim.y slabs so enough blocks run; each slab adds
+// its partial sums into the (zeroed) outputs.
 __global__ __launch_bounds__(256) void k_direct(BodiesDev b, int n, const int *__restrict__ tg,
-                                                int nt, double *__restrict__ pot,
+                                                int nt, int slab, double *__restrict__ pot,
                                                 double *__restrict__ acc,
                                                 double *__restrict__ jerk) {
-  __shared__ double sx[256], sy[256], sz[256], svx[256], svy[256], svz[256], sq[256];
+  __shared__ double4 sp[256], sv[256];
   const int k = blockIdx.x * 256 + threadIdx.x;
   const int i = k < nt ? tg[k] : 0;
-  double xi = b.x[i], yi = b.y[i], zi = b.z[i], vxi = b.vx[i], vyi = b.vy[i], vzi = b.vz[i];
+  const double xi = b.x[i], yi = b.y[i], zi = b.z[i], vxi = b.vx[i], vyi = b.vy[i],
+               vzi = b.vz[i];
   Acc7 a = {0, 0, 0, 0, 0, 0, 0};
-  for (int base = 0; base < n; base += 256) {
+  const int j0 = blockIdx.y * slab, j1 = min(n, j0 + slab);
+  for (int base = j0; base < j1; base += 256) {
     __syncthreads();
     int j = base + threadIdx.x;
-    if (j < n) {
-      sx[threadIdx.x] = b.x[j];
-      sy[threadIdx.x] = b.y[j];
-      sz[threadIdx.x] = b.z[j];
-      svx[threadIdx.x] = b.vx[j];
-      svy[threadIdx.x] = b.vy[j];
-      svz[threadIdx.x] = b.vz[j];
-      sq[threadIdx.x] = (b.flags[j] & kSource) ? b.q[j] : 0.0;
+    if (j < j1) {
+      double mq = (b.flags[j] & kSource) ? b.q[j] : 0.0;
+      sp[threadIdx.x] = make_double4(b.x[j], b.y[j], b.z[j], mq);
+      sv[threadIdx.x] = make_double4(b.vx[j], b.vy[j], b.vz[j], 0.0);
     }
     __syncthreads();
-    int lim = n - base < 256 ? n - base : 256;
-    for (int q = 0; q < lim; q++)
-      pair_fp64(xi, yi, zi, vxi, vyi, vzi, sx[q], sy[q], sz[q], svx[q], svy[q], svz[q], sq[q], a);
+    int lim = min(256, j1 - base);
+    for (int q = 0; q < lim; q++) {
+      const double4 P = sp[q], V = sv[q];
+      pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
+    }
   }
   if (k < nt) {
-    pot[k] = a.p;
-    acc[3 * k] = a.ax;
-    acc[3 * k + 1] = a.ay;
-    acc[3 * k + 2] = a.az;
-    jerk[3 * k] = a.jx;
-    jerk[3 * k + 1] = a.jy;
-    jerk[3 * k + 2] = a.jz;
+    atomicAdd(&pot[k], a.p);
+    atomicAdd(&acc[3 * k], a.ax);
+    atomicAdd(&acc[3 * k + 1], a.ay);
+    atomicAdd(&acc[3 * k + 2], a.az);
+    atomicAdd(&jerk[3 * k], a.jx);
+    atomicAdd(&jerk[3 * k + 1], a.jy);
+    atomicAdd(&jerk[3 * k + 2], a.jz);
   }
 }
 
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s) {
   if (nt <= 0) return 0;
-  hipLaunchKernelGGL(k_direct, dim3((nt + 255) / 256), dim3(256), 0, s, b, n, targets, nt, pot,
+  hipMemsetAsync(pot, 0, sizeof(double) * nt, s);
+  hipMemsetAsync(acc, 0, sizeof(double) * 3 * nt, s);
+  hipMemsetAsync(jerk, 0, sizeof(double) * 3 * nt, s);
+  int bx = (nt + 255) / 256;
+  int slabs = (2048 + bx - 1) / bx;  // aim for >= 2048 blocks
+  int slab = (n + slabs - 1) / slabs;
+  slab = ((slab + 255) / 256) * 256;
+  slabs = (n + slab - 1) / slab;
+  hipLaunchKernelGGL(k_direct, dim3(bx, slabs), dim3(256), 0, s, b, n, targets, nt, slab, pot,
                      acc, jerk);
   return 0;
 }

@@ -17,8 +17,10 @@
 //   recursion from (root, root) is run breadth-first over a frontier of cell
 //   pairs; each pair is accepted for M2L by the theta-MAC, sent to P2P when
 //   both are leaves, or split (Ci if Cj is a leaf or Ci is larger, else Cj).
-//   Appends are wave-aggregated atomics; the lists are then grouped by target
-//   cell (counting sort) and the M2L list is cut into chunks of <= 64 sources.
+//   Each step runs in two passes (count per block, scan, write at the scanned
+//   offsets), so the lists are written without atomics and in a fixed order;
+//   they are then grouped by target cell with a stable radix sort, and the
+//   M2L list is cut into chunks of <= 64 sources.
 #include <hipcub/hipcub.hpp>
 
 #include "tc_common.h"
@@ -332,92 +334,160 @@ int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, h
 }
 
 // ------------------------------------------------------------ traversal -----
-// Appends `k` items for this lane to a list with one atomic per wave; returns
-// the lane's first slot (may exceed the capacity; callers check).
-__device__ __forceinline__ int wave_reserve(int *counter, int k) {
-  int tot;
-  int pre = wave_excl_scan(k, &tot);
-  int base = 0;
-  int lane = threadIdx.x & (kWave - 1);
-  if (lane == 0 && tot > 0) base = atomicAdd(counter, tot);
-  base = __shfl(base, 0, kWave);
-  return base + pre;
-}
-
-__global__ __launch_bounds__(256) void k_traverse(CellsDev c, TravArgs a) {
+// One breadth-first step, in two passes so that no shared counter is hit by
+// every wave: pass 1 classifies each frontier pair and writes per-block
+// totals (M2L, P2P, child pairs); the totals are scanned over blocks; pass 2
+// classifies again and writes every output at its scanned position, so the
+// lists come out in frontier order (deterministic).
+__device__ __forceinline__ int trav_classify(const CellsDev &c, int i, int j, double theta,
+                                             int *nsplit) {
 #pragma clang fp contract(off)
+  double dx = c.X[3 * i] - c.X[3 * j];
+  double dy = c.X[3 * i + 1] - c.X[3 * j + 1];
+  double dz = c.X[3 * i + 2] - c.X[3 * j + 2];
+  double R2 = (dx * dx + dy * dy + dz * dz) * theta * theta;
+  double Ri = c.R[i], Rj = c.R[j];
+  int nci = c.nchild[i], ncj = c.nchild[j];
+  *nsplit = 0;
+  if (R2 > (Ri + Rj) * (Ri + Rj)) return 1;  // M2L
+  if (nci == 0 && ncj == 0) return 2;        // P2P
+  if (ncj == 0 || (Ri >= Rj && nci != 0)) {  // split the target
+    *nsplit = nci;
+    return 3;
+  }
+  *nsplit = ncj;  // split the source
+  return 4;
+}
+
+__device__ __forceinline__ int3 block_excl_scan3(int3 v, int3 *tot) {
+  __shared__ int3 wsum[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int tx, ty, tz;
+  int ex = wave_excl_scan(v.x, &tx), ey = wave_excl_scan(v.y, &ty), ez = wave_excl_scan(v.z, &tz);
+  if (lane == 0) wsum[w] = make_int3(tx, ty, tz);
+  __syncthreads();
+  int3 base = make_int3(0, 0, 0), all = make_int3(0, 0, 0);
+  for (int k = 0; k < 4; k++) {
+    int3 q = wsum[k];
+    if (k < w) base = make_int3(base.x + q.x, base.y + q.y, base.z + q.z);
+    all = make_int3(all.x + q.x, all.y + q.y, all.z + q.z);
+  }
+  *tot = all;
+  __syncthreads();
+  return make_int3(base.x + ex, base.y + ey, base.z + ez);
+}
+
+__global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__restrict__ fin,
+                                                    int nin, double theta,
+                                                    int *__restrict__ bm2l,
+                                                    int *__restrict__ bp2p,
+                                                    int *__restrict__ bsplit) {
   int t = blockIdx.x * 256 + threadIdx.x;
-  bool live = t < a.nin;
-  int2 pr = live ? a.fin[t] : make_int2(0, 0);
-  int i = pr.x, j = pr.y;
-  int kind = 0;  // 1 m2l, 2 p2p, 3 split i, 4 split j
-  int nsplit = 0;
-  if (live) {
-    double dx = c.X[3 * i] - c.X[3 * j];
-    double dy = c.X[3 * i + 1] - c.X[3 * j + 1];
-    double dz = c.X[3 * i + 2] - c.X[3 * j + 2];
-    double R2 = (dx * dx + dy * dy + dz * dz) * a.theta * a.theta;
-    double Ri = c.R[i], Rj = c.R[j];
-    int nci = c.nchild[i], ncj = c.nchild[j];
-    if (R2 > (Ri + Rj) * (Ri + Rj)) {
-      kind = 1;
-    } else if (nci == 0 && ncj == 0) {
-      kind = 2;
-    } else if (ncj == 0 || (Ri >= Rj && nci != 0)) {
-      kind = 3;
-      nsplit = nci;
-    } else {
-      kind = 4;
-      nsplit = ncj;
-    }
+  int3 v = make_int3(0, 0, 0);
+  if (t < nin) {
+    int2 pr = fin[t];
+    int ns;
+    int kind = trav_classify(c, pr.x, pr.y, theta, &ns);
+    v = make_int3(kind == 1, kind == 2, ns);
   }
-  int s1 = wave_reserve(&a.cnt[1], kind == 1 ? 1 : 0);
-  if (kind == 1 && s1 < a.cap_m2l) a.m2l[s1] = make_int2(i, j);
-  int s2 = wave_reserve(&a.cnt[2], kind == 2 ? 1 : 0);
-  if (kind == 2 && s2 < a.cap_p2p) a.p2p[s2] = make_int2(i, j);
-  int s3 = wave_reserve(&a.cnt[0], nsplit);
-  if (kind == 3) {
-    int c0 = c.child[i];
-    for (int k = 0; k < nsplit; k++)
-      if (s3 + k < a.cap_front) a.fout[s3 + k] = make_int2(c0 + k, j);
-  } else if (kind == 4) {
-    int c0 = c.child[j];
-    for (int k = 0; k < nsplit; k++)
-      if (s3 + k < a.cap_front) a.fout[s3 + k] = make_int2(i, c0 + k);
+  int3 tot;
+  block_excl_scan3(v, &tot);
+  if (threadIdx.x == 0) {
+    bm2l[blockIdx.x] = tot.x;
+    bp2p[blockIdx.x] = tot.y;
+    bsplit[blockIdx.x] = tot.z;
   }
 }
 
-int launch_traverse(CellsDev c, const TravArgs &a, hipStream_t s) {
+__global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
+                                                    const int *__restrict__ om2l,
+                                                    const int *__restrict__ op2p,
+                                                    const int *__restrict__ osplit, int nblk) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int kind = 0, ns = 0;
+  int2 pr = make_int2(0, 0);
+  if (t < a.nin) {
+    pr = a.fin[t];
+    kind = trav_classify(c, pr.x, pr.y, a.theta, &ns);
+  }
+  int3 tot;
+  int3 ex = block_excl_scan3(make_int3(kind == 1, kind == 2, ns), &tot);
+  const int b = blockIdx.x;
+  if (kind == 1) a.m2l[a.cnt[1] + om2l[b] + ex.x] = pr;
+  if (kind == 2) a.p2p[a.cnt[2] + op2p[b] + ex.y] = pr;
+  if (kind >= 3) {
+    int base = osplit[b] + ex.z;
+    int c0 = c.child[kind == 3 ? pr.x : pr.y];
+    for (int k = 0; k < ns; k++)
+      a.fout[base + k] = kind == 3 ? make_int2(c0 + k, pr.y) : make_int2(pr.x, c0 + k);
+  }
+  (void)nblk;
+}
+
+int launch_trav_count(CellsDev c, const int2 *fin, int nin, double theta, int *bm2l, int *bp2p,
+                      int *bsplit, hipStream_t s) {
+  if (nin <= 0) return 0;
+  hipLaunchKernelGGL(k_trav_count, dim3((nin + 255) / 256), dim3(256), 0, s, c, fin, nin, theta,
+                     bm2l, bp2p, bsplit);
+  return 0;
+}
+
+int launch_trav_write(CellsDev c, const TravArgs &a, const int *om2l, const int *op2p,
+                      const int *osplit, hipStream_t s) {
   if (a.nin <= 0) return 0;
-  hipLaunchKernelGGL(k_traverse, dim3((a.nin + 255) / 256), dim3(256), 0, s, c, a);
+  int nblk = (a.nin + 255) / 256;
+  hipLaunchKernelGGL(k_trav_write, dim3(nblk), dim3(256), 0, s, c, a, om2l, op2p, osplit, nblk);
   return 0;
 }
 
-__global__ void k_count_targets(const int2 *__restrict__ pairs, int n, int *__restrict__ count) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  if (k < n) atomicAdd(&count[pairs[k].x], 1);
+// Group (target, source) pairs by target: stable radix sort on the target,
+// then the start and end of each target's run.
+size_t pair_sort_temp_bytes(int n) {
+  size_t b = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, b, (int *)nullptr, (int *)nullptr, (int *)nullptr,
+                                     (int *)nullptr, n, 0, 32);
+  return b;
 }
 
-int launch_count_targets(const int2 *pairs, int n, int *count, hipStream_t s) {
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_count_targets, dim3((n + 255) / 256), dim3(256), 0, s, pairs, n, count);
-  return 0;
-}
-
-__global__ void k_scatter_csr(const int2 *__restrict__ pairs, int n, const int *__restrict__ offs,
-                              int *__restrict__ cursor, int *__restrict__ src) {
+__global__ void k_split_pairs(const int2 *__restrict__ pairs, int n, int *__restrict__ tg,
+                              int *__restrict__ sr) {
   int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
   int2 p = pairs[k];
-  int pos = offs[p.x] + atomicAdd(&cursor[p.x], 1);
-  src[pos] = p.y;
+  tg[k] = p.x;
+  sr[k] = p.y;
 }
 
-int launch_scatter_csr(const int2 *pairs, int n, const int *offs, int *cursor, int *src,
-                       hipStream_t s) {
+// Marks runs of equal targets: start[t] = first index, endp[t] = one past the
+// last (both zero-initialized, so targets without entries keep 0 / 0).
+__global__ void k_runs(const int *__restrict__ tg, int n, int *__restrict__ start,
+                       int *__restrict__ endp) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int t = tg[k];
+  if (k == 0 || tg[k - 1] != t) start[t] = k;
+  if (k == n - 1 || tg[k + 1] != t) endp[t] = k + 1;
+}
+
+__global__ void k_run_counts(const int *__restrict__ start, int *__restrict__ cnt, int ncells) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncells) cnt[c] -= start[c];
+}
+
+// After the call: src[] holds sources grouped by target (stable), start[] /
+// count[] (zeroed by the caller) the run of every target.
+int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *tg_a, int *tg_b,
+                       int *src_a, int *src, int *start, int *count, void *temp,
+                       size_t temp_bytes, hipStream_t s) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_scatter_csr, dim3((n + 255) / 256), dim3(256), 0, s, pairs, n, offs,
-                     cursor, src);
+  hipLaunchKernelGGL(k_split_pairs, dim3((n + 255) / 256), dim3(256), 0, s, pairs, n, tg_a, src_a);
+  size_t tb = temp_bytes;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, tg_a, tg_b, src_a, src, n, 0, key_bits, s) !=
+      hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_runs, dim3((n + 255) / 256), dim3(256), 0, s, tg_b, n, start, count);
+  hipLaunchKernelGGL(k_run_counts, dim3((ncells + 255) / 256), dim3(256), 0, s, start, count,
+                     ncells);
   return 0;
 }
 

@@ -178,9 +178,10 @@ def test_gpu_drop_in_body_records_match_soa(torch_cuda, oracle_lib):
     f.set_shift(u)
     f.get_force_and_potential(b)
     gp, ga, gj, _ = _run_gpu(torch_cuda, x, v, m, 8, 0.5, u)
-    assert np.array_equal(b["F"], ga) and np.array_equal(b["J"], gj)
-    assert np.array_equal(b["p"], gp)
-    assert np.array_equal(b["X"], x)
+    # identical up to the summation order of the M2L accumulation (atomics)
+    _check((b["p"], b["F"], b["J"]), (gp, ga, gj), "drop-in")
+    assert relerr_vec(b["F"], ga)[1] < 1e-13 and relerr_vec(b["J"], gj)[1] < 1e-12
+    assert np.array_equal(b["X"], x) and np.array_equal(b["index"], np.arange(n))
 
 
 def test_gpu_full_size_error_vs_direct(torch_cuda):
