@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-timeout", type=float, default=150.0)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "m2l_traffic.json"))
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size once (a plain copy kernel "
+                         "of known size for scaling HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -157,6 +160,12 @@ def main():
            "jerk_rms": float(torch.sqrt(torch.mean(ej ** 2))), "jerk_max": float(ej.max()),
            "subset": int(tg.numel()), "gate_force_rms_1e-7": bool(torch.sqrt(torch.mean(ea ** 2)) <= 1e-7)}
 
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     traffic = load_traffic(args.traffic_file)
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,

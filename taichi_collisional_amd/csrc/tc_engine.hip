@@ -89,8 +89,8 @@ struct tc_context {
   std::vector<int> level_start;
   DBuf leaf_ids;
   std::vector<int> h_leaves;
-  std::vector<int4> h_groups;
-  DBuf p2p_groups, p2p_ranges, npairs;
+  std::vector<int4> h_groups, h_items;
+  DBuf p2p_items, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists
   DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan;
@@ -251,9 +251,6 @@ static int build_tree(tc_context *ctx, int n) {
   ctx->st.nleaves = ctx->nleaves;
   TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves.data(), sizeof(int) * leaves.size(),
                           hipMemcpyHostToDevice, s));
-  TC_CHECK(ctx->p2p_groups.ensure(sizeof(int4) * std::max<size_t>(groups.size(), 1)));
-  TC_CHECK(hipMemcpyAsync(ctx->p2p_groups.p, groups.data(), sizeof(int4) * groups.size(),
-                          hipMemcpyHostToDevice, s));
   return 0;
 }
 
@@ -390,6 +387,21 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   TC_CHECK(ctx->m2l_chunks.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
   launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
                      ctx->m2l_chunks.as<int4>(), s);
+  // P2P work items: every target group x slices of <= kItemList list entries
+  constexpr int kItemList = 128;
+  std::vector<int> pcnt(nc);
+  TC_CHECK(hipMemcpyAsync(pcnt.data(), ctx->p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamSynchronize(s));
+  std::vector<int4> &items = ctx->h_items;
+  items.clear();
+  for (const int4 &g : ctx->h_groups) {
+    int cnt = pcnt[g.x];
+    for (int e0 = 0; e0 < cnt; e0 += kItemList)
+      items.push_back(make_int4(g.x, g.y, e0, std::min(kItemList, cnt - e0) | (g.z << 16)));
+  }
+  TC_CHECK(ctx->p2p_items.ensure(sizeof(int4) * std::max<size_t>(items.size(), 1)));
+  TC_CHECK(hipMemcpyAsync(ctx->p2p_items.p, items.data(), sizeof(int4) * items.size(),
+                          hipMemcpyHostToDevice, s));
   return 0;
 }
 
@@ -500,8 +512,8 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   TC_CHECK(hipMemsetAsync(ctx->npairs.p, 0, sizeof(unsigned long long), s));
   launch_p2p_ranges(cells_of(ctx), ctx->p2p_src.as<int>(), (int)ctx->st.n_p2p_cells,
                     ctx->p2p_ranges.as<int2>(), s);
-  launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_groups.as<int4>(),
-             (int)ctx->h_groups.size(), ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
+  launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
+             (int)ctx->h_items.size(), ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
              ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s);
   TC_CHECK(hipEventRecord(ctx->ev[6], s));
   // ---- downward
@@ -595,8 +607,8 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
-                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->p2p_groups,
-                 &ctx->p2p_ranges, &ctx->npairs};
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, 
+                 &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items};
   for (DBuf *b : all) b->release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);

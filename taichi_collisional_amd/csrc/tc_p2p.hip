@@ -46,7 +46,10 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
 
 constexpr int kTile = 256;
 
-// One block (one wave) per target group: <= 32 consecutive bodies of one leaf.
+// One block (one wave) per work item: a target group (<= 32 consecutive bodies
+// of one leaf) and a slice of at most kItemList entries of the leaf's P2P list
+// (long lists -- a sparse halo leaf can touch most of the system -- are split
+// so no single wave serializes them; partial sums are added atomically).
 // The leaf's P2P list is a list of source body ranges (one per source leaf);
 // up to 64 ranges are fetched at once (one per lane), a wave prefix sum over
 // their lengths places as many whole ranges as fit into an LDS tile of kTile
@@ -64,10 +67,11 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
   __shared__ int rstart[kWave], rexcl[kWave];
   __shared__ double red[7][kWave];
   const int lane = threadIdx.x;
+  // work item: target leaf, first target, list slice start, slice length | group size << 16
   const int4 gw = groups[blockIdx.x];
   const int ci = gw.x;
   if (!c.has_sink[ci]) return;
-  const int nt = gw.z;
+  const int nt = gw.w >> 16;
   const int G = kWave / nt;
   const int s = lane / nt, t = lane - s * nt;
   const bool act = s < G;
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
   const double xi = b.x[ib], yi = b.y[ib], zi = b.z[ib];
   const double vxi = b.vx[ib], vyi = b.vy[ib], vzi = b.vz[ib];
   Acc7 a = {0, 0, 0, 0, 0, 0, 0};
-  const int l0 = offs[ci], ln = count[ci];
+  const int l0 = offs[ci] + gw.z, ln = gw.w & 0xffff;
   long long nsrc = 0;
   int e = 0, off = 0;
   while (e < ln) {
@@ -149,13 +153,14 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
       for (int g = 0; g < G; g++) v += red[q][g * nt + t];
       o[q] = v;
     }
-    r.pot[ib] += o[0];
-    r.ax[ib] += o[1];
-    r.ay[ib] += o[2];
-    r.az[ib] += o[3];
-    r.jx[ib] += o[4];
-    r.jy[ib] += o[5];
-    r.jz[ib] += o[6];
+    // several work items may cover one target group (long lists are split)
+    atomicAdd(&r.pot[ib], o[0]);
+    atomicAdd(&r.ax[ib], o[1]);
+    atomicAdd(&r.ay[ib], o[2]);
+    atomicAdd(&r.az[ib], o[3]);
+    atomicAdd(&r.jx[ib], o[4]);
+    atomicAdd(&r.jy[ib], o[5]);
+    atomicAdd(&r.jz[ib], o[6]);
   }
 }
 
