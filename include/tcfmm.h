@@ -67,6 +67,9 @@ typedef struct tc_stats {
   long long n_p2p_pairs; /* body pairs in those leaf pairs (incl. self pairs)      */
   int ncells, nleaves, depth, nlevels;
   double ms_tree, ms_upward, ms_traverse, ms_lists, ms_m2l, ms_p2p, ms_downward, ms_total;
+  long long n_m2l_low;   /* TC_MAC_ERROR: pre-pass (p = 2) M2L cell pairs         */
+  long long n_p2p_low;   /* TC_MAC_ERROR: pre-pass leaf pairs                      */
+  double ms_prepass;     /* TC_MAC_ERROR: pre-pass time (not part of ms_tree)      */
 } tc_stats;
 
 typedef struct tc_context tc_context;
@@ -98,6 +101,13 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n);
  * is queued on the context's stream and the call returns after it finishes. */
 int tc_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                 const unsigned char *flags, double *pot, double *acc, double *jerk);
+
+/* |a| estimates of the last TC_MAC_ERROR evaluation's pre-pass (the values
+ * the reference stores in Body.acc_old, integrator.h:350-356), in the
+ * caller's body order, into acc_old[n] (device pointer).  Returns -2 when the
+ * last evaluation did not run the pre-pass (theta MAC or the all-pairs
+ * branch). */
+int tc_get_acc_old(tc_context *ctx, double *acc_old);
 
 /* FP64 all-pairs sum for nt targets (indices into the n bodies), device
  * pointers; used to measure the FMM error. */

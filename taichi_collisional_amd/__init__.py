@@ -46,7 +46,9 @@ class TcStats(C.Structure):
                 ("depth", C.c_int), ("nlevels", C.c_int), ("ms_tree", C.c_double),
                 ("ms_upward", C.c_double), ("ms_traverse", C.c_double),
                 ("ms_lists", C.c_double), ("ms_m2l", C.c_double), ("ms_p2p", C.c_double),
-                ("ms_downward", C.c_double), ("ms_total", C.c_double)]
+                ("ms_downward", C.c_double), ("ms_total", C.c_double),
+                ("n_m2l_low", C.c_longlong), ("n_p2p_low", C.c_longlong),
+                ("ms_prepass", C.c_double)]
 
 
 _lib = None
@@ -69,6 +71,7 @@ def lib():
         L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.tc_get_acc_old.argtypes = [vp, vp]
         L.tc_get_stats.argtypes = [vp, C.POINTER(TcStats)]
         L.tc_last_error.argtypes = [vp]
         L.tc_last_error.restype = C.c_char_p
@@ -78,7 +81,8 @@ def lib():
 
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
-                    "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_stats",
+                    "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_acc_old",
+                    "tc_get_stats",
                     "tc_last_error", "tc_supported_orders"]
 
 
@@ -107,6 +111,7 @@ class Fmm:
                                             supported_orders()))
         self.h = h
         self.p, self.ncrit, self.theta = p, ncrit, theta
+        self._last_n = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -154,9 +159,21 @@ class Fmm:
             acc = torch.empty(n, 3, dtype=torch.float64, device=dev)
         if jerk is None:
             jerk = torch.empty(n, 3, dtype=torch.float64, device=dev)
+        self._last_n = n
         self._check(lib().tc_eval_soa(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(flags),
                                       _ptr(pot), _ptr(acc), _ptr(jerk)), "tc_eval_soa")
         return pot, acc, jerk
+
+    def acc_old(self):
+        """|a| from the last error-MAC evaluation's pre-pass (caller order), cuda float64."""
+        import torch
+        n = self.stats_n()
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        self._check(lib().tc_get_acc_old(self.h, _ptr(out)), "tc_get_acc_old")
+        return out
+
+    def stats_n(self):
+        return self._last_n
 
     def direct(self, x, v, m, targets, flags=None):
         """FP64 all-pairs potential/acceleration/jerk for the given target indices."""

@@ -5,9 +5,12 @@
 //
 //   L2L (kernel_fast_lean.h:1001-1058): for each child,
 //       L_child[n,m] += sum_{k<=p-n} sum_{|l|<=k} L[n+k, m+l] conj Y_k^l(X - X_child)
-//       one wave per parent cell, lanes over output terms, harmonics in LDS.
+//       one block of 256 threads per parent cell: 32 threads per child, each
+//       thread a strided share of the child's (output term, L or Ldot) tasks;
+//       the eight harmonic tables and the parent's expansion sit in LDS.
 //   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
-//       dY/dt with velocity -v_b; only n <= 1 of the local field is formed:
+//       dY/dt with velocity -v_b, streamed order by order (three harmonic
+//       rows in registers); only n <= 1 of the local field is formed:
 //       p += Re Phi_0^0, a -= (Re Phi_1^1, Im Phi_1^1, Re Phi_1^0),
 //       j -= the same of Phidot = L conj dY + Ldot conj Y.
 #include "tc_common.h"
@@ -17,62 +20,51 @@
 namespace tcfmm {
 
 template <int P>
-__global__ __launch_bounds__(64) void k_l2l(CellsDev c, int ls) {
+__global__ __launch_bounds__(256) void k_l2l(CellsDev c, int ls) {
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
-  constexpr int RO = (NO + kWave - 1) / kWave;
-  __shared__ cplx G[NT], cl[NT], cld[NT];
-  const int lane = threadIdx.x;
+  constexpr int kGroup = 32;  // threads per child
+  static_assert(P + 1 <= kGroup, "one thread per harmonic column");
+  __shared__ cplx G[8][NT], cl[NT], cld[NT];
+  const int tid = threadIdx.x;
   const int ci = ls + blockIdx.x;
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
   const double *Lp = c.L + (size_t)ci * NT;
   const double *Ldp = c.Ld + (size_t)ci * NT;
-  for (int t = lane; t < NT; t += kWave) {
+  for (int t = tid; t < NT; t += blockDim.x) {
     int n = 0;
     while ((n + 1) * (n + 1) <= t) n++;
-    int m = t - n * n - n;
+    const int m = t - n * n - n;
     cl[t] = packed_to_c(Lp, n, m);
     cld[t] = packed_to_c(Ldp, n, m);
   }
-  int on[RO], om[RO];
-#pragma unroll
-  for (int r = 0; r < RO; r++) {
-    int o = lane + kWave * r, n = 0;
+  const int k = tid / kGroup, u = tid - k * kGroup;
+  const int ch = ch0 + k;
+  if (k < nch)
+    harm_table_col<P>(u, c.X[3 * ci] - c.X[3 * ch], c.X[3 * ci + 1] - c.X[3 * ch + 1],
+                      c.X[3 * ci + 2] - c.X[3 * ch + 2], G[k]);
+  __syncthreads();
+  if (k >= nch) return;
+  double *Lc = c.L + (size_t)ch * NT;
+  double *Ldc = c.Ld + (size_t)ch * NT;
+  const cplx *Gk = G[k];
+  // tasks (output term, L or Ldot) of this child, strided over the group
+  for (int j = u; j < 2 * NO; j += kGroup) {
+    const int which = j >= NO, o = j - which * NO;
+    int n = 0;
     while (tri(n + 1, 0) <= o) n++;
-    on[r] = n;
-    om[r] = o - tri(n, 0);
-  }
-  const double X0 = c.X[3 * ci], X1 = c.X[3 * ci + 1], X2 = c.X[3 * ci + 2];
-  for (int k = 0; k < nch; k++) {
-    const int ch = ch0 + k;
-    __syncthreads();
-    harm_table_lds<P>(X0 - c.X[3 * ch], X1 - c.X[3 * ch + 1], X2 - c.X[3 * ch + 2], G);
-    __syncthreads();
-    double *Lc = c.L + (size_t)ch * NT;
-    double *Ldc = c.Ld + (size_t)ch * NT;
-#pragma unroll
-    for (int r = 0; r < RO; r++) {
-      int o = lane + kWave * r;
-      if (o >= NO) continue;
-      const int n = on[r], m = om[r];
-      cplx a = {0, 0}, ad = {0, 0};
-      for (int kk = 0; kk <= P - n; kk++)
-        for (int l = -kk; l <= kk; l++) {
-          cplx g = cconj(G[ixnm(kk, l)]);
-          int sidx = ixnm(n + kk, m + l);
-          a = cadd(a, cmul(cl[sidx], g));
-          ad = cadd(ad, cmul(cld[sidx], g));
-        }
-      const int s = n * n + n;
-      Lc[s + m] += a.re;
-      Ldc[s + m] += ad.re;
-      if (m > 0) {
-        Lc[s - m] += a.im;
-        Ldc[s - m] += ad.im;
-      }
-    }
+    const int m = o - tri(n, 0);
+    const cplx *src = which ? cld : cl;
+    cplx a = {0, 0};
+    for (int kk = 0; kk <= P - n; kk++)
+      for (int l = -kk; l <= kk; l++)
+        a = cadd(a, cmul(src[ixnm(n + kk, m + l)], cconj(Gk[ixnm(kk, l)])));
+    double *dst = which ? Ldc : Lc;
+    const int s = n * n + n;
+    dst[s + m] += a.re;
+    if (m > 0) dst[s - m] += a.im;
   }
 }
 
@@ -80,11 +72,13 @@ template <int P>
 __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev r,
                                             const int *__restrict__ leaf_ids) {
   constexpr int NT = nterm(P);
-  constexpr int NY = tri(P, P) + 1;
+  constexpr int NQ = (P + 1) * (P + 1);
   __shared__ cplx cl[NT], cld[NT];
+  __shared__ double s_ra[NQ], s_rb[NQ];
   const int lane = threadIdx.x;
   const int ci = leaf_ids[blockIdx.x];
   if (!c.has_sink[ci]) return;
+  harm_coef_lds<P>(s_ra, s_rb);
   const int b0 = c.body[ci], nb = c.nbody[ci];
   const double *Lp = c.L + (size_t)ci * NT;
   const double *Ldp = c.Ld + (size_t)ci * NT;
@@ -102,31 +96,52 @@ __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev 
     if (base + lane >= nb) continue;
     if (!(b.flags[i] & kSink)) continue;
     const double vx = -b.vx[i], vy = -b.vy[i], vz = -b.vz[i];
-    cplx y[NY];
-    harm_all<P>(X0 - b.x[i], X1 - b.y[i], X2 - b.z[i], y);
+    const double dx = X0 - b.x[i], dy = X1 - b.y[i], dz = X2 - b.z[i];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    cplx y1[P + 1], y2[P + 1];
+#pragma unroll
+    for (int m = 0; m <= P; m++) y1[m] = y2[m] = {0.0, 0.0};
     cplx p00 = {0, 0}, p10 = {0, 0}, p11 = {0, 0};
     cplx d10 = {0, 0}, d11 = {0, 0};
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k <= P; k++) {
+      cplx yk[P + 1];
+      harm_row<P>(k, dx, dy, dz, r2, s_ra, s_rb, y1, y2, yk);
+      const cplx *L0 = cl + k * k + k, *L1 = cl + (k + 1) * (k + 1) + (k + 1);
+      const cplx *D1 = cld + (k + 1) * (k + 1) + (k + 1);
+      const bool up = k < P;
 #pragma unroll
-      for (int l = -k; l <= k; l++) {
-        const int al = l < 0 ? -l : l;
-        cplx g = harm_at(y, k, l);
-        cplx gd = harm_dot(y, k, al, vx, vy, vz);
-        if (l < 0) {
-          gd = cconj(gd);
-          if (al & 1) gd = {-gd.re, -gd.im};
+      for (int m = 0; m <= P; m++) {
+        if (m > k) continue;
+        const cplx dy_m = harm_dot_row<P>(y1, k, m, vx, vy, vz);
+        // l = +m: conj Y_k^m, conj dY_k^m
+        {
+          const cplx g = cconj(yk[m]), gd = cconj(dy_m);
+          p00 = cadd(p00, cmul(L0[m], g));
+          if (up) {
+            p10 = cadd(p10, cmul(L1[m], g));
+            p11 = cadd(p11, cmul(L1[m + 1], g));
+            d10 = cadd(d10, cadd(cmul(L1[m], gd), cmul(D1[m], g)));
+            d11 = cadd(d11, cadd(cmul(L1[m + 1], gd), cmul(D1[m + 1], g)));
+          }
         }
-        g = cconj(g);
-        gd = cconj(gd);
-        p00 = cadd(p00, cmul(cl[ixnm(k, l)], g));
-        if (k + 1 <= P) {
-          p10 = cadd(p10, cmul(cl[ixnm(k + 1, l)], g));
-          p11 = cadd(p11, cmul(cl[ixnm(k + 1, l + 1)], g));
-          d10 = cadd(d10, cadd(cmul(cl[ixnm(k + 1, l)], gd), cmul(cld[ixnm(k + 1, l)], g)));
-          d11 = cadd(d11,
-                     cadd(cmul(cl[ixnm(k + 1, l + 1)], gd), cmul(cld[ixnm(k + 1, l + 1)], g)));
+        // l = -m: conj Y_k^{-m} = (-1)^m Y_k^m, likewise for dY
+        if (m > 0) {
+          const double sg = (m & 1) ? -1.0 : 1.0;
+          const cplx g = cscale(sg, yk[m]), gd = cscale(sg, dy_m);
+          p00 = cadd(p00, cmul(L0[-m], g));
+          if (up) {
+            p10 = cadd(p10, cmul(L1[-m], g));
+            p11 = cadd(p11, cmul(L1[1 - m], g));
+            d10 = cadd(d10, cadd(cmul(L1[-m], gd), cmul(D1[-m], g)));
+            d11 = cadd(d11, cadd(cmul(L1[1 - m], gd), cmul(D1[1 - m], g)));
+          }
         }
+      }
+#pragma unroll
+      for (int m = 0; m <= P; m++) {
+        y2[m] = y1[m];
+        y1[m] = yk[m];
       }
     }
     r.pot[i] += p00.re;
@@ -167,7 +182,7 @@ int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double
 
 template <int P>
 static void l2l_go(CellsDev c, int ls, int le, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2l<P>, dim3(le - ls), dim3(64), 0, s, c, ls);
+  hipLaunchKernelGGL(k_l2l<P>, dim3(le - ls), dim3(256), 0, s, c, ls);
 }
 template <int P>
 static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {

@@ -98,6 +98,10 @@ struct tc_context {
   int *h_pin = nullptr;  // small pinned host buffer for per-step counts
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
+  // error-controlled MAC: p = 2 pre-pass, multipole norms, acc_old output
+  DBuf lowM, lowL, lowF, cPn, accout;
+  bool has_acc_old = false;  // the last evaluation ran the pre-pass
+  int last_n = 0;
   // direct
   DBuf dtg, dpot, dacc, djerk, dx3, dv3, dm, dfl;
   hipEvent_t ev[9];
@@ -140,6 +144,7 @@ static int ensure_bodies(tc_context *ctx, int n) {
                   &ctx->bacc, &ctx->rpot, &ctx->rax, &ctx->ray, &ctx->raz, &ctx->rjx, &ctx->rjy,
                   &ctx->rjz})
     TC_CHECK(b->ensure(d));
+  TC_CHECK(ctx->lowF.ensure(sizeof(double) * 3 * (size_t)n));
   TC_CHECK(ctx->bflags.ensure(n));
   TC_CHECK(ctx->border.ensure(sizeof(int) * (size_t)n));
   TC_CHECK(ctx->key_hi.ensure(8 * (size_t)n));
@@ -174,6 +179,9 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
   TC_CHECK(ctx->csink.ensure(cap, keep, s));
   for (DBuf *b : {&ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd})
     TC_CHECK(b->ensure(sizeof(double) * NT * cap));
+  TC_CHECK(ctx->lowM.ensure(sizeof(double) * 9 * cap));
+  TC_CHECK(ctx->lowL.ensure(sizeof(double) * 9 * cap));
+  TC_CHECK(ctx->cPn.ensure(sizeof(double) * (ctx->cfg.p + 1) * cap));
   for (DBuf *b : {&ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur, &ctx->m2l_coff, &ctx->p2p_cnt,
                   &ctx->p2p_off, &ctx->p2p_cur})
     TC_CHECK(b->ensure(sizeof(int) * (cap + 1)));
@@ -284,7 +292,7 @@ __global__ void k_trav_totals(const int *__restrict__ blk, int nblk, int stride,
 }
 
 // Breadth-first dual traversal from (root, root), one launch pair per step.
-static int traverse(tc_context *ctx, int n) {
+static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   hipStream_t s = ctx->stream;
   int *h = ctx->h_pin;
   TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 8));
@@ -300,7 +308,7 @@ static int traverse(tc_context *ctx, int n) {
     TC_CHECK(ctx->tscan.ensure(scan_temp_bytes(nblk) + 256));
     int *blk = ctx->tblk.as<int>();
     CellsDev c = cells_of(ctx);
-    launch_trav_count(c, ctx->front_a.as<int2>(), nin, ctx->cfg.theta, blk, blk + stride,
+    launch_trav_count(c, ctx->front_a.as<int2>(), nin, mac, blk, blk + stride,
                       blk + 2 * stride, s);
     for (int k = 0; k < 3; k++)
       if (launch_scan(blk + k * stride, blk + (3 + k) * stride, nblk, ctx->tscan.p,
@@ -327,7 +335,7 @@ static int traverse(tc_context *ctx, int n) {
     a.cnt = ctx->tcnt.as<int>();
     a.m2l = ctx->m2l.as<int2>();
     a.p2p = ctx->p2p.as<int2>();
-    a.theta = ctx->cfg.theta;
+    a.mac = mac;
     launch_trav_write(cells_of(ctx), a, blk + 3 * stride, blk + 4 * stride, blk + 5 * stride, s);
     nm2l += tm;
     np2p += tp;
@@ -405,11 +413,77 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   return 0;
 }
 
+// The reference's factorial_table (kernel_fast_lean.h:124-131): products of
+// doubles up to index 2p-1; with the quirk index 2p stays 0.
+static FactTable fact_table(const tc_config &cfg) {
+  FactTable f;
+  for (double &v : f.fact) v = 0.0;
+  f.fact[0] = 1.0;
+  const int top = cfg.quirk_factorial ? 2 * cfg.p : 2 * cfg.p + 1;
+  for (int i = 1; i < top; i++) f.fact[i] = (double)i * f.fact[i - 1];
+  return f;
+}
+
+static MacArgs mac_args(tc_context *ctx, int mode) {
+  MacArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.mode = mode;
+  a.p = ctx->cfg.p;
+  a.theta = ctx->cfg.theta;
+  a.facc = ctx->cfg.force_accuracy;
+  a.Pn = ctx->cPn.as<double>();
+  const FactTable f = fact_table(ctx->cfg);
+  for (int k = 0; k <= a.p; k++) a.comb[k] = f.fact[a.p] / f.fact[k] / f.fact[a.p - k];
+  return a;
+}
+
+// p = 2 pre-pass of the error-MAC path (integrator.h:326-356): acc_old of
+// every body (sorted order) from a cheap FMM on the tree geometry.
+static int low_pass(tc_context *ctx, int n) {
+  hipStream_t s = ctx->stream;
+  const int nc = ctx->st.ncells;
+  CellsDev c = cells_of(ctx);
+  BodiesDev b = bodies_of(ctx);
+  TC_CHECK(hipMemsetAsync(ctx->lowM.p, 0, sizeof(double) * 9 * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->lowL.p, 0, sizeof(double) * 9 * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->lowF.p, 0, sizeof(double) * 3 * (size_t)n, s));
+  launch_low_upward(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
+                    ctx->lowM.as<double>(), s);
+  if (int e = traverse(ctx, n, mac_args(ctx, kMacLow))) return e;
+  ctx->st.n_m2l_low = ctx->st.n_m2l;
+  ctx->st.n_p2p_low = ctx->st.n_p2p_cells;
+  if (int e = build_csr(ctx, ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->m2l_cnt, ctx->m2l_off,
+                        ctx->m2l_src))
+    return e;
+  if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
+                        ctx->p2p_off, ctx->p2p_src))
+    return e;
+  launch_low_horizontal(c, b, nc, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->m2l_off.as<int>(),
+                        ctx->m2l_cnt.as<int>(), ctx->m2l_src.as<int>(), ctx->p2p_off.as<int>(),
+                        ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowM.as<double>(),
+                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), s);
+  launch_low_downward(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
+                      ctx->lowL.as<double>(), ctx->lowF.as<double>(), n, s);
+  return 0;
+}
+
+// acc_old of the last evaluation into out[n] in the caller's body order.
+static int copy_acc_old(tc_context *ctx, int n, double *out) {
+  if (!ctx->has_acc_old || n != ctx->last_n) {
+    ctx->err = "acc_old is only computed by an error-MAC evaluation (TC_MAC_ERROR)";
+    return -2;
+  }
+  launch_scatter_acc_old(bodies_of(ctx), n, out, ctx->stream);
+  return 0;
+}
+
 static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                        const unsigned char *flags, double *pot, double *acc, double *jerk) {
   hipStream_t s = ctx->stream;
   const tc_config &cfg = ctx->cfg;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
+  ctx->has_acc_old = false;
+  ctx->last_n = n;
   if (n <= 0) {
     ctx->err = "n must be positive";
     return -1;
@@ -458,10 +532,6 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
     TC_CHECK(hipStreamSynchronize(s));
     return 0;
   }
-  if (cfg.mac != TC_MAC_THETA) {
-    ctx->err = "error-controlled MAC is not available in this build";
-    return -2;
-  }
 
   // ---- tree
   launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
@@ -476,12 +546,21 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   }
   launch_gather(x, v, m, flags, ctx->val_a.as<int>(), n, bodies_of(ctx), s);
   if (int e = build_tree(ctx, n)) return e;
+  const bool err_mac = cfg.mac == TC_MAC_ERROR;
+  if (err_mac) {
+    TC_CHECK(hipEventRecord(ctx->ev[8], s));
+    if (int e = low_pass(ctx, n)) return e;
+    ctx->has_acc_old = true;
+  }
   TC_CHECK(hipEventRecord(ctx->ev[1], s));
   // ---- upward
   if (int e = upward(ctx)) return e;
+  if (err_mac)
+    launch_pn(ctx->cM.as<double>(), ctx->st.ncells, cfg.p, fact_table(cfg), ctx->cPn.as<double>(),
+              s);
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
   // ---- traversal and lists
-  if (int e = traverse(ctx, n)) return e;
+  if (int e = traverse(ctx, n, mac_args(ctx, err_mac ? kMacError : kMacTheta))) return e;
   TC_CHECK(hipEventRecord(ctx->ev[3], s));
   int nchunks = 0;
   if (int e = build_lists(ctx, nchunks)) return e;
@@ -531,6 +610,13 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   float ms[8];
   for (int k = 0; k < 7; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
   hipEventElapsedTime(&ms[7], ctx->ev[0], ctx->ev[7]);
+  if (err_mac) {
+    // tree phase ends where the pre-pass starts
+    float pre = 0;
+    hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[8]);
+    hipEventElapsedTime(&pre, ctx->ev[8], ctx->ev[1]);
+    ctx->st.ms_prepass = pre;
+  }
   ctx->st.ms_tree = ms[0];
   ctx->st.ms_upward = ms[1];
   ctx->st.ms_traverse = ms[2];
@@ -572,6 +658,8 @@ static bool order_supported(int p) {
 int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
   if (!cfg || !out) return -1;
   if (!order_supported(cfg->p) || cfg->ncrit < 1 || !(cfg->theta > 0)) return -1;
+  if (cfg->mac != TC_MAC_THETA && cfg->mac != TC_MAC_ERROR) return -1;
+  if (cfg->mac == TC_MAC_ERROR && !(cfg->force_accuracy > 0)) return -1;
   tc_context *ctx = new tc_context();
   ctx->cfg = *cfg;
   if (stream) {
@@ -608,7 +696,8 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
                  &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, 
-                 &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items};
+                 &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
+                 &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
@@ -687,10 +776,17 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
                       ctx->dfl.as<unsigned char>(), op.as<double>(), oa.as<double>(),
                       oj.as<double>());
   if (e == 0) {
-    std::vector<double> hp(n), ha(3 * (size_t)n), hj(3 * (size_t)n);
+    std::vector<double> hp(n), ha(3 * (size_t)n), hj(3 * (size_t)n), ho;
     hipMemcpyAsync(hp.data(), op.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(ha.data(), oa.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(hj.data(), oj.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
+    // acc_old is set by the pre-pass of the error-MAC path (integrator.h:350-356)
+    const bool want_acc = ctx->has_acc_old;
+    if (want_acc) {
+      ho.resize(n);
+      e = copy_acc_old(ctx, n, op.as<double>());
+      if (e == 0) hipMemcpyAsync(ho.data(), op.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+    }
     hipStreamSynchronize(s);
     for (int i = 0; i < n; i++) {
       bodies[i].p = hp[i];
@@ -698,6 +794,7 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
         bodies[i].F[d] = ha[3 * i + d];
         bodies[i].J[d] = hj[3 * i + d];
       }
+      if (want_acc) bodies[i].acc_old = ho[i];
     }
   }
   op.release();
@@ -724,6 +821,17 @@ int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, cons
   launch_direct(bodies_of(ctx), n, targets, nt, pot, acc, jerk, s);
   TC_CHECK(hipStreamSynchronize(s));
   TC_CHECK(hipGetLastError());
+  return 0;
+}
+
+int tc_get_acc_old(tc_context *ctx, double *acc_old) {
+  if (!ctx) return -1;
+  if (!acc_old) {
+    ctx->err = "acc_old must be a device pointer";
+    return -1;
+  }
+  if (int e = copy_acc_old(ctx, ctx->last_n, acc_old)) return e;
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

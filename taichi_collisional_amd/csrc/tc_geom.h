@@ -1,17 +1,17 @@
-// Smallest enclosing ball, computed by one wave (device side of the cell
-// geometry of P2M / M2M, taichi_with_jerks/kernel_fast_lean.h:733-763 and
-// :875-935, which use the Miniball of miniball.hpp).
+// Smallest enclosing ball, computed by one wave or one block (device side of
+// the cell geometry of P2M / M2M, taichi_with_jerks/kernel_fast_lean.h:733-763
+// and :875-935, which use the Miniball of miniball.hpp).
 //
 // Algorithm (same as the CPU oracle, so both produce the same ball up to
-// rounding): start from point 0; repeatedly find the point of largest excess
-// |q-c|^2 - r^2 (lanes scan strided subsets, then a wave argmax with ties to
-// the lowest index); if it lies outside, the new ball is the smallest ball of
+// rounding; the minimum ball is unique): start from point 0; repeatedly find
+// the point of largest excess |q-c|^2 - r^2 (threads scan subsets, then an
+// argmax with ties to the lowest index); if it lies outside, the new ball is the smallest ball of
 // (support set + q) with q on its boundary, found by testing the circumball of
 // q with every subset of the support (<= 15 candidates, each of <= 4 points).
 // The radius grows strictly, so the loop ends; the final ball encloses every
 // point and is the minimum ball of its own support, hence of all points.
-// Every lane runs the small solves on identical data, so the result is
-// uniform across the wave.  Contraction is off to match the CPU oracle.
+// Every thread runs the small solves on identical data, so the result is
+// uniform across the block.  Contraction is off to match the CPU oracle.
 #pragma once
 #include "tc_common.h"
 
@@ -83,10 +83,20 @@ __device__ inline bool circumball_dev(const double (*p)[3], int k, DBall &b) {
 
 constexpr double kBallTol = 1e-13;
 
-template <class Get>
-__device__ DBall wave_minball(int n, Get get) {
+// Smallest ball over the points one block of NW waves can enumerate.
+//   get(i, q)   writes point i (uniform index; used for point 0 and the
+//               selected point of each iteration),
+//   scan(f)     calls f(i, q) for this thread's share of the points, in
+//               increasing i.
+// Per iteration the point of largest excess is found with ties going to the
+// lowest index (thread, wave, then block argmax), so the sequence of support
+// sets -- and hence the ball -- does not depend on NW.  s_e / s_i hold 2 x NW
+// slots (double-buffered: one barrier per iteration).
+template <int NW, class Get, class Scan>
+__device__ DBall block_minball(Get get, Scan scan, double *s_e, int *s_i) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
   DBall b;
   double sup[4][3];
   int ns = 1;
@@ -96,9 +106,7 @@ __device__ DBall wave_minball(int n, Get get) {
   for (int iter = 0; iter < 10000; iter++) {
     double best = -1;
     int bidx = 0x7fffffff;
-    for (int i = lane; i < n; i += kWave) {
-      double q[3];
-      get(i, q);
+    scan([&](int i, const double *q) {
       double e = 0;
       for (int d = 0; d < 3; d++) e += (q[d] - b.c[d]) * (q[d] - b.c[d]);
       e -= b.r2;
@@ -106,8 +114,26 @@ __device__ DBall wave_minball(int n, Get get) {
         best = e;
         bidx = i;
       }
-    }
+    });
     wave_argmax(best, bidx);
+    if (NW > 1) {
+      const int par = iter & 1;
+      if (lane == 0) {
+        s_e[par * NW + wid] = best;
+        s_i[par * NW + wid] = bidx;
+      }
+      __syncthreads();
+      best = s_e[par * NW];
+      bidx = s_i[par * NW];
+      for (int w = 1; w < NW; w++) {
+        const double e2 = s_e[par * NW + w];
+        const int i2 = s_i[par * NW + w];
+        if (e2 > best || (e2 == best && i2 < bidx)) {
+          best = e2;
+          bidx = i2;
+        }
+      }
+    }
     if (best <= kBallTol * b.r2 || best <= 0) break;
     double bq[3];
     get(bidx, bq);
@@ -160,6 +186,22 @@ __device__ DBall wave_minball(int n, Get get) {
     b = bestb;
   }
   return b;
+}
+
+// One wave, points 0..n-1 from get(), lane-strided scan.
+template <class Get>
+__device__ DBall wave_minball(int n, Get get) {
+  const int lane = threadIdx.x & (kWave - 1);
+  return block_minball<1>(
+      get,
+      [&](auto f) {
+        for (int i = lane; i < n; i += kWave) {
+          double q[3];
+          get(i, q);
+          f(i, q);
+        }
+      },
+      nullptr, nullptr);
 }
 
 }  // namespace tcfmm

@@ -35,6 +35,64 @@ __device__ __forceinline__ void harm_all(double dx, double dy, double dz, cplx *
   }
 }
 
+// Recurrence coefficients for harm_row, index n*(P+1)+m (used for m < n-1):
+// ra = (2n-1)/(n^2-m^2), rb = 1/(n^2-m^2).  Filled by the whole block; the
+// caller synchronizes before use.
+template <int P>
+__device__ __forceinline__ void harm_coef_lds(double *ra, double *rb) {
+  for (int t = threadIdx.x; t < (P + 1) * (P + 1); t += blockDim.x) {
+    const int n = t / (P + 1), m = t - n * (P + 1);
+    const double q = (double)(n * n - m * m);
+    ra[t] = (m + 1 < n) ? (2 * n - 1.0) / q : 0.0;
+    rb[t] = (m + 1 < n) ? 1.0 / q : 0.0;
+  }
+}
+
+// Row n (runtime) of the solid harmonics, yn[m] = Y_n^m for m = 0..P (zero
+// for m > n), from rows n-1 (y1) and n-2 (y2): the recurrences of harm_all,
+// so orders can be streamed with three rows of registers.
+template <int P>
+__device__ __forceinline__ void harm_row(int n, double dx, double dy, double dz, double r2,
+                                         const double *ra, const double *rb, const cplx *y1,
+                                         const cplx *y2, cplx *yn) {
+  const double tn = n > 0 ? 2.0 * n : 1.0;
+  const cplx f = {dx / tn, dy / tn};
+  const double *a_n = ra + n * (P + 1), *b_n = rb + n * (P + 1);
+#pragma unroll
+  for (int m = 0; m <= P; m++) {
+    cplx v = {0.0, 0.0};
+    if (n == 0) {
+      if (m == 0) v = {1.0, 0.0};
+    } else if (m + 1 < n) {
+      const double a = a_n[m] * dz, b = r2 * b_n[m];
+      v = {a * y1[m].re - b * y2[m].re, a * y1[m].im - b * y2[m].im};
+    } else if (m + 1 == n) {
+      v = cscale(dz, y1[m]);
+    } else if (m == n && m >= 1) {
+      v = cmul(y1[m >= 1 ? m - 1 : 0], f);
+    }
+    yn[m] = v;
+  }
+}
+
+// dY_n^m/dt (m >= 0, compile-time; n runtime) from row n-1 (y1): harm_dot.
+template <int P>
+__device__ __forceinline__ cplx harm_dot_row(const cplx *y1, int n, int m, double vx, double vy,
+                                             double vz) {
+  cplx r = {0.0, 0.0};
+  if (n == 0) return r;
+  const cplx eta = {-vx * 0.5, vy * 0.5}, xi = {vx * 0.5, vy * 0.5};
+  if (m + 2 <= n) r = cadd(r, cmul(y1[m + 1 <= P ? m + 1 : P], eta));
+  if (m + 1 <= n) r = cadd(r, cscale(vz, y1[m]));
+  if (m >= 1) {
+    r = cadd(r, cmul(y1[m >= 1 ? m - 1 : 0], xi));
+  } else if (n >= 2) {
+    const cplx c1 = cconj(y1[1]);
+    r = cadd(r, cmul({-c1.re, -c1.im}, xi));
+  }
+  return r;
+}
+
 // Y_k^l for any l (|l| <= k) from the m >= 0 table.
 __device__ __forceinline__ cplx harm_at(const cplx *y, int k, int l) {
   if (l >= 0) return y[tri(k, l)];
@@ -58,11 +116,11 @@ __device__ __forceinline__ cplx harm_dot(const cplx *y, int n, int m, double vx,
 }
 
 // Cooperative table of Y_n^m for all n <= P, -n <= m <= n into LDS
-// (real-packed index order, complex values): lane m computes column m.
+// (real-packed index order, complex values): the caller's column m (0..P)
+// is computed by one thread; other values of m do nothing.
 template <int P>
-__device__ __forceinline__ void harm_table_lds(double dx, double dy, double dz, cplx *G) {
-  const int m = threadIdx.x;
-  if (m <= P) {
+__device__ __forceinline__ void harm_table_col(int m, double dx, double dy, double dz, cplx *G) {
+  if (m >= 0 && m <= P) {
     double r2 = dx * dx + dy * dy + dz * dz;
     cplx y = {1.0, 0.0};
     for (int k = 1; k <= m; k++) y = cmul(y, {dx / (2 * k), dy / (2 * k)});
@@ -89,6 +147,12 @@ __device__ __forceinline__ void harm_table_lds(double dx, double dy, double dz, 
       }
     }
   }
+}
+
+// Whole table by the block's first P+1 threads.
+template <int P>
+__device__ __forceinline__ void harm_table_lds(double dx, double dy, double dz, cplx *G) {
+  harm_table_col<P>((int)threadIdx.x, dx, dy, dz, G);
 }
 
 // Complex form of a real-packed expansion (real_2_complex,

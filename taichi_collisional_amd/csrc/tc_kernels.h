@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace tcfmm {
 
 constexpr int kMaxP = 20;
@@ -70,16 +72,30 @@ size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 
 // ---- traversal (tc_tree.hip)
+// Multipole acceptance: kMacTheta  |dX|^2 theta^2 > (Ri+Rj)^2 on the balls
+// (traverse_eager.h:125-156); kMacLow  |dXg|^2 > 3 (Rgi+Rgj)^2 on the tree
+// geometry (:172-212); kMacError  the error estimate of horizontalPassHigh
+// (:59-122) from Pn of the source and min_acc of the target.
+enum { kMacTheta = 0, kMacLow = 1, kMacError = 2 };
+struct MacArgs {
+  int mode, p;
+  double theta, facc;
+  const double *Pn;         // [cell][p+1], kMacError
+  double comb[kMaxP + 1];   // C(p,k), kMacError
+};
+struct FactTable {
+  double fact[2 * kMaxP + 1];  // the reference's factorial_table (quirk applied)
+};
 struct TravArgs {
   const int2 *fin;
   int nin;
   int2 *fout;
   int *cnt;  // [1] M2L pairs so far, [2] P2P pairs so far (device copy)
   int2 *m2l, *p2p;
-  double theta;
+  MacArgs mac;
 };
-int launch_trav_count(CellsDev c, const int2 *fin, int nin, double theta, int *bm2l, int *bp2p,
-                      int *bsplit, hipStream_t s);
+int launch_trav_count(CellsDev c, const int2 *fin, int nin, const MacArgs &mac, int *bm2l,
+                      int *bp2p, int *bsplit, hipStream_t s);
 int launch_trav_write(CellsDev c, const TravArgs &a, const int *om2l, const int *op2p,
                       const int *osplit, hipStream_t s);
 size_t pair_sort_temp_bytes(int n);
@@ -113,6 +129,19 @@ int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ng
 int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s);
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s);
+
+// ---- p = 2 pre-pass and multipole norms (tc_low.hip)
+int launch_low_upward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                      const std::vector<int> &level_start, double *lowM, hipStream_t s);
+int launch_low_horizontal(CellsDev c, BodiesDev b, int nc, const int *leaf_ids, int nleaves,
+                          const int *m2l_off, const int *m2l_cnt, const int *m2l_src,
+                          const int *p2p_off, const int *p2p_cnt, const int *p2p_src,
+                          const double *lowM, double *lowL, double *F, hipStream_t s);
+int launch_low_downward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                        const std::vector<int> &level_start, double *lowL, double *F, int n,
+                        hipStream_t s);
+int launch_pn(const double *M, int nc, int p, const FactTable &ft, double *Pn, hipStream_t s);
+int launch_scatter_acc_old(BodiesDev b, int n, double *out, hipStream_t s);
 
 // ---- downward (tc_downward.hip)
 int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s);

@@ -339,19 +339,53 @@ int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, h
 // totals (M2L, P2P, child pairs); the totals are scanned over blocks; pass 2
 // classifies again and writes every output at its scanned position, so the
 // lists come out in frontier order (deterministic).
-__device__ __forceinline__ int trav_classify(const CellsDev &c, int i, int j, double theta,
+__device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a, int i, int j,
                                              int *nsplit) {
 #pragma clang fp contract(off)
-  double dx = c.X[3 * i] - c.X[3 * j];
-  double dy = c.X[3 * i + 1] - c.X[3 * j + 1];
-  double dz = c.X[3 * i + 2] - c.X[3 * j + 2];
-  double R2 = (dx * dx + dy * dy + dz * dz) * theta * theta;
-  double Ri = c.R[i], Rj = c.R[j];
-  int nci = c.nchild[i], ncj = c.nchild[j];
+  const int nci = c.nchild[i], ncj = c.nchild[j];
+  double Ri, Rj;
+  bool accept;
+  if (a.mode == kMacLow) {
+    const double dx = c.Xg[3 * i] - c.Xg[3 * j];
+    const double dy = c.Xg[3 * i + 1] - c.Xg[3 * j + 1];
+    const double dz = c.Xg[3 * i + 2] - c.Xg[3 * j + 2];
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    Ri = c.Rg[i];
+    Rj = c.Rg[j];
+    accept = R2 > (Ri + Rj) * (Ri + Rj) * 3;
+  } else {
+    const double dx = c.X[3 * i] - c.X[3 * j];
+    const double dy = c.X[3 * i + 1] - c.X[3 * j + 1];
+    const double dz = c.X[3 * i + 2] - c.X[3 * j + 2];
+    Ri = c.R[i];
+    Rj = c.R[j];
+    if (a.mode == kMacTheta) {
+      const double R2 = (dx * dx + dy * dy + dz * dz) * a.theta * a.theta;
+      accept = R2 > (Ri + Rj) * (Ri + Rj);
+    } else {
+      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double s = Ri + Rj;
+      double f2 = 10;
+      if (R2 * 0.4 > s * s) {
+        const int P = a.p;
+        const double R = sqrt(R2);
+        const double Rp = pow(R, -(double)(P + 1));
+        const double *Pn = a.Pn + (size_t)j * (P + 1);
+        double pa = 1, E = 0;
+        for (int k = 0; k <= P; k++) {
+          E += Pn[P - k] * pa * a.comb[k];
+          pa *= Ri;
+        }
+        E *= 8 * fmax(Ri, Rj) * Rp;
+        f2 = E / (a.facc * c.min_acc[i]);
+      }
+      accept = f2 < 1;
+    }
+  }
   *nsplit = 0;
-  if (R2 > (Ri + Rj) * (Ri + Rj)) return 1;  // M2L
-  if (nci == 0 && ncj == 0) return 2;        // P2P
-  if (ncj == 0 || (Ri >= Rj && nci != 0)) {  // split the target
+  if (accept) return 1;                       // M2L
+  if (nci == 0 && ncj == 0) return 2;         // P2P
+  if (ncj == 0 || (Ri >= Rj && nci != 0)) {   // split the target
     *nsplit = nci;
     return 3;
   }
@@ -378,7 +412,7 @@ __device__ __forceinline__ int3 block_excl_scan3(int3 v, int3 *tot) {
 }
 
 __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__restrict__ fin,
-                                                    int nin, double theta,
+                                                    int nin, MacArgs mac,
                                                     int *__restrict__ bm2l,
                                                     int *__restrict__ bp2p,
                                                     int *__restrict__ bsplit) {
@@ -387,7 +421,7 @@ __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__re
   if (t < nin) {
     int2 pr = fin[t];
     int ns;
-    int kind = trav_classify(c, pr.x, pr.y, theta, &ns);
+    int kind = trav_classify(c, mac, pr.x, pr.y, &ns);
     v = make_int3(kind == 1, kind == 2, ns);
   }
   int3 tot;
@@ -408,7 +442,7 @@ __global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
   int2 pr = make_int2(0, 0);
   if (t < a.nin) {
     pr = a.fin[t];
-    kind = trav_classify(c, pr.x, pr.y, a.theta, &ns);
+    kind = trav_classify(c, a.mac, pr.x, pr.y, &ns);
   }
   int3 tot;
   int3 ex = block_excl_scan3(make_int3(kind == 1, kind == 2, ns), &tot);
@@ -424,10 +458,10 @@ __global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
   (void)nblk;
 }
 
-int launch_trav_count(CellsDev c, const int2 *fin, int nin, double theta, int *bm2l, int *bp2p,
-                      int *bsplit, hipStream_t s) {
+int launch_trav_count(CellsDev c, const int2 *fin, int nin, const MacArgs &mac, int *bm2l,
+                      int *bp2p, int *bsplit, hipStream_t s) {
   if (nin <= 0) return 0;
-  hipLaunchKernelGGL(k_trav_count, dim3((nin + 255) / 256), dim3(256), 0, s, c, fin, nin, theta,
+  hipLaunchKernelGGL(k_trav_count, dim3((nin + 255) / 256), dim3(256), 0, s, c, fin, nin, mac,
                      bm2l, bp2p, bsplit);
   return 0;
 }

@@ -2,16 +2,19 @@
 // M2M for moments M and their time derivatives Mdot (taichi_with_jerks/
 // kernel_fast_lean.h:701-999, driven post-order by traverse_eager.h:9-32).
 //
-//   P2M  one wave per leaf: the ball of the leaf's bodies (centre X, radius
-//        r + 1e-6), then lane = body: M += q Y(x_b - X), Mdot += q dY/dt(v_b)
-//        with the absolute body velocity; per order the contributions are
-//        summed across the wave.  min_acc / has_sink are reduced alongside.
-//   M2M  one wave per internal cell (one launch per level, deepest first):
-//        the ball of the bodies of leaf children and of 64 design points on
-//        each grandchild sphere of non-leaf children, then for each child
-//        M_parent[n,m] += sum_{k,l} M_child[n-k, m-l] Y_k^l(X_child - X_parent)
-//        with lanes over the output terms and the child's harmonics staged
-//        in LDS.
+//   P2M  one wave per leaf: the leaf's bodies are staged in LDS, their ball
+//        (centre X, radius r + 1e-6) is found, then lane = body walks the
+//        orders n = 0..P with rolling harmonic rows; per order the terms
+//        q Y_n^m(x_b - X) and q dY_n^m/dt(v_b) are summed over the bodies
+//        through an LDS transpose (lane j sums column j).  min_acc and
+//        has_sink are reduced alongside.
+//   M2M  one block of 4 waves per internal cell (one launch per level,
+//        deepest first): the ball of the bodies of leaf children and of 64
+//        design points on each grandchild sphere (points generated from LDS),
+//        then for each child
+//          M_parent[n,m] += sum_{k,l} M_child[n-k, m-l] Y_k^l(X_child - X_parent)
+//        with one thread per (output term, M or Mdot) and the child's
+//        harmonics and moments staged in LDS.
 #include "tc_common.h"
 #include "tc_geom.h"
 #include "tc_harm.h"
@@ -20,22 +23,40 @@
 
 namespace tcfmm {
 
-__constant__ double c_tdesign[192];
-
 template <int P>
 __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
                                             const int *__restrict__ leaf_ids) {
   constexpr int NT = nterm(P);
-  constexpr int NY = tri(P, P) + 1;
+  constexpr int NQ = (P + 1) * (P + 1);
+  constexpr int NR = 2 * (P + 1);  // real, imaginary part for m = 0..P
+  constexpr int kStage = 128;
+  __shared__ double s_q[3][kStage];
+  __shared__ double s_ra[NQ], s_rb[NQ];
+  __shared__ double red[kWave][NR + 1];
   const int lane = threadIdx.x;
+  harm_coef_lds<P>(s_ra, s_rb);
   const int ci = leaf_ids[blockIdx.x];
   const int b0 = c.body[ci], nb = c.nbody[ci];
+  const bool staged = nb <= kStage;
+  if (staged)
+    for (int i = lane; i < nb; i += kWave) {
+      s_q[0][i] = b.x[b0 + i];
+      s_q[1][i] = b.y[b0 + i];
+      s_q[2][i] = b.z[b0 + i];
+    }
+  __syncthreads();
   double X[3], R;
   if (nb > 0) {
     DBall ball = wave_minball(nb, [&](int i, double *q) {
-      q[0] = b.x[b0 + i];
-      q[1] = b.y[b0 + i];
-      q[2] = b.z[b0 + i];
+      if (staged) {
+        q[0] = s_q[0][i];
+        q[1] = s_q[1][i];
+        q[2] = s_q[2][i];
+      } else {
+        q[0] = b.x[b0 + i];
+        q[1] = b.y[b0 + i];
+        q[2] = b.z[b0 + i];
+      }
     });
     for (int d = 0; d < 3; d++) X[d] = ball.c[d];
     R = sqrt(ball.r2) + 1e-6;
@@ -64,9 +85,9 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
   double *Mdc = c.Md + (size_t)ci * NT;
   for (int base = 0; base < nb || base == 0; base += kWave) {
     const int i = b0 + base + lane;
-    const bool act = base + lane < nb;
+    const int cnt = min(kWave, nb - base);
     double dx = 0, dy = 0, dz = 0, vx = 0, vy = 0, vz = 0, w = 0;
-    if (act) {
+    if (lane < cnt) {
       dx = b.x[i] - X[0];
       dy = b.y[i] - X[1];
       dz = b.z[i] - X[2];
@@ -75,34 +96,41 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
       vz = b.vz[i];
       w = (b.flags[i] & kSource) ? b.q[i] : 0.0;
     }
-    cplx y[NY];
-    harm_all<P>(dx, dy, dz, y);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    cplx y1[P + 1], y2[P + 1];
 #pragma unroll
+    for (int m = 0; m <= P; m++) y1[m] = y2[m] = {0.0, 0.0};
+#pragma unroll 1
     for (int n = 0; n <= P; n++) {
+      cplx yn[P + 1];
+      harm_row<P>(n, dx, dy, dz, r2, s_ra, s_rb, y1, y2, yn);
       const int s = n * n + n;
 #pragma unroll
-      for (int m = 0; m <= n; m++) {
-        cplx g = y[tri(n, m)];
-        cplx gd = harm_dot(y, n, m, vx, vy, vz);
-        double a0 = wave_sum(w * g.re), a1 = (m > 0) ? wave_sum(w * g.im) : 0.0;
-        double d0 = wave_sum(w * gd.re), d1 = (m > 0) ? wave_sum(w * gd.im) : 0.0;
-        if (lane == 0) {
-          if (base == 0) {
-            Mc[s + m] = a0;
-            Mdc[s + m] = d0;
-            if (m > 0) {
-              Mc[s - m] = a1;
-              Mdc[s - m] = d1;
-            }
-          } else {
-            Mc[s + m] += a0;
-            Mdc[s + m] += d0;
-            if (m > 0) {
-              Mc[s - m] += a1;
-              Mdc[s - m] += d1;
-            }
+      for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int m = 0; m <= P; m++)
+          if (m <= n) {
+            const cplx v = pass == 0 ? yn[m] : harm_dot_row<P>(y1, n, m, vx, vy, vz);
+            red[lane][2 * m] = w * v.re;
+            red[lane][2 * m + 1] = w * v.im;
           }
+        __syncthreads();
+        const int m = lane >> 1, im = lane & 1;
+        if (lane < 2 * (n + 1) && !(m == 0 && im)) {
+          double acc = 0;
+          for (int r = 0; r < cnt; r++) acc += red[r][lane];
+          double *dst = (pass == 0 ? Mc : Mdc) + (im ? s - m : s + m);
+          if (base == 0)
+            *dst = acc;
+          else
+            *dst += acc;
         }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int m = 0; m <= P; m++) {
+        y2[m] = y1[m];
+        y1[m] = yn[m];
       }
     }
     if (nb == 0) break;
@@ -110,122 +138,157 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
 }
 
 template <int P>
-__global__ __launch_bounds__(64) void k_m2m(CellsDev c, BodiesDev b, int ls) {
+__global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, int ls) {
 #pragma clang fp contract(off)
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
-  constexpr int RO = (NO + kWave - 1) / kWave;
+  constexpr int NW = 4;
+  constexpr int kStage = 512;  // body points of leaf children staged in LDS
+  __shared__ double s_td[192];
+  __shared__ double s_g[64][4];  // grandchild ball (X, R), children in order
+  __shared__ double s_p[3][kStage];
+  __shared__ int s_pre[9], s_base[8], s_leaf[8], s_body[8], s_nstage;
+  __shared__ double s_e[2 * NW];
+  __shared__ int s_i[2 * NW];
   __shared__ cplx G[NT], cm[NT], cmd[NT];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int ci = ls + blockIdx.x;
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
-  int pre[9];
-  pre[0] = 0;
-  double mina = 1e30;
-  int sink = 0;
-  for (int k = 0; k < 8; k++) {
-    int cnt = 0;
-    if (k < nch) {
-      int ch = ch0 + k;
-      cnt = c.nchild[ch] == 0 ? c.nbody[ch] : 64 * c.nchild[ch];
-      mina = fmin(c.min_acc[ch], mina);
-      sink |= c.has_sink[ch];
+  if (tid < 192) s_td[tid] = kTDesign[tid];
+  if (tid == 0) {
+    int pre = 0, nbs = 0, ng = 0;
+    for (int k = 0; k < nch; k++) {
+      const int ch = ch0 + k;
+      const int nc2 = c.nchild[ch];
+      s_pre[k] = pre;
+      s_leaf[k] = nc2 == 0;
+      s_body[k] = c.body[ch];
+      if (nc2 == 0) {
+        s_base[k] = nbs;
+        nbs += c.nbody[ch];
+        pre += c.nbody[ch];
+      } else {
+        s_base[k] = ng;
+        ng += nc2;
+        pre += 64 * nc2;
+      }
     }
-    pre[k + 1] = pre[k] + cnt;
+    s_pre[nch] = pre;
+    s_nstage = nbs;
   }
-  auto get = [&](int i, double *q) {
-    int k = 0;
-    while (k < nch - 1 && i >= pre[k + 1]) k++;
-    int ch = ch0 + k;
-    int local = i - pre[k];
-    if (c.nchild[ch] == 0) {
-      int bi = c.body[ch] + local;
-      q[0] = b.x[bi];
-      q[1] = b.y[bi];
-      q[2] = b.z[bi];
+  __syncthreads();
+  const bool staged = s_nstage <= kStage;
+  for (int k = 0; k < nch; k++) {
+    const int ch = ch0 + k;
+    if (s_leaf[k]) {
+      if (staged) {
+        const int nbk = s_pre[k + 1] - s_pre[k], o = s_base[k], bb = s_body[k];
+        for (int i = tid; i < nbk; i += blockDim.x) {
+          s_p[0][o + i] = b.x[bb + i];
+          s_p[1][o + i] = b.y[bb + i];
+          s_p[2][o + i] = b.z[bb + i];
+        }
+      }
     } else {
-      int g = c.child[ch] + (local >> 6);
-      int t = local & 63;
-      double Rg = c.R[g];
-      for (int d = 0; d < 3; d++) q[d] = c.X[3 * g + d] + Rg * c_tdesign[3 * t + d];
+      const int g0 = c.child[ch], ngk = (s_pre[k + 1] - s_pre[k]) >> 6;
+      if (tid < ngk) {
+        const int g = g0 + tid;
+        s_g[s_base[k] + tid][0] = c.X[3 * g];
+        s_g[s_base[k] + tid][1] = c.X[3 * g + 1];
+        s_g[s_base[k] + tid][2] = c.X[3 * g + 2];
+        s_g[s_base[k] + tid][3] = c.R[g];
+      }
+    }
+  }
+  __syncthreads();
+  // point `local` of child k, in the order the reference enumerates them
+  auto point = [&](int k, int local, double *q) {
+    if (s_leaf[k]) {
+      if (staged) {
+        const int o = s_base[k] + local;
+        q[0] = s_p[0][o];
+        q[1] = s_p[1][o];
+        q[2] = s_p[2][o];
+      } else {
+        const int bi = s_body[k] + local;
+        q[0] = b.x[bi];
+        q[1] = b.y[bi];
+        q[2] = b.z[bi];
+      }
+    } else {
+      const int g = s_base[k] + (local >> 6), t = local & 63;
+      const double Rg = s_g[g][3];
+      for (int d = 0; d < 3; d++) q[d] = s_g[g][d] + Rg * s_td[3 * t + d];
     }
   };
-  DBall ball = wave_minball(pre[nch], get);
-  double X[3] = {ball.c[0], ball.c[1], ball.c[2]};
-  if (lane == 0) {
+  DBall ball = block_minball<NW>(
+      [&](int i, double *q) {
+        int k = 0;
+        while (k < nch - 1 && i >= s_pre[k + 1]) k++;
+        point(k, i - s_pre[k], q);
+      },
+      [&](auto f) {
+        for (int k = 0; k < nch; k++) {
+          const int p0 = s_pre[k], cnt = s_pre[k + 1] - p0;
+          for (int l = tid; l < cnt; l += NW * kWave) {
+            double q[3];
+            point(k, l, q);
+            f(p0 + l, q);
+          }
+        }
+      },
+      s_e, s_i);
+  const double X[3] = {ball.c[0], ball.c[1], ball.c[2]};
+  if (tid == 0) {
+    double mina = 1e30;
+    int sink = 0;
+    for (int k = 0; k < nch; k++) {
+      mina = fmin(c.min_acc[ch0 + k], mina);
+      sink |= c.has_sink[ch0 + k];
+    }
     for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
     c.R[ci] = sqrt(ball.r2) + 1e-6;
     c.min_acc[ci] = mina;
     c.has_sink[ci] = (unsigned char)sink;
   }
-  cplx acc[RO], accd[RO];
-  int on[RO], om[RO];
-#pragma unroll
-  for (int r = 0; r < RO; r++) {
-    acc[r] = {0, 0};
-    accd[r] = {0, 0};
-    int o = lane + kWave * r, n = 0;
-    while (tri(n + 1, 0) <= o) n++;
-    on[r] = n;
-    om[r] = o - tri(n, 0);
-  }
+  // translation: thread = (output term o, M or Mdot)
+  const int which = tid >> 7, o = tid & 127;
+  const bool act = o < NO;
+  int n = 0;
+  while (tri(n + 1, 0) <= o && n < P) n++;
+  const int m = o - tri(n, 0);
+  cplx a = {0, 0};
   for (int k = 0; k < nch; k++) {
     const int ch = ch0 + k;
     __syncthreads();
     harm_table_lds<P>(c.X[3 * ch] - X[0], c.X[3 * ch + 1] - X[1], c.X[3 * ch + 2] - X[2], G);
     const double *Mch = c.M + (size_t)ch * NT;
     const double *Mdch = c.Md + (size_t)ch * NT;
-    for (int t = lane; t < NT; t += kWave) {
-      int n = 0;
-      while ((n + 1) * (n + 1) <= t) n++;
-      int m = t - n * n - n;
-      cm[t] = packed_to_c(Mch, n, m);
-      cmd[t] = packed_to_c(Mdch, n, m);
+    for (int t = tid; t < NT; t += blockDim.x) {
+      int nn = 0;
+      while ((nn + 1) * (nn + 1) <= t) nn++;
+      const int mm = t - nn * nn - nn;
+      cm[t] = packed_to_c(Mch, nn, mm);
+      cmd[t] = packed_to_c(Mdch, nn, mm);
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RO; r++) {
-      int o = lane + kWave * r;
-      if (o >= NO) continue;
-      const int n = on[r], m = om[r];
-      cplx a = acc[r], ad = accd[r];
+    if (act) {
+      const cplx *src = which ? cmd : cm;
       for (int kk = 0; kk <= n; kk++) {
-        int lo = -kk > m - n + kk ? -kk : m - n + kk;
-        int hi = kk < m + n - kk ? kk : m + n - kk;
-        for (int l = lo; l <= hi; l++) {
-          cplx g = G[ixnm(kk, l)];
-          int src = ixnm(n - kk, m - l);
-          a = cadd(a, cmul(cm[src], g));
-          ad = cadd(ad, cmul(cmd[src], g));
-        }
+        const int lo = -kk > m - n + kk ? -kk : m - n + kk;
+        const int hi = kk < m + n - kk ? kk : m + n - kk;
+        for (int l = lo; l <= hi; l++) a = cadd(a, cmul(src[ixnm(n - kk, m - l)], G[ixnm(kk, l)]));
       }
-      acc[r] = a;
-      accd[r] = ad;
     }
   }
-  double *Mc = c.M + (size_t)ci * NT;
-  double *Mdc = c.Md + (size_t)ci * NT;
-#pragma unroll
-  for (int r = 0; r < RO; r++) {
-    int o = lane + kWave * r;
-    if (o >= NO) continue;
-    const int n = on[r], m = om[r], s = n * n + n;
-    Mc[s + m] = acc[r].re;
-    Mdc[s + m] = accd[r].re;
-    if (m > 0) {
-      Mc[s - m] = acc[r].im;
-      Mdc[s - m] = accd[r].im;
-    }
+  if (act) {
+    double *dst = (which ? c.Md : c.M) + (size_t)ci * NT;
+    const int s = n * n + n;
+    dst[s + m] = a.re;
+    if (m > 0) dst[s - m] = a.im;
   }
-}
-
-static void upload_tdesign() {
-  static bool done = false;
-  if (done) return;
-  hipMemcpyToSymbol(HIP_SYMBOL(c_tdesign), kTDesign, sizeof(double) * 192);
-  done = true;
 }
 
 template <int P>
@@ -235,7 +298,7 @@ static void p2m_launch(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves
 
 template <int P>
 static void m2m_launch(CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
-  hipLaunchKernelGGL(k_m2m<P>, dim3(le - ls), dim3(64), 0, s, c, b, ls);
+  hipLaunchKernelGGL(k_m2m<P>, dim3(le - ls), dim3(256), 0, s, c, b, ls);
 }
 
 int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s) {
@@ -253,7 +316,6 @@ int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
 }
 
 int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
-  upload_tdesign();
   if (le <= ls) return 0;
   switch (p) {
 #define TC_CASE(P)                     \
