@@ -115,6 +115,43 @@ int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, cons
                   const unsigned char *flags, int nt, const int *targets, double *pot,
                   double *acc, double *jerk);
 
+/* ---- Per-kernel entry points over the reference's Cell records -----------
+ * The reference's kernels take Cell pointers (kernel_fast_lean.h): P2M(C)
+ * :701, M2M(C) :826, M2L_rotate(Ci, Cj) :1130, L2L(C) :1001, L2P(C) :1060,
+ * P2P(Ci, Cj) :499.  Each entry point below applies that kernel to every
+ * listed cell (ids) or (targets[k], sources[k]) pair on the GPU, with the
+ * result of calling the reference kernel once per entry in list order:
+ *   tc_P2M  sets X, R (smallest ball + 1e-6), min_acc; ORs has_sink;
+ *           M, Mdot += (bodies of the cell);
+ *   tc_M2M  the same from the children (and grandchildren's balls);
+ *   tc_M2L  L, Ldot of the target += (skipped when !has_sink);
+ *   tc_L2L  L, Ldot of the children += ;
+ *   tc_L2P  p, F, J of the sink bodies += / -= ;
+ *   tc_P2P  p, F, J of the target's sink bodies += (skipped when !has_sink).
+ * `cells` holds ncells records laid out as the reference's struct Cell for
+ * the context's order p (tc_get_cell_layout); CHILD and BODY hold pointers
+ * into `cells` and `bodies` (nbodies Body records), as buildTree leaves them.
+ * Host memory; the call returns when the records are updated.  Returns 0, -1
+ * for bad arguments (pointers outside the arrays, ids out of range). */
+typedef struct tc_cell_layout {
+  int size;               /* sizeof(Cell)                                    */
+  int nchild, nbody, child, body, X, R, min_acc, cell_mass, has_sink;
+  int M, L, Mdot, Ldot, Pn; /* byte offsets                                  */
+} tc_cell_layout;
+int tc_get_cell_layout(int p, tc_cell_layout *out);
+
+int tc_P2M(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncells,
+           const int *ids, int nids);
+int tc_M2M(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncells,
+           const int *ids, int nids);
+int tc_M2L(tc_context *ctx, void *cells, int ncells, const int *targets, const int *sources,
+           int npairs);
+int tc_L2L(tc_context *ctx, void *cells, int ncells, const int *ids, int nids);
+int tc_L2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncells,
+           const int *ids, int nids);
+int tc_P2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncells,
+           const int *targets, const int *sources, int npairs);
+
 int tc_get_stats(const tc_context *ctx, tc_stats *out);
 const char *tc_last_error(const tc_context *ctx);
 

@@ -51,6 +51,12 @@ class TcStats(C.Structure):
                 ("ms_prepass", C.c_double)]
 
 
+class TcCellLayout(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("size", "nchild", "nbody", "child", "body", "X", "R",
+                                       "min_acc", "cell_mass", "has_sink", "M", "L", "Mdot",
+                                       "Ldot", "Pn")]
+
+
 _lib = None
 
 
@@ -72,6 +78,12 @@ def lib():
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.tc_get_acc_old.argtypes = [vp, vp]
+        L.tc_get_cell_layout.argtypes = [C.c_int, C.POINTER(TcCellLayout)]
+        for k in ("tc_P2M", "tc_M2M", "tc_L2P"):
+            getattr(L, k).argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.tc_L2L.argtypes = [vp, vp, C.c_int, vp, C.c_int]
+        L.tc_M2L.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int]
+        L.tc_P2P.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int]
         L.tc_get_stats.argtypes = [vp, C.POINTER(TcStats)]
         L.tc_last_error.argtypes = [vp]
         L.tc_last_error.restype = C.c_char_p
@@ -82,8 +94,26 @@ def lib():
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_acc_old",
-                    "tc_get_stats",
+                    "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
+                    "tc_P2P", "tc_get_stats",
                     "tc_last_error", "tc_supported_orders"]
+
+
+def cell_dtype(p):
+    """numpy dtype of the reference's struct Cell for order p (exafmm.h:50-71)."""
+    lay = TcCellLayout()
+    if lib().tc_get_cell_layout(int(p), C.byref(lay)) != 0:
+        raise ValueError("no cell layout for p=%r" % p)
+    nt = (p + 1) ** 2
+    return np.dtype({"names": ["NCHILD", "NBODY", "CHILD", "BODY", "X", "R", "min_acc",
+                               "cell_mass", "has_sink", "M", "L", "Mdot", "Ldot", "Pn"],
+                     "formats": ["<i4", "<i4", "<u8", "<u8", ("<f8", 3), "<f8", "<f8", "<f8",
+                                 "u1", ("<f8", nt), ("<f8", nt), ("<f8", nt), ("<f8", nt),
+                                 ("<f8", p + 1)],
+                     "offsets": [lay.nchild, lay.nbody, lay.child, lay.body, lay.X, lay.R,
+                                 lay.min_acc, lay.cell_mass, lay.has_sink, lay.M, lay.L,
+                                 lay.Mdot, lay.Ldot, lay.Pn],
+                     "itemsize": lay.size})
 
 
 def supported_orders():
@@ -188,6 +218,55 @@ class Fmm:
                                         _ptr(flags), nt, _ptr(tg), _ptr(pot), _ptr(acc),
                                         _ptr(jerk)), "tc_direct_soa")
         return pot, acc, jerk
+
+    # ---- per-kernel entry points over the reference's Cell / Body records ----
+    def _recs(self, bodies, cells):
+        if cells.dtype != cell_dtype(self.p) or not cells.flags.c_contiguous:
+            raise TypeError("cells must be a C-contiguous array of cell_dtype(p)")
+        if bodies is not None and (bodies.dtype != body_dtype or not bodies.flags.c_contiguous):
+            raise TypeError("bodies must be a C-contiguous array of body_dtype")
+
+    @staticmethod
+    def _ids(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        return a, a.ctypes.data
+
+    def P2M(self, bodies, cells, ids):
+        self._recs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_P2M(self.h, bodies.ctypes.data, bodies.size, cells.ctypes.data,
+                                 cells.size, pa, a.size), "tc_P2M")
+
+    def M2M(self, bodies, cells, ids):
+        self._recs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_M2M(self.h, bodies.ctypes.data, bodies.size, cells.ctypes.data,
+                                 cells.size, pa, a.size), "tc_M2M")
+
+    def M2L(self, cells, targets, sources):
+        self._recs(None, cells)
+        t, pt = self._ids(targets)
+        s, ps = self._ids(sources)
+        self._check(lib().tc_M2L(self.h, cells.ctypes.data, cells.size, pt, ps, t.size),
+                    "tc_M2L")
+
+    def L2L(self, cells, ids):
+        self._recs(None, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_L2L(self.h, cells.ctypes.data, cells.size, pa, a.size), "tc_L2L")
+
+    def L2P(self, bodies, cells, ids):
+        self._recs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_L2P(self.h, bodies.ctypes.data, bodies.size, cells.ctypes.data,
+                                 cells.size, pa, a.size), "tc_L2P")
+
+    def P2P(self, bodies, cells, targets, sources):
+        self._recs(bodies, cells)
+        t, pt = self._ids(targets)
+        s, ps = self._ids(sources)
+        self._check(lib().tc_P2P(self.h, bodies.ctypes.data, bodies.size, cells.ctypes.data,
+                                 cells.size, pt, ps, t.size), "tc_P2P")
 
     def stats(self):
         st = TcStats()

@@ -20,14 +20,14 @@
 namespace tcfmm {
 
 template <int P>
-__global__ __launch_bounds__(256) void k_l2l(CellsDev c, int ls) {
+__global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
   constexpr int kGroup = 32;  // threads per child
   static_assert(P + 1 <= kGroup, "one thread per harmonic column");
   __shared__ cplx G[8][NT], cl[NT], cld[NT];
   const int tid = threadIdx.x;
-  const int ci = ls + blockIdx.x;
+  const int ci = ids ? ids[blockIdx.x] : ls + blockIdx.x;
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
@@ -181,8 +181,8 @@ int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double
 }
 
 template <int P>
-static void l2l_go(CellsDev c, int ls, int le, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2l<P>, dim3(le - ls), dim3(256), 0, s, c, ls);
+static void l2l_go(CellsDev c, const int *ids, int ls, int ncell, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2l<P>, dim3(ncell), dim3(256), 0, s, c, ids, ls);
 }
 template <int P>
 static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {
@@ -192,9 +192,23 @@ static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n,
 int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s) {
   if (le <= ls) return 0;
   switch (p) {
-#define TC_CASE(P)              \
-  case P:                       \
-    l2l_go<P>(c, ls, le, s);    \
+#define TC_CASE(P)                            \
+  case P:                                     \
+    l2l_go<P>(c, nullptr, ls, le - ls, s);    \
+    return 0;
+    TC_FOR_EACH_P(TC_CASE)
+#undef TC_CASE
+    default:
+      return -1;
+  }
+}
+
+int launch_l2l_ids(int p, CellsDev c, const int *ids, int nids, hipStream_t s) {
+  if (nids <= 0) return 0;
+  switch (p) {
+#define TC_CASE(P)                          \
+  case P:                                   \
+    l2l_go<P>(c, ids, 0, nids, s);          \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE

@@ -17,125 +17,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tcfmm.h"
-#include "tc_common.h"
-#include "tc_kernels.h"
+#include "tc_engine.h"
 
 using namespace tcfmm;
-
-namespace {
-
-struct DBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  // grow to at least b bytes; keep `keep` bytes of old content
-  hipError_t ensure(size_t b, size_t keep = 0, hipStream_t s = nullptr) {
-    if (b <= bytes) return hipSuccess;
-    size_t nb = std::max(b, bytes + bytes / 2);
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, nb);
-    if (e != hipSuccess) return e;
-    if (keep && p) {
-      e = hipMemcpyAsync(q, p, std::min(keep, bytes), hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) return e;
-      hipStreamSynchronize(s);
-    }
-    if (p) hipFree(p);
-    p = q;
-    bytes = nb;
-    return hipSuccess;
-  }
-  void release() {
-    if (p) hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T *as() const {
-    return (T *)p;
-  }
-};
-
-struct Lcg48 {  // srand48 / drand48: X' = (0x5DEECE66D X + 0xB) mod 2^48
-  uint64_t X = 0x1234ABCD330EULL;
-  void seed(long s) { X = (((uint64_t)(uint32_t)s) << 16) | 0x330E; }
-  double next() {
-    X = (0x5DEECE66DULL * X + 0xB) & ((1ULL << 48) - 1);
-    return std::ldexp((double)X, -48);
-  }
-};
-
-}  // namespace
-
-struct tc_context {
-  tc_config cfg;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  std::string err;
-  tc_stats st{};
-  Lcg48 rng;
-  bool fixed_shift = false;
-  double shift[3] = {0, 0, 0};
-  // bodies
-  int cap_n = 0;
-  DBuf bx, by, bz, bvx, bvy, bvz, bq, bflags, border, bacc;
-  DBuf rpot, rax, ray, raz, rjx, rjy, rjz;
-  DBuf key_hi, key_a, key_b, val_a, val_b, sort_tmp, part, tp, dshift;
-  // cells
-  int cap_c = 0;
-  DBuf clevel, cnchild, cchild, cbody, cnbody, cXg, cRg, cX, cR, cmin, csink;
-  DBuf cM, cMd, cL, cLd;
-  DBuf split_offs, split_nch, split_scan, scan_tmp;
-  std::vector<int> level_start;
-  DBuf leaf_ids;
-  std::vector<int> h_leaves;
-  std::vector<int4> h_groups, h_items;
-  DBuf p2p_items, p2p_ranges, npairs;
-  int nleaves = 0;
-  // traversal / lists
-  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan;
-  DBuf gtg_a, gtg_b, gsrc_a, gsort_tmp;
-  int *h_pin = nullptr;  // small pinned host buffer for per-step counts
-  DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
-  DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
-  // error-controlled MAC: p = 2 pre-pass, multipole norms, acc_old output
-  DBuf lowM, lowL, lowF, cPn, accout;
-  bool has_acc_old = false;  // the last evaluation ran the pre-pass
-  int last_n = 0;
-  // direct
-  DBuf dtg, dpot, dacc, djerk, dx3, dv3, dm, dfl;
-  hipEvent_t ev[9];
-};
-
-#define TC_CHECK(expr)                                                   \
-  do {                                                                   \
-    hipError_t _e = (expr);                                              \
-    if (_e != hipSuccess) {                                              \
-      ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);      \
-      return -3;                                                         \
-    }                                                                    \
-  } while (0)
-
-static BodiesDev bodies_of(tc_context *ctx) {
-  return {ctx->bx.as<double>(),   ctx->by.as<double>(),  ctx->bz.as<double>(),
-          ctx->bvx.as<double>(),  ctx->bvy.as<double>(), ctx->bvz.as<double>(),
-          ctx->bq.as<double>(),   ctx->bflags.as<unsigned char>(), ctx->border.as<int>(),
-          ctx->bacc.as<double>()};
-}
-
-static CellsDev cells_of(tc_context *ctx) {
-  return {ctx->clevel.as<int>(), ctx->cnchild.as<int>(), ctx->cchild.as<int>(),
-          ctx->cbody.as<int>(),  ctx->cnbody.as<int>(),  ctx->cXg.as<double>(),
-          ctx->cRg.as<double>(), ctx->cX.as<double>(),   ctx->cR.as<double>(),
-          ctx->cmin.as<double>(), ctx->csink.as<unsigned char>(), ctx->cM.as<double>(),
-          ctx->cMd.as<double>(), ctx->cL.as<double>(),   ctx->cLd.as<double>()};
-}
-
-static ResultsDev results_of(tc_context *ctx) {
-  return {ctx->rpot.as<double>(), ctx->rax.as<double>(), ctx->ray.as<double>(),
-          ctx->raz.as<double>(),  ctx->rjx.as<double>(), ctx->rjy.as<double>(),
-          ctx->rjz.as<double>()};
-}
 
 static int ensure_bodies(tc_context *ctx, int n) {
   if (n <= ctx->cap_n) return 0;
@@ -271,6 +155,7 @@ static int upward(tc_context *ctx) {
   TC_CHECK(hipMemsetAsync(ctx->cMd.p, 0, sizeof(double) * NT * nc, s));
   TC_CHECK(hipMemsetAsync(ctx->cL.p, 0, sizeof(double) * NT * nc, s));
   TC_CHECK(hipMemsetAsync(ctx->cLd.p, 0, sizeof(double) * NT * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->csink.p, 0, nc, s));
   CellsDev c = cells_of(ctx);
   BodiesDev b = bodies_of(ctx);
   if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s)) {

@@ -22,7 +22,10 @@ struct DBall {
   double r2;
 };
 
-__device__ inline bool circumball_dev(const double (*p)[3], int k, DBall &b) {
+// Smallest ball with the k (1..4) points p[0..k-1] on its boundary: the
+// circumcentre in their affine hull.  Loops have fixed bounds with guards so
+// every array stays in registers; the arithmetic is that of the CPU oracle.
+__device__ inline bool circumball_dev(const double (&p)[4][3], int k, DBall &b) {
 #pragma clang fp contract(off)
   if (k == 1) {
     for (int d = 0; d < 3; d++) b.c[d] = p[0][d];
@@ -30,12 +33,16 @@ __device__ inline bool circumball_dev(const double (*p)[3], int k, DBall &b) {
     return true;
   }
   double v[3][3];
-  for (int i = 1; i < k; i++)
-    for (int d = 0; d < 3; d++) v[i - 1][d] = p[i][d] - p[0][d];
-  int q = k - 1;
+#pragma unroll
+  for (int i = 1; i < 4; i++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[i - 1][d] = i < k ? p[i][d] - p[0][d] : 0.0;
+  const int q = k - 1;
   double A[3][3], rhs[3];
-  for (int i = 0; i < q; i++) {
-    for (int j = 0; j < q; j++) A[i][j] = v[i][0] * v[j][0] + v[i][1] * v[j][1] + v[i][2] * v[j][2];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) A[i][j] = v[i][0] * v[j][0] + v[i][1] * v[j][1] + v[i][2] * v[j][2];
     rhs[i] = 0.5 * A[i][i];
   }
   double lam[3] = {0, 0, 0};
@@ -63,21 +70,73 @@ __device__ inline bool circumball_dev(const double (*p)[3], int k, DBall &b) {
     inv[2][0] = c02;
     inv[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
     inv[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+#pragma unroll
     for (int i = 0; i < 3; i++)
       lam[i] = (inv[i][0] * rhs[0] + inv[i][1] * rhs[1] + inv[i][2] * rhs[2]) / det;
   }
+#pragma unroll
   for (int d = 0; d < 3; d++) {
     double c = p[0][d];
-    for (int i = 0; i < q; i++) c += lam[i] * v[i][d];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      if (i < q) c += lam[i] * v[i][d];
     b.c[d] = c;
   }
   double r2 = 0;
-  for (int i = 0; i < k; i++) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (i >= k) continue;
     double dd = 0;
+#pragma unroll
     for (int d = 0; d < 3; d++) dd += (p[i][d] - b.c[d]) * (p[i][d] - b.c[d]);
     r2 = fmax(r2, dd);
   }
   b.r2 = r2;
+  return true;
+}
+
+// Points bq, then the support points selected by mask, in support order.
+__device__ inline int gather_support(int mask, const double (&bq)[3], const double (&sup)[4][3],
+                                     double (&p)[4][3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    p[0][d] = bq[d];
+    p[1][d] = p[2][d] = p[3][d] = 0.0;
+  }
+  int k = 1;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    if (mask & (1 << s)) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double v = sup[s][d];
+        if (k == 1) p[1][d] = v;
+        if (k == 2) p[2][d] = v;
+        if (k == 3) p[3][d] = v;
+      }
+      k++;
+    }
+  }
+  return k;
+}
+
+// Candidate `mask` of one refinement step: the ball through bq and the
+// selected support points, valid when it also holds the other support points.
+__device__ inline bool candidate_ball(int mask, int ns, const double (&bq)[3],
+                                      const double (&sup)[4][3], DBall &cb) {
+#pragma clang fp contract(off)
+  if (mask >= (1 << ns) || __popc(mask) > 3) return false;
+  double pts[4][3];
+  const int k = gather_support(mask, bq, sup, pts);
+  if (!circumball_dev(pts, k, cb)) return false;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    if (s >= ns || (mask & (1 << s))) continue;
+    double dd = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) dd += (sup[s][d] - cb.c[d]) * (sup[s][d] - cb.c[d]);
+    if (dd > cb.r2 * (1 + 1e-12)) return false;
+  }
   return true;
 }
 
@@ -137,47 +196,31 @@ __device__ DBall block_minball(Get get, Scan scan, double *s_e, int *s_i) {
     if (best <= kBallTol * b.r2 || best <= 0) break;
     double bq[3];
     get(bidx, bq);
-    DBall bestb;
-    bestb.r2 = -1;
-    int bestmask = 0;
-    for (int mask = 0; mask < (1 << ns); mask++) {
-      int cnt = __popc(mask);
-      if (cnt > 3) continue;
-      double pts[4][3];
-      int k = 0;
-      for (int d = 0; d < 3; d++) pts[k][d] = bq[d];
-      k++;
-      for (int s = 0; s < ns; s++)
-        if (mask & (1 << s)) {
-          for (int d = 0; d < 3; d++) pts[k][d] = sup[s][d];
-          k++;
-        }
-      DBall cb;
-      if (!circumball_dev(pts, k, cb)) continue;
-      if (bestb.r2 >= 0 && cb.r2 >= bestb.r2) continue;
-      bool ok = true;
-      for (int s = 0; s < ns && ok; s++) {
-        if (mask & (1 << s)) continue;
-        double dd = 0;
-        for (int d = 0; d < 3; d++) dd += (sup[s][d] - cb.c[d]) * (sup[s][d] - cb.c[d]);
-        if (dd > cb.r2 * (1 + 1e-12)) ok = false;
+    // the <= 15 candidates (subsets of the support) one per lane; the valid
+    // one of smallest radius wins, ties to the lowest mask (the order a
+    // sequential scan over masks would keep)
+    DBall cb;
+    const bool ok = candidate_ball(lane, ns, bq, sup, cb);
+    double key = ok ? cb.r2 : INFINITY;
+    int bestmask = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double k2 = __shfl_xor(key, o, kWave);
+      const int m2 = __shfl_xor(bestmask, o, kWave);
+      if (k2 < key || (k2 == key && m2 < bestmask)) {
+        key = k2;
+        bestmask = m2;
       }
-      if (!ok) continue;
-      bestb = cb;
-      bestmask = mask;
     }
-    if (bestb.r2 < 0) break;
+    if (!(key < INFINITY)) break;  // numerically degenerate; keep the current ball
+    DBall bestb;
+    candidate_ball(bestmask, ns, bq, sup, bestb);
     double nsup[4][3];
-    int k = 0;
-    for (int d = 0; d < 3; d++) nsup[k][d] = bq[d];
-    k++;
-    for (int s = 0; s < ns; s++)
-      if (bestmask & (1 << s)) {
-        for (int d = 0; d < 3; d++) nsup[k][d] = sup[s][d];
-        k++;
-      }
-    for (int s = 0; s < 4; s++)
-      for (int d = 0; d < 3; d++) sup[s][d] = nsup[s][d];
+    const int k = gather_support(bestmask, bq, sup, nsup);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) sup[s2][d] = nsup[s2][d];
     ns = k;
     if (bestb.r2 <= b.r2) {
       b = bestb;

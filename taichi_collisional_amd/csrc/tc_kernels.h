@@ -108,6 +108,7 @@ int launch_make_chunks(const int *offs, const int *count, const int *chunk_offs,
 // ---- upward (tc_upward.hip)
 int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s);
 int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s);
+int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hipStream_t s);
 
 // ---- M2L (tc_m2l.hip)
 struct M2LArgs {
@@ -145,6 +146,7 @@ int launch_scatter_acc_old(BodiesDev b, int n, double *out, hipStream_t s);
 
 // ---- downward (tc_downward.hip)
 int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s);
+int launch_l2l_ids(int p, CellsDev c, const int *ids, int nids, hipStream_t s);
 int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids, int nleaves,
                hipStream_t s);
 int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,

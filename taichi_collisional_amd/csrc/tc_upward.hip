@@ -13,8 +13,8 @@
 //        design points on each grandchild sphere (points generated from LDS),
 //        then for each child
 //          M_parent[n,m] += sum_{k,l} M_child[n-k, m-l] Y_k^l(X_child - X_parent)
-//        with one thread per (output term, M or Mdot) and the child's
-//        harmonics and moments staged in LDS.
+//        with the (output term, M or Mdot) tasks strided over the block and
+//        the child's harmonics and moments staged in LDS.
 #include "tc_common.h"
 #include "tc_geom.h"
 #include "tc_harm.h"
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
     for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
     c.R[ci] = R;
     c.min_acc[ci] = mina;
-    c.has_sink[ci] = (unsigned char)sink;
+    c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
   }
   double *Mc = c.M + (size_t)ci * NT;
   double *Mdc = c.Md + (size_t)ci * NT;
@@ -120,10 +120,7 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
           double acc = 0;
           for (int r = 0; r < cnt; r++) acc += red[r][lane];
           double *dst = (pass == 0 ? Mc : Mdc) + (im ? s - m : s + m);
-          if (base == 0)
-            *dst = acc;
-          else
-            *dst += acc;
+          *dst += acc;
         }
         __syncthreads();
       }
@@ -138,7 +135,8 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, int ls) {
+__global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int *__restrict__ ids,
+                                             int ls) {
 #pragma clang fp contract(off)
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
@@ -152,7 +150,7 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, int ls) {
   __shared__ int s_i[2 * NW];
   __shared__ cplx G[NT], cm[NT], cmd[NT];
   const int tid = threadIdx.x;
-  const int ci = ls + blockIdx.x;
+  const int ci = ids ? ids[blockIdx.x] : ls + blockIdx.x;
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
@@ -251,15 +249,21 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, int ls) {
     for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
     c.R[ci] = sqrt(ball.r2) + 1e-6;
     c.min_acc[ci] = mina;
-    c.has_sink[ci] = (unsigned char)sink;
+    c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
   }
-  // translation: thread = (output term o, M or Mdot)
-  const int which = tid >> 7, o = tid & 127;
-  const bool act = o < NO;
-  int n = 0;
-  while (tri(n + 1, 0) <= o && n < P) n++;
-  const int m = o - tri(n, 0);
-  cplx a = {0, 0};
+  // translation: tasks (output term o, M or Mdot), strided over the block
+  constexpr int NTASK = 2 * NO, NR = (NTASK + 255) / 256;
+  cplx a[NR];
+  int tn[NR], tm[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    a[r] = {0, 0};
+    const int j = tid + 256 * r, o = j % NO;
+    int n = 0;
+    while (tri(n + 1, 0) <= o && n < P) n++;
+    tn[r] = n;
+    tm[r] = o - tri(n, 0);
+  }
   for (int k = 0; k < nch; k++) {
     const int ch = ch0 + k;
     __syncthreads();
@@ -274,20 +278,30 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, int ls) {
       cmd[t] = packed_to_c(Mdch, nn, mm);
     }
     __syncthreads();
-    if (act) {
-      const cplx *src = which ? cmd : cm;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int j = tid + 256 * r;
+      if (j >= NTASK) continue;
+      const cplx *src = j >= NO ? cmd : cm;
+      const int n = tn[r], m = tm[r];
+      cplx acc = a[r];
       for (int kk = 0; kk <= n; kk++) {
         const int lo = -kk > m - n + kk ? -kk : m - n + kk;
         const int hi = kk < m + n - kk ? kk : m + n - kk;
-        for (int l = lo; l <= hi; l++) a = cadd(a, cmul(src[ixnm(n - kk, m - l)], G[ixnm(kk, l)]));
+        for (int l = lo; l <= hi; l++)
+          acc = cadd(acc, cmul(src[ixnm(n - kk, m - l)], G[ixnm(kk, l)]));
       }
+      a[r] = acc;
     }
   }
-  if (act) {
-    double *dst = (which ? c.Md : c.M) + (size_t)ci * NT;
-    const int s = n * n + n;
-    dst[s + m] = a.re;
-    if (m > 0) dst[s - m] = a.im;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = tid + 256 * r;
+    if (j >= NTASK) continue;
+    double *dst = (j >= NO ? c.Md : c.M) + (size_t)ci * NT;
+    const int n = tn[r], m = tm[r], s = n * n + n;
+    dst[s + m] += a[r].re;
+    if (m > 0) dst[s - m] += a[r].im;
   }
 }
 
@@ -297,8 +311,8 @@ static void p2m_launch(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves
 }
 
 template <int P>
-static void m2m_launch(CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
-  hipLaunchKernelGGL(k_m2m<P>, dim3(le - ls), dim3(256), 0, s, c, b, ls);
+static void m2m_launch(CellsDev c, BodiesDev b, const int *ids, int ls, int ncell, hipStream_t s) {
+  hipLaunchKernelGGL(k_m2m<P>, dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
 }
 
 int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s) {
@@ -318,9 +332,23 @@ int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
 int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
   if (le <= ls) return 0;
   switch (p) {
-#define TC_CASE(P)                     \
-  case P:                              \
-    m2m_launch<P>(c, b, ls, le, s);    \
+#define TC_CASE(P)                                 \
+  case P:                                          \
+    m2m_launch<P>(c, b, nullptr, ls, le - ls, s);  \
+    return 0;
+    TC_FOR_EACH_P(TC_CASE)
+#undef TC_CASE
+    default:
+      return -1;
+  }
+}
+
+int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hipStream_t s) {
+  if (nids <= 0) return 0;
+  switch (p) {
+#define TC_CASE(P)                             \
+  case P:                                      \
+    m2m_launch<P>(c, b, ids, 0, nids, s);      \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE

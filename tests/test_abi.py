@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "tcfmm.h")
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(tc_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(tc_[A-Za-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -75,3 +75,15 @@ def test_compiled_orders_and_loud_failure_without_library(monkeypatch, tmp_path)
     monkeypatch.setattr(tca, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError):
         tca.lib()
+
+
+def test_cell_layout_matches_reference_cell():
+    """struct Cell (exafmm.h:50-71) at p = 12 on x86-64: 5592 bytes, M@80,
+    L@1432, Mdot@2784, Ldot@4136, Pn@5488."""
+    import taichi_collisional_amd as tca
+    dt = tca.cell_dtype(12)
+    assert dt.itemsize == 5592
+    off = {k: dt.fields[k][1] for k in dt.names}
+    assert (off["NCHILD"], off["NBODY"], off["CHILD"], off["BODY"]) == (0, 4, 8, 16)
+    assert (off["X"], off["R"], off["min_acc"], off["cell_mass"], off["has_sink"]) == (24, 48, 56, 64, 72)
+    assert (off["M"], off["L"], off["Mdot"], off["Ldot"], off["Pn"]) == (80, 1432, 2784, 4136, 5488)

@@ -59,7 +59,7 @@ def _check(a, b, what=""):
 
 @pytest.mark.parametrize("n,p,theta,ncrit", [
     (3000, 12, 0.5, 64), (20000, 4, 0.5, 64), (20000, 8, 0.4, 64), (65536, 10, 0.5, 64),
-    (30000, 6, 0.6, 16), (30000, 8, 0.5, 200),
+    (30000, 6, 0.6, 16), (30000, 8, 0.5, 200), (20000, 16, 0.5, 64), (20000, 14, 0.4, 64),
 ])
 def test_gpu_matches_oracle(torch_cuda, oracle_lib, n, p, theta, ncrit):
     from taichi_collisional_amd import ics
@@ -298,3 +298,90 @@ def test_gpu_error_mac_drop_in_sets_acc_old(torch_cuda):
     g.eval(dx, dv, dm)
     with pytest.raises(RuntimeError):
         g.acc_old()
+
+
+# ---------------------------------------------------------------------------
+# Per-kernel entry points (tc_P2M ... tc_P2P) on the reference's own Cell and
+# Body records: the oracle's tree and interaction lists are replayed kernel by
+# kernel through the C ABI and must reproduce the oracle's coefficients and
+# results.
+
+def _records(tca, cl, x, v, m, p):
+    order = cl["order"]
+    n = order.size
+    bodies = np.zeros(n, tca.body_dtype)
+    bodies["X"], bodies["V"], bodies["q"] = x[order], v[order], m[order]
+    bodies["issink"], bodies["issource"], bodies["index"] = 1, 1, order
+    nc = cl["level"].size
+    cells = np.zeros(nc, tca.cell_dtype(p))
+    cells["NCHILD"], cells["NBODY"] = cl["nchild"], cl["nbody"]
+    csz = np.uint64(cells.dtype.itemsize)
+    cells["CHILD"] = np.where(cl["nchild"] > 0,
+                              np.uint64(cells.ctypes.data) + cl["child"].astype(np.uint64) * csz, 0)
+    cells["BODY"] = np.uint64(bodies.ctypes.data) + cl["body"].astype(np.uint64) * np.uint64(152)
+    # buildTree's geometry; P2M / M2M replace it with the balls
+    cells["X"], cells["R"] = cl["Xg"], cl["Rg"]
+    return bodies, cells
+
+
+def test_gpu_per_kernel_entry_points_replay_oracle(torch_cuda, oracle_lib):
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    n, p = 20000, 8
+    x, v, m = ics.plummer(n, seed=31)
+    u = oracle_lib.shift_from_seed(5)
+    o = oracle_lib.Oracle(p, ncrit=64, theta=0.5, shift_frac=u)
+    op, oa, oj, _ = o.eval(x, v, m)
+    cl = o.cells()
+    nc = cl["level"].size
+    OM, OMd, OL, OLd = o.coefs(nc)
+    bodies, cells = _records(tca, cl, x, v, m, p)
+    f = tca.Fmm(p=p, ncrit=64, theta=0.5, direct_below=0)
+    leaves = np.where(cl["nchild"] == 0)[0]
+    internal = np.where(cl["nchild"] > 0)[0]
+    up = internal[np.argsort(-cl["level"][internal], kind="stable")]
+    down = internal[np.argsort(cl["level"][internal], kind="stable")]
+
+    def rel(a, b):
+        return np.abs(a - b).max() / np.abs(b).max()
+
+    f.P2M(bodies, cells, leaves)
+    f.M2M(bodies, cells, up)
+    assert np.abs(cells["X"] - cl["X"]).max() < 1e-12
+    assert np.abs(cells["R"] - cl["R"]).max() < 1e-12
+    assert rel(cells["M"], OM) < 1e-12 and rel(cells["Mdot"], OMd) < 1e-12
+    assert cells["has_sink"].all()
+    tm, sm = o.lists(0)
+    tp, sp = o.lists(1)
+    f.M2L(cells, tm, sm)
+    f.L2L(cells, down)
+    assert rel(cells["L"], OL) < 1e-11 and rel(cells["Ldot"], OLd) < 1e-11
+    f.L2P(bodies, cells, leaves)
+    f.P2P(bodies, cells, tp, sp)
+    order = cl["order"]
+    _check((bodies["p"], bodies["F"], bodies["J"]), (op[order], oa[order], oj[order]),
+           "per-kernel replay")
+
+
+def test_gpu_per_kernel_accumulates_and_orders(torch_cuda, oracle_lib):
+    """Entries run as sequential reference calls: calling twice doubles the sums;
+    a parent listed before its child in M2M sees the child's old moments."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    n, p = 12000, 4
+    x, v, m = ics.plummer(n, seed=32)
+    o = oracle_lib.Oracle(p, ncrit=64, theta=0.5, shift_frac=(0.0, 0.0, 0.0))
+    o.eval(x, v, m)
+    cl = o.cells()
+    bodies, cells = _records(tca, cl, x, v, m, p)
+    f = tca.Fmm(p=p, ncrit=64, theta=0.5, direct_below=0)
+    leaves = np.where(cl["nchild"] == 0)[0]
+    f.P2M(bodies, cells, leaves)
+    once = cells["M"][leaves].copy()
+    f.P2M(bodies, cells, np.concatenate([leaves, leaves]))
+    assert np.allclose(cells["M"][leaves], 3 * once, rtol=1e-13, atol=0)
+    # a bad pointer is rejected, nothing is written
+    bad = cells.copy()
+    bad["BODY"][leaves[0]] += np.uint64(8)
+    with pytest.raises(RuntimeError):
+        f.P2M(bodies, bad, leaves[:1])
