@@ -23,6 +23,17 @@ __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 __device__ __forceinline__ cplx cscale(double s, cplx a) { return {s * a.re, s * a.im}; }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
 
+// Work-group dispatch is round-robin over the 8 XCDs (linear block id mod 8),
+// and each XCD has its own L2.  Neighbouring work items (target cells close
+// in tree order) share most of their sources, so give every XCD one
+// contiguous range of items: a bijection of [0, n) onto itself.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous(int bid, int n) {
+  const int x = bid % kXcds, k = bid / kXcds;
+  const int per = n / kXcds, rem = n % kXcds;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 // Wave-wide sum of a double, result in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -19,8 +19,9 @@
 // M_p^{+-p} from the sum; with quirk=1 those inputs are zeroed.
 //
 // Work decomposition: a block = one wave = one chunk of <= 64 sources of a
-// single target cell (the chunk list is grouped by target); blockIdx.y picks
-// the array (0: M -> L, 1: Mdot -> Ldot).  Each order's 2n+1 contributions are
+// single target cell (the chunk list is grouped by target) for one array
+// (M -> L or Mdot -> Ldot); each XCD takes a contiguous range of the
+// (array, chunk) items so neighbouring targets share its L2.  Each order's 2n+1 contributions are
 // summed across the wave through an LDS transpose; the chunk adds its sum
 // into the target's L with one atomic per coefficient.
 #include "tc_common.h"
@@ -85,6 +86,10 @@ __device__ __forceinline__ void swap_block(const double *in, double *out) {
   }
 }
 
+// Row stride (doubles) of the LDS reduction buffer: 64 lanes + 16 padding,
+// so rows r and r+1 start 32 banks apart.
+constexpr int kRedStride = kWave + 16;
+
 template <int P>
 struct M2LRegs {
   static constexpr int q(int n) { return n < P - n ? n : P - n; }
@@ -103,29 +108,44 @@ __device__ __forceinline__ void load_order(const double *__restrict__ src, doubl
   for (int r = 0; r < 2 * N + 1; r++) v[r] = src[N * N + r];
 }
 
-// Forward half for order N of the source moments (already loaded into vin):
-// scale, rotate by alpha_z, swap, rotate by alpha_x, swap (rows |m| <=
-// min(N, p-N) only), unscale into the rotated-moment store.  The loads of
-// order N-1 are issued first and fenced with scheduling barriers, so one
-// order's loads are in flight while this order computes and the compiler
-// cannot hoist every order's loads to the top of the kernel.
+// Prefetch distance of the forward half, in orders: the source gathers of
+// orders N-1 .. N-kAhead are in flight while order N computes (one wave per
+// SIMD leaves registers to spare, and a 64-lane gather takes thousands of
+// cycles).
+#ifndef TC_M2L_AHEAD
+#define TC_M2L_AHEAD 1
+#endif
+constexpr int kAhead = TC_M2L_AHEAD;
+static_assert(kAhead >= 1 && kAhead <= 4, "prefetch distance");
+
+template <int N>
+struct OrderBuf {
+  static constexpr int size = N >= 0 ? 2 * N + 1 : 1;
+};
+
+// Forward half for order N of the source moments: scale, rotate by alpha_z,
+// swap, rotate by alpha_x, swap (rows |m| <= min(N, p-N) only), unscale into
+// the rotated-moment store.  b0 holds order N; b1..b3 hold orders N-1..N-3
+// when loaded (kAhead of them); the gather of order N-kAhead is issued first,
+// fenced with scheduling barriers so the compiler neither hoists every
+// order's gathers to the top nor sinks them to their use.
 template <int P, int N>
 __device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src, bool quirk,
                                                   double ezr, double ezi, double exr,
-                                                  double exi, double *vin, double *mp) {
+                                                  double exi, double *b0, double *b1,
+                                                  double *b2, double *b3, double *mp) {
   constexpr int K = 2 * N + 1;
   constexpr int Q = M2LRegs<P>::q(N);
   constexpr int O = M2LRegs<P>::off(N);
-  constexpr int KN = N > 0 ? 2 * N - 1 : 1;
-  double nxt[KN];
-  if constexpr (N > 0) {
+  double nb[OrderBuf<N - kAhead>::size];
+  if constexpr (N - kAhead >= 0) {
     __builtin_amdgcn_sched_barrier(0);
-    load_order<N - 1>(src, nxt);
+    load_order<N - kAhead>(src, nb);
     __builtin_amdgcn_sched_barrier(0);
   }
   double v[K], w[K];
 #pragma unroll
-  for (int r = 0; r < K; r++) v[r] = vin[r] * c_scale[N * N + r];
+  for (int r = 0; r < K; r++) v[r] = b0[r] * c_scale[N * N + r];
   if (N == P && quirk) {
     v[0] = 0.0;
     v[2 * N] = 0.0;
@@ -136,11 +156,21 @@ __device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src
   swap_block<N, Q, N>(w, v);
 #pragma unroll
   for (int m = -Q; m <= Q; m++) mp[O + Q + m] = v[N + m] * c_scale_inv[N * N + N + m];
-  if constexpr (N > 0) m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, nxt, mp);
+  if constexpr (N > 0) {
+    if constexpr (kAhead == 1)
+      m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, nb, nullptr, nullptr, nullptr,
+                                  mp);
+    else if constexpr (kAhead == 2)
+      m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, b1, nb, nullptr, nullptr, mp);
+    else if constexpr (kAhead == 3)
+      m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, b1, b2, nb, nullptr, mp);
+    else
+      m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, b1, b2, b3, nb, mp);
+  }
 }
 
 template <int P, int N>
-__device__ __forceinline__ void m2l_backward_order(const double *mp, const double *f,
+__device__ __forceinline__ void m2l_backward_order(const double *mp, double &rpn, double ri,
                                                    double ezr, double ezi, double exr,
                                                    double exi, bool active, double *acc_lds,
                                                    double *red, int lane) {
@@ -149,6 +179,18 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
   // one order at a time: keeps the scheduler from interleaving the
   // register-only work of several orders (which multiplies live state)
   __builtin_amdgcn_sched_barrier(0);
+  // translation factors f[N+k] = (N+k)! r^-(N+k+1), k = 0..P-N, from the
+  // power chain r^-(j+1) = r^-j * r^-1 (rpn = r^-(N+1) on entry)
+  double f[P - N + 1];
+  {
+    double rp = rpn;
+#pragma unroll
+    for (int k = 0; k <= P - N; k++) {
+      f[k] = c_fact[N + k] * rp;
+      rp *= ri;
+    }
+    rpn *= ri;
+  }
   double u[K], w[K];
 #pragma unroll
   for (int r = 0; r < K; r++) u[r] = 0.0;
@@ -158,7 +200,7 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
   for (int k = 0; k <= P - N; k++) {
     const int qk = M2LRegs<P>::q(k);
     const int ok = M2LRegs<P>::off(k);
-    u[N] += mp[ok + qk] * f[N + k];
+    u[N] += mp[ok + qk] * f[k];
   }
 #pragma unroll
   for (int m = 1; m <= Q; m++) {
@@ -166,7 +208,7 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
     for (int k = m; k <= P - N; k++) {
       const int qk = M2LRegs<P>::q(k);
       const int ok = M2LRegs<P>::off(k);
-      double fk = (m & 1) ? -f[N + k] : f[N + k];
+      double fk = (m & 1) ? -f[k] : f[k];
       u[N - m] += mp[ok + qk - m] * fk;
       u[N + m] += mp[ok + qk + m] * fk;
     }
@@ -176,10 +218,14 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
   swap_block<N, N, N>(w, u);
   rotate<N, true>(u, ezr, ezi);
   // Sum the K contributions over the wave: each lane writes its values to
-  // LDS rows, then groups of 8 lanes each sum one row (8 values per lane, a
-  // 3-step butterfly inside the group).
+  // LDS rows (row stride kRedStride), then groups of 8 lanes each sum one row
+  // (8 values per lane, a 3-step butterfly inside the group).  A group reads
+  // its row as four contiguous 128-byte runs and the padded stride puts
+  // neighbouring rows in opposite halves of the 64 banks, so the eight
+  // groups of one read fill the banks evenly (no conflicts beyond the 4
+  // passes a 16-byte-per-lane read needs).
 #pragma unroll
-  for (int r = 0; r < K; r++) red[r * kWave + lane] = u[r];
+  for (int r = 0; r < K; r++) red[r * kRedStride + lane] = u[r];
   __syncthreads();
   const int seg = lane & 7, grp = lane >> 3;
 #pragma unroll
@@ -187,10 +233,10 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
     const int r = g * 8 + grp;
     double sum = 0.0;
     if (r < K) {
-      const double2 *row = reinterpret_cast<const double2 *>(red + r * kWave + seg * 8);
+      const double2 *row = reinterpret_cast<const double2 *>(red + r * kRedStride);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        double2 t = row[q];
+        double2 t = row[seg + 8 * q];
         sum += t.x + t.y;
       }
     }
@@ -203,11 +249,11 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, const doubl
 }
 
 template <int P, int... Ns>
-__device__ __forceinline__ void m2l_backward_all(const double *mp, const double *f, double ezr,
-                                                 double ezi, double exr, double exi,
+__device__ __forceinline__ void m2l_backward_all(const double *mp, double rpn, double ri,
+                                                 double ezr, double ezi, double exr, double exi,
                                                  bool active, double *acc, double *red,
                                                  int lane, IntSeq<Ns...>) {
-  (m2l_backward_order<P, Ns>(mp, f, ezr, ezi, exr, exi, active, acc, red, lane), ...);
+  (m2l_backward_order<P, Ns>(mp, rpn, ri, ezr, ezi, exr, exi, active, acc, red, lane), ...);
 }
 
 template <int P>
@@ -221,13 +267,17 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
                                             int quirk) {
   constexpr int NT = nterm(P);
   __shared__ double acc[NT];
-  __shared__ __attribute__((aligned(16))) double red[(2 * P + 1) * kWave];
+  __shared__ __attribute__((aligned(16))) double red[(2 * P + 1) * kRedStride];
   const int lane = threadIdx.x;
-  const int4 ch = chunks[blockIdx.x];
+  // (chunk, array) from the linear block id, contiguous per XCD
+  const int nchunk = gridDim.x;
+  const int item = xcd_contiguous(blockIdx.y * nchunk + blockIdx.x, 2 * nchunk);
+  const int which = item >= nchunk;
+  const int4 ch = chunks[item - which * nchunk];
   const int tgt = ch.x, start = ch.y, cnt = ch.z;
   if (!has_sink[tgt]) return;
-  const double *Msrc = blockIdx.y ? Md : M;
-  double *Lout = blockIdx.y ? Ld : L;
+  const double *Msrc = which ? Md : M;
+  double *Lout = which ? Ld : L;
   const bool active = lane < cnt;
   const int j = src_list[start + (active ? lane : 0)];
 
@@ -250,25 +300,26 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
   const double *src = Msrc + (size_t)j * NT;
   // Orders run from p down to 0: the large orders (biggest working set) run
   // while few rotated moments are live.
-  double v0[2 * P + 1];
+  // orders P .. P-kAhead+1 are gathered up front
+  double v0[OrderBuf<P>::size], v1[OrderBuf<P - 1>::size], v2[OrderBuf<P - 2>::size],
+      v3[OrderBuf<P - 3>::size];
   load_order<P>(src, v0);
-  m2l_forward_order<P, P>(src, quirk != 0, ezr, ezi, exr, exi, v0, mp);
+  if constexpr (kAhead >= 2 && P - 1 >= 0) load_order<P - 1>(src, v1);
+  if constexpr (kAhead >= 3 && P - 2 >= 0) load_order<P - 2>(src, v2);
+  if constexpr (kAhead >= 4 && P - 3 >= 0) load_order<P - 3>(src, v3);
+  m2l_forward_order<P, P>(src, quirk != 0, ezr, ezi, exr, exi, v0, v1, v2, v3, mp);
   // Fence the rotated moments: without it, IR-level code motion interleaves
   // the forward and backward halves and roughly doubles the live registers
   // (p = 12: 512 registers plus scratch spills instead of ~310, no spills).
 #pragma unroll
   for (int t = 0; t < M2LRegs<P>::total; t++) asm volatile("" : "+v"(mp[t]));
   __builtin_amdgcn_sched_barrier(0);
-  // translation factors (n+k)! r^-(n+k+1), n+k <= p; zero on idle lanes,
-  // which makes their whole contribution zero
-  double f[P + 1];
-  double rp = active ? ri : 0.0;
-#pragma unroll
-  for (int k = 0; k <= P; k++) {
-    f[k] = c_fact[k] * rp;
-    rp *= ri;
-  }
-  m2l_backward_all<P>(mp, f, ezr, ezi, exr, exi, active, acc, red, lane, MakeIntSeq<P + 1>{});
+  // translation factors (n+k)! r^-(n+k+1) are formed per order from the power
+  // chain starting at r^-1; zero on idle lanes, which zeroes their whole
+  // contribution
+  const double rp0 = active ? ri : 0.0;
+  m2l_backward_all<P>(mp, rp0, ri, ezr, ezi, exr, exi, active, acc, red, lane,
+                      MakeIntSeq<P + 1>{});
   double *dst = Lout + (size_t)tgt * NT;
   for (int t = lane; t < NT; t += kWave) atomicAdd(dst + t, acc[t]);
 }

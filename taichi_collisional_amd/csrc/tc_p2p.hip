@@ -20,6 +20,17 @@ struct Acc7 {
   double p, ax, ay, az, jx, jy, jz;
 };
 
+// 1/sqrt(x) for x > 0 in FP64 without a branch: the hardware estimate
+// y0 = v_rsq_f64(x), then one second-order Newton-Householder step on
+// f(y) = 1/y^2 - x:  e = 1 - x y0^2,  y = y0 + y0 e (1/2 + 3/8 e), which
+// takes the estimate's relative error eps to ~(5/16) eps^3.  x = 0 gives a
+// non-finite value that the caller discards.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+  const double y0 = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y0, y0, 1.0);
+  return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
 __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double vxi, double vyi,
                                           double vzi, double xj, double yj, double zj, double vxj,
                                           double vyj, double vzj, double mj, Acc7 &a) {
@@ -27,7 +38,8 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
   double dvx = vxj - vxi, dvy = vyj - vyi, dvz = vzj - vzi;
   double r2 = dx * dx + dy * dy + dz * dz;
   double vr = 3.0 * (dx * dvx + dy * dvy + dz * dvz);
-  double ri = r2 > 0.0 ? rsqrt(r2) : 0.0;
+  const double rq = rsqrt_fp64(r2);
+  double ri = r2 > 0.0 ? rq : 0.0;
   double ri2 = ri * ri;
   double f = mj * ri;
   a.p += f;
@@ -44,7 +56,14 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
   a.jz += dz * f;
 }
 
-constexpr int kTile = 256;
+// Source bodies per LDS tile.  The final slice reduction reuses the tile's
+// memory, so LDS per wave is 64 * kTile bytes + 512 (occupancy is set by
+// LDS, not registers).
+#ifndef TC_P2P_TILE
+#define TC_P2P_TILE 128
+#endif
+constexpr int kTile = TC_P2P_TILE;
+static_assert(kTile >= 64 && 2 * kTile * 32 >= 7 * kWave * 8, "tile holds the reduction");
 
 // One block (one wave) per work item: a target group (<= 32 consecutive bodies
 // of one leaf) and a slice of at most kItemList entries of the leaf's P2P list
@@ -63,11 +82,14 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
                                             const int *__restrict__ count,
                                             const int2 *__restrict__ ranges,
                                             unsigned long long *__restrict__ npairs) {
-  __shared__ double4 sp[kTile], sv[kTile];
+  __shared__ double4 sbuf[2 * kTile];
   __shared__ int rstart[kWave], rexcl[kWave];
-  __shared__ double red[7][kWave];
+  double4 *sp = sbuf, *sv = sbuf + kTile;
+  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(sbuf);  // after the loop
   const int lane = threadIdx.x;
   // work item: target leaf, first target, list slice start, slice length | group size << 16
+  // round-robin dispatch spreads the consecutive, heavy items of one large
+  // list over all XCDs; a contiguous per-XCD range would pile them on one
   const int4 gw = groups[blockIdx.x];
   const int ci = gw.x;
   if (!c.has_sink[ci]) return;
