@@ -38,10 +38,19 @@ __constant__ double c_fact[2 * kMaxP + 2];
 
 // z-rotation of one order by e^{i m alpha} (forward) or e^{-i m alpha}
 // (backward), m = 1..N.  The powers come from the recurrence
-// e^{i(m+1)a} = e^{ima} e^{ia} (kernel_fast_lean.h:1163-1169), recomputed per
-// order so no power table has to stay live in registers.
+// e^{i(m+1)a} = e^{ima} e^{ia} (kernel_fast_lean.h:1163-1169).
+//
+// rotate(): the chain is recomputed inside the call from an opaque copy of
+// e^{ia}.  Without the copy the compiler merges the identical chains of all
+// unrolled orders into one table of 4p powers that stays live across the
+// kernel (p = 12: ~90 extra registers, one wave per SIMD instead of two).
+// The forward half uses it: its orders run p..0, so a table would be largest
+// exactly where the working set is.  The backward half runs 0..p and grows a
+// table by one power per order instead (rotate_tab), while the rotated
+// moments it consumes die off.
 template <int N, bool kBack>
 __device__ __forceinline__ void rotate(double *v, double c1, double s1) {
+  asm volatile("" : "+v"(c1), "+v"(s1));
   double c = c1, s = s1;
 #pragma unroll
   for (int m = 1; m <= N; m++) {
@@ -58,6 +67,36 @@ __device__ __forceinline__ void rotate(double *v, double c1, double s1) {
       s = c * s1 + s * c1;
       c = cn;
     }
+  }
+}
+
+template <int N, bool kBack>
+__device__ __forceinline__ void rotate_tab(double *v, const double *cp, const double *sp) {
+#pragma unroll
+  for (int m = 1; m <= N; m++) {
+    const double c = cp[m], s = sp[m];
+    double im = v[N - m], re = v[N + m];
+    if (kBack) {
+      v[N - m] = c * im - s * re;
+      v[N + m] = c * re + s * im;
+    } else {
+      v[N - m] = c * im + s * re;
+      v[N + m] = c * re - s * im;
+    }
+  }
+}
+
+// Powers e^{iNa} from e^{i(N-1)a} (same recurrence and operation order as
+// rotate()'s chain), stored at index N.
+template <int N>
+__device__ __forceinline__ void extend_powers(double *cp, double *sp, double c1, double s1) {
+  if constexpr (N == 1) {
+    cp[1] = c1;
+    sp[1] = s1;
+  } else if constexpr (N > 1) {
+    const double c = cp[N - 1], s = sp[N - 1];
+    cp[N] = c * c1 - s * s1;
+    sp[N] = c * s1 + s * c1;
   }
 }
 
@@ -89,6 +128,17 @@ __device__ __forceinline__ void swap_block(const double *in, double *out) {
 // Row stride (doubles) of the LDS reduction buffer: 64 lanes + 16 padding,
 // so rows r and r+1 start 32 banks apart.
 constexpr int kRedStride = kWave + 16;
+// One reduction buffer holds the rows of the largest order.  Orders up to 12
+// fit two waves per SIMD in registers, and a second buffer would then halve
+// occupancy through LDS; from 14 on, one wave per SIMD runs anyway and two
+// buffers alternate between consecutive orders so the reduction of one order
+// overlaps the next order's arithmetic.
+template <int P>
+struct RedBuf {
+  static constexpr int size = (2 * P + 1) * kRedStride;
+  static constexpr bool pipelined = P >= 14;
+  static constexpr int count = pipelined ? 2 : 1;
+};
 
 template <int P>
 struct M2LRegs {
@@ -169,11 +219,43 @@ __device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src
   }
 }
 
+// Sum order N's K = 2N+1 contributions over the wave: the lanes' values sit
+// in LDS rows (row stride kRedStride); groups of 8 lanes each sum one row
+// (8 values per lane, a 3-step butterfly inside the group).  A group reads
+// its row as four contiguous 128-byte runs and the padded stride puts
+// neighbouring rows in opposite halves of the 64 banks, so the eight groups of
+// one read fill the banks evenly.  Rows past K are read as row K-1 and
+// written to a per-lane spare slot, which keeps the loop free of divergent
+// branches.
+template <int P, int N>
+__device__ __forceinline__ void reduce_order(const double *red, double *acc_lds, int lane) {
+  constexpr int K = 2 * N + 1;
+  __syncthreads();
+  const int seg = lane & 7, grp = lane >> 3;
+#pragma unroll
+  for (int g = 0; g < (K + 7) / 8; g++) {
+    const int r = g * 8 + grp;
+    const int rr = r < K ? r : K - 1;
+    double sum = 0.0;
+    const double2 *row = reinterpret_cast<const double2 *>(red + rr * kRedStride);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      double2 t = row[seg + 8 * q];
+      sum += t.x + t.y;
+    }
+    sum += __shfl_xor(sum, 1, kWave);
+    sum += __shfl_xor(sum, 2, kWave);
+    sum += __shfl_xor(sum, 4, kWave);
+    acc_lds[(seg == 0 && r < K) ? N * N + r : nterm(P) + lane] = sum;
+  }
+}
+
 template <int P, int N>
 __device__ __forceinline__ void m2l_backward_order(const double *mp, double &rpn, double ri,
                                                    double ezr, double ezi, double exr,
-                                                   double exi, bool active, double *acc_lds,
-                                                   double *red, int lane) {
+                                                   double exi, double *czp, double *szp,
+                                                   double *cxp, double *sxp, bool active,
+                                                   double *acc_lds, double *red, int lane) {
   constexpr int K = 2 * N + 1;
   constexpr int Q = M2LRegs<P>::q(N);
   // one order at a time: keeps the scheduler from interleaving the
@@ -213,39 +295,28 @@ __device__ __forceinline__ void m2l_backward_order(const double *mp, double &rpn
       u[N + m] += mp[ok + qk + m] * fk;
     }
   }
+  extend_powers<N>(cxp, sxp, exr, exi);
+  extend_powers<N>(czp, szp, ezr, ezi);
   swap_block<N, N, Q>(u, w);
-  rotate<N, true>(w, exr, exi);
+  rotate_tab<N, true>(w, cxp, sxp);
   swap_block<N, N, N>(w, u);
-  rotate<N, true>(u, ezr, ezi);
-  // Sum the K contributions over the wave: each lane writes its values to
-  // LDS rows (row stride kRedStride), then groups of 8 lanes each sum one row
-  // (8 values per lane, a 3-step butterfly inside the group).  A group reads
-  // its row as four contiguous 128-byte runs and the padded stride puts
-  // neighbouring rows in opposite halves of the 64 banks, so the eight
-  // groups of one read fill the banks evenly (no conflicts beyond the 4
-  // passes a 16-byte-per-lane read needs).
+  rotate_tab<N, true>(u, czp, szp);
+  if constexpr (RedBuf<P>::pipelined) {
+    // The previous order's rows were written one order ago: reduce them now,
+    // so the barrier finds those writes long complete; then store this
+    // order's rows in the other buffer (reduced after the next order, or at
+    // the end).
+    if constexpr (N > 0)
+      reduce_order<P, N - 1>(red + ((N - 1) & 1) * RedBuf<P>::size, acc_lds, lane);
 #pragma unroll
-  for (int r = 0; r < K; r++) red[r * kRedStride + lane] = u[r];
-  __syncthreads();
-  const int seg = lane & 7, grp = lane >> 3;
+    for (int r = 0; r < K; r++) red[(N & 1) * RedBuf<P>::size + r * kRedStride + lane] = u[r];
+    if constexpr (N == P) reduce_order<P, N>(red + (N & 1) * RedBuf<P>::size, acc_lds, lane);
+  } else {
 #pragma unroll
-  for (int g = 0; g < (K + 7) / 8; g++) {
-    const int r = g * 8 + grp;
-    double sum = 0.0;
-    if (r < K) {
-      const double2 *row = reinterpret_cast<const double2 *>(red + r * kRedStride);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        double2 t = row[seg + 8 * q];
-        sum += t.x + t.y;
-      }
-    }
-    sum += __shfl_xor(sum, 1, kWave);
-    sum += __shfl_xor(sum, 2, kWave);
-    sum += __shfl_xor(sum, 4, kWave);
-    if (seg == 0 && r < K) acc_lds[N * N + r] = sum;
+    for (int r = 0; r < K; r++) red[r * kRedStride + lane] = u[r];
+    reduce_order<P, N>(red, acc_lds, lane);
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 template <int P, int... Ns>
@@ -253,7 +324,10 @@ __device__ __forceinline__ void m2l_backward_all(const double *mp, double rpn, d
                                                  double ezr, double ezi, double exr, double exi,
                                                  bool active, double *acc, double *red,
                                                  int lane, IntSeq<Ns...>) {
-  (m2l_backward_order<P, Ns>(mp, rpn, ri, ezr, ezi, exr, exi, active, acc, red, lane), ...);
+  double czp[P + 1], szp[P + 1], cxp[P + 1], sxp[P + 1];
+  (m2l_backward_order<P, Ns>(mp, rpn, ri, ezr, ezi, exr, exi, czp, szp, cxp, sxp, active, acc,
+                             red, lane),
+   ...);
 }
 
 template <int P>
@@ -266,8 +340,8 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
                                             double *__restrict__ L, double *__restrict__ Ld,
                                             int quirk) {
   constexpr int NT = nterm(P);
-  __shared__ double acc[NT];
-  __shared__ __attribute__((aligned(16))) double red[(2 * P + 1) * kRedStride];
+  __shared__ double acc[NT + kWave];  // + one spare slot per lane
+  __shared__ __attribute__((aligned(16))) double red[RedBuf<P>::count * RedBuf<P>::size];
   const int lane = threadIdx.x;
   // (chunk, array) from the linear block id, contiguous per XCD
   const int nchunk = gridDim.x;
@@ -320,6 +394,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
   const double rp0 = active ? ri : 0.0;
   m2l_backward_all<P>(mp, rp0, ri, ezr, ezi, exr, exi, active, acc, red, lane,
                       MakeIntSeq<P + 1>{});
+  __syncthreads();
   double *dst = Lout + (size_t)tgt * NT;
   for (int t = lane; t < NT; t += kWave) atomicAdd(dst + t, acc[t]);
 }

@@ -58,17 +58,27 @@ __global__ __launch_bounds__(256) void k_bounds_partial(const double *__restrict
     }
 }
 
-__global__ void k_bounds_final(const double *__restrict__ part, int nb,
-                               const double *__restrict__ shift_frac, TreeParams *tp) {
+// One wave: lane-strided min / max over the block partials, then a wave
+// reduction (min and max do not depend on the order); lane 0 finishes.
+__global__ __launch_bounds__(64) void k_bounds_final(const double *__restrict__ part, int nb,
+                                                     const double *__restrict__ shift_frac,
+                                                     TreeParams *tp) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
+  const int lane = threadIdx.x;
   double mn[3], mx[3];
   for (int d = 0; d < 3; d++) mn[d] = part[d], mx[d] = part[3 + d];
-  for (int b = 1; b < nb; b++)
+  for (int b = lane; b < nb; b += kWave)
     for (int d = 0; d < 3; d++) {
       mn[d] = fmin(part[6 * b + d], mn[d]);
       mx[d] = fmax(part[6 * b + 3 + d], mx[d]);
     }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = fmin(mn[d], __shfl_xor(mn[d], o, kWave));
+      mx[d] = fmax(mx[d], __shfl_xor(mx[d], o, kWave));
+    }
+  if (lane != 0) return;
   double X0[3];
   for (int d = 0; d < 3; d++) X0[d] = (mx[d] + mn[d]) / 2;
   double box = -1;
