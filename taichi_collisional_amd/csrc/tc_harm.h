@@ -93,6 +93,44 @@ __device__ __forceinline__ cplx harm_dot_row(const cplx *y1, int n, int m, doubl
   return r;
 }
 
+// Row N (compile-time) of the solid harmonics from rows N-1 (y1) and N-2 (y2):
+// the recurrences of harm_all, unrolled with no case tests.  yn[m] for m > N
+// is left unset.
+template <int N>
+__device__ __forceinline__ void harm_row_c(double dx, double dy, double dz, double r2,
+                                           const cplx *y1, const cplx *y2, cplx *yn) {
+  if constexpr (N == 0) {
+    yn[0] = {1.0, 0.0};
+  } else {
+#pragma unroll
+    for (int m = 0; m < N - 1; m++) {
+      const double q = (double)(N * N - m * m);
+      const double a = (2 * N - 1.0) / q * dz, b = r2 / q;
+      yn[m] = {a * y1[m].re - b * y2[m].re, a * y1[m].im - b * y2[m].im};
+    }
+    yn[N - 1] = cscale(dz, y1[N - 1]);
+    yn[N] = cmul(y1[N - 1], {dx / (2 * N), dy / (2 * N)});
+  }
+}
+
+// dY_N^m/dt (compile-time N, m) from row N-1 (y1): harm_dot.
+template <int N, int m>
+__device__ __forceinline__ cplx harm_dot_c(const cplx *y1, double vx, double vy, double vz) {
+  cplx r = {0.0, 0.0};
+  if constexpr (N > 0) {
+    const cplx eta = {-vx * 0.5, vy * 0.5}, xi = {vx * 0.5, vy * 0.5};
+    if constexpr (m <= N - 2) r = cadd(r, cmul(y1[m + 1], eta));
+    if constexpr (m <= N - 1) r = cadd(r, cscale(vz, y1[m]));
+    if constexpr (m >= 1) {
+      r = cadd(r, cmul(y1[m - 1], xi));
+    } else if constexpr (N >= 2) {
+      const cplx c1 = cconj(y1[1]);
+      r = cadd(r, cmul({-c1.re, -c1.im}, xi));
+    }
+  }
+  return r;
+}
+
 // Y_k^l for any l (|l| <= k) from the m >= 0 table.
 __device__ __forceinline__ cplx harm_at(const cplx *y, int k, int l) {
   if (l >= 0) return y[tri(k, l)];

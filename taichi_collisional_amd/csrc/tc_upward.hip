@@ -4,7 +4,8 @@
 //
 //   P2M  one wave per leaf: the leaf's bodies are staged in LDS, their ball
 //        (centre X, radius r + 1e-6) is found, then lane = body walks the
-//        orders n = 0..P with rolling harmonic rows; per order the terms
+//        orders n = 0..P (compile-time, unrolled) with rolling harmonic rows;
+//        per order the terms
 //        q Y_n^m(x_b - X) and q dY_n^m/dt(v_b) are summed over the bodies
 //        through an LDS transpose (lane j sums column j).  min_acc and
 //        has_sink are reduced alongside.
@@ -23,18 +24,71 @@
 
 namespace tcfmm {
 
+// Orders N..P of P2M for one lane's body (w = its mass, 0 when idle): row N of
+// the harmonics from rows N-1, N-2 (compile-time N, so no case tests), then
+// q Y_N^m and q dY_N^m/dt summed over the bodies through the LDS transpose
+// (lane j sums column j; cnt bodies).
+template <int P, int N>
+__device__ __forceinline__ void p2m_orders(double dx, double dy, double dz, double r2, double vx,
+                                           double vy, double vz, double w, cplx *y1, cplx *y2,
+                                           double (*red)[2 * (P + 1) + 1], int lane, int cnt,
+                                           double *Mc, double *Mdc) {
+  cplx yn[P + 1];
+  harm_row_c<N>(dx, dy, dz, r2, y1, y2, yn);
+  constexpr int s = N * N + N;
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+    for (int m = 0; m <= N; m++) {
+      cplx v;
+      if (pass == 0) {
+        v = yn[m];
+      } else {
+        v = {0.0, 0.0};
+        if constexpr (N > 0) {
+          const cplx eta = {-vx * 0.5, vy * 0.5}, xi = {vx * 0.5, vy * 0.5};
+          if (m <= N - 2) v = cadd(v, cmul(y1[m + 1 <= P ? m + 1 : P], eta));
+          if (m <= N - 1) v = cadd(v, cscale(vz, y1[m]));
+          if (m >= 1) {
+            v = cadd(v, cmul(y1[m >= 1 ? m - 1 : 0], xi));
+          } else if (N >= 2) {
+            const cplx c1 = cconj(y1[1]);
+            v = cadd(v, cmul({-c1.re, -c1.im}, xi));
+          }
+        }
+      }
+      red[lane][2 * m] = w * v.re;
+      red[lane][2 * m + 1] = w * v.im;
+    }
+    __syncthreads();
+    const int m = lane >> 1, im = lane & 1;
+    if (lane < 2 * (N + 1) && !(m == 0 && im)) {
+      double acc = 0;
+      for (int r = 0; r < cnt; r++) acc += red[r][lane];
+      double *dst = (pass == 0 ? Mc : Mdc) + (im ? s - m : s + m);
+      *dst += acc;
+    }
+    __syncthreads();
+  }
+  if constexpr (N < P) {
+#pragma unroll
+    for (int m = 0; m <= N; m++) {
+      y2[m] = y1[m];
+      y1[m] = yn[m];
+    }
+    p2m_orders<P, N + 1>(dx, dy, dz, r2, vx, vy, vz, w, y1, y2, red, lane, cnt, Mc, Mdc);
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
                                             const int *__restrict__ leaf_ids) {
   constexpr int NT = nterm(P);
-  constexpr int NQ = (P + 1) * (P + 1);
   constexpr int NR = 2 * (P + 1);  // real, imaginary part for m = 0..P
   constexpr int kStage = 128;
   __shared__ double s_q[3][kStage];
-  __shared__ double s_ra[NQ], s_rb[NQ];
   __shared__ double red[kWave][NR + 1];
   const int lane = threadIdx.x;
-  harm_coef_lds<P>(s_ra, s_rb);
   const int ci = leaf_ids[blockIdx.x];
   const int b0 = c.body[ci], nb = c.nbody[ci];
   const bool staged = nb <= kStage;
@@ -98,38 +152,7 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
     }
     const double r2 = dx * dx + dy * dy + dz * dz;
     cplx y1[P + 1], y2[P + 1];
-#pragma unroll
-    for (int m = 0; m <= P; m++) y1[m] = y2[m] = {0.0, 0.0};
-#pragma unroll 1
-    for (int n = 0; n <= P; n++) {
-      cplx yn[P + 1];
-      harm_row<P>(n, dx, dy, dz, r2, s_ra, s_rb, y1, y2, yn);
-      const int s = n * n + n;
-#pragma unroll
-      for (int pass = 0; pass < 2; pass++) {
-#pragma unroll
-        for (int m = 0; m <= P; m++)
-          if (m <= n) {
-            const cplx v = pass == 0 ? yn[m] : harm_dot_row<P>(y1, n, m, vx, vy, vz);
-            red[lane][2 * m] = w * v.re;
-            red[lane][2 * m + 1] = w * v.im;
-          }
-        __syncthreads();
-        const int m = lane >> 1, im = lane & 1;
-        if (lane < 2 * (n + 1) && !(m == 0 && im)) {
-          double acc = 0;
-          for (int r = 0; r < cnt; r++) acc += red[r][lane];
-          double *dst = (pass == 0 ? Mc : Mdc) + (im ? s - m : s + m);
-          *dst += acc;
-        }
-        __syncthreads();
-      }
-#pragma unroll
-      for (int m = 0; m <= P; m++) {
-        y2[m] = y1[m];
-        y1[m] = yn[m];
-      }
-    }
+    p2m_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, w, y1, y2, red, lane, cnt, Mc, Mdc);
     if (nb == 0) break;
   }
 }
