@@ -68,17 +68,78 @@ __global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__
   }
 }
 
+// Orders K..P of L2P for one body: row K of Y(X_c - x_b) (compile-time K, no
+// case tests) and its time derivative, contracted with L / Ldot of orders K
+// and K+1 for the n <= 1 part of the local field; l = +m and l = -m use
+// Y_k^{-m} = (-1)^m conj(Y_k^m).  A scheduling barrier per order keeps the
+// later orders' coefficient reads from being hoisted (which multiplies the
+// live registers).
+template <int P, int K>
+__device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, double r2, double vx,
+                                           double vy, double vz, const cplx *cl, const cplx *cld,
+                                           cplx *y1, cplx *y2, cplx &p00, cplx &p10, cplx &p11,
+                                           cplx &d10, cplx &d11) {
+  __builtin_amdgcn_sched_barrier(0);
+  cplx yk[P + 1];
+  harm_row_c<K>(dx, dy, dz, r2, y1, y2, yk);
+  const cplx *L0 = cl + K * K + K;
+  const cplx *L1 = cl + (K + 1) * (K + 1) + (K + 1);
+  const cplx *D1 = cld + (K + 1) * (K + 1) + (K + 1);
+  constexpr bool up = K < P;
+#pragma unroll
+  for (int m = 0; m <= K; m++) {
+    cplx dy_m = {0.0, 0.0};
+    if constexpr (K > 0) {
+      const cplx eta = {-vx * 0.5, vy * 0.5}, xi = {vx * 0.5, vy * 0.5};
+      if (m <= K - 2) dy_m = cadd(dy_m, cmul(y1[m + 1 <= P ? m + 1 : P], eta));
+      if (m <= K - 1) dy_m = cadd(dy_m, cscale(vz, y1[m]));
+      if (m >= 1) {
+        dy_m = cadd(dy_m, cmul(y1[m >= 1 ? m - 1 : 0], xi));
+      } else if (K >= 2) {
+        const cplx c1 = cconj(y1[1]);
+        dy_m = cadd(dy_m, cmul({-c1.re, -c1.im}, xi));
+      }
+    }
+    {
+      const cplx g = cconj(yk[m]), gd = cconj(dy_m);
+      p00 = cadd(p00, cmul(L0[m], g));
+      if (up) {
+        p10 = cadd(p10, cmul(L1[m], g));
+        p11 = cadd(p11, cmul(L1[m + 1], g));
+        d10 = cadd(d10, cadd(cmul(L1[m], gd), cmul(D1[m], g)));
+        d11 = cadd(d11, cadd(cmul(L1[m + 1], gd), cmul(D1[m + 1], g)));
+      }
+    }
+    if (m > 0) {
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      const cplx g = cscale(sg, yk[m]), gd = cscale(sg, dy_m);
+      p00 = cadd(p00, cmul(L0[-m], g));
+      if (up) {
+        p10 = cadd(p10, cmul(L1[-m], g));
+        p11 = cadd(p11, cmul(L1[1 - m], g));
+        d10 = cadd(d10, cadd(cmul(L1[-m], gd), cmul(D1[-m], g)));
+        d11 = cadd(d11, cadd(cmul(L1[1 - m], gd), cmul(D1[1 - m], g)));
+      }
+    }
+  }
+  if constexpr (K < P) {
+#pragma unroll
+    for (int m = 0; m <= K; m++) {
+      y2[m] = y1[m];
+      y1[m] = yk[m];
+    }
+    l2p_orders<P, K + 1>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, p00, p10, p11, d10, d11);
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev r,
                                             const int *__restrict__ leaf_ids) {
   constexpr int NT = nterm(P);
-  constexpr int NQ = (P + 1) * (P + 1);
   __shared__ cplx cl[NT], cld[NT];
-  __shared__ double s_ra[NQ], s_rb[NQ];
   const int lane = threadIdx.x;
   const int ci = leaf_ids[blockIdx.x];
   if (!c.has_sink[ci]) return;
-  harm_coef_lds<P>(s_ra, s_rb);
   const int b0 = c.body[ci], nb = c.nbody[ci];
   const double *Lp = c.L + (size_t)ci * NT;
   const double *Ldp = c.Ld + (size_t)ci * NT;
@@ -99,51 +160,9 @@ __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev 
     const double dx = X0 - b.x[i], dy = X1 - b.y[i], dz = X2 - b.z[i];
     const double r2 = dx * dx + dy * dy + dz * dz;
     cplx y1[P + 1], y2[P + 1];
-#pragma unroll
-    for (int m = 0; m <= P; m++) y1[m] = y2[m] = {0.0, 0.0};
     cplx p00 = {0, 0}, p10 = {0, 0}, p11 = {0, 0};
     cplx d10 = {0, 0}, d11 = {0, 0};
-#pragma unroll 1
-    for (int k = 0; k <= P; k++) {
-      cplx yk[P + 1];
-      harm_row<P>(k, dx, dy, dz, r2, s_ra, s_rb, y1, y2, yk);
-      const cplx *L0 = cl + k * k + k, *L1 = cl + (k + 1) * (k + 1) + (k + 1);
-      const cplx *D1 = cld + (k + 1) * (k + 1) + (k + 1);
-      const bool up = k < P;
-#pragma unroll
-      for (int m = 0; m <= P; m++) {
-        if (m > k) continue;
-        const cplx dy_m = harm_dot_row<P>(y1, k, m, vx, vy, vz);
-        // l = +m: conj Y_k^m, conj dY_k^m
-        {
-          const cplx g = cconj(yk[m]), gd = cconj(dy_m);
-          p00 = cadd(p00, cmul(L0[m], g));
-          if (up) {
-            p10 = cadd(p10, cmul(L1[m], g));
-            p11 = cadd(p11, cmul(L1[m + 1], g));
-            d10 = cadd(d10, cadd(cmul(L1[m], gd), cmul(D1[m], g)));
-            d11 = cadd(d11, cadd(cmul(L1[m + 1], gd), cmul(D1[m + 1], g)));
-          }
-        }
-        // l = -m: conj Y_k^{-m} = (-1)^m Y_k^m, likewise for dY
-        if (m > 0) {
-          const double sg = (m & 1) ? -1.0 : 1.0;
-          const cplx g = cscale(sg, yk[m]), gd = cscale(sg, dy_m);
-          p00 = cadd(p00, cmul(L0[-m], g));
-          if (up) {
-            p10 = cadd(p10, cmul(L1[-m], g));
-            p11 = cadd(p11, cmul(L1[1 - m], g));
-            d10 = cadd(d10, cadd(cmul(L1[-m], gd), cmul(D1[-m], g)));
-            d11 = cadd(d11, cadd(cmul(L1[1 - m], gd), cmul(D1[1 - m], g)));
-          }
-        }
-      }
-#pragma unroll
-      for (int m = 0; m <= P; m++) {
-        y2[m] = y1[m];
-        y1[m] = yk[m];
-      }
-    }
+    l2p_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, p00, p10, p11, d10, d11);
     r.pot[i] += p00.re;
     r.ax[i] -= p11.re;
     r.ay[i] -= p11.im;
