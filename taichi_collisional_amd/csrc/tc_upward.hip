@@ -14,8 +14,9 @@
 //        design points on each grandchild sphere (points generated from LDS),
 //        then for each child
 //          M_parent[n,m] += sum_{k,l} M_child[n-k, m-l] Y_k^l(X_child - X_parent)
-//        with the (output term, M or Mdot) tasks strided over the block and
-//        the child's harmonics and moments staged in LDS.
+//        with the eight children's harmonics and moments in LDS and lanes
+//        over (output term, child), the children's sums added across lanes
+//        (M first, then Mdot); one child at a time when that does not fit.
 #include "tc_common.h"
 #include "tc_geom.h"
 #include "tc_harm.h"
@@ -165,13 +166,18 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
   constexpr int NO = tri(P, P) + 1;
   constexpr int NW = 4;
   constexpr int kStage = 512;  // body points of leaf children staged in LDS
-  __shared__ double s_td[192];
-  __shared__ double s_g[64][4];  // grandchild ball (X, R), children in order
-  __shared__ double s_p[3][kStage];
+  // Eight children's harmonic tables and moments (complex) side by side, when
+  // they fit one work-group's LDS; else one child at a time.
+  constexpr bool kPar = 16 * 16 * NT <= 60 * 1024;
+  constexpr int kBall = 192 + 64 * 4 + 3 * kStage;       // doubles, ball phase
+  constexpr int kTrans = kPar ? 2 * 8 * 2 * NT : 3 * 2 * NT;  // doubles, translation
+  __shared__ __attribute__((aligned(16))) double smem[kBall > kTrans ? kBall : kTrans];
+  double *s_td = smem;                                            // t-design
+  double(*s_g)[4] = reinterpret_cast<double(*)[4]>(smem + 192);  // grandchild (X, R)
+  double(*s_p)[kStage] = reinterpret_cast<double(*)[kStage]>(smem + 192 + 256);
   __shared__ int s_pre[9], s_base[8], s_leaf[8], s_body[8], s_nstage;
   __shared__ double s_e[2 * NW];
   __shared__ int s_i[2 * NW];
-  __shared__ cplx G[NT], cm[NT], cmd[NT];
   const int tid = threadIdx.x;
   const int ci = ids ? ids[blockIdx.x] : ls + blockIdx.x;
   const int nch = c.nchild[ci];
@@ -274,57 +280,112 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
     c.min_acc[ci] = mina;
     c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
   }
-  // translation: tasks (output term o, M or Mdot), strided over the block
-  constexpr int NTASK = 2 * NO, NR = (NTASK + 255) / 256;
-  cplx a[NR];
-  int tn[NR], tm[NR];
+  __syncthreads();  // the ball's buffers are reused below
+  if constexpr (kPar) {
+    // translation, children in parallel: lane = (task group g, child k) with
+    // k fastest, each lane a strided share of the output terms of child k;
+    // the 8 children's partial sums of one term sit in 8 neighbouring lanes
+    // and are added with a 3-step butterfly
+    cplx *Gall = reinterpret_cast<cplx *>(smem), *Call = Gall + 8 * NT;
+    const int k = tid & 7, g = tid >> 3;
+    const bool haveK = k < nch;
+    const int ch = ch0 + (haveK ? k : 0);
+    if (haveK)
+      harm_table_col<P>(g, c.X[3 * ch] - X[0], c.X[3 * ch + 1] - X[1], c.X[3 * ch + 2] - X[2],
+                        Gall + k * NT);
+    for (int pass = 0; pass < 2; pass++) {
+      const double *Msrc = pass ? c.Md : c.M;
+      for (int t = tid; t < 8 * NT; t += blockDim.x) {
+        const int kc = t / NT, idx = t - kc * NT;
+        if (kc < nch) {
+          int nn = 0;
+          while ((nn + 1) * (nn + 1) <= idx) nn++;
+          Call[t] = packed_to_c(Msrc + (size_t)(ch0 + kc) * NT, nn, idx - nn * nn - nn);
+        }
+      }
+      __syncthreads();
+      double *dst = (pass ? c.Md : c.M) + (size_t)ci * NT;
+      const cplx *src = Call + k * NT, *Gk = Gall + k * NT;
+      for (int o = g; o < NO; o += 32) {
+        int n = 0;
+        while (tri(n + 1, 0) <= o && n < P) n++;
+        const int m = o - tri(n, 0);
+        cplx acc = {0, 0};
+        if (haveK)
+          for (int kk = 0; kk <= n; kk++) {
+            const int lo = -kk > m - n + kk ? -kk : m - n + kk;
+            const int hi = kk < m + n - kk ? kk : m + n - kk;
+            for (int l = lo; l <= hi; l++)
+              acc = cadd(acc, cmul(src[ixnm(n - kk, m - l)], Gk[ixnm(kk, l)]));
+          }
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    a[r] = {0, 0};
-    const int j = tid + 256 * r, o = j % NO;
-    int n = 0;
-    while (tri(n + 1, 0) <= o && n < P) n++;
-    tn[r] = n;
-    tm[r] = o - tri(n, 0);
-  }
-  for (int k = 0; k < nch; k++) {
-    const int ch = ch0 + k;
-    __syncthreads();
-    harm_table_lds<P>(c.X[3 * ch] - X[0], c.X[3 * ch + 1] - X[1], c.X[3 * ch + 2] - X[2], G);
-    const double *Mch = c.M + (size_t)ch * NT;
-    const double *Mdch = c.Md + (size_t)ch * NT;
-    for (int t = tid; t < NT; t += blockDim.x) {
-      int nn = 0;
-      while ((nn + 1) * (nn + 1) <= t) nn++;
-      const int mm = t - nn * nn - nn;
-      cm[t] = packed_to_c(Mch, nn, mm);
-      cmd[t] = packed_to_c(Mdch, nn, mm);
+        for (int x = 1; x < 8; x <<= 1) {
+          acc.re += __shfl_xor(acc.re, x, kWave);
+          acc.im += __shfl_xor(acc.im, x, kWave);
+        }
+        if (k == 0) {
+          const int s = n * n + n;
+          dst[s + m] += acc.re;
+          if (m > 0) dst[s - m] += acc.im;
+        }
+      }
+      __syncthreads();  // Call is refilled by the next pass
     }
-    __syncthreads();
+  } else {
+    // translation, one child at a time: tasks (output term, M or Mdot)
+    // strided over the block, the child's harmonics and moments in LDS
+    cplx *G = reinterpret_cast<cplx *>(smem), *cm = G + NT, *cmd = cm + NT;
+    constexpr int NTASK = 2 * NO, NRT = (NTASK + 255) / 256;
+    cplx a[NRT];
+    int tn[NRT], tm[NRT];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
+    for (int r = 0; r < NRT; r++) {
+      a[r] = {0, 0};
+      const int j = tid + 256 * r, o = j % NO;
+      int n = 0;
+      while (tri(n + 1, 0) <= o && n < P) n++;
+      tn[r] = n;
+      tm[r] = o - tri(n, 0);
+    }
+    for (int kc = 0; kc < nch; kc++) {
+      const int ch = ch0 + kc;
+      __syncthreads();
+      harm_table_lds<P>(c.X[3 * ch] - X[0], c.X[3 * ch + 1] - X[1], c.X[3 * ch + 2] - X[2], G);
+      const double *Mch = c.M + (size_t)ch * NT;
+      const double *Mdch = c.Md + (size_t)ch * NT;
+      for (int t = tid; t < NT; t += blockDim.x) {
+        int nn = 0;
+        while ((nn + 1) * (nn + 1) <= t) nn++;
+        const int mm = t - nn * nn - nn;
+        cm[t] = packed_to_c(Mch, nn, mm);
+        cmd[t] = packed_to_c(Mdch, nn, mm);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < NRT; r++) {
+        const int j = tid + 256 * r;
+        if (j >= NTASK) continue;
+        const cplx *src = j >= NO ? cmd : cm;
+        const int n = tn[r], m = tm[r];
+        cplx acc = a[r];
+        for (int kk = 0; kk <= n; kk++) {
+          const int lo = -kk > m - n + kk ? -kk : m - n + kk;
+          const int hi = kk < m + n - kk ? kk : m + n - kk;
+          for (int l = lo; l <= hi; l++)
+            acc = cadd(acc, cmul(src[ixnm(n - kk, m - l)], G[ixnm(kk, l)]));
+        }
+        a[r] = acc;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NRT; r++) {
       const int j = tid + 256 * r;
       if (j >= NTASK) continue;
-      const cplx *src = j >= NO ? cmd : cm;
-      const int n = tn[r], m = tm[r];
-      cplx acc = a[r];
-      for (int kk = 0; kk <= n; kk++) {
-        const int lo = -kk > m - n + kk ? -kk : m - n + kk;
-        const int hi = kk < m + n - kk ? kk : m + n - kk;
-        for (int l = lo; l <= hi; l++)
-          acc = cadd(acc, cmul(src[ixnm(n - kk, m - l)], G[ixnm(kk, l)]));
-      }
-      a[r] = acc;
+      double *dst = (j >= NO ? c.Md : c.M) + (size_t)ci * NT;
+      const int n = tn[r], m = tm[r], s = n * n + n;
+      dst[s + m] += a[r].re;
+      if (m > 0) dst[s - m] += a[r].im;
     }
-  }
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int j = tid + 256 * r;
-    if (j >= NTASK) continue;
-    double *dst = (j >= NO ? c.Md : c.M) + (size_t)ci * NT;
-    const int n = tn[r], m = tm[r], s = n * n + n;
-    dst[s + m] += a[r].re;
-    if (m > 0) dst[s - m] += a[r].im;
   }
 }
 
