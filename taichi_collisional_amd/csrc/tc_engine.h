@@ -45,6 +45,31 @@ struct DBuf {
   }
 };
 
+// Page-locked host memory for the arrays the engine moves every evaluation
+// (level counts, child counts, leaf ids, P2P work items): copies from or to
+// it are asynchronous and skip the driver's staging.
+template <class T>
+struct HBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t m) {
+    if (m <= n) return hipSuccess;
+    size_t nn = std::max(m, n + n / 2);
+    T *q = nullptr;
+    hipError_t e = hipHostMalloc((void **)&q, sizeof(T) * nn, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    if (p) hipHostFree(p);
+    p = q;
+    n = nn;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 struct Lcg48 {  // srand48 / drand48: X' = (0x5DEECE66D X + 0xB) mod 2^48
   uint64_t X = 0x1234ABCD330EULL;
   void seed(long s) { X = (((uint64_t)(uint32_t)s) << 16) | 0x330E; }
@@ -79,8 +104,10 @@ struct tc_context {
   DBuf split_offs, split_nch, split_scan, scan_tmp;
   std::vector<int> level_start;
   DBuf leaf_ids;
-  std::vector<int> h_leaves;
-  std::vector<int4> h_groups, h_items;
+  HBuf<int> h_leaves, h_nch, h_nbd, h_pcnt;
+  HBuf<int4> h_items;
+  size_t n_items = 0;
+  std::vector<int4> h_groups;
   DBuf p2p_items, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists

@@ -82,6 +82,7 @@ static int build_tree(tc_context *ctx, int n) {
   hipStream_t s = ctx->stream;
   BodiesDev b = bodies_of(ctx);
   int ncrit = ctx->cfg.ncrit;
+  TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 8));  // [7]: the children of one level
   ctx->level_start.assign(1, 0);
   int ls = 0, le = 1;
   TC_CHECK(hipStreamSynchronize(s));
@@ -96,16 +97,12 @@ static int build_tree(tc_context *ctx, int n) {
     if (int e = ensure_cells(ctx, le + 8 * nl, le)) return e;
     CellsDev c = cells_of(ctx);
     launch_split(c, b, ls, le, ncrit, ctx->split_offs.as<int>(), ctx->split_nch.as<int>(), s);
-    if (launch_scan(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, ctx->scan_tmp.p,
-                    ctx->scan_tmp.bytes, s))
-      return -4;
-    int last[2];
-    TC_CHECK(hipMemcpyAsync(&last[0], ctx->split_scan.as<int>() + nl - 1, sizeof(int),
-                            hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipMemcpyAsync(&last[1], ctx->split_nch.as<int>() + nl - 1, sizeof(int),
-                            hipMemcpyDeviceToHost, s));
+    launch_scan_small(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, 0, 1,
+                      ctx->tcnt.as<int>() + 7, s);
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 8, ctx->tcnt.as<int>() + 7, sizeof(int),
+                            hipMemcpyDeviceToHost, s));  // pinned: a true asynchronous copy
     TC_CHECK(hipStreamSynchronize(s));
-    int total = last[0] + last[1];
+    int total = ctx->h_pin[8];
     if (total == 0) break;
     launch_children(cells_of(ctx), ls, le, ctx->split_offs.as<int>(), ctx->split_scan.as<int>(),
                     ctx->tp.as<TreeParams>(), s);
@@ -119,18 +116,20 @@ static int build_tree(tc_context *ctx, int n) {
   ctx->st.depth = depth;
   ctx->st.nlevels = (int)ctx->level_start.size() - 1;
   // leaf list (host side, one copy of the child counts)
-  std::vector<int> nch(nc), nbd(nc);
-  std::vector<int> &leaves = ctx->h_leaves;  // outlive the asynchronous uploads
+  TC_CHECK(ctx->h_nch.ensure(nc));
+  TC_CHECK(ctx->h_nbd.ensure(nc));
+  TC_CHECK(ctx->h_leaves.ensure(nc));
+  const int *nch = ctx->h_nch.p, *nbd = ctx->h_nbd.p;
+  int *leaves = ctx->h_leaves.p;  // outlives the asynchronous upload
   std::vector<int4> &groups = ctx->h_groups;
-  leaves.clear();
   groups.clear();
-  TC_CHECK(hipMemcpyAsync(nch.data(), ctx->cnchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipMemcpyAsync(nbd.data(), ctx->cnbody.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(ctx->h_nch.p, ctx->cnchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(ctx->h_nbd.p, ctx->cnbody.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
   TC_CHECK(hipStreamSynchronize(s));
-  leaves.reserve(nc);
+  int nleaf = 0;
   for (int i = 0; i < nc; i++)
     if (nch[i] == 0) {
-      leaves.push_back(i);
+      leaves[nleaf++] = i;
       // near-field target groups of <= 32 bodies, balanced within the leaf
       int nb = nbd[i], ng = (nb + 31) / 32, o = 0;
       for (int g = 0; g < ng; g++) {
@@ -139,10 +138,9 @@ static int build_tree(tc_context *ctx, int n) {
         o += k;
       }
     }
-  ctx->nleaves = (int)leaves.size();
+  ctx->nleaves = nleaf;
   ctx->st.nleaves = ctx->nleaves;
-  TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves.data(), sizeof(int) * leaves.size(),
-                          hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves, sizeof(int) * nleaf, hipMemcpyHostToDevice, s));
   return 0;
 }
 
@@ -168,14 +166,6 @@ static int upward(tc_context *ctx) {
   return 0;
 }
 
-__global__ void k_trav_totals(const int *__restrict__ blk, int nblk, int stride,
-                              int *__restrict__ out) {
-  // blk holds three count arrays then their three exclusive scans
-  if (threadIdx.x < 3)
-    out[threadIdx.x] = blk[(3 + threadIdx.x) * stride + nblk - 1] +
-                       blk[threadIdx.x * stride + nblk - 1];
-}
-
 // Breadth-first dual traversal from (root, root), one launch pair per step.
 static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   hipStream_t s = ctx->stream;
@@ -190,17 +180,11 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   while (nin > 0) {
     const int nblk = (nin + 255) / 256, stride = nblk + 1;
     TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * (size_t)stride));
-    TC_CHECK(ctx->tscan.ensure(scan_temp_bytes(nblk) + 256));
     int *blk = ctx->tblk.as<int>();
     CellsDev c = cells_of(ctx);
     launch_trav_count(c, ctx->front_a.as<int2>(), nin, mac, blk, blk + stride,
                       blk + 2 * stride, s);
-    for (int k = 0; k < 3; k++)
-      if (launch_scan(blk + k * stride, blk + (3 + k) * stride, nblk, ctx->tscan.p,
-                      ctx->tscan.bytes, s))
-        return -4;
-    hipLaunchKernelGGL(k_trav_totals, dim3(1), dim3(64), 0, s, blk, nblk, stride,
-                       ctx->tcnt.as<int>() + 4);
+    launch_scan_small(blk, blk + 3 * stride, nblk, stride, 3, ctx->tcnt.as<int>() + 4, s);
     TC_CHECK(hipMemcpyAsync(h + 4, ctx->tcnt.as<int>() + 4, 3 * sizeof(int),
                             hipMemcpyDeviceToHost, s));
     TC_CHECK(hipStreamSynchronize(s));
@@ -268,33 +252,47 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
                         ctx->p2p_off, ctx->p2p_src))
     return e;
+  // per-leaf P2P list lengths for the host-built work items (see build_p2p_items)
+  TC_CHECK(ctx->h_pcnt.ensure(nc));
+  TC_CHECK(hipMemcpyAsync(ctx->h_pcnt.p, ctx->p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost,
+                          s));
   // chunks of <= 64 sources per target for M2L
   hipLaunchKernelGGL(k_chunk_counts, dim3((nc + 1 + 255) / 256), dim3(256), 0, s,
                      ctx->m2l_cnt.as<int>(), nc, ctx->m2l_cur.as<int>());
   if (launch_scan(ctx->m2l_cur.as<int>(), ctx->m2l_coff.as<int>(), nc + 1, ctx->scan_tmp.p,
                   ctx->scan_tmp.bytes, s))
     return -4;
-  TC_CHECK(hipMemcpyAsync(&nchunks, ctx->m2l_coff.as<int>() + nc, sizeof(int),
+  TC_CHECK(hipMemcpyAsync(ctx->h_pin + 10, ctx->m2l_coff.as<int>() + nc, sizeof(int),
                           hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(hipStreamSynchronize(s));  // also completes the list-length copy above
+  nchunks = ctx->h_pin[10];
   TC_CHECK(ctx->m2l_chunks.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
   launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
                      ctx->m2l_chunks.as<int4>(), s);
-  // P2P work items: every target group x slices of <= kItemList list entries
+  return 0;
+}
+
+// P2P work items: every target group x slices of <= kItemList list entries.
+// The per-leaf list lengths were copied to pinned memory before the M2L
+// launch (and the stream synchronized after that copy), so this host loop runs
+// while the GPU is busy with M2L; the upload is queued behind it.
+static int build_p2p_items(tc_context *ctx) {
   constexpr int kItemList = 128;
-  std::vector<int> pcnt(nc);
-  TC_CHECK(hipMemcpyAsync(pcnt.data(), ctx->p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));
-  std::vector<int4> &items = ctx->h_items;
-  items.clear();
+  const int *pcnt = ctx->h_pcnt.p;
+  size_t nitem = 0;
+  for (const int4 &g : ctx->h_groups) nitem += (pcnt[g.x] + kItemList - 1) / kItemList;
+  TC_CHECK(ctx->h_items.ensure(std::max<size_t>(nitem, 1)));
+  int4 *items = ctx->h_items.p;
+  size_t k = 0;
   for (const int4 &g : ctx->h_groups) {
-    int cnt = pcnt[g.x];
+    const int cnt = pcnt[g.x];
     for (int e0 = 0; e0 < cnt; e0 += kItemList)
-      items.push_back(make_int4(g.x, g.y, e0, std::min(kItemList, cnt - e0) | (g.z << 16)));
+      items[k++] = make_int4(g.x, g.y, e0, std::min(kItemList, cnt - e0) | (g.z << 16));
   }
-  TC_CHECK(ctx->p2p_items.ensure(sizeof(int4) * std::max<size_t>(items.size(), 1)));
-  TC_CHECK(hipMemcpyAsync(ctx->p2p_items.p, items.data(), sizeof(int4) * items.size(),
-                          hipMemcpyHostToDevice, s));
+  ctx->n_items = nitem;
+  TC_CHECK(ctx->p2p_items.ensure(sizeof(int4) * std::max<size_t>(nitem, 1)));
+  TC_CHECK(hipMemcpyAsync(ctx->p2p_items.p, items, sizeof(int4) * nitem, hipMemcpyHostToDevice,
+                          ctx->stream));
   return 0;
 }
 
@@ -466,6 +464,8 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
     ctx->err = "expansion order not compiled";
     return -2;
   }
+  // the GPU runs M2L while the host builds the P2P work items
+  if (int e = build_p2p_items(ctx)) return e;
   TC_CHECK(hipEventRecord(ctx->ev[5], s));
   // ---- near field
   ResultsDev r = results_of(ctx);
@@ -477,7 +477,7 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   launch_p2p_ranges(cells_of(ctx), ctx->p2p_src.as<int>(), (int)ctx->st.n_p2p_cells,
                     ctx->p2p_ranges.as<int2>(), s);
   launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
-             (int)ctx->h_items.size(), ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
+             (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
              ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s);
   TC_CHECK(hipEventRecord(ctx->ev[6], s));
   // ---- downward
@@ -584,6 +584,11 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
+  ctx->h_leaves.release();
+  ctx->h_nch.release();
+  ctx->h_nbd.release();
+  ctx->h_pcnt.release();
+  ctx->h_items.release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);

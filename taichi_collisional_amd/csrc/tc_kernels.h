@@ -70,6 +70,9 @@ int launch_children(CellsDev c, int ls, int le, const int *offs, const int *nch_
                     const TreeParams *d_tp, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
+// narrays exclusive scans of n ints (arrays stride apart), totals to total[k]
+int launch_scan_small(const int *in, int *out, int n, int stride, int narrays, int *total,
+                      hipStream_t s);
 
 // ---- traversal (tc_tree.hip)
 // Multipole acceptance: kMacTheta  |dX|^2 theta^2 > (Ri+Rj)^2 on the balls

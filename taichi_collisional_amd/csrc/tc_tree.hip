@@ -343,6 +343,45 @@ int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, h
   return hipcub::DeviceScan::ExclusiveSum(temp, tb, in, out, n, s) == hipSuccess ? 0 : -1;
 }
 
+// Exclusive prefix sums of small int arrays (the cells of one level, the
+// blocks of one traversal step), one 1024-thread block per array (blockIdx.x
+// selects array k at in + k * stride, out + k * stride): each thread sums a
+// contiguous chunk, the chunk sums are scanned across the block, then every
+// thread writes its chunk.  The array's total goes to total[k].  One launch,
+// no temporary storage and none of a library call's host-side dispatch.
+__global__ __launch_bounds__(1024) void k_scan_small(const int *__restrict__ in,
+                                                     int *__restrict__ out, int n, int stride,
+                                                     int *__restrict__ total) {
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int *a = in + (size_t)blockIdx.x * stride;
+  int *o = out + (size_t)blockIdx.x * stride;
+  const int per = (n + 1023) / 1024;
+  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+  int sum = 0;
+  for (int i = b0; i < b1; i++) sum += a[i];
+  int wt;
+  const int ex = wave_excl_scan(sum, &wt);
+  if (lane == 0) wsum[w] = wt;
+  __syncthreads();
+  int base = 0;
+  for (int k = 0; k < w; k++) base += wsum[k];
+  int run = base + ex;
+  for (int i = b0; i < b1; i++) {
+    const int v = a[i];
+    o[i] = run;
+    run += v;
+  }
+  if (t == 1023) total[blockIdx.x] = run;
+}
+
+int launch_scan_small(const int *in, int *out, int n, int stride, int narrays, int *total,
+                      hipStream_t s) {
+  if (n <= 0 || narrays <= 0) return 0;
+  hipLaunchKernelGGL(k_scan_small, dim3(narrays), dim3(1024), 0, s, in, out, n, stride, total);
+  return 0;
+}
+
 // ------------------------------------------------------------ traversal -----
 // One breadth-first step, in two passes so that no shared counter is hit by
 // every wave: pass 1 classifies each frontier pair and writes per-block
