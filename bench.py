@@ -29,23 +29,34 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(n, p, theta, ncrit, seed, threads, timeout):
-    """Time the reference (oracle/_ref) on the host in a child process."""
+def cpu_baseline(n, p, theta, ncrit, seed, threads, timeout, repeats=3, default_path=True):
+    """Time the reference (oracle/_ref) on the host in a child process: one
+    warm-up and `repeats` timed evaluations of the theta-MAC workload (median
+    reported), then one evaluation of the reference's default path (p=2
+    pre-pass + error-controlled MAC, get_force_and_potential as shipped)."""
     code = r"""
 import sys, time, json, os
 sys.path.insert(0, %r); sys.path.insert(0, os.path.join(%r, "oracle"))
 import numpy as np, pyoracle as po
 from taichi_collisional_amd import ics
-n, p, theta, ncrit, seed = %d, %d, %r, %d, %d
+n, p, theta, ncrit, seed, repeats, dflt = %d, %d, %r, %d, %d, %d, %d
 x, v, m = ics.plummer(n, seed=seed)
 if not po.ref_available(p):
     print(json.dumps({"error": "reference harness for p=%%d not built" %% p})); sys.exit(0)
 L = po.reference(p)
-t = time.time()
-pot, acc, jerk, acco, counts, shift = po.ref_eval(p, x, v, m, seed, ncrit=ncrit, theta=theta, mode=0)
-dt = time.time() - t
-print(json.dumps({"seconds": dt, "isa": L.isa, "counts": [int(c) for c in counts]}))
-""" % (ROOT, ROOT, n, p, theta, ncrit, seed)
+po.ref_eval(p, x, v, m, seed, ncrit=ncrit, theta=theta, mode=0)  # warm-up
+ts = []
+for _ in range(repeats):
+    t = time.time()
+    pot, acc, jerk, acco, counts, shift = po.ref_eval(p, x, v, m, seed, ncrit=ncrit, theta=theta, mode=0)
+    ts.append(time.time() - t)
+out = {"seconds": sorted(ts)[len(ts) // 2], "all": ts, "isa": L.isa, "counts": [int(c) for c in counts]}
+if dflt:
+    t = time.time()
+    po.ref_eval(p, x, v, m, seed, ncrit=ncrit, theta=theta, mode=1)
+    out["default_seconds"] = time.time() - t
+print(json.dumps(out))
+""" % (ROOT, ROOT, n, p, theta, ncrit, seed, repeats, int(default_path))
     env = dict(os.environ)
     env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
                OMP_WAIT_POLICY="active", OMP_STACKSIZE="100M")
@@ -67,12 +78,18 @@ print(json.dumps({"seconds": dt, "isa": L.isa, "counts": [int(c) for c in counts
                 break
     except OSError:
         pass
-    return {"value": n / out["seconds"], "unit": "particles/s", "cores": threads,
-            "kind": "reference",
-            "sample": "one full evaluation of the same workload (theta-MAC passes, N=%d, p=%d, "
-                      "theta=%g, ncrit=%d), %.2f s wall, %s build, OpenMP %d threads, %s"
-                      % (n, p, theta, ncrit, out["seconds"], out["isa"], threads, cpu_model),
-            "seconds": out["seconds"], "m2l_pairs": out["counts"][0]}
+    res = {"value": n / out["seconds"], "unit": "particles/s", "cores": threads,
+           "kind": "reference",
+           "sample": "median of %d full evaluations after one warm-up, same workload (theta-MAC "
+                     "passes, N=%d, p=%d, theta=%g, ncrit=%d): %s s wall, %s build, OpenMP %d "
+                     "threads, %s" % (len(out["all"]), n, p, theta, ncrit,
+                                      ", ".join("%.2f" % t for t in out["all"]), out["isa"],
+                                      threads, cpu_model),
+           "seconds": out["seconds"], "m2l_pairs": out["counts"][0]}
+    if "default_seconds" in out:
+        res["default_path_seconds"] = out["default_seconds"]
+        res["default_path_value"] = n / out["default_seconds"]
+    return res
 
 
 def load_traffic(path):
@@ -95,7 +112,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-timeout", type=float, default=150.0)
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the default-path and host-copy rows")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "m2l_traffic.json"))
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once (a plain copy kernel "
@@ -110,7 +130,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.n, args.p, args.theta, args.ncrit, args.seed, args.cpu_threads,
-                           args.cpu_timeout)
+                           args.cpu_timeout, repeats=args.cpu_repeats,
+                           default_path=not args.no_extra)
 
     import numpy as np
     import torch
@@ -160,6 +181,45 @@ def main():
            "jerk_rms": float(torch.sqrt(torch.mean(ej ** 2))), "jerk_max": float(ej.max()),
            "subset": int(tg.numel()), "gate_force_rms_1e-7": bool(torch.sqrt(torch.mean(ea ** 2)) <= 1e-7)}
 
+    extra = {}
+    if not args.no_extra:
+        # the reference's default path on the GPU: p=2 pre-pass + error-controlled MAC
+        fd = tca.Fmm(p=args.p, ncrit=args.ncrit, theta=args.theta, mac=tca.TC_MAC_ERROR,
+                     force_accuracy=2e-8, direct_below=0)
+        fd.set_seed(args.seed)
+        fd.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            fd.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+        torch.cuda.synchronize()
+        dms = (time.perf_counter() - t) * 1e3 / 3
+        sd = fd.stats()
+        ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
+        extra["default_path"] = {
+            "what": "reference default pass: p=2 pre-pass (acc_old), error-controlled MAC "
+                    "(force_accuracy=2e-8), same bodies and p",
+            "ms_per_eval": dms, "particles_per_s": args.n / (dms * 1e-3),
+            "force_rms": float(torch.sqrt(torch.mean(ea ** 2))), "force_max": float(ea.max()),
+            "m2l_pairs": sd["n_m2l"], "p2p_leaf_pairs": sd["n_p2p_cells"],
+            "ms_prepass": sd["ms_prepass"]}
+        if cpu and cpu.get("default_path_seconds"):
+            extra["default_path"]["cpu_reference_seconds"] = cpu["default_path_seconds"]
+            extra["default_path"]["gpu_over_cpu"] = cpu["default_path_seconds"] / (dms * 1e-3)
+        del fd
+        # host Body records in and out (AoS <-> device SoA, PCIe both ways)
+        bodies = np.zeros(args.n, tca.body_dtype)
+        bodies["X"], bodies["V"], bodies["q"] = x, v, m
+        bodies["issink"], bodies["issource"] = 1, 1
+        fmm.get_force_and_potential(bodies)
+        t = time.perf_counter()
+        for _ in range(2):
+            fmm.get_force_and_potential(bodies)
+        hms = (time.perf_counter() - t) * 1e3 / 2
+        extra["host_path"] = {"what": "tc_eval_bodies on host Body records: host AoS to device "
+                                      "SoA, evaluation, results back (PCIe both ways)",
+                              "ms_per_eval": hms, "particles_per_s": args.n / (hms * 1e-3)}
+
     if args.calibrate_copy_mib > 0:
         src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -193,6 +253,7 @@ def main():
     }
     if cpu and cpu.get("value"):
         line["gpu_over_cpu"] = value / cpu["value"]
+    line.update(extra)
     print(json.dumps(line))
 
 
