@@ -122,6 +122,9 @@ struct tc_context {
   int last_n = 0;
   // direct
   DBuf dtg, dpot, dacc, djerk, dx3, dv3, dm, dfl;
+  // host Body records: page-locked staging and device result buffers
+  HBuf<double> h_stage;
+  DBuf hop, hoa, hoj;
   hipEvent_t ev[9];
 };
 

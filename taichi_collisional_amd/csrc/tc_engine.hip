@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tc_engine.h"
@@ -351,6 +352,28 @@ static int low_pass(tc_context *ctx, int n) {
 }
 
 // acc_old of the last evaluation into out[n] in the caller's body order.
+// Split [0, n) into contiguous slices over a few host threads.  Packing and
+// unpacking the 152-byte Body records is bound by host memory bandwidth,
+// which one core does not reach.
+template <class F>
+static void host_parallel(int n, F f) {
+  const int kMinPerThread = 1 << 15;
+  int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+  nt = std::max(1, std::min(nt, n / kMinPerThread));
+  if (nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (int t = 1; t < nt; t++) {
+    const int lo = (int)((int64_t)n * t / nt), hi = (int)((int64_t)n * (t + 1) / nt);
+    th.emplace_back([=, &f] { f(lo, hi); });
+  }
+  f(0, (int)((int64_t)n / nt));
+  for (auto &t : th) t.join();
+}
+
 static int copy_acc_old(tc_context *ctx, int n, double *out) {
   if (!ctx->has_acc_old || n != ctx->last_n) {
     ctx->err = "acc_old is only computed by an error-MAC evaluation (TC_MAC_ERROR)";
@@ -580,7 +603,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
-                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, 
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
@@ -589,6 +612,7 @@ int tc_destroy(tc_context *ctx) {
   ctx->h_nbd.release();
   ctx->h_pcnt.release();
   ctx->h_items.release();
+  ctx->h_stage.release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
@@ -636,61 +660,68 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
     return -1;
   }
   hipStream_t s = ctx->stream;
-  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), m(n);
-  std::vector<unsigned char> fl(n);
-  for (int i = 0; i < n; i++) {
-    for (int d = 0; d < 3; d++) {
-      x[3 * i + d] = bodies[i].X[d];
-      v[3 * i + d] = bodies[i].V[d];
+  const size_t N = (size_t)n;
+  // page-locked staging: inputs x[3n] v[3n] m[n] flags[n bytes]; outputs
+  // pot[n] acc[3n] jerk[3n] acc_old[n] reuse the same block afterwards
+  TC_CHECK(ctx->h_stage.ensure(8 * N + (N + 7) / 8));
+  double *hx = ctx->h_stage.p, *hv = hx + 3 * N, *hm = hv + 3 * N;
+  unsigned char *hf = (unsigned char *)(hm + N);
+  host_parallel(n, [&](int lo, int hi) {
+    for (int i = lo; i < hi; i++) {
+      const tc_body &b = bodies[i];
+      hx[3 * i] = b.X[0];
+      hx[3 * i + 1] = b.X[1];
+      hx[3 * i + 2] = b.X[2];
+      hv[3 * i] = b.V[0];
+      hv[3 * i + 1] = b.V[1];
+      hv[3 * i + 2] = b.V[2];
+      hm[i] = b.q;
+      hf[i] = (b.issink ? kSink : 0) | (b.issource ? kSource : 0);
     }
-    m[i] = bodies[i].q;
-    fl[i] = (bodies[i].issink ? kSink : 0) | (bodies[i].issource ? kSource : 0);
-  }
-  TC_CHECK(ctx->dx3.ensure(24 * (size_t)n));
-  TC_CHECK(ctx->dv3.ensure(24 * (size_t)n));
-  TC_CHECK(ctx->dm.ensure(8 * (size_t)n));
-  TC_CHECK(ctx->dfl.ensure(n));
-  TC_CHECK(ctx->dpot.ensure(8 * (size_t)n));
-  TC_CHECK(ctx->dacc.ensure(24 * (size_t)n));
-  TC_CHECK(ctx->djerk.ensure(24 * (size_t)n));
-  TC_CHECK(hipMemcpyAsync(ctx->dx3.p, x.data(), 24 * (size_t)n, hipMemcpyHostToDevice, s));
-  TC_CHECK(hipMemcpyAsync(ctx->dv3.p, v.data(), 24 * (size_t)n, hipMemcpyHostToDevice, s));
-  TC_CHECK(hipMemcpyAsync(ctx->dm.p, m.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
-  TC_CHECK(hipMemcpyAsync(ctx->dfl.p, fl.data(), n, hipMemcpyHostToDevice, s));
-  // the all-pairs branch writes into dpot/dacc/djerk itself; use separate outputs
-  DBuf op, oa, oj;
-  TC_CHECK(op.ensure(8 * (size_t)n));
-  TC_CHECK(oa.ensure(24 * (size_t)n));
-  TC_CHECK(oj.ensure(24 * (size_t)n));
+  });
+  TC_CHECK(ctx->dx3.ensure(24 * N));
+  TC_CHECK(ctx->dv3.ensure(24 * N));
+  TC_CHECK(ctx->dm.ensure(8 * N));
+  TC_CHECK(ctx->dfl.ensure(N));
+  TC_CHECK(ctx->hop.ensure(8 * N));
+  TC_CHECK(ctx->hoa.ensure(24 * N));
+  TC_CHECK(ctx->hoj.ensure(24 * N));
+  TC_CHECK(hipMemcpyAsync(ctx->dx3.p, hx, 24 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dv3.p, hv, 24 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dm.p, hm, 8 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dfl.p, hf, N, hipMemcpyHostToDevice, s));
+  // the all-pairs branch of eval_device uses dpot/dacc/djerk as its own
+  // scratch, so the results land in separate buffers
+  double *op = ctx->hop.as<double>(), *oa = ctx->hoa.as<double>(), *oj = ctx->hoj.as<double>();
   int e = eval_device(ctx, n, ctx->dx3.as<double>(), ctx->dv3.as<double>(), ctx->dm.as<double>(),
-                      ctx->dfl.as<unsigned char>(), op.as<double>(), oa.as<double>(),
-                      oj.as<double>());
-  if (e == 0) {
-    std::vector<double> hp(n), ha(3 * (size_t)n), hj(3 * (size_t)n), ho;
-    hipMemcpyAsync(hp.data(), op.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(ha.data(), oa.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(hj.data(), oj.p, 24 * (size_t)n, hipMemcpyDeviceToHost, s);
-    // acc_old is set by the pre-pass of the error-MAC path (integrator.h:350-356)
-    const bool want_acc = ctx->has_acc_old;
-    if (want_acc) {
-      ho.resize(n);
-      e = copy_acc_old(ctx, n, op.as<double>());
-      if (e == 0) hipMemcpyAsync(ho.data(), op.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
-    }
-    hipStreamSynchronize(s);
-    for (int i = 0; i < n; i++) {
-      bodies[i].p = hp[i];
-      for (int d = 0; d < 3; d++) {
-        bodies[i].F[d] = ha[3 * i + d];
-        bodies[i].J[d] = hj[3 * i + d];
-      }
-      if (want_acc) bodies[i].acc_old = ho[i];
-    }
+                      ctx->dfl.as<unsigned char>(), op, oa, oj);
+  if (e != 0) return e;
+  double *rp = ctx->h_stage.p, *ra = rp + N, *rj = ra + 3 * N, *ro = rj + 3 * N;
+  TC_CHECK(hipMemcpyAsync(rp, op, 8 * N, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(ra, oa, 24 * N, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(rj, oj, 24 * N, hipMemcpyDeviceToHost, s));
+  // acc_old is set by the pre-pass of the error-MAC path (integrator.h:350-356)
+  const bool want_acc = ctx->has_acc_old;
+  if (want_acc) {
+    // op has been copied out above (same stream), so it can hold acc_old
+    if ((e = copy_acc_old(ctx, n, op)) != 0) return e;
+    TC_CHECK(hipMemcpyAsync(ro, op, 8 * N, hipMemcpyDeviceToHost, s));
   }
-  op.release();
-  oa.release();
-  oj.release();
-  return e;
+  TC_CHECK(hipStreamSynchronize(s));
+  host_parallel(n, [&](int lo, int hi) {
+    for (int i = lo; i < hi; i++) {
+      tc_body &b = bodies[i];
+      b.p = rp[i];
+      b.F[0] = ra[3 * i];
+      b.F[1] = ra[3 * i + 1];
+      b.F[2] = ra[3 * i + 2];
+      b.J[0] = rj[3 * i];
+      b.J[1] = rj[3 * i + 1];
+      b.J[2] = rj[3 * i + 2];
+      if (want_acc) b.acc_old = ro[i];
+    }
+  });
+  return 0;
 }
 
 int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
