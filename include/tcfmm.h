@@ -152,6 +152,66 @@ int tc_L2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncell
 int tc_P2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncells,
            const int *targets, const int *sources, int npairs);
 
+/* ---- Master (non-jerk) variant ----------------------------------------------
+ * The reference's top-level sources (the "master" code, its own exafmm.h /
+ * integrator.h / traverse_eager.h / kernel_fast_lean.h) evaluate force and
+ * potential without jerk, plus an optional per-body time-step estimate:
+ *   tc_master_eval_bodies <- void get_force_and_potential(Bodies&, bool get_steps)
+ *                            (master integrator.h:459-562)
+ *   tc_master_body        <- master struct Body (exafmm.h:21-48), byte-for-byte
+ * mode bits:
+ *   TC_MASTER_STEPS      get_steps = true: timestep += (min over each source leaf
+ *                        of tau)^-4, tau = min(free-fall, fly-by) estimate
+ *                        (master P2P, kernel_fast_lean.h:462-674).  The value
+ *                        written is that sum; the caller converts it with
+ *                        1/sqrt(sqrt(.)) as kick_naive does (integrator.h:265).
+ *   TC_MASTER_FMM        the build with -DFMM (integrator.h:461-556): below
+ *                        config.ncrit bodies direct(), else the p = 2
+ *                        pre-pass, the symmetric error-controlled walk
+ *                        (traverse_eager.h:61-149, both f1 and f2 < 1 at :103) with
+ *                        symmetric M2L_rotate (kernel_fast_lean.h:1034-1356);
+ *                        with STEPS the walk's accepted far pairs add nothing,
+ *                        as in the reference (traverse_eager.h:116-120).
+ *                        Without this bit: the master Makefile's default build
+ *                        (FMM undefined, integrator.h:557-559): direct() over
+ *                        every pair of leaves.
+ *   TC_MASTER_PAIR_MASS  the scalar form of the time-step estimate (pair mass
+ *                        q_i + q_j, kernel_fast_lean.h:593-672) instead of the
+ *                        vector form the master build compiles (SIMD_P2P,
+ *                        :474-592), whose mass term accumulates along the
+ *                        source leaf.
+ * The tree is the master one: ncrit = config.ncrit, no root shift.  dt_param
+ * is the global of the same name (master exafmm.h:109, set from argv by
+ * main.cxx:75); it must be > 0 when TC_MASTER_STEPS is set. */
+enum { TC_MASTER_STEPS = 1, TC_MASTER_FMM = 2, TC_MASTER_PAIR_MASS = 4 };
+
+typedef struct tc_master_body {
+  double X[3];             /* position            @0   */
+  double q;                /* mass                @24  */
+  double F[3];             /* acceleration (out)  @32  */
+  double p;                /* potential (out)     @56  */
+  double V[3];             /* velocity            @64  */
+  double acc_old;          /* |a| estimate        @88  */
+  unsigned int Morton[3];  /* tree keys (scratch) @96  */
+  double timestep;         /* step sum (out)      @112 */
+  int index;               /* caller's index      @120 */
+  TC_BOOL issink;          /*                     @124 */
+  TC_BOOL issource;        /*                     @125 */
+} tc_master_body;
+
+/* Host Body records: F, p and (with TC_MASTER_STEPS) timestep are
+ * overwritten, acc_old is written by the TC_MASTER_FMM pre-pass; the array is
+ * not reordered.  Copies host<->device inside. */
+int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mode,
+                          double dt_param);
+
+/* Device-resident form: x, v [n][3], m [n], flags [n] (bit0 sink, bit1
+ * source; NULL = all both), pot [n], acc [n][3], timestep [n] (needed with
+ * TC_MASTER_STEPS, else ignored).  Returns after the work is done. */
+int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                       const unsigned char *flags, int mode, double dt_param, double *pot,
+                       double *acc, double *timestep);
+
 int tc_get_stats(const tc_context *ctx, tc_stats *out);
 const char *tc_last_error(const tc_context *ctx);
 
@@ -173,7 +233,15 @@ static_assert(__builtin_offsetof(tc_body, V) == 64 && __builtin_offsetof(tc_body
 static_assert(__builtin_offsetof(tc_body, J) == 120 && __builtin_offsetof(tc_body, index) == 144 &&
                   __builtin_offsetof(tc_body, issink) == 148 && __builtin_offsetof(tc_body, issource) == 149,
               "tc_body layout");
+static_assert(sizeof(tc_master_body) == 128, "tc_master_body must match the master Body (128 bytes)");
+static_assert(__builtin_offsetof(tc_master_body, acc_old) == 88 &&
+                  __builtin_offsetof(tc_master_body, timestep) == 112 &&
+                  __builtin_offsetof(tc_master_body, index) == 120 &&
+                  __builtin_offsetof(tc_master_body, issink) == 124 &&
+                  __builtin_offsetof(tc_master_body, issource) == 125,
+              "tc_master_body layout");
 #else
+_Static_assert(sizeof(tc_master_body) == 128, "tc_master_body layout");
 _Static_assert(sizeof(tc_body) == 152, "tc_body must match the reference Body (152 bytes)");
 _Static_assert(__builtin_offsetof(tc_body, J) == 120 && __builtin_offsetof(tc_body, issink) == 148, "tc_body layout");
 #endif

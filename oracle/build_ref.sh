@@ -32,5 +32,23 @@ for P in $ORDERS; do
     pids+=($!)
   done
 done
+# Master (non-jerk) variant: the top-level sources with the master Makefile's
+# flags (Makefile:1-12, minus DEBUG and the HDF5 I/O) plus -DFMM so that both
+# of get_force_and_potential's branches can be driven (ref_master_harness.cpp).
+MREF=${TC_REFERENCE_MASTER_DIR:-$(dirname "$REF")}
+MORDERS=${TC_REFM_ORDERS:-"8 12"}
+MFLAGS="-O3 -fopenmp -fcx-limited-range -funroll-loops -Wno-format -w -fPIC -shared \
+  -DFMM -DGRADIENT -DOUTPUTPOT -DOUTPUTACC -DSIMD_P2P -DDOUBLE_P2P -DMINIBALL -DHUGE=3.40282347e+38F"
+if [ -f "$MREF/kernel_fast_lean.h" ]; then
+  for P in $MORDERS; do
+    for V in v4 v3; do
+      tgt="$OUT/libtcrefm_p${P}_${V}.so"
+      if [ -f "$tgt" ] && [ "$tgt" -nt "$HERE/ref_master_harness.cpp" ]; then continue; fi
+      g++ -march=x86-64-${V} $MFLAGS -DEXPANSION=$P -I"$MREF" "$HERE/ref_master_harness.cpp" \
+        -o "$tgt.tmp" && mv "$tgt.tmp" "$tgt" &
+      pids+=($!)
+    done
+  done
+fi
 for p in "${pids[@]}"; do wait "$p"; done
 echo "build_ref: libraries in $OUT"

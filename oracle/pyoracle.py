@@ -63,6 +63,11 @@ def lib():
         L.orc_harmonics.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.orc_swap.argtypes = [C.c_int, _dp]
         L.orc_shift_from_seed.argtypes = [C.c_long, C.c_int, _dp]
+        L.orc_master_direct.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p,
+                                        C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_master_fmm.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp,
+                                     C.POINTER(OrcStats)]
         _lib = L
     return _lib
 
@@ -103,6 +108,45 @@ class Oracle:
         self._n = n
         self.stats = {k: getattr(st, k) for k, _ in OrcStats._fields_}
         return pot, acc.reshape(n, 3), jerk.reshape(n, 3), acco
+
+    def master_direct(self, x, v, m, issink=None, issource=None, steps=True, pair_mass=False,
+                      dt_param=0.025):
+        """Master default build's get_force_and_potential (direct over leaf pairs).
+        Construct the Oracle with shift_frac=(0,0,0): the master tree has no root shift.
+        Returns pot, acc, timestep (the per-body sum of leaf-minimum tau^-4)."""
+        n = m.size
+        pot, acc, ts = np.zeros(n), np.zeros(3 * n), np.zeros(n)
+        s = _opt(issink, np.uint8)
+        so = _opt(issource, np.uint8)
+        rc = lib().orc_master_direct(self.h, n, np.ascontiguousarray(x).reshape(-1),
+                                     np.ascontiguousarray(v).reshape(-1), np.ascontiguousarray(m),
+                                     None if s is None else s.ctypes.data,
+                                     None if so is None else so.ctypes.data, int(steps),
+                                     int(pair_mass), dt_param, pot, acc, ts)
+        assert rc == 0
+        self._n = n
+        return pot, acc.reshape(n, 3), ts
+
+    def master_fmm(self, x, v, m, issink=None, issource=None, steps=False, pair_mass=False,
+                   dt_param=0.025, acc_old=None):
+        """Master get_force_and_potential built with -DFMM (the handle's p, ncrit,
+        force_accuracy; construct with shift_frac=(0,0,0)).  acc_old: the caller's
+        values the pre-pass adds to.  Returns pot, acc, timestep, acc_old."""
+        n = m.size
+        pot, acc, ts = np.zeros(n), np.zeros(3 * n), np.zeros(n)
+        ao = np.zeros(n) if acc_old is None else np.array(acc_old, dtype=np.float64)
+        st = OrcStats()
+        s = _opt(issink, np.uint8)
+        so = _opt(issource, np.uint8)
+        rc = lib().orc_master_fmm(self.h, n, np.ascontiguousarray(x).reshape(-1),
+                                  np.ascontiguousarray(v).reshape(-1), np.ascontiguousarray(m),
+                                  None if s is None else s.ctypes.data,
+                                  None if so is None else so.ctypes.data, int(steps),
+                                  int(pair_mass), dt_param, ao, pot, acc, ts, C.byref(st))
+        assert rc == 0
+        self._n = n
+        self.stats = {k: getattr(st, k) for k, _ in OrcStats._fields_}
+        return pot, acc.reshape(n, 3), ts, ao
 
     def cells(self):
         n = -lib().orc_cells(self.h, 0, *([np.zeros(1, np.int32)] * 5),
@@ -198,6 +242,48 @@ def ref_eval(p, x, v, m, seed, ncrit=64, theta=0.5, mode=0, nthreads=0):
                     pot, acc, jerk, acco, counts, shift)
     assert rc == 0
     return pot, acc.reshape(n, 3), jerk.reshape(n, 3), acco, counts, shift
+
+
+_mrefs = {}
+
+
+def master_ref_available(p):
+    d = os.path.join(HERE, "_ref")
+    return any(os.path.exists(os.path.join(d, "libtcrefm_p%d_%s.so" % (p, v))) for v in ("v4", "v3"))
+
+
+def master_reference(p):
+    """ctypes handle on the master-variant reference harness (ref_master_harness.cpp)."""
+    if p in _mrefs:
+        return _mrefs[p]
+    isa = "v4" if _cpu_has_avx512() else "v3"
+    path = os.path.join(HERE, "_ref", "libtcrefm_p%d_%s.so" % (p, isa))
+    if not os.path.exists(path):
+        raise RuntimeError("master reference harness for p=%d not built (%s)" % (p, path))
+    L = C.CDLL(path)
+    L.refm_eval.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                            C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    L.refm_kick_steps.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+    L.isa = isa
+    _mrefs[p] = L
+    return L
+
+
+def master_ref_eval(p, x, v, m, issink=None, issource=None, ncrit=300, dt_param=0.025,
+                    force_accuracy=2e-7, mode=0, get_steps=True, nthreads=0):
+    """Run the compiled master reference once (mode 0: direct, 1: FMM
+    get_force_and_potential); returns pot, acc, timestep, acc_old."""
+    L = master_reference(p)
+    n = m.size
+    pot, acc, ts, acco = np.zeros(n), np.zeros(3 * n), np.zeros(n), np.zeros(n)
+    s = _opt(issink, np.uint8)
+    so = _opt(issource, np.uint8)
+    rc = L.refm_eval(n, np.ascontiguousarray(x).reshape(-1), np.ascontiguousarray(v).reshape(-1),
+                     np.ascontiguousarray(m), None if s is None else s.ctypes.data,
+                     None if so is None else so.ctypes.data, ncrit, dt_param, force_accuracy,
+                     mode, int(get_steps), nthreads, pot, acc, ts, acco)
+    assert rc == 0
+    return pot, acc.reshape(n, 3), ts, acco
 
 
 def ref_cells(p, ncells_max, nbodies):

@@ -320,6 +320,7 @@ static const real *tdesign() { return tcfmm::kTDesign; }
 struct Body {
   real X[3], V[3], q;
   real F[3], J[3], p, acc_old;
+  real ts;  // master variant: time-step sum
   unsigned Morton[3];
   int index;
   bool issink, issource;
@@ -965,7 +966,12 @@ struct Fmm {
   // (kernel_fast_lean.h:1446-1780) with the passes of traverse_eager.h:34-56,
   // :172-212, :249-269.  Geometry is the tree's (centre, half-width).  The
   // reference runs P2P_low in FP32 (Vec16f); so does this restatement.
-  void low_pass() {
+  // master = true: the master sources' P2P_low and L2P_low (master
+  // kernel_fast_lean.h:1479-1556, 1609-1646), which add to acc_old instead of
+  // forming |a|: per listed source cell q_last / r^2 to the source cell's last
+  // body (the vector loop overwrites its register every source, FP64 under
+  // DOUBLE_P2P; r = 0 gives 0), and the magnitude of the far-field gradient.
+  void low_pass(bool master = false) {
     const int P = 2, NT = 9;
     std::vector<real> lowM(9 * C.size(), 0), lowL(9 * C.size(), 0);
     std::vector<int> order(C.size());
@@ -1052,6 +1058,15 @@ struct Fmm {
       }
       for (int j : gp[i]) {
         Cell &b = C[j];
+        if (master) {
+          const Body &bl = B[b.BODY + b.NBODY - 1];
+          for (int x = a.BODY; x < a.BODY + a.NBODY; x++) {
+            real dx = B[x].X[0] - bl.X[0], dy = B[x].X[1] - bl.X[1], dz = B[x].X[2] - bl.X[2];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 0) F[3 * x] += bl.q / r2;
+          }
+          continue;
+        }
         for (int x = a.BODY; x < a.BODY + a.NBODY; x++) {
           float xi = -(float)B[x].X[0], yi = -(float)B[x].X[1], zi = -(float)B[x].X[2];
           float ax = 0, ay = 0, az = 0;
@@ -1085,6 +1100,12 @@ struct Fmm {
             for (int m = 0; m <= n; m++)
               for (int k = 0; k <= P - n; k++)
                 for (int l = -k; l <= k; l++) phi[ix(n, m)] += cl[ix(n + k, m + l)] * G[ix(k, l)];
+          if (master) {
+            F[3 * i + 1] += std::sqrt(phi[3].real() * phi[3].real() +
+                                      phi[3].imag() * phi[3].imag() +
+                                      phi[2].real() * phi[2].real());
+            continue;
+          }
           F[3 * i] -= phi[3].real();
           F[3 * i + 1] -= phi[3].imag();
           F[3 * i + 2] -= phi[2].real();
@@ -1105,6 +1126,10 @@ struct Fmm {
           for (int t = 0; t < 9; t++) lowL[9 * k + t] += r[t];
         }
       }
+    }
+    if (master) {  // F[3i]: near terms, F[3i+1]: far-field magnitude
+      for (size_t i = 0; i < B.size(); i++) B[i].acc_old += F[3 * i] + F[3 * i + 1];
+      return;
     }
     for (size_t i = 0; i < B.size(); i++)
       B[i].acc_old = std::sqrt(F[3 * i] * F[3 * i] + F[3 * i + 1] * F[3 * i + 1] +
@@ -1127,6 +1152,243 @@ struct Fmm {
         B[i].J[0] += jx, B[i].J[1] += jy, B[i].J[2] += jz;
       }
     }
+  }
+
+  // ---- master (non-jerk) variant ------------------------------------------
+  // Timestep estimate of one (target, source) pair, folded into the running
+  // minimum ts.  Two forms (master kernel_fast_lean.h, the file at the top of
+  // the reference tree):
+  //  * vector form (:474-592, SIMD_P2P, the master Makefile's build): the
+  //    target mass register accumulates every source met with r2 > 0 within
+  //    the current source leaf (mi += mj), r2 == 0 is replaced by 1e38, v2
+  //    carries +1e-20 and both estimates are always taken;
+  //  * scalar form (:593-672): pairs with r2 == 0 skipped, pair mass
+  //    q_i + q_j, the fly-by estimate only for v2 > 0.
+  // tau_ff = dt/sqrt2 * sqrt(r^3 / M) and tau_fb = dt * r / |dv|, each divided
+  // by (1 - dtau/2) with dtau = its relative rate of change capped at 1.
+  static inline void master_tau(const real *xi, const real *vi, real qi, const real *xj,
+                                const real *vj, real qj, bool pair_mass, real dt, real &mi,
+                                real &ts) {
+    real dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
+    real dvx = vj[0] - vi[0], dvy = vj[1] - vi[1], dvz = vj[2] - vi[2];
+    real r2 = dx * dx + dy * dy + dz * dz;
+    if (!pair_mass) {
+      real v2 = dvx * dvx + dvy * dvy + dvz * dvz + 1e-20;
+      if (r2 > 0) mi += qj;
+      else r2 = 1e38;
+      real vdr = (dx * dvx + dy * dvy + dz * dvz) / r2;
+      real r = std::sqrt(r2);
+      real tau = dt / M_SQRT2 * r * std::sqrt(r / mi);
+      real dtau = 3 * tau * vdr / 2;
+      dtau = dtau < 1 ? dtau : 1;
+      tau /= (1 - dtau / 2);
+      ts = std::fmin(tau, ts);
+      tau = dt * r / std::sqrt(v2);
+      dtau = tau * vdr * (1 + mi / (v2 * r));
+      dtau = dtau < 1 ? dtau : 1;
+      tau /= (1 - dtau / 2);
+      ts = std::fmin(tau, ts);
+    } else {
+      if (!(r2 > 0)) return;
+      real r = std::sqrt(r2), v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+      real vdr = (dx * dvx + dy * dvy + dz * dvz) / r2, mm = qi + qj;
+      real tau = dt / M_SQRT2 * std::sqrt(r * r2 / mm);
+      real dtau = 3 * tau * vdr / 2;
+      if (dtau > 1) dtau = 1;
+      tau /= (1 - dtau / 2);
+      if (tau < ts) ts = tau;
+      if (v2 > 0) {
+        tau = dt * r / std::sqrt(v2);
+        dtau = tau * vdr * (1 + mm / (v2 * r));
+        if (dtau > 1) dtau = 1;
+        tau /= (1 - dtau / 2);
+        if (tau < ts) ts = tau;
+      }
+    }
+  }
+
+  // Master get_force_and_potential in the master Makefile's default build
+  // (FMM undefined, integrator.h:557-559): direct() (traverse_eager.h:349-370)
+  // builds the tree (getBounds without the root shift) and runs P2P_simple
+  // (kernel_fast_lean.h:268-460) or, with get_steps, P2P (:462-674) on every
+  // pair of leaves (directPass, :262-297).  Per sink body: p += sum m/r,
+  // F += sum m dx/r^3 over sources with r > 0; with get_steps,
+  // timestep += (min over the source leaf of tau)^-4 for every leaf.
+  void master_direct(bool steps, bool pair_mass, real dt) {
+    std::memset(&st, 0, sizeof(st));
+    for (auto &b : B) {
+      b.F[0] = b.F[1] = b.F[2] = 0;
+      b.J[0] = b.J[1] = b.J[2] = 0;
+      b.p = 0;
+      b.acc_old = 0;  // holds the timestep sum
+    }
+    build_tree();
+    std::vector<int> leaves;
+    for (int c = 0; c < (int)C.size(); c++)
+      if (C[c].NCHILD == 0) leaves.push_back(c);
+    int n = (int)B.size();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; i++) {
+      Body &bi = B[i];
+      if (!bi.issink) continue;
+      real ax = 0, ay = 0, az = 0, pot = 0, tsum = 0;
+      for (int c : leaves) {
+        real mi = bi.q, ts = 1e38;
+        for (int j = C[c].BODY; j < C[c].BODY + C[c].NBODY; j++) {
+          const Body &bj = B[j];
+          if (steps) master_tau(bi.X, bi.V, bi.q, bj.X, bj.V, bj.q, pair_mass, dt, mi, ts);
+          if (!bj.issource) continue;
+          real dx = bj.X[0] - bi.X[0], dy = bj.X[1] - bi.X[1], dz = bj.X[2] - bi.X[2];
+          real r2 = dx * dx + dy * dy + dz * dz;
+          if (!(r2 > 0)) continue;
+          real ir = 1 / std::sqrt(r2), f = bj.q * ir;
+          pot += f;
+          f *= ir * ir;
+          ax += dx * f, ay += dy * f, az += dz * f;
+        }
+        if (steps) {
+          ts *= ts;
+          ts *= ts;
+          tsum += 1 / ts;
+        }
+      }
+      bi.p = pot;
+      bi.F[0] = ax, bi.F[1] = ay, bi.F[2] = az;
+      bi.acc_old = tsum;
+    }
+  }
+
+  // master horizontalPassHigh (traverse_eager.h:61-149): the symmetric
+  // error-controlled walk.  f1 bounds the error of Ci's expansion at Cj, f2
+  // that of Cj's at Ci, both with R^-(P+3) and a 1/(Ra+Rb) factor; the pair
+  // is accepted when both are below 1.  Accepted pairs of tiny cells (2 and 8
+  // bodies) go to P2P; with steps, accepted pairs add nothing else.
+  void trav_master(int i, int j, bool steps) {
+    Cell &ci = C[i], &cj = C[j];
+    int P = prm.p;
+    real dx[3] = {ci.X[0] - cj.X[0], ci.X[1] - cj.X[1], ci.X[2] - cj.X[2]};
+    real R2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    real f1 = 10, f2 = 10;
+    real Ra = ci.R, Rb = cj.R, s = Ra + Rb;
+    if (R2 > s * s) {
+      real R = std::sqrt(R2), Rp = R2, pa = 1, pb = 1, Eab = 0, Eba = 0;
+      for (int k = 0; k <= P; k++) {
+        Eab += ci.Pn[P - k] * pb * tab.comb[k];
+        Eba += cj.Pn[P - k] * pa * tab.comb[k];
+        pa *= Ra;
+        pb *= Rb;
+        Rp *= R;
+      }
+      real fac = 8 * std::fmax(Ra, Rb) / s / Rp;
+      Eab *= fac;
+      f1 = Eab / (prm.force_accuracy * cj.min_acc);
+      Eba *= fac;
+      f2 = Eba / (prm.force_accuracy * ci.min_acc);
+    }
+    if (f2 < 1 && f1 < 1) {
+      if ((ci.NBODY <= 2 && cj.NBODY <= 8) || (cj.NBODY <= 2 && ci.NBODY <= 8))
+        p2p.push_back({i, j});
+      else if (!steps)
+        m2l.push_back({i, j});
+    } else if (ci.NCHILD == 0 && cj.NCHILD == 0) {
+      p2p.push_back({i, j});
+    } else if (cj.NCHILD == 0 || (ci.R >= cj.R && ci.NCHILD != 0)) {
+      for (int k = ci.CHILD; k < ci.CHILD + ci.NCHILD; k++) trav_master(k, j, steps);
+    } else {
+      for (int k = cj.CHILD; k < cj.CHILD + cj.NCHILD; k++) trav_master(i, k, steps);
+    }
+  }
+
+  // master P2P_simple / P2P of target cell i with source cell j
+  // (kernel_fast_lean.h:268-674): force and potential on the sink bodies and,
+  // with steps, one (min tau)^-4 term per body.
+  void master_p2p(const Cell &ci, const Cell &cj, bool steps, bool pair_mass, real dt) {
+    for (int a = ci.BODY; a < ci.BODY + ci.NBODY; a++) {
+      Body &bi = B[a];
+      if (!bi.issink) continue;
+      real ax = 0, ay = 0, az = 0, pot = 0, mi = bi.q, ts = 1e38;
+      for (int b = cj.BODY; b < cj.BODY + cj.NBODY; b++) {
+        const Body &bj = B[b];
+        if (steps) master_tau(bi.X, bi.V, bi.q, bj.X, bj.V, bj.q, pair_mass, dt, mi, ts);
+        if (!bj.issource) continue;
+        real dx = bj.X[0] - bi.X[0], dy = bj.X[1] - bi.X[1], dz = bj.X[2] - bi.X[2];
+        real r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 > 0)) continue;
+        real ir = 1 / std::sqrt(r2), f = bj.q * ir;
+        pot += f;
+        f *= ir * ir;
+        ax += dx * f, ay += dy * f, az += dz * f;
+      }
+      bi.p += pot;
+      bi.F[0] += ax, bi.F[1] += ay, bi.F[2] += az;
+      if (steps) {
+        ts *= ts;
+        ts *= ts;
+        bi.ts += 1 / ts;
+      }
+    }
+  }
+
+  // Master get_force_and_potential built with -DFMM (integrator.h:461-556):
+  // below ncrit bodies direct(); else the master p = 2 pre-pass (acc_old
+  // accumulates onto the caller's values), the upward pass, Pn, the symmetric
+  // walk, symmetric M2L_rotate (kernel_fast_lean.h:1034-1356: each accepted
+  // pair with Ci < Cj, in cell order, updates the locals of both cells; the
+  // pair in the other order does nothing), P2P and the downward pass.
+  void master_fmm(bool steps, bool pair_mass, real dt) {
+    if ((int)B.size() < prm.ncrit) {
+      for (auto &b : B) b.ts = 0;
+      master_direct(steps, pair_mass, dt);
+      for (auto &b : B) b.ts = b.acc_old, b.acc_old = 0;
+      return;
+    }
+    std::memset(&st, 0, sizeof(st));
+    for (auto &b : B) {
+      b.F[0] = b.F[1] = b.F[2] = 0;
+      b.J[0] = b.J[1] = b.J[2] = 0;
+      b.p = 0;
+      b.ts = 0;
+    }
+    build_tree();
+    low_pass(true);
+    upward();
+    pn();
+    m2l.clear();
+    p2p.clear();
+    trav_master(0, 0, steps);
+    std::vector<std::vector<int>> lm(C.size()), lp(C.size());
+    for (auto &e : m2l) {
+      int i = e.first, j = e.second;
+      if (i > j || (!C[i].has_sink && !C[j].has_sink)) continue;
+      lm[i].push_back(j);
+      lm[j].push_back(i);
+    }
+    for (auto &e : p2p) lp[e.first].push_back(e.second);
+    st.n_m2l = (long long)m2l.size();
+    st.n_p2p_cells = (long long)p2p.size();
+    long long pairs = 0;
+    for (auto &e : p2p) pairs += (long long)C[e.first].NBODY * C[e.second].NBODY;
+    st.n_p2p_pairs = pairs;
+    // P2P lists may hold a tiny non-leaf target whose bodies also belong to
+    // its descendants: run targets level by level so no body is written by
+    // two threads at once
+    std::vector<int> order(C.size());
+    for (size_t i = 0; i < C.size(); i++) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return C[a].level < C[b].level; });
+    size_t i0 = 0;
+    while (i0 < order.size()) {
+      size_t i1 = i0;
+      while (i1 < order.size() && C[order[i1]].level == C[order[i0]].level) i1++;
+#pragma omp parallel for schedule(dynamic, 1)
+      for (size_t k = i0; k < i1; k++) {
+        Cell &ci = C[order[k]];
+        for (int j : lm[order[k]]) m2l_rotate(ci, C[j], ci.L, ci.Ld);
+        if (!ci.has_sink) continue;
+        for (int j : lp[order[k]]) master_p2p(ci, C[j], steps, pair_mass, dt);
+      }
+      i0 = i1;
+    }
+    downward();
   }
 
   // get_force_and_potential (integrator.h:308-432) with the MAC chosen by prm.mac.
@@ -1214,6 +1476,77 @@ int orc_eval(void *h, int n, const double *x, const double *v, const double *m,
       if (jerk) jerk[3 * i + d] = b.J[d];
     }
     if (acc_old) acc_old[i] = b.acc_old;
+  }
+  if (st) std::memcpy(st, &F.st, sizeof(*st));
+  return 0;
+}
+
+// Master variant, default build: all-pairs force and potential over the leaf
+// pairs of the master tree (no root shift; the caller passes shift_frac = 0)
+// and, with steps, the per-body timestep sum.  pair_mass selects the scalar
+// form of the timestep estimate.  Outputs in the caller's order.
+int orc_master_direct(void *h, int n, const double *x, const double *v, const double *m,
+                      const unsigned char *issink, const unsigned char *issource, int steps,
+                      int pair_mass, double dt_param, double *pot, double *acc,
+                      double *timestep) {
+  orc::Fmm &F = *(orc::Fmm *)h;
+  if (n <= 0) return 1;
+  if (F.prm.nthreads > 0) omp_set_num_threads(F.prm.nthreads);
+  F.B.resize(n);
+  for (int i = 0; i < n; i++) {
+    orc::Body &b = F.B[i];
+    for (int d = 0; d < 3; d++) {
+      b.X[d] = x[3 * i + d];
+      b.V[d] = v[3 * i + d];
+    }
+    b.q = m[i];
+    b.index = i;
+    b.issink = issink ? issink[i] != 0 : true;
+    b.issource = issource ? issource[i] != 0 : true;
+  }
+  F.master_direct(steps != 0, pair_mass != 0, dt_param);
+  for (auto &b : F.B) {
+    int i = b.index;
+    if (pot) pot[i] = b.p;
+    for (int d = 0; d < 3; d++)
+      if (acc) acc[3 * i + d] = b.F[d];
+    if (timestep) timestep[i] = b.acc_old;
+  }
+  return 0;
+}
+
+// Master variant built with -DFMM: acc_old [n] is read (the caller's
+// values, onto which the pre-pass adds) and written; outputs in the caller's
+// order.  Uses the handle's p, ncrit and force_accuracy; shift_frac must be 0.
+int orc_master_fmm(void *h, int n, const double *x, const double *v, const double *m,
+                   const unsigned char *issink, const unsigned char *issource, int steps,
+                   int pair_mass, double dt_param, double *acc_old, double *pot, double *acc,
+                   double *timestep, orc_stats *st) {
+  orc::Fmm &F = *(orc::Fmm *)h;
+  if (n <= 0) return 1;
+  if (F.prm.nthreads > 0) omp_set_num_threads(F.prm.nthreads);
+  F.B.resize(n);
+  for (int i = 0; i < n; i++) {
+    orc::Body &b = F.B[i];
+    for (int d = 0; d < 3; d++) {
+      b.X[d] = x[3 * i + d];
+      b.V[d] = v[3 * i + d];
+    }
+    b.q = m[i];
+    b.index = i;
+    b.issink = issink ? issink[i] != 0 : true;
+    b.issource = issource ? issource[i] != 0 : true;
+    b.acc_old = acc_old ? acc_old[i] : 0;
+  }
+  const bool direct = n < F.prm.ncrit;
+  F.master_fmm(steps != 0, pair_mass != 0, dt_param);
+  for (auto &b : F.B) {
+    int i = b.index;
+    if (pot) pot[i] = b.p;
+    for (int d = 0; d < 3; d++)
+      if (acc) acc[3 * i + d] = b.F[d];
+    if (timestep) timestep[i] = b.ts;
+    if (acc_old && !direct) acc_old[i] = b.acc_old;
   }
   if (st) std::memcpy(st, &F.st, sizeof(*st));
   return 0;

@@ -33,6 +33,21 @@ body_dtype = np.dtype({
     "itemsize": 152,
 })
 
+# struct Body of the reference's master variant (exafmm.h:21-48), 128 bytes.
+master_body_dtype = np.dtype({
+    "names": ["X", "q", "F", "p", "V", "acc_old", "Morton", "timestep", "index", "issink",
+              "issource"],
+    "formats": [("<f8", 3), "<f8", ("<f8", 3), "<f8", ("<f8", 3), "<f8", ("<u4", 3), "<f8",
+                "<i4", "u1", "u1"],
+    "offsets": [0, 24, 32, 56, 64, 88, 96, 112, 120, 124, 125],
+    "itemsize": 128,
+})
+
+# mode bits of tc_master_eval_* (include/tcfmm.h)
+TC_MASTER_STEPS = 1
+TC_MASTER_FMM = 2
+TC_MASTER_PAIR_MASS = 4
+
 
 class TcConfig(C.Structure):
     _fields_ = [("p", C.c_int), ("ncrit", C.c_int), ("theta", C.c_double), ("mac", C.c_int),
@@ -88,6 +103,9 @@ def lib():
         L.tc_last_error.argtypes = [vp]
         L.tc_last_error.restype = C.c_char_p
         L.tc_supported_orders.argtypes = [C.POINTER(C.c_int), C.c_int]
+        L.tc_master_eval_bodies.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double]
+        L.tc_master_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_double, vp,
+                                         vp, vp]
         _lib = L
     return _lib
 
@@ -95,7 +113,7 @@ def lib():
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_acc_old",
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
-                    "tc_P2P", "tc_get_stats",
+                    "tc_P2P", "tc_master_eval_bodies", "tc_master_eval_soa", "tc_get_stats",
                     "tc_last_error", "tc_supported_orders"]
 
 
@@ -218,6 +236,36 @@ class Fmm:
                                         _ptr(flags), nt, _ptr(tg), _ptr(pot), _ptr(acc),
                                         _ptr(jerk)), "tc_direct_soa")
         return pot, acc, jerk
+
+    # ---- master (non-jerk) variant ----------------------------------------
+    def master_get_force_and_potential(self, bodies, get_steps=False, dt_param=0.025, mode=0):
+        """Drop-in for the master variant's get_force_and_potential(bodies, get_steps)
+        on a host array of master_body_dtype records (F, p and, with get_steps,
+        timestep are overwritten)."""
+        if bodies.dtype != master_body_dtype or not bodies.flags.c_contiguous:
+            raise TypeError("bodies must be a C-contiguous array of master_body_dtype")
+        mode = int(mode) | (TC_MASTER_STEPS if get_steps else 0)
+        self._check(lib().tc_master_eval_bodies(self.h, bodies.ctypes.data, bodies.size, mode,
+                                                float(dt_param)), "tc_master_eval_bodies")
+        return bodies
+
+    def master_eval(self, x, v, m, flags=None, get_steps=False, dt_param=0.025, mode=0):
+        """Device tensors in, (pot, acc, timestep-or-None) cuda float64 tensors out."""
+        import torch
+        n = m.shape[0]
+        dev = x.device
+        for t in (x, v, m):
+            if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
+                raise TypeError("x, v, m must be contiguous float64 cuda tensors")
+        mode = int(mode) | (TC_MASTER_STEPS if get_steps else 0)
+        pot = torch.empty(n, dtype=torch.float64, device=dev)
+        acc = torch.empty(n, 3, dtype=torch.float64, device=dev)
+        ts = torch.empty(n, dtype=torch.float64, device=dev) if get_steps else None
+        self._last_n = n
+        self._check(lib().tc_master_eval_soa(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(flags),
+                                             mode, float(dt_param), _ptr(pot), _ptr(acc),
+                                             _ptr(ts)), "tc_master_eval_soa")
+        return pot, acc, ts
 
     # ---- per-kernel entry points over the reference's Cell / Body records ----
     def _recs(self, bodies, cells):

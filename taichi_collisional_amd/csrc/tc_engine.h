@@ -125,6 +125,8 @@ struct tc_context {
   // host Body records: page-locked staging and device result buffers
   HBuf<double> h_stage;
   DBuf hop, hoa, hoj;
+  // master variant: last-body-of-leaf flags, time-step sums
+  DBuf mlend, mts;
   hipEvent_t ev[9];
 };
 

@@ -351,7 +351,6 @@ static int low_pass(tc_context *ctx, int n) {
   return 0;
 }
 
-// acc_old of the last evaluation into out[n] in the caller's body order.
 // Split [0, n) into contiguous slices over a few host threads.  Packing and
 // unpacking the 152-byte Body records is bound by host memory bandwidth,
 // which one core does not reach.
@@ -374,6 +373,7 @@ static void host_parallel(int n, F f) {
   for (auto &t : th) t.join();
 }
 
+// acc_old of the last evaluation into out[n] in the caller's body order.
 static int copy_acc_old(tc_context *ctx, int n, double *out) {
   if (!ctx->has_acc_old || n != ctx->last_n) {
     ctx->err = "acc_old is only computed by an error-MAC evaluation (TC_MAC_ERROR)";
@@ -381,6 +381,25 @@ static int copy_acc_old(tc_context *ctx, int n, double *out) {
   }
   launch_scatter_acc_old(bodies_of(ctx), n, out, ctx->stream);
   return 0;
+}
+
+// Bounds, keys, sort, gather into tree order, cells (buildTree,
+// build_tree.h:239-487); the root shift is whatever ctx->dshift holds.
+static int sort_and_build(tc_context *ctx, int n, const double *x, const double *v,
+                          const double *m, const unsigned char *flags) {
+  hipStream_t s = ctx->stream;
+  launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
+                ctx->dshift.as<double>(), s);
+  launch_morton(x, n, ctx->tp.as<TreeParams>(), ctx->key_hi.as<uint64_t>(),
+                ctx->key_a.as<uint64_t>(), ctx->val_a.as<int>(), s);
+  if (launch_sort(ctx->key_a.as<uint64_t>(), ctx->key_b.as<uint64_t>(), ctx->val_a.as<int>(),
+                  ctx->val_b.as<int>(), ctx->key_hi.as<uint64_t>(), n, ctx->sort_tmp.p,
+                  ctx->sort_tmp.bytes, s)) {
+    ctx->err = "radix sort failed";
+    return -4;
+  }
+  launch_gather(x, v, m, flags, ctx->val_a.as<int>(), n, bodies_of(ctx), s);
+  return build_tree(ctx, n);
 }
 
 static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
@@ -440,18 +459,7 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   }
 
   // ---- tree
-  launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
-                ctx->dshift.as<double>(), s);
-  launch_morton(x, n, ctx->tp.as<TreeParams>(), ctx->key_hi.as<uint64_t>(),
-                ctx->key_a.as<uint64_t>(), ctx->val_a.as<int>(), s);
-  if (launch_sort(ctx->key_a.as<uint64_t>(), ctx->key_b.as<uint64_t>(), ctx->val_a.as<int>(),
-                  ctx->val_b.as<int>(), ctx->key_hi.as<uint64_t>(), n, ctx->sort_tmp.p,
-                  ctx->sort_tmp.bytes, s)) {
-    ctx->err = "radix sort failed";
-    return -4;
-  }
-  launch_gather(x, v, m, flags, ctx->val_a.as<int>(), n, bodies_of(ctx), s);
-  if (int e = build_tree(ctx, n)) return e;
+  if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
   const bool err_mac = cfg.mac == TC_MAC_ERROR;
   if (err_mac) {
     TC_CHECK(hipEventRecord(ctx->ev[8], s));
@@ -541,6 +549,66 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   return 0;
 }
 
+// ---- master (non-jerk) variant -------------------------------------------
+// get_force_and_potential of the reference's top-level sources (master
+// integrator.h:459-562).  The tree is the master one: getBounds there has no
+// root shift (master build_tree.h), so the shift is zero here.
+static int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
+                              const double *m, const unsigned char *flags, int mode, double dt,
+                              double *pot, double *acc, double *tstep) {
+  hipStream_t s = ctx->stream;
+  std::memset(&ctx->st, 0, sizeof(ctx->st));
+  ctx->has_acc_old = false;
+  ctx->last_n = n;
+  const bool steps = mode & TC_MASTER_STEPS, pair_mass = mode & TC_MASTER_PAIR_MASS;
+  if (n <= 0 || !x || !v || !m || !pot || !acc) {
+    ctx->err = "n > 0 and x, v, m, pot, acc are required";
+    return -1;
+  }
+  if (mode & ~(TC_MASTER_STEPS | TC_MASTER_FMM | TC_MASTER_PAIR_MASS)) {
+    ctx->err = "unknown mode bits";
+    return -1;
+  }
+  if (steps && (!tstep || !(dt > 0))) {
+    ctx->err = "TC_MASTER_STEPS needs a timestep array and dt_param > 0";
+    return -1;
+  }
+  if ((mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit) {
+    ctx->err = "TC_MASTER_FMM above ncrit bodies is not built; the master default build "
+               "(direct) is";
+    return -2;
+  }
+  if (int e = ensure_bodies(ctx, n)) return e;
+  const double zero[3] = {0, 0, 0};
+  std::memcpy(ctx->shift, zero, sizeof(zero));
+  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+  TC_CHECK(hipEventRecord(ctx->ev[0], s));
+  if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
+  TC_CHECK(ctx->mlend.ensure((size_t)n));
+  launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
+                   ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
+  TC_CHECK(hipEventRecord(ctx->ev[1], s));
+  launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, steps, pair_mass, dt,
+                       pot, acc, tstep, s);
+  TC_CHECK(hipEventRecord(ctx->ev[2], s));
+  TC_CHECK(hipStreamSynchronize(s));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
+    return -3;
+  }
+  float ms[3];
+  hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[1]);
+  hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]);
+  hipEventElapsedTime(&ms[2], ctx->ev[0], ctx->ev[2]);
+  ctx->st.ms_tree = ms[0];
+  ctx->st.ms_p2p = ms[1];
+  ctx->st.ms_total = ms[2];
+  ctx->st.n_p2p_cells = (long long)ctx->nleaves * ctx->nleaves;
+  ctx->st.n_p2p_pairs = (long long)n * n;
+  return 0;
+}
+
 // ------------------------------------------------------------------ C ABI ---
 extern "C" {
 
@@ -603,7 +671,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
-                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj,
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
@@ -719,6 +787,69 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
       b.J[1] = rj[3 * i + 1];
       b.J[2] = rj[3 * i + 2];
       if (want_acc) b.acc_old = ro[i];
+    }
+  });
+  return 0;
+}
+
+int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                       const unsigned char *flags, int mode, double dt_param, double *pot,
+                       double *acc, double *timestep) {
+  if (!ctx) return -1;
+  return master_eval_device(ctx, n, x, v, m, flags, mode, dt_param, pot, acc, timestep);
+}
+
+int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mode,
+                          double dt_param) {
+  if (!ctx) return -1;
+  if (!bodies || n <= 0) {
+    ctx->err = "bodies must be non-empty";
+    return -1;
+  }
+  hipStream_t s = ctx->stream;
+  const size_t N = (size_t)n;
+  const bool steps = mode & TC_MASTER_STEPS;
+  TC_CHECK(ctx->h_stage.ensure(8 * N + (N + 7) / 8));
+  double *hx = ctx->h_stage.p, *hv = hx + 3 * N, *hm = hv + 3 * N;
+  unsigned char *hf = (unsigned char *)(hm + N);
+  host_parallel(n, [&](int lo, int hi) {
+    for (int i = lo; i < hi; i++) {
+      const tc_master_body &b = bodies[i];
+      for (int d = 0; d < 3; d++) {
+        hx[3 * i + d] = b.X[d];
+        hv[3 * i + d] = b.V[d];
+      }
+      hm[i] = b.q;
+      hf[i] = (b.issink ? kSink : 0) | (b.issource ? kSource : 0);
+    }
+  });
+  TC_CHECK(ctx->dx3.ensure(24 * N));
+  TC_CHECK(ctx->dv3.ensure(24 * N));
+  TC_CHECK(ctx->dm.ensure(8 * N));
+  TC_CHECK(ctx->dfl.ensure(N));
+  TC_CHECK(ctx->hop.ensure(8 * N));
+  TC_CHECK(ctx->hoa.ensure(24 * N));
+  TC_CHECK(ctx->mts.ensure(8 * N));
+  TC_CHECK(hipMemcpyAsync(ctx->dx3.p, hx, 24 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dv3.p, hv, 24 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dm.p, hm, 8 * N, hipMemcpyHostToDevice, s));
+  TC_CHECK(hipMemcpyAsync(ctx->dfl.p, hf, N, hipMemcpyHostToDevice, s));
+  double *op = ctx->hop.as<double>(), *oa = ctx->hoa.as<double>(), *ot = ctx->mts.as<double>();
+  int e = master_eval_device(ctx, n, ctx->dx3.as<double>(), ctx->dv3.as<double>(),
+                             ctx->dm.as<double>(), ctx->dfl.as<unsigned char>(), mode, dt_param,
+                             op, oa, ot);
+  if (e != 0) return e;
+  double *rp = ctx->h_stage.p, *ra = rp + N, *rt = ra + 3 * N;
+  TC_CHECK(hipMemcpyAsync(rp, op, 8 * N, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(ra, oa, 24 * N, hipMemcpyDeviceToHost, s));
+  if (steps) TC_CHECK(hipMemcpyAsync(rt, ot, 8 * N, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamSynchronize(s));
+  host_parallel(n, [&](int lo, int hi) {
+    for (int i = lo; i < hi; i++) {
+      tc_master_body &b = bodies[i];
+      b.p = rp[i];
+      for (int d = 0; d < 3; d++) b.F[d] = ra[3 * i + d];
+      if (steps) b.timestep = rt[i];
     }
   });
   return 0;

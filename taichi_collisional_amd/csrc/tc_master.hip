@@ -1,0 +1,183 @@
+// Master (non-jerk) variant of the force evaluation: what the reference's
+// top-level sources run by default (master Makefile:4 leaves FMM undefined,
+// so get_force_and_potential is direct(), integrator.h:557-559 and
+// traverse_eager.h:349-370).  File and line citations below refer to those
+// top-level sources.
+//
+// direct() visits every pair of leaves of the tree (directPass,
+// traverse_eager.h:262-297) and runs P2P_simple (kernel_fast_lean.h:268-460)
+// or, when the caller asks for time steps, P2P (:462-674) on it.  Summed over
+// the leaf pairs that is, per sink body i:
+//   p_i += sum_j q_j / r_ij,  F_i += sum_j q_j (x_j - x_i) / r_ij^3
+// over sources j with r_ij > 0, and, with time steps,
+//   timestep_i += sum over leaves J of (min_{j in J} tau_ij)^-4
+// where tau_ij is the smaller of a free-fall and a fly-by estimate (see
+// pair_tau).  The leaf structure matters only for the time steps: the minimum
+// is taken per source leaf, and in the vector form the mass in the free-fall
+// estimate accumulates along the leaf.
+//
+// Layout: one thread per target body in tree (sorted) order, so each block's
+// targets are neighbours; sources stream through LDS in tiles of kMT bodies in
+// the same sorted order, with a flag byte per source marking "is a source" and
+// "last body of its leaf".  Results are written at the caller's index.
+#include <hip/hip_runtime.h>
+
+#include "tc_common.h"
+#include "tc_kernels.h"
+
+namespace tcfmm {
+
+constexpr int kMT = 256;
+
+__global__ void k_fill_u8(unsigned char *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// lend[last body of every leaf] = 1
+__global__ void k_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody,
+                            const int *cnbody, unsigned char *lend) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int c = leaf_ids[k];
+  if (cnbody[c] > 0) lend[cbody[c] + cnbody[c] - 1] = 1;
+}
+
+// Fold one (target, source) pair into the running minimum ts of the current
+// source leaf.
+//  kPairMass = false: the vector P2P of the master build (SIMD_P2P,
+//    kernel_fast_lean.h:474-592): mi, reset to q_i at every source leaf,
+//    accumulates every source met with r2 > 0 (mi += mj); r2 = 0 is replaced by
+//    1e38; v2 carries +1e-20; both estimates are always taken.
+//  kPairMass = true: the scalar P2P (:593-672): r2 = 0 skipped, pair mass
+//    q_i + q_j, the fly-by estimate only for v2 > 0.
+// tau_ff = dt/sqrt(2) r sqrt(r / M), tau_fb = dt r / |dv|, each divided by
+// (1 - dtau/2) with its rate term dtau capped at 1.
+template <bool kPairMass>
+__device__ __forceinline__ void pair_tau(double dx, double dy, double dz, double dvx, double dvy,
+                                         double dvz, double r2, double qi, double qj, double dt,
+                                         double &mi, double &ts) {
+  const double dts2 = dt / M_SQRT2;
+  if (!kPairMass) {
+    const double v2 = dvx * dvx + dvy * dvy + dvz * dvz + 1e-20;
+    const bool pos = r2 > 0;
+    mi += pos ? qj : 0.0;
+    const double rr = pos ? r2 : 1e38;
+    const double vdr = (dx * dvx + dy * dvy + dz * dvz) / rr;
+    const double r = sqrt(rr);
+    double tau = dts2 * r * sqrt(r / mi);
+    double dtau = 3.0 * tau * vdr / 2.0;
+    dtau = dtau < 1.0 ? dtau : 1.0;
+    tau /= (1.0 - dtau / 2.0);
+    ts = fmin(tau, ts);
+    tau = dt * r / sqrt(v2);
+    dtau = tau * vdr * (1.0 + mi / (v2 * r));
+    dtau = dtau < 1.0 ? dtau : 1.0;
+    tau /= (1.0 - dtau / 2.0);
+    ts = fmin(tau, ts);
+  } else {
+    if (!(r2 > 0)) return;
+    const double r = sqrt(r2), v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const double vdr = (dx * dvx + dy * dvy + dz * dvz) / r2, mm = qi + qj;
+    double tau = dts2 * sqrt(r * r2 / mm);
+    double dtau = 3.0 * tau * vdr / 2.0;
+    if (dtau > 1.0) dtau = 1.0;
+    tau /= (1.0 - dtau / 2.0);
+    if (tau < ts) ts = tau;
+    if (v2 > 0) {
+      tau = dt * r / sqrt(v2);
+      dtau = tau * vdr * (1.0 + mm / (v2 * r));
+      if (dtau > 1.0) dtau = 1.0;
+      tau /= (1.0 - dtau / 2.0);
+      if (tau < ts) ts = tau;
+    }
+  }
+}
+
+template <bool kSteps, bool kPairMass>
+__global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsigned char *lend,
+                                                       int n, double dt, double *pot, double *acc,
+                                                       double *tstep) {
+  __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT], svx[kMT], svy[kMT], svz[kMT];
+  __shared__ unsigned char sf[kMT];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kMT + t;
+  const bool live = i < n;
+  // lanes past n run the loop on a harmless stand-in body and write nothing
+  const double xi = live ? b.x[i] : 0.0, yi = live ? b.y[i] : 0.0, zi = live ? b.z[i] : 0.0;
+  const double vxi = live ? b.vx[i] : 0.0, vyi = live ? b.vy[i] : 0.0,
+               vzi = live ? b.vz[i] : 0.0;
+  const double qi = live ? b.q[i] : 1.0;
+  double ax = 0, ay = 0, az = 0, pt = 0, tsum = 0, mi = qi, ts = 1e38;
+  for (int base = 0; base < n; base += kMT) {
+    __syncthreads();
+    const int j = base + t;
+    if (j < n) {
+      sx[t] = b.x[j];
+      sy[t] = b.y[j];
+      sz[t] = b.z[j];
+      sq[t] = b.q[j];
+      if (kSteps) {
+        svx[t] = b.vx[j];
+        svy[t] = b.vy[j];
+        svz[t] = b.vz[j];
+      }
+      sf[t] = ((b.flags[j] & kSource) ? 1 : 0) | (lend[j] ? 2 : 0);
+    }
+    __syncthreads();
+    const int cnt = min(kMT, n - base);
+    for (int k = 0; k < cnt; k++) {
+      const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const unsigned char f = sf[k];
+      if (kSteps)
+        pair_tau<kPairMass>(dx, dy, dz, svx[k] - vxi, svy[k] - vyi, svz[k] - vzi, r2, qi, sq[k],
+                            dt, mi, ts);
+      // P2P_simple / P2P force: sources only, r2 = 0 masked
+      const bool use = (f & 1) && r2 > 0;
+      const double ir = rsqrt_fp64(use ? r2 : 1.0);
+      const double w = use ? sq[k] * ir : 0.0;
+      pt += w;
+      const double w3 = w * ir * ir;
+      ax += dx * w3;
+      ay += dy * w3;
+      az += dz * w3;
+      if (kSteps && (f & 2)) {
+        double t4 = ts * ts;
+        t4 *= t4;
+        tsum += 1.0 / t4;
+        ts = 1e38;
+        mi = qi;
+      }
+    }
+  }
+  if (!live) return;
+  const bool sink = b.flags[i] & kSink;
+  const int o = b.order[i];
+  pot[o] = sink ? pt : 0.0;
+  acc[3 * o] = sink ? ax : 0.0;
+  acc[3 * o + 1] = sink ? ay : 0.0;
+  acc[3 * o + 2] = sink ? az : 0.0;
+  if (kSteps) tstep[o] = sink ? tsum : 0.0;
+}
+
+void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
+                      int n, unsigned char *lend, hipStream_t s) {
+  k_fill_u8<<<(n + 255) / 256, 256, 0, s>>>(lend, n);
+  if (nleaves > 0)
+    k_leaf_ends<<<(nleaves + 255) / 256, 256, 0, s>>>(leaf_ids, nleaves, cbody, cnbody, lend);
+}
+
+void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
+                          bool pair_mass, double dt, double *pot, double *acc, double *tstep,
+                          hipStream_t s) {
+  const int nb = (n + kMT - 1) / kMT;
+  if (!steps)
+    k_master_direct<false, false><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+  else if (!pair_mass)
+    k_master_direct<true, false><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+  else
+    k_master_direct<true, true><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+}
+
+}  // namespace tcfmm

@@ -20,17 +20,6 @@ struct Acc7 {
   double p, ax, ay, az, jx, jy, jz;
 };
 
-// 1/sqrt(x) for x > 0 in FP64 without a branch: the hardware estimate
-// y0 = v_rsq_f64(x), then one second-order Newton-Householder step on
-// f(y) = 1/y^2 - x:  e = 1 - x y0^2,  y = y0 + y0 e (1/2 + 3/8 e), which
-// takes the estimate's relative error eps to ~(5/16) eps^3.  x = 0 gives a
-// non-finite value that the caller discards.
-__device__ __forceinline__ double rsqrt_fp64(double x) {
-  const double y0 = __builtin_amdgcn_rsq(x);
-  const double e = fma(-x * y0, y0, 1.0);
-  return fma(y0 * e, fma(e, 0.375, 0.5), y0);
-}
-
 __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double vxi, double vyi,
                                           double vzi, double xj, double yj, double zj, double vxj,
                                           double vyj, double vzj, double mj, Acc7 &a) {

@@ -1,0 +1,173 @@
+"""GPU parity of the master (non-jerk) variant through the C ABI: the master
+default build's get_force_and_potential (direct over leaf pairs, with and
+without the time-step estimate) against the oracle on the same inputs and
+against the compiled master reference's outputs (golden fixtures).
+
+Tolerances (FP64): forces and potentials are the same sums in another order
+(max relative error per body); the time-step sum takes a minimum per source
+leaf and the fourth power of it, so its relative error is about four times
+that of one estimate."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+
+sys.path.insert(0, GOLDEN)
+import make_master_golden as mmg  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+POT_TOL = 1e-12
+ACC_TOL = 1e-11
+TS_TOL = 1e-11
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _gpu(torch, x, v, m, sink=None, src=None, ncrit=64, steps=True, dt=0.025, mode=0):
+    import taichi_collisional_amd as tca
+    f = tca.Fmm(p=8, ncrit=ncrit, direct_below=0)
+    dx, dv, dm = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, v, m))
+    fl = None
+    if sink is not None or src is not None:
+        n = m.shape[0]
+        s1 = np.ones(n, np.uint8) if sink is None else sink
+        s2 = np.ones(n, np.uint8) if src is None else src
+        fl = torch.from_numpy((s1 | (s2 << 1)).astype(np.uint8)).cuda()
+    pot, acc, ts = f.master_eval(dx, dv, dm, flags=fl, get_steps=steps, dt_param=dt, mode=mode)
+    torch.cuda.synchronize()
+    return (pot.cpu().numpy(), acc.cpu().numpy(), None if ts is None else ts.cpu().numpy(),
+            f.stats())
+
+
+def _rel(a, b, mask):
+    a, b = a[mask], b[mask]
+    if a.ndim == 2:
+        return float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def _check(got, want, live, steps, what):
+    gp, ga, gt = got
+    wp, wa, wt = want
+    assert np.all(gp[~live] == 0) and np.all(ga[~live] == 0)
+    ep, ea = _rel(gp, wp, live), _rel(ga, wa, live)
+    et = _rel(gt, wt, live) if steps else 0.0
+    print("%s pot %.2e acc %.2e ts %.2e" % (what, ep, ea, et))
+    assert ep < POT_TOL and ea < ACC_TOL and et < TS_TOL, (what, ep, ea, et)
+    if steps:
+        assert np.all(gt[~live] == 0)
+
+
+@pytest.mark.parametrize("k", range(len(mmg.CASES)))
+def test_gpu_master_matches_oracle_and_reference(torch_cuda, oracle_lib, k):
+    g = golden("master_direct.npz")
+    c = mmg.CASES[k]
+    x, v, m, sink, src = mmg.inputs(c)
+    assert mmg.sha(x, v, m, sink, src) == str(g["case%d_sha" % k])
+    gp, ga, gt, st = _gpu(torch_cuda, x, v, m, sink, src, ncrit=c["ncrit"], steps=c["steps"],
+                          dt=c["dt"])
+    o = oracle_lib.Oracle(8, ncrit=c["ncrit"])
+    op, oa, ot = o.master_direct(x, v, m, sink, src, steps=c["steps"], dt_param=c["dt"])
+    live = sink.astype(bool)
+    _check((gp, ga, gt), (op, oa, ot), live, c["steps"], "case %d vs oracle" % k)
+    _check((gp, ga, gt), (g["case%d_pot" % k], g["case%d_acc" % k], g["case%d_ts" % k]), live,
+           c["steps"], "case %d vs reference" % k)
+
+
+def test_gpu_master_pair_mass_form(torch_cuda, oracle_lib):
+    """Scalar form: against the oracle with several leaves, and with one leaf
+    against the integrator's own all-pairs step (kick_cpu_with_steps)."""
+    import taichi_collisional_amd as tca
+    g = golden("master_direct.npz")
+    x, v, m = mmg.ics.plummer(600, seed=15)
+    assert mmg.sha(x, v, m) == str(g["kick_sha"])
+    _, _, gt, _ = _gpu(torch_cuda, x, v, m, ncrit=1000, steps=True, dt=float(g["kick_dt"]),
+                       mode=tca.TC_MASTER_PAIR_MASS)
+    np.testing.assert_allclose(gt ** -0.25, g["kick_ts"], rtol=1e-12)
+    x, v, m = mmg.ics.plummer(3000, seed=21)
+    gp, ga, gt, _ = _gpu(torch_cuda, x, v, m, ncrit=40, steps=True, mode=tca.TC_MASTER_PAIR_MASS)
+    o = oracle_lib.Oracle(8, ncrit=40)
+    op, oa, ot = o.master_direct(x, v, m, steps=True, pair_mass=True)
+    _check((gp, ga, gt), (op, oa, ot), np.ones(3000, bool), True, "pair mass")
+
+
+def test_gpu_master_body_records(torch_cuda):
+    """The host Body-record drop-in equals the device-array form."""
+    import taichi_collisional_amd as tca
+    x, v, m = mmg.ics.plummer(4000, seed=22)
+    rng = np.random.default_rng(22)
+    sink = (rng.random(4000) < 0.9).astype(np.uint8)
+    src = (rng.random(4000) < 0.9).astype(np.uint8)
+    gp, ga, gt, _ = _gpu(torch_cuda, x, v, m, sink, src, ncrit=100, steps=True)
+    b = np.zeros(4000, tca.master_body_dtype)
+    b["X"], b["V"], b["q"], b["issink"], b["issource"] = x, v, m, sink, src
+    b["index"] = np.arange(4000)
+    b["acc_old"] = 7.0
+    f = tca.Fmm(p=8, ncrit=100, direct_below=0)
+    f.master_get_force_and_potential(b, get_steps=True, dt_param=0.025)
+    np.testing.assert_array_equal(b["p"], gp)
+    np.testing.assert_array_equal(b["F"], ga)
+    np.testing.assert_array_equal(b["timestep"], gt)
+    assert np.all(b["acc_old"] == 7.0) and np.array_equal(b["index"], np.arange(4000))
+    np.testing.assert_array_equal(b["X"], x)
+
+
+def test_gpu_master_full_size_forces(torch_cuda):
+    """N = 2^16: forces equal the FP64 all-pairs sum of tc_direct_soa."""
+    import taichi_collisional_amd as tca
+    n = 1 << 16
+    x, v, m = mmg.ics.plummer(n, seed=23)
+    gp, ga, gt, st = _gpu(torch_cuda, x, v, m, ncrit=300, steps=True)
+    assert np.all(np.isfinite(gt)) and np.all(gt > 0)
+    f = tca.Fmm(p=8, direct_below=0)
+    t = torch_cuda
+    dx, dv, dm = (t.from_numpy(a).cuda() for a in (x, v, m))
+    tg = t.arange(0, n, 7, dtype=t.int32)
+    dp, da, _ = f.direct(dx, dv, dm, tg)
+    idx = tg.numpy()
+    assert _rel(gp[idx], dp.cpu().numpy(), np.ones(idx.size, bool)) < POT_TOL
+    assert _rel(ga[idx], da.cpu().numpy(), np.ones(idx.size, bool)) < ACC_TOL
+    assert st["nleaves"] > 200
+
+
+def test_gpu_master_edges_and_errors(torch_cuda, oracle_lib):
+    import taichi_collisional_amd as tca
+    # one and two bodies
+    x = np.array([[0.1, 0.2, 0.3]])
+    v = np.array([[0.0, 1.0, 0.0]])
+    gp, ga, gt, _ = _gpu(torch_cuda, x, v, np.array([1.0]), steps=True)
+    assert gp[0] == 0 and np.all(ga == 0) and gt[0] < 1e-100
+    x2 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    v2 = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.0]])
+    m2 = np.array([1.0, 2.0])
+    gp, ga, gt, _ = _gpu(torch_cuda, x2, v2, m2, steps=True)
+    o = oracle_lib.Oracle(8, ncrit=64)
+    op, oa, ot = o.master_direct(x2, v2, m2, steps=True)
+    np.testing.assert_allclose(gp, [2.0, 1.0], rtol=1e-15)
+    np.testing.assert_allclose(ga[:, 0], [2.0, -1.0], rtol=1e-15)
+    np.testing.assert_allclose(gt, ot, rtol=1e-13)
+    # errors come back as exceptions, not faults
+    f = tca.Fmm(p=8, ncrit=64, direct_below=0)
+    t = torch_cuda
+    dx, dv, dm = (t.from_numpy(a).cuda() for a in mmg.ics.plummer(500, seed=1))
+    with pytest.raises(RuntimeError, match="dt_param"):
+        f.master_eval(dx, dv, dm, get_steps=True, dt_param=0.0)
+    with pytest.raises(RuntimeError, match="not built"):
+        f.master_eval(dx, dv, dm, mode=tca.TC_MASTER_FMM)
+    with pytest.raises(RuntimeError, match="mode"):
+        f.master_eval(dx, dv, dm, mode=64)
+    # below ncrit the FMM build runs direct(), as the reference does
+    f2 = tca.Fmm(p=8, ncrit=600, direct_below=0)
+    p1, a1, t1 = f2.master_eval(dx, dv, dm, get_steps=True, mode=tca.TC_MASTER_FMM)
+    p0, a0, t0 = f2.master_eval(dx, dv, dm, get_steps=True)
+    assert t.equal(p1, p0) and t.equal(a1, a0) and t.equal(t1, t0)

@@ -1,0 +1,100 @@
+"""Pinning of the oracle's master (non-jerk) variant: the master default
+build's get_force_and_potential (direct over leaf pairs, P2P_simple / P2P with
+the time-step estimate) against the compiled master reference's outputs
+(golden fixtures), the integrator's own all-pairs time step, and brute force."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+
+sys.path.insert(0, GOLDEN)
+import make_master_golden as mmg  # noqa: E402
+
+
+def _case(g, k):
+    c = mmg.CASES[k]
+    x, v, m, sink, src = mmg.inputs(c)
+    assert mmg.sha(x, v, m, sink, src) == str(g["case%d_sha" % k]), "IC generator changed"
+    return c, x, v, m, sink, src
+
+
+def _rel(a, b, mask):
+    a, b = a[mask], b[mask]
+    if a.ndim == 2:
+        return float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("k", range(len(mmg.CASES)))
+def test_master_oracle_matches_reference(oracle_lib, k):
+    g = golden("master_direct.npz")
+    c, x, v, m, sink, src = _case(g, k)
+    o = oracle_lib.Oracle(8, ncrit=c["ncrit"])
+    pot, acc, ts = o.master_direct(x, v, m, sink, src, steps=c["steps"], dt_param=c["dt"])
+    live = sink.astype(bool)
+    assert np.all(pot[~live] == 0) and np.all(g["case%d_pot" % k][~live] == 0)
+    assert _rel(pot, g["case%d_pot" % k], live) < 1e-13
+    assert _rel(acc, g["case%d_acc" % k], live) < 1e-12
+    if c["steps"]:
+        ref = g["case%d_ts" % k]
+        assert np.all(ref[live] > 0)
+        assert _rel(ts, ref, live) < 1e-12
+    else:
+        assert np.all(ts == 0)
+
+
+def test_master_pair_mass_form_is_the_integrator_step(oracle_lib):
+    """With every body in one leaf, the scalar (pair-mass) form gives
+    timestep = tau_min^-4 with tau_min the integrator's own all-pairs step
+    (kick_cpu_with_steps, integrator.h:101-180)."""
+    g = golden("master_direct.npz")
+    x, v, m = mmg.ics.plummer(600, seed=15)
+    assert mmg.sha(x, v, m) == str(g["kick_sha"])
+    o = oracle_lib.Oracle(8, ncrit=1000)
+    _, _, ts = o.master_direct(x, v, m, steps=True, pair_mass=True,
+                               dt_param=float(g["kick_dt"]))
+    tau = ts ** -0.25
+    np.testing.assert_allclose(tau, g["kick_ts"], rtol=1e-13)
+
+
+def test_master_vector_form_accumulates_mass_along_the_leaf(oracle_lib):
+    """The vector form's mass term grows along the source leaf, so it differs
+    from the pair-mass form; with one source per leaf (ncrit = 1 cannot hold
+    coincident bodies, so use well-separated ones) both agree."""
+    rng = np.random.default_rng(3)
+    n = 400
+    x = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3)) * 0.3
+    m = rng.uniform(0.5, 1.5, n) / n
+    o = oracle_lib.Oracle(8, ncrit=64)
+    _, _, tv = o.master_direct(x, v, m, steps=True, pair_mass=False)
+    _, _, tp = o.master_direct(x, v, m, steps=True, pair_mass=True)
+    assert np.max(np.abs(tv - tp) / tp) > 1e-2
+    o1 = oracle_lib.Oracle(8, ncrit=1)
+    _, _, tv1 = o1.master_direct(x, v, m, steps=True, pair_mass=False)
+    _, _, tp1 = o1.master_direct(x, v, m, steps=True, pair_mass=True)
+    # one body per leaf: mi = q_i + q_j in both forms; the vector form adds
+    # 1e-20 to |dv|^2 only
+    np.testing.assert_allclose(tv1, tp1, rtol=1e-12)
+
+
+def test_master_forces_brute_force(oracle_lib):
+    rng = np.random.default_rng(9)
+    n = 900
+    x = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    src = (rng.random(n) < 0.7).astype(np.uint8)
+    o = oracle_lib.Oracle(8, ncrit=50)
+    pot, acc, _ = o.master_direct(x, v, m, None, src, steps=False)
+    d = x[None, :, :] - x[:, None, :]
+    r2 = np.sum(d * d, -1)
+    np.fill_diagonal(r2, np.inf)
+    w = m * src
+    pb = np.sum(w[None, :] / np.sqrt(r2), 1)
+    ab = np.sum(w[None, :, None] * d / r2[..., None] ** 1.5, 1)
+    np.testing.assert_allclose(pot, pb, rtol=1e-13)
+    assert np.max(np.linalg.norm(acc - ab, axis=1) / np.linalg.norm(ab, axis=1)) < 1e-12
