@@ -207,10 +207,13 @@ int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mo
 
 /* Device-resident form: x, v [n][3], m [n], flags [n] (bit0 sink, bit1
  * source; NULL = all both), pot [n], acc [n][3], timestep [n] (needed with
- * TC_MASTER_STEPS, else ignored).  Returns after the work is done. */
+ * TC_MASTER_STEPS, else ignored), acc_old [n] (TC_MASTER_FMM above ncrit
+ * bodies: read as the values the pre-pass adds to, then overwritten with the
+ * result; NULL = start from 0 and return nothing).  Returns after the work is
+ * done. */
 int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                        const unsigned char *flags, int mode, double dt_param, double *pot,
-                       double *acc, double *timestep);
+                       double *acc, double *timestep, double *acc_old);
 
 int tc_get_stats(const tc_context *ctx, tc_stats *out);
 const char *tc_last_error(const tc_context *ctx);

@@ -66,7 +66,7 @@ def lib():
         L.orc_master_direct.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
         L.orc_master_fmm.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
-                                     C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp,
+                                     C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp,
                                      C.POINTER(OrcStats)]
         _lib = L
     return _lib
@@ -128,10 +128,11 @@ class Oracle:
         return pot, acc.reshape(n, 3), ts
 
     def master_fmm(self, x, v, m, issink=None, issource=None, steps=False, pair_mass=False,
-                   dt_param=0.025, acc_old=None):
+                   dt_param=0.025, acc_old=None, prepass=True):
         """Master get_force_and_potential built with -DFMM (the handle's p, ncrit,
         force_accuracy; construct with shift_frac=(0,0,0)).  acc_old: the caller's
-        values the pre-pass adds to.  Returns pot, acc, timestep, acc_old."""
+        values the pre-pass adds to (prepass=False: used as they are).
+        Returns pot, acc, timestep, acc_old."""
         n = m.size
         pot, acc, ts = np.zeros(n), np.zeros(3 * n), np.zeros(n)
         ao = np.zeros(n) if acc_old is None else np.array(acc_old, dtype=np.float64)
@@ -142,7 +143,8 @@ class Oracle:
                                   np.ascontiguousarray(v).reshape(-1), np.ascontiguousarray(m),
                                   None if s is None else s.ctypes.data,
                                   None if so is None else so.ctypes.data, int(steps),
-                                  int(pair_mass), dt_param, ao, pot, acc, ts, C.byref(st))
+                                  int(pair_mass), int(prepass), dt_param, ao, pot, acc, ts,
+                                  C.byref(st))
         assert rc == 0
         self._n = n
         self.stats = {k: getattr(st, k) for k, _ in OrcStats._fields_}
@@ -270,12 +272,14 @@ def master_reference(p):
 
 
 def master_ref_eval(p, x, v, m, issink=None, issource=None, ncrit=300, dt_param=0.025,
-                    force_accuracy=2e-7, mode=0, get_steps=True, nthreads=0):
+                    force_accuracy=2e-7, mode=0, get_steps=True, nthreads=0, acc_old=None):
     """Run the compiled master reference once (mode 0: direct, 1: FMM
-    get_force_and_potential); returns pot, acc, timestep, acc_old."""
+    get_force_and_potential, 2: the FMM passes after the pre-pass with the given
+    acc_old); returns pot, acc, timestep, acc_old."""
     L = master_reference(p)
     n = m.size
-    pot, acc, ts, acco = np.zeros(n), np.zeros(3 * n), np.zeros(n), np.zeros(n)
+    pot, acc, ts = np.zeros(n), np.zeros(3 * n), np.zeros(n)
+    acco = np.zeros(n) if acc_old is None else np.array(acc_old, dtype=np.float64)
     s = _opt(issink, np.uint8)
     so = _opt(issource, np.uint8)
     rc = L.refm_eval(n, np.ascontiguousarray(x).reshape(-1), np.ascontiguousarray(v).reshape(-1),

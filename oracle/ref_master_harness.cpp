@@ -15,6 +15,11 @@
 //                      (integrator.h:459-556): direct below ncrit bodies, else
 //                      the p=2 pre-pass, the symmetric error-controlled walk
 //                      and symmetric M2L_rotate
+//              mode 2: the FMM branch's passes after the pre-pass, driven
+//                      directly with the caller's acc_old (integrator.h:
+//                      491-552: reset, upwardPass, Pn, horizontalPass(.., 1,
+//                      get_steps), downwardPass), so the symmetric walk, M2L
+//                      and P2P can be compared without the pre-pass
 //   refm_kick_steps    kick_cpu_with_steps(0, s, s, 0, true) (integrator.h:101-180):
 //                      the integrator's own all-pairs time step, min over every
 //                      source of the pair-mass estimate
@@ -76,12 +81,35 @@ int refm_eval(int n, const double *x, const double *v, const double *m,
     bodies[i].issource = issource ? (issource[i] != 0) : true;
     bodies[i].timestep = 0;
   }
-  if (mode == 0)
+  if (mode == 0) {
     direct(bodies, bodies, get_steps != 0);
-  else if (mode == 1)
+  } else if (mode == 1) {
     get_force_and_potential(bodies, get_steps != 0);
-  else
+  } else if (mode == 2) {
+    for (int i = 0; i < n; i++) bodies[i].acc_old = acc_old ? acc_old[i] : 0;
+    Cells cells = buildTree(bodies);
+    for (size_t i = 0; i < cells.size(); i++) {
+      for (int t = 0; t < NTERM; t++) cells[i].L[t] = cells[i].M[t] = 0;
+      for (int k = 0; k <= P; k++) cells[i].Pn[k] = 0;
+      cells[i].min_acc = HUGE;
+      cells[i].has_sink = false;
+    }
+    upwardPass(cells);
+    for (size_t i = 0; i < cells.size(); i++) {
+      for (int k = 0; k <= P; k++) {
+        for (int m = -k; m <= k; m++) {
+          int index = k * k + k + m;
+          cells[i].Pn[k] += cells[i].M[index] * cells[i].M[index] * factorial_table[k + m] *
+                            factorial_table[k - m];
+        }
+        cells[i].Pn[k] = sqrt(cells[i].Pn[k]);
+      }
+    }
+    horizontalPass(cells, cells, 1, get_steps != 0);
+    downwardPass(cells);
+  } else {
     return 1;
+  }
   for (size_t b = 0; b < bodies.size(); b++) {
     int i = bodies[b].index;
     if (pot) pot[i] = bodies[b].p;

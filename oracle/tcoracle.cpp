@@ -1335,7 +1335,9 @@ struct Fmm {
   // walk, symmetric M2L_rotate (kernel_fast_lean.h:1034-1356: each accepted
   // pair with Ci < Cj, in cell order, updates the locals of both cells; the
   // pair in the other order does nothing), P2P and the downward pass.
-  void master_fmm(bool steps, bool pair_mass, real dt) {
+  // prepass = false: skip the pre-pass and use acc_old as given (the
+  // reference harness's mode 2 does the same).
+  void master_fmm(bool steps, bool pair_mass, real dt, bool prepass = true) {
     if ((int)B.size() < prm.ncrit) {
       for (auto &b : B) b.ts = 0;
       master_direct(steps, pair_mass, dt);
@@ -1350,7 +1352,7 @@ struct Fmm {
       b.ts = 0;
     }
     build_tree();
-    low_pass(true);
+    if (prepass) low_pass(true);
     upward();
     pn();
     m2l.clear();
@@ -1520,8 +1522,8 @@ int orc_master_direct(void *h, int n, const double *x, const double *v, const do
 // order.  Uses the handle's p, ncrit and force_accuracy; shift_frac must be 0.
 int orc_master_fmm(void *h, int n, const double *x, const double *v, const double *m,
                    const unsigned char *issink, const unsigned char *issource, int steps,
-                   int pair_mass, double dt_param, double *acc_old, double *pot, double *acc,
-                   double *timestep, orc_stats *st) {
+                   int pair_mass, int prepass, double dt_param, double *acc_old, double *pot,
+                   double *acc, double *timestep, orc_stats *st) {
   orc::Fmm &F = *(orc::Fmm *)h;
   if (n <= 0) return 1;
   if (F.prm.nthreads > 0) omp_set_num_threads(F.prm.nthreads);
@@ -1539,7 +1541,7 @@ int orc_master_fmm(void *h, int n, const double *x, const double *v, const doubl
     b.acc_old = acc_old ? acc_old[i] : 0;
   }
   const bool direct = n < F.prm.ncrit;
-  F.master_fmm(steps != 0, pair_mass != 0, dt_param);
+  F.master_fmm(steps != 0, pair_mass != 0, dt_param, prepass != 0);
   for (auto &b : F.B) {
     int i = b.index;
     if (pot) pot[i] = b.p;

@@ -105,7 +105,7 @@ def lib():
         L.tc_supported_orders.argtypes = [C.POINTER(C.c_int), C.c_int]
         L.tc_master_eval_bodies.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double]
         L.tc_master_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_double, vp,
-                                         vp, vp]
+                                         vp, vp, vp]
         _lib = L
     return _lib
 
@@ -249,8 +249,10 @@ class Fmm:
                                                 float(dt_param)), "tc_master_eval_bodies")
         return bodies
 
-    def master_eval(self, x, v, m, flags=None, get_steps=False, dt_param=0.025, mode=0):
-        """Device tensors in, (pot, acc, timestep-or-None) cuda float64 tensors out."""
+    def master_eval(self, x, v, m, flags=None, get_steps=False, dt_param=0.025, mode=0,
+                    acc_old=None):
+        """Device tensors in, (pot, acc, timestep-or-None) cuda float64 tensors out.
+        acc_old (n,) cuda float64, TC_MASTER_FMM only: read, then overwritten."""
         import torch
         n = m.shape[0]
         dev = x.device
@@ -264,7 +266,7 @@ class Fmm:
         self._last_n = n
         self._check(lib().tc_master_eval_soa(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(flags),
                                              mode, float(dt_param), _ptr(pot), _ptr(acc),
-                                             _ptr(ts)), "tc_master_eval_soa")
+                                             _ptr(ts), _ptr(acc_old)), "tc_master_eval_soa")
         return pot, acc, ts
 
     # ---- per-kernel entry points over the reference's Cell / Body records ----

@@ -125,8 +125,9 @@ struct tc_context {
   // host Body records: page-locked staging and device result buffers
   HBuf<double> h_stage;
   DBuf hop, hoa, hoj;
-  // master variant: last-body-of-leaf flags, time-step sums
-  DBuf mlend, mts;
+  // master variant: last-body-of-leaf flags, time-step sums (caller order,
+  // tree order), symmetric M2L list and its scan
+  DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
   hipEvent_t ev[9];
 };
 

@@ -549,13 +549,150 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   return 0;
 }
 
+// The master build with -DFMM above ncrit bodies (master integrator.h:466-556):
+// the master p = 2 pre-pass (acc_old accumulates onto the caller's values),
+// upward pass, Pn, the symmetric error-controlled walk, symmetric M2L,
+// P2P_simple / P2P over the near list, downward pass.  Jerk-free outputs; the
+// M2L and downward kernels also carry Mdot/Ldot, whose results are dropped.
+static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass, double dt,
+                             double *pot, double *acc, double *tstep, double *acc_old) {
+  hipStream_t s = ctx->stream;
+  const tc_config &cfg = ctx->cfg;
+  const int nc = ctx->st.ncells;
+  CellsDev c = cells_of(ctx);
+  BodiesDev b = bodies_of(ctx);
+  // ---- pre-pass
+  TC_CHECK(hipEventRecord(ctx->ev[8], s));
+  TC_CHECK(hipMemsetAsync(ctx->lowM.p, 0, sizeof(double) * 9 * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->lowL.p, 0, sizeof(double) * 9 * nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->lowF.p, 0, sizeof(double) * 3 * (size_t)n, s));
+  launch_low_upward(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
+                    ctx->lowM.as<double>(), s);
+  if (int e = traverse(ctx, n, mac_args(ctx, kMacLow))) return e;
+  ctx->st.n_m2l_low = ctx->st.n_m2l;
+  ctx->st.n_p2p_low = ctx->st.n_p2p_cells;
+  if (int e = build_csr(ctx, ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->m2l_cnt, ctx->m2l_off,
+                        ctx->m2l_src))
+    return e;
+  if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
+                        ctx->p2p_off, ctx->p2p_src))
+    return e;
+  launch_low_horizontal(c, b, nc, ctx->leaf_ids.as<int>(), 0, ctx->m2l_off.as<int>(),
+                        ctx->m2l_cnt.as<int>(), ctx->m2l_src.as<int>(), ctx->p2p_off.as<int>(),
+                        ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowM.as<double>(),
+                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), s);
+  launch_low_p2p_master(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->p2p_off.as<int>(),
+                        ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowF.as<double>(), s);
+  launch_low_downward_master(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
+                             ctx->lowL.as<double>(), ctx->lowF.as<double>(), n, acc_old, s);
+  ctx->has_acc_old = true;
+  TC_CHECK(hipEventRecord(ctx->ev[1], s));
+  // ---- upward, norms
+  if (int e = upward(ctx)) return e;
+  launch_pn(ctx->cM.as<double>(), nc, cfg.p, fact_table(cfg), ctx->cPn.as<double>(), s);
+  TC_CHECK(hipEventRecord(ctx->ev[2], s));
+  // ---- symmetric walk
+  MacArgs mac = mac_args(ctx, kMacMaster);
+  mac.steps = steps;
+  if (int e = traverse(ctx, n, mac)) return e;
+  const long long accepted = ctx->st.n_m2l;
+  if (accepted > 0) {
+    const int na = (int)accepted;
+    TC_CHECK(ctx->msym_cnt.ensure(sizeof(int) * ((size_t)na + 1)));
+    TC_CHECK(ctx->msym_off.ensure(sizeof(int) * ((size_t)na + 1)));
+    TC_CHECK(ctx->msym_tmp.ensure(scan_temp_bytes(na + 1) + 256));
+    launch_sym_count(ctx->m2l.as<int2>(), na, ctx->csink.as<unsigned char>(),
+                     ctx->msym_cnt.as<int>(), s);
+    if (launch_scan(ctx->msym_cnt.as<int>(), ctx->msym_off.as<int>(), na + 1, ctx->msym_tmp.p,
+                    ctx->msym_tmp.bytes, s)) {
+      ctx->err = "scan failed";
+      return -4;
+    }
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 11, ctx->msym_off.as<int>() + na, sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    const int nsym = ctx->h_pin[11];
+    TC_CHECK(ctx->m2l_sym.ensure(sizeof(int2) * (size_t)std::max(nsym, 1)));
+    launch_sym_write(ctx->m2l.as<int2>(), na, ctx->csink.as<unsigned char>(),
+                     ctx->msym_off.as<int>(), ctx->m2l_sym.as<int2>(), s);
+    // the lists below read ctx->m2l: hand them the symmetric list
+    std::swap(ctx->m2l, ctx->m2l_sym);
+    ctx->st.n_m2l = nsym;
+  }
+  TC_CHECK(hipEventRecord(ctx->ev[3], s));
+  int nchunks = 0;
+  if (int e = build_lists(ctx, nchunks)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[4], s));
+  // ---- far field
+  if (nchunks > 0) {
+    M2LArgs ma;
+    ma.chunks = ctx->m2l_chunks.as<int4>();
+    ma.nchunks = nchunks;
+    ma.src = ctx->m2l_src.as<int>();
+    ma.cX = ctx->cX.as<double>();
+    ma.has_sink = ctx->csink.as<unsigned char>();
+    ma.M = ctx->cM.as<double>();
+    ma.Md = ctx->cMd.as<double>();
+    ma.L = ctx->cL.as<double>();
+    ma.Ld = ctx->cLd.as<double>();
+    ma.quirk = cfg.quirk_factorial;
+    if (launch_m2l(cfg.p, ma, s)) {
+      ctx->err = "expansion order not compiled";
+      return -2;
+    }
+  }
+  TC_CHECK(hipEventRecord(ctx->ev[5], s));
+  // ---- near field
+  ResultsDev r = results_of(ctx);
+  for (double *q : {r.pot, r.ax, r.ay, r.az, r.jx, r.jy, r.jz})
+    TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
+  TC_CHECK(ctx->mts_sorted.ensure(sizeof(double) * (size_t)n));
+  TC_CHECK(hipMemsetAsync(ctx->mts_sorted.p, 0, sizeof(double) * n, s));
+  launch_master_p2p(c, b, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, steps, pair_mass, dt, r,
+                    ctx->mts_sorted.as<double>(), s);
+  TC_CHECK(hipEventRecord(ctx->ev[6], s));
+  // ---- downward
+  int L = (int)ctx->level_start.size() - 1;
+  for (int l = 0; l < L; l++)
+    launch_l2l(cfg.p, c, ctx->level_start[l], ctx->level_start[l + 1], s);
+  launch_l2p(cfg.p, c, b, r, ctx->leaf_ids.as<int>(), ctx->nleaves, s);
+  launch_scatter_results(b, r, n, pot, acc, nullptr, s);
+  if (steps) launch_scatter_ts(b, ctx->mts_sorted.as<double>(), n, tstep, s);
+  if (acc_old) launch_scatter_acc_old(b, n, acc_old, s);
+  TC_CHECK(hipEventRecord(ctx->ev[7], s));
+  TC_CHECK(hipStreamSynchronize(s));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
+    return -3;
+  }
+  float ms[8], pre = 0;
+  for (int k = 1; k < 7; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
+  hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[8]);
+  hipEventElapsedTime(&pre, ctx->ev[8], ctx->ev[1]);
+  hipEventElapsedTime(&ms[7], ctx->ev[0], ctx->ev[7]);
+  ctx->st.ms_prepass = pre;
+  ctx->st.ms_tree = ms[0];
+  ctx->st.ms_upward = ms[1];
+  ctx->st.ms_traverse = ms[2];
+  ctx->st.ms_lists = ms[3];
+  ctx->st.ms_m2l = ms[4];
+  ctx->st.ms_p2p = ms[5];
+  ctx->st.ms_downward = ms[6];
+  ctx->st.ms_total = ms[7];
+  long long pairs = 0;
+  (void)pairs;
+  ctx->st.n_m2l = accepted;
+  return 0;
+}
+
 // ---- master (non-jerk) variant -------------------------------------------
 // get_force_and_potential of the reference's top-level sources (master
 // integrator.h:459-562).  The tree is the master one: getBounds there has no
 // root shift (master build_tree.h), so the shift is zero here.
 static int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                               const double *m, const unsigned char *flags, int mode, double dt,
-                              double *pot, double *acc, double *tstep) {
+                              double *pot, double *acc, double *tstep, double *acc_old) {
   hipStream_t s = ctx->stream;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->has_acc_old = false;
@@ -573,17 +710,15 @@ static int master_eval_device(tc_context *ctx, int n, const double *x, const dou
     ctx->err = "TC_MASTER_STEPS needs a timestep array and dt_param > 0";
     return -1;
   }
-  if ((mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit) {
-    ctx->err = "TC_MASTER_FMM above ncrit bodies is not built; the master default build "
-               "(direct) is";
-    return -2;
-  }
   if (int e = ensure_bodies(ctx, n)) return e;
   const double zero[3] = {0, 0, 0};
   std::memcpy(ctx->shift, zero, sizeof(zero));
   TC_CHECK(hipMemcpyAsync(ctx->dshift.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
   TC_CHECK(hipEventRecord(ctx->ev[0], s));
   if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
+  // the -DFMM build runs direct() below ncrit bodies (integrator.h:462-465)
+  if ((mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit)
+    return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
   TC_CHECK(ctx->mlend.ensure((size_t)n));
   launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
                    ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
@@ -671,7 +806,8 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
-                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts,
+                 &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts, &ctx->mts_sorted,
+                 &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
@@ -794,9 +930,9 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {
 
 int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                        const unsigned char *flags, int mode, double dt_param, double *pot,
-                       double *acc, double *timestep) {
+                       double *acc, double *timestep, double *acc_old) {
   if (!ctx) return -1;
-  return master_eval_device(ctx, n, x, v, m, flags, mode, dt_param, pot, acc, timestep);
+  return master_eval_device(ctx, n, x, v, m, flags, mode, dt_param, pot, acc, timestep, acc_old);
 }
 
 int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mode,
@@ -809,9 +945,13 @@ int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mo
   hipStream_t s = ctx->stream;
   const size_t N = (size_t)n;
   const bool steps = mode & TC_MASTER_STEPS;
-  TC_CHECK(ctx->h_stage.ensure(8 * N + (N + 7) / 8));
-  double *hx = ctx->h_stage.p, *hv = hx + 3 * N, *hm = hv + 3 * N;
-  unsigned char *hf = (unsigned char *)(hm + N);
+  // the -DFMM build's pre-pass adds to Body.acc_old (above ncrit bodies)
+  const bool fmm = (mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit;
+  // page-locked staging: x[3n] v[3n] m[n] acc_old[n] flags[n bytes]; the
+  // outputs pot[n] acc[3n] timestep[n] acc_old[n] reuse it afterwards
+  TC_CHECK(ctx->h_stage.ensure(9 * N));
+  double *hx = ctx->h_stage.p, *hv = hx + 3 * N, *hm = hv + 3 * N, *ha = hm + N;
+  unsigned char *hf = (unsigned char *)(ha + N);
   host_parallel(n, [&](int lo, int hi) {
     for (int i = lo; i < hi; i++) {
       const tc_master_body &b = bodies[i];
@@ -820,6 +960,7 @@ int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mo
         hv[3 * i + d] = b.V[d];
       }
       hm[i] = b.q;
+      ha[i] = b.acc_old;
       hf[i] = (b.issink ? kSink : 0) | (b.issource ? kSource : 0);
     }
   });
@@ -829,20 +970,24 @@ int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mo
   TC_CHECK(ctx->dfl.ensure(N));
   TC_CHECK(ctx->hop.ensure(8 * N));
   TC_CHECK(ctx->hoa.ensure(24 * N));
+  TC_CHECK(ctx->hoj.ensure(8 * N));
   TC_CHECK(ctx->mts.ensure(8 * N));
   TC_CHECK(hipMemcpyAsync(ctx->dx3.p, hx, 24 * N, hipMemcpyHostToDevice, s));
   TC_CHECK(hipMemcpyAsync(ctx->dv3.p, hv, 24 * N, hipMemcpyHostToDevice, s));
   TC_CHECK(hipMemcpyAsync(ctx->dm.p, hm, 8 * N, hipMemcpyHostToDevice, s));
   TC_CHECK(hipMemcpyAsync(ctx->dfl.p, hf, N, hipMemcpyHostToDevice, s));
   double *op = ctx->hop.as<double>(), *oa = ctx->hoa.as<double>(), *ot = ctx->mts.as<double>();
+  double *oo = ctx->hoj.as<double>();
+  if (fmm) TC_CHECK(hipMemcpyAsync(oo, ha, 8 * N, hipMemcpyHostToDevice, s));
   int e = master_eval_device(ctx, n, ctx->dx3.as<double>(), ctx->dv3.as<double>(),
                              ctx->dm.as<double>(), ctx->dfl.as<unsigned char>(), mode, dt_param,
-                             op, oa, ot);
+                             op, oa, ot, fmm ? oo : nullptr);
   if (e != 0) return e;
-  double *rp = ctx->h_stage.p, *ra = rp + N, *rt = ra + 3 * N;
+  double *rp = ctx->h_stage.p, *ra = rp + N, *rt = ra + 3 * N, *ro = rt + N;
   TC_CHECK(hipMemcpyAsync(rp, op, 8 * N, hipMemcpyDeviceToHost, s));
   TC_CHECK(hipMemcpyAsync(ra, oa, 24 * N, hipMemcpyDeviceToHost, s));
   if (steps) TC_CHECK(hipMemcpyAsync(rt, ot, 8 * N, hipMemcpyDeviceToHost, s));
+  if (fmm) TC_CHECK(hipMemcpyAsync(ro, oo, 8 * N, hipMemcpyDeviceToHost, s));
   TC_CHECK(hipStreamSynchronize(s));
   host_parallel(n, [&](int lo, int hi) {
     for (int i = lo; i < hi; i++) {
@@ -850,6 +995,7 @@ int tc_master_eval_bodies(tc_context *ctx, tc_master_body *bodies, int n, int mo
       b.p = rp[i];
       for (int d = 0; d < 3; d++) b.F[d] = ra[3 * i + d];
       if (steps) b.timestep = rt[i];
+      if (fmm) b.acc_old = ro[i];
     }
   });
   return 0;

@@ -78,10 +78,14 @@ int launch_scan_small(const int *in, int *out, int n, int stride, int narrays, i
 // Multipole acceptance: kMacTheta  |dX|^2 theta^2 > (Ri+Rj)^2 on the balls
 // (traverse_eager.h:125-156); kMacLow  |dXg|^2 > 3 (Rgi+Rgj)^2 on the tree
 // geometry (:172-212); kMacError  the error estimate of horizontalPassHigh
-// (:59-122) from Pn of the source and min_acc of the target.
-enum { kMacTheta = 0, kMacLow = 1, kMacError = 2 };
+// (:59-122) from Pn of the source and min_acc of the target; kMacMaster  the
+// symmetric estimate of the master sources' horizontalPassHigh (master
+// traverse_eager.h:61-149): both directions below 1, accepted pairs of tiny
+// cells to P2P, and with `steps` accepted far pairs dropped.
+enum { kMacTheta = 0, kMacLow = 1, kMacError = 2, kMacMaster = 3 };
 struct MacArgs {
   int mode, p;
+  int steps;                // kMacMaster: the get_steps walk
   double theta, facc;
   const double *Pn;         // [cell][p+1], kMacError
   double comb[kMaxP + 1];   // C(p,k), kMacError
@@ -144,6 +148,15 @@ int launch_low_horizontal(CellsDev c, BodiesDev b, int nc, const int *leaf_ids, 
 int launch_low_downward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
                         const std::vector<int> &level_start, double *lowL, double *F, int n,
                         hipStream_t s);
+// master pre-pass variants (master kernel_fast_lean.h:1479-1556, 1609-1646):
+// F[3i] collects the near terms, F[3i+1] the far-field magnitude; acc_old =
+// ain (caller order, may be null) + both
+int launch_low_p2p_master(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                          const int *p2p_off, const int *p2p_cnt, const int *p2p_src, double *F,
+                          hipStream_t s);
+int launch_low_downward_master(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                               const std::vector<int> &level_start, double *lowL, double *F,
+                               int n, const double *ain, hipStream_t s);
 int launch_pn(const double *M, int nc, int p, const FactTable &ft, double *Pn, hipStream_t s);
 int launch_scatter_acc_old(BodiesDev b, int n, double *out, hipStream_t s);
 
@@ -158,6 +171,18 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
 void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
                           bool pair_mass, double dt, double *pot, double *acc, double *tstep,
                           hipStream_t s);
+// symmetric M2L list: for every accepted (i, j) with i < j and a sink in
+// either cell, (i, j) and (j, i); cnt/off hold npairs + 1 ints
+void launch_sym_count(const int2 *pairs, int npairs, const unsigned char *has_sink, int *cnt,
+                      hipStream_t s);
+void launch_sym_write(const int2 *pairs, int npairs, const unsigned char *has_sink,
+                      const int *off, int2 *out, hipStream_t s);
+// P2P_simple / P2P of the master sources over a (target, source) cell list
+void launch_master_p2p(const CellsDev &c, const BodiesDev &b, const int2 *pairs, int npairs,
+                       bool steps, bool pair_mass, double dt, ResultsDev r, double *ts_sorted,
+                       hipStream_t s);
+void launch_scatter_ts(const BodiesDev &b, const double *ts_sorted, int n, double *out,
+                       hipStream_t s);
 
 int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
                            double *jerk3, hipStream_t s);

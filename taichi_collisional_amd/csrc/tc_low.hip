@@ -293,7 +293,97 @@ __global__ void k_scatter_acc_old(BodiesDev b, int n, double *__restrict__ out) 
   if (k < n) out[b.order[k]] = b.acc_old[k];
 }
 
+// Master P2P_low (master kernel_fast_lean.h:1479-1556): the vector loop
+// overwrites its register at every source, so each listed source leaf adds
+// q / r^2 of its last body only (0 at r = 0), in FP64.  One lane per target.
+__global__ __launch_bounds__(64) void k_low_p2p_master(CellsDev c, BodiesDev b,
+                                                       const int *__restrict__ leaf_ids,
+                                                       const int *__restrict__ off,
+                                                       const int *__restrict__ cnt,
+                                                       const int *__restrict__ src,
+                                                       double *__restrict__ F) {
+#pragma clang fp contract(off)
+  const int ci = leaf_ids[blockIdx.x];
+  const int b0 = c.body[ci], nb = c.nbody[ci];
+  const int e0 = off[ci], ne = cnt[ci];
+  for (int base = 0; base < nb; base += kWave) {
+    const int i = b0 + base + (int)threadIdx.x;
+    if (base + (int)threadIdx.x >= nb) break;
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    double near = 0;
+    for (int e = e0; e < e0 + ne; e++) {
+      const int j = src[e];
+      const int y = c.body[j] + c.nbody[j] - 1;
+      const double dx = xi - b.x[y], dy = yi - b.y[y], dz = zi - b.z[y];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > 0) near += b.q[y] / r2;
+    }
+    F[3 * i] += near;
+  }
+}
+
+// Master L2P_low (:1609-1646): every body of the leaf gets the magnitude of
+// the far-field gradient, sqrt(Re Phi_1^1^2 + Im Phi_1^1^2 + Re Phi_1^0^2).
+__global__ void k_low_l2p_master(CellsDev c, BodiesDev b, const int *__restrict__ leaf_ids,
+                                 int nleaves, const double *__restrict__ lowL,
+                                 double *__restrict__ F) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int ci = leaf_ids[k];
+  cplx cl[LNT], G[LNT];
+  r2c2(lowL + LNT * ci, cl);
+  const double X0 = c.Xg[3 * ci], X1 = c.Xg[3 * ci + 1], X2 = c.Xg[3 * ci + 2];
+  const int b0 = c.body[ci], nb = c.nbody[ci];
+  for (int i = b0; i < b0 + nb; i++) {
+    make_g2(X0 - b.x[i], X1 - b.y[i], X2 - b.z[i], G);
+    for (int t = 0; t < LNT; t++) G[t] = cconj(G[t]);
+    cplx phi[4];
+    for (int t = 0; t < 4; t++) phi[t] = {0.0, 0.0};
+    for (int n = 0; n <= 1; n++)
+      for (int m = 0; m <= n; m++)
+        for (int kk = 0; kk <= LP - n; kk++)
+          for (int l = -kk; l <= kk; l++)
+            phi[lix(n, m)] = cadd(phi[lix(n, m)], cmul(cl[lix(n + kk, m + l)], G[lix(kk, l)]));
+    F[3 * i + 1] +=
+        sqrt(phi[3].re * phi[3].re + phi[3].im * phi[3].im + phi[2].re * phi[2].re);
+  }
+}
+
+__global__ void k_acc_old_master(BodiesDev b, int n, const double *__restrict__ F,
+                                 const double *__restrict__ ain) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a0 = ain ? ain[b.order[i]] : 0.0;
+  b.acc_old[i] = a0 + (F[3 * i] + F[3 * i + 1]);
+}
+
 static inline dim3 blocks_for(int n, int t) { return dim3((n + t - 1) / t); }
+
+int launch_low_p2p_master(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                          const int *p2p_off, const int *p2p_cnt, const int *p2p_src, double *F,
+                          hipStream_t s) {
+  if (nleaves > 0)
+    hipLaunchKernelGGL(k_low_p2p_master, dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids, p2p_off,
+                       p2p_cnt, p2p_src, F);
+  return 0;
+}
+
+int launch_low_downward_master(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                               const std::vector<int> &level_start, double *lowL, double *F,
+                               int n, const double *ain, hipStream_t s) {
+  const int L = (int)level_start.size() - 1;
+  for (int l = 0; l < L; l++) {
+    const int ls = level_start[l], le = level_start[l + 1];
+    if (le > ls)
+      hipLaunchKernelGGL(k_low_l2l, blocks_for(le - ls, 128), dim3(128), 0, s, c, ls, le, lowL);
+  }
+  if (nleaves > 0)
+    hipLaunchKernelGGL(k_low_l2p_master, blocks_for(nleaves, 128), dim3(128), 0, s, c, b,
+                       leaf_ids, nleaves, lowL, F);
+  if (n > 0)
+    hipLaunchKernelGGL(k_acc_old_master, blocks_for(n, 256), dim3(256), 0, s, b, n, F, ain);
+  return 0;
+}
 
 int launch_low_upward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
                       const std::vector<int> &level_start, double *lowM, hipStream_t s) {

@@ -16,7 +16,7 @@
 // is taken per source leaf, and in the vector form the mass in the free-fall
 // estimate accumulates along the leaf.
 //
-// Layout: one thread per target body in tree (sorted) order, so each block's
+// Layout (default build): one thread per target body in tree (sorted) order, so each block's
 // targets are neighbours; sources stream through LDS in tiles of kMT bodies in
 // the same sorted order, with a flag byte per source marking "is a source" and
 // "last body of its leaf".  Results are written at the caller's index.
@@ -159,6 +159,114 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
   acc[3 * o + 1] = sink ? ay : 0.0;
   acc[3 * o + 2] = sink ? az : 0.0;
   if (kSteps) tstep[o] = sink ? tsum : 0.0;
+}
+
+// ---- the build with -DFMM ---------------------------------------------------
+// Symmetric M2L_rotate (kernel_fast_lean.h:1034-1356) returns at once unless
+// Ci < Cj (cell order) and one of the two cells holds a sink; it then adds to
+// the locals of both cells.  As a list of one-sided translations: every
+// accepted (i, j) with i < j and a sink in i or j gives (i, j) and (j, i).
+__global__ void k_sym_count(const int2 *__restrict__ pairs, int npairs,
+                            const unsigned char *__restrict__ has_sink, int *__restrict__ cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > npairs) return;
+  if (k == npairs) {
+    cnt[k] = 0;
+    return;
+  }
+  const int2 pr = pairs[k];
+  cnt[k] = (pr.x < pr.y && (has_sink[pr.x] || has_sink[pr.y])) ? 2 : 0;
+}
+
+__global__ void k_sym_write(const int2 *__restrict__ pairs, int npairs,
+                            const unsigned char *__restrict__ has_sink,
+                            const int *__restrict__ off, int2 *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npairs) return;
+  const int2 pr = pairs[k];
+  if (!(pr.x < pr.y && (has_sink[pr.x] || has_sink[pr.y]))) return;
+  out[off[k]] = pr;
+  out[off[k] + 1] = make_int2(pr.y, pr.x);
+}
+
+// P2P_simple / P2P (kernel_fast_lean.h:268-674) of one listed (target,
+// source) cell pair per block: lanes take target bodies, every lane walks the
+// source cell's bodies in order; results are added with atomics (a body's
+// leaf and a tiny ancestor may both be listed targets).  Only pairs whose
+// target holds a sink run (Ci->has_sink), only sink bodies receive.
+template <bool kSteps, bool kPairMass>
+__global__ __launch_bounds__(64) void k_master_p2p(CellsDev c, BodiesDev b,
+                                                   const int2 *__restrict__ pairs, double dt,
+                                                   ResultsDev r, double *__restrict__ ts_sorted) {
+  const int2 pr = pairs[blockIdx.x];
+  if (!c.has_sink[pr.x]) return;
+  const int i0 = c.body[pr.x], ni = c.nbody[pr.x];
+  const int j0 = c.body[pr.y], nj = c.nbody[pr.y];
+  for (int base = 0; base < ni; base += kWave) {
+    const int a = i0 + base + (int)threadIdx.x;
+    if (base + (int)threadIdx.x >= ni) break;
+    if (!(b.flags[a] & kSink)) continue;
+    const double xi = b.x[a], yi = b.y[a], zi = b.z[a], qi = b.q[a];
+    const double vxi = b.vx[a], vyi = b.vy[a], vzi = b.vz[a];
+    double ax = 0, ay = 0, az = 0, pt = 0, mi = qi, ts = 1e38;
+    for (int j = j0; j < j0 + nj; j++) {
+      const double dx = b.x[j] - xi, dy = b.y[j] - yi, dz = b.z[j] - zi;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (kSteps)
+        pair_tau<kPairMass>(dx, dy, dz, b.vx[j] - vxi, b.vy[j] - vyi, b.vz[j] - vzi, r2, qi,
+                            b.q[j], dt, mi, ts);
+      const bool use = (b.flags[j] & kSource) && r2 > 0;
+      const double ir = rsqrt_fp64(use ? r2 : 1.0);
+      const double w = use ? b.q[j] * ir : 0.0;
+      pt += w;
+      const double w3 = w * ir * ir;
+      ax += dx * w3;
+      ay += dy * w3;
+      az += dz * w3;
+    }
+    atomicAdd(r.pot + a, pt);
+    atomicAdd(r.ax + a, ax);
+    atomicAdd(r.ay + a, ay);
+    atomicAdd(r.az + a, az);
+    if (kSteps) {
+      double t4 = ts * ts;
+      t4 *= t4;
+      atomicAdd(ts_sorted + a, 1.0 / t4);
+    }
+  }
+}
+
+__global__ void k_scatter_ts(BodiesDev b, const double *__restrict__ ts, int n,
+                             double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[b.order[k]] = ts[k];
+}
+
+void launch_sym_count(const int2 *pairs, int npairs, const unsigned char *has_sink, int *cnt,
+                      hipStream_t s) {
+  k_sym_count<<<(npairs + 1 + 255) / 256, 256, 0, s>>>(pairs, npairs, has_sink, cnt);
+}
+
+void launch_sym_write(const int2 *pairs, int npairs, const unsigned char *has_sink,
+                      const int *off, int2 *out, hipStream_t s) {
+  if (npairs > 0) k_sym_write<<<(npairs + 255) / 256, 256, 0, s>>>(pairs, npairs, has_sink, off, out);
+}
+
+void launch_master_p2p(const CellsDev &c, const BodiesDev &b, const int2 *pairs, int npairs,
+                       bool steps, bool pair_mass, double dt, ResultsDev r, double *ts_sorted,
+                       hipStream_t s) {
+  if (npairs <= 0) return;
+  if (!steps)
+    k_master_p2p<false, false><<<npairs, 64, 0, s>>>(c, b, pairs, dt, r, ts_sorted);
+  else if (!pair_mass)
+    k_master_p2p<true, false><<<npairs, 64, 0, s>>>(c, b, pairs, dt, r, ts_sorted);
+  else
+    k_master_p2p<true, true><<<npairs, 64, 0, s>>>(c, b, pairs, dt, r, ts_sorted);
+}
+
+void launch_scatter_ts(const BodiesDev &b, const double *ts_sorted, int n, double *out,
+                       hipStream_t s) {
+  if (n > 0) k_scatter_ts<<<(n + 255) / 256, 256, 0, s>>>(b, ts_sorted, n, out);
 }
 
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,

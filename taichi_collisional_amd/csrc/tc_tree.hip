@@ -411,6 +411,35 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
     if (a.mode == kMacTheta) {
       const double R2 = (dx * dx + dy * dy + dz * dz) * a.theta * a.theta;
       accept = R2 > (Ri + Rj) * (Ri + Rj);
+    } else if (a.mode == kMacMaster) {
+      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double s = Ri + Rj;
+      double f1 = 10, f2 = 10;
+      if (R2 > s * s) {
+        const int P = a.p;
+        const double R = sqrt(R2);
+        const double *Pi = a.Pn + (size_t)i * (P + 1), *Pj = a.Pn + (size_t)j * (P + 1);
+        double Rp = R2, pa = 1, pb = 1, Eab = 0, Eba = 0;
+        for (int k = 0; k <= P; k++) {
+          Eab += Pi[P - k] * pb * a.comb[k];
+          Eba += Pj[P - k] * pa * a.comb[k];
+          pa *= Ri;
+          pb *= Rj;
+          Rp *= R;
+        }
+        const double fac = 8 * fmax(Ri, Rj) / s / Rp;
+        Eab *= fac;
+        f1 = Eab / (a.facc * c.min_acc[j]);
+        Eba *= fac;
+        f2 = Eba / (a.facc * c.min_acc[i]);
+      }
+      if (f2 < 1 && f1 < 1) {
+        *nsplit = 0;
+        const int ni = c.nbody[i], nj = c.nbody[j];
+        if ((ni <= 2 && nj <= 8) || (nj <= 2 && ni <= 8)) return 2;  // P2P
+        return a.steps ? 0 : 1;                                       // M2L, or nothing
+      }
+      accept = false;
     } else {
       const double R2 = dx * dx + dy * dy + dz * dz;
       const double s = Ri + Rj;

@@ -1,6 +1,7 @@
 """GPU parity of the master (non-jerk) variant through the C ABI: the master
 default build's get_force_and_potential (direct over leaf pairs, with and
-without the time-step estimate) against the oracle on the same inputs and
+without the time-step estimate) and the build with -DFMM (master pre-pass,
+symmetric walk, symmetric M2L) against the oracle on the same inputs and
 against the compiled master reference's outputs (golden fixtures).
 
 Tolerances (FP64): forces and potentials are the same sums in another order
@@ -162,8 +163,6 @@ def test_gpu_master_edges_and_errors(torch_cuda, oracle_lib):
     dx, dv, dm = (t.from_numpy(a).cuda() for a in mmg.ics.plummer(500, seed=1))
     with pytest.raises(RuntimeError, match="dt_param"):
         f.master_eval(dx, dv, dm, get_steps=True, dt_param=0.0)
-    with pytest.raises(RuntimeError, match="not built"):
-        f.master_eval(dx, dv, dm, mode=tca.TC_MASTER_FMM)
     with pytest.raises(RuntimeError, match="mode"):
         f.master_eval(dx, dv, dm, mode=64)
     # below ncrit the FMM build runs direct(), as the reference does
@@ -171,3 +170,66 @@ def test_gpu_master_edges_and_errors(torch_cuda, oracle_lib):
     p1, a1, t1 = f2.master_eval(dx, dv, dm, get_steps=True, mode=tca.TC_MASTER_FMM)
     p0, a0, t0 = f2.master_eval(dx, dv, dm, get_steps=True)
     assert t.equal(p1, p0) and t.equal(a1, a0) and t.equal(t1, t0)
+
+
+def _fmm_gpu(torch, c, x, v, m, sink, src, ao, mode_extra=0):
+    import taichi_collisional_amd as tca
+    f = tca.Fmm(p=c["p"], ncrit=c["ncrit"], mac=tca.TC_MAC_ERROR, force_accuracy=2e-7,
+                direct_below=0)
+    dx, dv, dm, dao = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, v, m, ao))
+    fl = torch.from_numpy((sink | (src << 1)).astype(np.uint8)).cuda()
+    pot, acc, ts = f.master_eval(dx, dv, dm, flags=fl, get_steps=c["steps"],
+                                 mode=tca.TC_MASTER_FMM | mode_extra, acc_old=dao)
+    torch.cuda.synchronize()
+    return (pot.cpu().numpy(), acc.cpu().numpy(), None if ts is None else ts.cpu().numpy(),
+            dao.cpu().numpy(), f.stats())
+
+
+@pytest.mark.parametrize("k", range(len(mmg.FMM_CASES)))
+def test_gpu_master_fmm_matches_oracle(torch_cuda, oracle_lib, k):
+    """The whole -DFMM path (pre-pass from the caller's acc_old included)
+    against the oracle on the same inputs: the same tie decisions in the
+    pre-pass and the walk, so results agree to round-off; the case with
+    the mode-1 reference golden is also checked against it (pre-pass ties
+    decided by the reference's rounding: see tests/test_master_oracle.py)."""
+    g = golden("master_direct.npz")
+    c = mmg.FMM_CASES[k]
+    x, v, m, sink, src, ao, idx = mmg.fmm_inputs(c)
+    assert mmg.sha(x, v, m, sink, src, ao, idx) == str(g["fmm%d_sha" % k])
+    gp, ga, gt, gao, st = _fmm_gpu(torch_cuda, c, x, v, m, sink, src, ao)
+    o = oracle_lib.Oracle(c["p"], ncrit=c["ncrit"], mac=1, force_accuracy=2e-7)
+    op, oa, ot, oao = o.master_fmm(x, v, m, sink, src, steps=c["steps"], acc_old=ao)
+    assert st["n_m2l"] == o.stats["n_m2l"] and st["n_p2p_cells"] == o.stats["n_p2p_cells"]
+    np.testing.assert_allclose(gao, oao, rtol=1e-12)
+    live = sink.astype(bool)
+    _check((gp, ga, gt), (op, oa, ot), live, c["steps"], "fmm case %d vs oracle" % k)
+    if c["mode"] == 1:
+        e = np.abs(gao[idx] - g["fmm%d_acc_old" % k]) / g["fmm%d_acc_old" % k]
+        assert np.median(e) < 1e-14 and np.mean(e < 1e-12) > 0.8
+        lv = live[idx]
+        ea = (np.linalg.norm(ga[idx][lv] - g["fmm%d_acc" % k][lv], axis=1) /
+              np.linalg.norm(g["fmm%d_acc" % k][lv], axis=1))
+        assert np.sqrt(np.mean(ea ** 2)) < 3e-8 and ea.max() < 3e-6
+
+
+def test_gpu_master_fmm_body_records(torch_cuda):
+    """Body records: acc_old accumulates onto the records' values in the FMM
+    build above ncrit, stays untouched below it and in the default build."""
+    import taichi_collisional_amd as tca
+    c = dict(p=8, ncrit=64, steps=True)
+    x, v, m = mmg.ics.plummer(6000, seed=51)
+    ones = np.ones(6000, np.uint8)
+    ao = np.full(6000, 0.25)
+    gp, ga, gt, gao, _ = _fmm_gpu(torch_cuda, c, x, v, m, ones, ones, ao.copy())
+    b = np.zeros(6000, tca.master_body_dtype)
+    b["X"], b["V"], b["q"], b["issink"], b["issource"] = x, v, m, 1, 1
+    b["acc_old"] = 0.25
+    f = tca.Fmm(p=8, ncrit=64, mac=tca.TC_MAC_ERROR, force_accuracy=2e-7, direct_below=0)
+    f.master_get_force_and_potential(b, get_steps=True, mode=tca.TC_MASTER_FMM)
+    np.testing.assert_allclose(b["acc_old"], gao, rtol=1e-13)
+    assert np.all(b["acc_old"] > 0.25)
+    np.testing.assert_allclose(b["p"], gp, rtol=1e-12)
+    np.testing.assert_allclose(b["timestep"], gt, rtol=1e-12)
+    b2 = b.copy()
+    f.master_get_force_and_potential(b2, get_steps=False)
+    assert np.array_equal(b2["acc_old"], b["acc_old"])

@@ -98,3 +98,66 @@ def test_master_forces_brute_force(oracle_lib):
     ab = np.sum(w[None, :, None] * d / r2[..., None] ** 1.5, 1)
     np.testing.assert_allclose(pot, pb, rtol=1e-13)
     assert np.max(np.linalg.norm(acc - ab, axis=1) / np.linalg.norm(ab, axis=1)) < 1e-12
+
+
+def _fmm_case(g, k):
+    c = mmg.FMM_CASES[k]
+    x, v, m, sink, src, ao, idx = mmg.fmm_inputs(c)
+    assert mmg.sha(x, v, m, sink, src, ao, idx) == str(g["fmm%d_sha" % k]), "IC generator changed"
+    return c, x, v, m, sink, src, ao, idx
+
+
+@pytest.mark.parametrize("k", [k for k, c in enumerate(mmg.FMM_CASES) if c["mode"] == 2])
+def test_master_fmm_passes_match_reference(oracle_lib, k):
+    """The -DFMM build after its pre-pass: symmetric error-controlled walk
+    (both f1, f2 < 1; tiny cells to P2P; with steps no far field), symmetric
+    M2L_rotate, P2P_simple / P2P with the time step, downward pass.  Same
+    acc_old in both, so every decision is reproduced: round-off agreement."""
+    g = golden("master_direct.npz")
+    c, x, v, m, sink, src, ao, idx = _fmm_case(g, k)
+    o = oracle_lib.Oracle(c["p"], ncrit=c["ncrit"], mac=1, force_accuracy=2e-7)
+    pot, acc, ts, _ = o.master_fmm(x, v, m, sink, src, steps=c["steps"], acc_old=ao,
+                                   prepass=False)
+    live = sink[idx].astype(bool)
+    assert np.all(pot[idx][~live] == 0)
+    assert _rel(pot[idx], g["fmm%d_pot" % k], live) < 1e-12
+    assert _rel(acc[idx], g["fmm%d_acc" % k], live) < 1e-11
+    if c["steps"]:
+        assert o.stats["n_m2l"] == 0
+        assert _rel(ts[idx], g["fmm%d_ts" % k], live) < 1e-12
+    else:
+        assert o.stats["n_m2l"] > 0
+
+
+def test_master_fmm_with_prepass_matches_reference(oracle_lib):
+    """The whole -DFMM get_force_and_potential.  The pre-pass's acceptance test
+    |dXg|^2 > 3 (Rg_i + Rg_j)^2 ties exactly for diagonal neighbours on the
+    tree grid, and the reference's compiled rounding of those ties is not
+    reproduced, so the leaves on such pairs get other acc_old values (the
+    master pre-pass adds a q/r^2 term per listed near leaf, so a flipped pair
+    moves acc_old by a large factor).  All other bodies agree to round-off,
+    and forces to the expansion error."""
+    g = golden("master_direct.npz")
+    k = [k for k, c in enumerate(mmg.FMM_CASES) if c["mode"] == 1][0]
+    c, x, v, m, sink, src, ao, idx = _fmm_case(g, k)
+    o = oracle_lib.Oracle(c["p"], ncrit=c["ncrit"], mac=1, force_accuracy=2e-7)
+    pot, acc, ts, aco = o.master_fmm(x, v, m, sink, src, steps=c["steps"], acc_old=ao)
+    e = np.abs(aco[idx] - g["fmm%d_acc_old" % k]) / g["fmm%d_acc_old" % k]
+    assert np.median(e) < 1e-14
+    assert np.mean(e < 1e-12) > 0.8
+    live = sink[idx].astype(bool)
+    ea = (np.linalg.norm(acc[idx][live] - g["fmm%d_acc" % k][live], axis=1) /
+          np.linalg.norm(g["fmm%d_acc" % k][live], axis=1))
+    assert np.sqrt(np.mean(ea ** 2)) < 3e-8 and ea.max() < 3e-6
+    assert _rel(pot[idx], g["fmm%d_pot" % k], live) < 1e-7
+
+
+def test_master_fmm_below_ncrit_is_direct(oracle_lib):
+    x, v, m = mmg.ics.plummer(300, seed=41)
+    o = oracle_lib.Oracle(8, ncrit=400, mac=1, force_accuracy=2e-7)
+    pf, af, tf, ao = o.master_fmm(x, v, m, steps=True, acc_old=np.full(300, 3.0))
+    pd, ad, td = oracle_lib.Oracle(8, ncrit=400).master_direct(x, v, m, steps=True)
+    np.testing.assert_array_equal(pf, pd)
+    np.testing.assert_array_equal(af, ad)
+    np.testing.assert_array_equal(tf, td)
+    assert np.all(ao == 3.0)
