@@ -92,6 +92,35 @@ print(json.dumps(out))
     return res
 
 
+def master_cpu_baseline(n, seed, threads, timeout):
+    """The master reference's default build (direct() with get_steps) on the
+    host, one evaluation after a warm-up on a small input."""
+    code = r"""
+import sys, time, json, os
+sys.path.insert(0, %r); sys.path.insert(0, os.path.join(%r, "oracle"))
+import numpy as np, pyoracle as po
+from taichi_collisional_amd import ics
+if not po.master_ref_available(8):
+    print(json.dumps({})); sys.exit(0)
+x, v, m = ics.plummer(%d, seed=%d)
+po.master_ref_eval(8, x[:4096], v[:4096], m[:4096], ncrit=300, mode=0, get_steps=True)
+t = time.time()
+po.master_ref_eval(8, x, v, m, ncrit=300, mode=0, get_steps=True)
+print(json.dumps({"cpu_reference_seconds": time.time() - t}))
+""" % (ROOT, ROOT, n, seed)
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               OMP_WAIT_POLICY="active", OMP_STACKSIZE="100M")
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=timeout, env=env)
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception:  # noqa: BLE001 - the row just lacks the CPU figure
+        return {}
+    out["cpu_cores"] = threads
+    return out
+
+
 def load_traffic(path):
     try:
         d = json.load(open(path))
@@ -219,6 +248,31 @@ def main():
         extra["host_path"] = {"what": "tc_eval_bodies on host Body records: host AoS to device "
                                       "SoA, evaluation, results back (PCIe both ways)",
                               "ms_per_eval": hms, "particles_per_s": args.n / (hms * 1e-3)}
+
+    if not args.no_extra:
+        # the master variant's default build: direct over leaf pairs with the
+        # time-step estimate (N = 2^16, ncrit 300, dt_param 0.025)
+        nm = 1 << 16
+        xm, vm, mm = ics.plummer(nm, seed=args.seed)
+        dxm, dvm, dmm = (torch.from_numpy(a).to(dev) for a in (xm, vm, mm))
+        fm = tca.Fmm(p=args.p, ncrit=300, direct_below=0)
+        fm.master_eval(dxm, dvm, dmm, get_steps=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            fm.master_eval(dxm, dvm, dmm, get_steps=True)
+        torch.cuda.synchronize()
+        mms = (time.perf_counter() - t) * 1e3 / 3
+        row = {"what": "master default build: direct() over every leaf pair, P2P with the "
+                       "per-leaf time-step estimate, N=65536, ncrit=300",
+               "ms_per_eval": mms, "body_pairs_per_s": nm * nm / (mms * 1e-3),
+               "ms_kernel": fm.stats()["ms_p2p"]}
+        del fm
+        if cpu is not None:
+            row.update(master_cpu_baseline(nm, args.seed, args.cpu_threads, args.cpu_timeout))
+            if row.get("cpu_reference_seconds"):
+                row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (mms * 1e-3)
+        extra["master_direct"] = row
 
     if args.calibrate_copy_mib > 0:
         src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)

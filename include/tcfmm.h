@@ -12,6 +12,8 @@
  *                      (taichi.cxx:32, build_tree.h:256-267)
  *   tc_body         <- struct Body (exafmm.h:22-46), byte-for-byte
  *   tc_direct_soa   <- direct(bodies, bodies) / directPass  traverse_eager.h:215-304
+ *   tc_master_*     <- the master sources' get_force_and_potential (top-level
+ *                      integrator.h:459-562) on their 128-byte Body (see below)
  *
  * Semantics kept from the reference: outputs are accelerations (no target
  * mass factor, G = 1), potential is +sum m/r, jerk is da/dt; bodies with
