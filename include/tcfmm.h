@@ -217,6 +217,56 @@ int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v,
                        const unsigned char *flags, int mode, double dt_param, double *pot,
                        double *acc, double *timestep, double *acc_old);
 
+/* ---- The master variant's step loop ------------------------------------------
+ * tc_master_evolve <- do_evolve(sys, dt) (master integrator.h:864-884) as the
+ * master Makefile builds it (GRADIENT): kick_naive for the time steps, then
+ * evolve_split_omelyan (:720-812), the fourth-order force-gradient scheme over
+ * a hierarchy of time-step levels (split :592-640, kick :378-457, kick_cpu
+ * :65-98 for subsystems of <= 128 bodies, kick_gradient :313-376, drift with
+ * compensated summation :642-655).  Every force is the master variant's
+ * get_force_and_potential (default build, or the FMM with TC_MASTER_FMM; mode
+ * as for tc_master_eval_*).  One call advances the particles by dt; *simtime
+ * is advanced as the reference advances diag->simtime.
+ * Particle arrays are device pointers, the fields of the reference's struct
+ * particle (exafmm.h:73-86) that the loop reads or writes; the arrays keep
+ * their order (the reference's split swaps records; identity is the index).
+ * timestep, acc, pot, acc_old are outputs (as kick_naive leaves them); pos_e
+ * holds the compensated-summation error terms and must start at 0. */
+typedef struct tc_particles {
+  int n;
+  double *pos;       /* [n][3] */
+  double *pos_e;     /* [n][3] */
+  double *vel;       /* [n][3] */
+  double *mass;      /* [n]    */
+  double *timestep;  /* [n]    */
+  double *acc;       /* [n][3] */
+  double *pot;       /* [n]    */
+  double *acc_old;   /* [n]    */
+} tc_particles;
+
+int tc_master_evolve(tc_context *ctx, const tc_particles *p, double *simtime, double dt,
+                     int mode, double dt_param);
+
+/* The reference's struct particle (master exafmm.h:73-86), 168 bytes. */
+typedef struct tc_particle {
+  double pos[3];
+  double pos_e[3];
+  double mass;
+  double vel[3];
+  double timestep;
+  double acc[3];
+  double pot;
+  double jerk[3];   /* not used by the master loop */
+  double postime;   /* end time of the last drift: the call's end time */
+  double acc_old;
+  unsigned int id;
+} tc_particle;
+
+/* Host-record form of tc_master_evolve on the reference's particle array
+ * (copies host<->device inside; the array is not reordered). */
+int tc_master_evolve_particles(tc_context *ctx, tc_particle *part, int n, double *simtime,
+                               double dt, int mode, double dt_param);
+
 int tc_get_stats(const tc_context *ctx, tc_stats *out);
 const char *tc_last_error(const tc_context *ctx);
 
@@ -245,7 +295,11 @@ static_assert(__builtin_offsetof(tc_master_body, acc_old) == 88 &&
                   __builtin_offsetof(tc_master_body, issink) == 124 &&
                   __builtin_offsetof(tc_master_body, issource) == 125,
               "tc_master_body layout");
+static_assert(sizeof(tc_particle) == 168 && __builtin_offsetof(tc_particle, acc_old) == 152 &&
+                  __builtin_offsetof(tc_particle, id) == 160,
+              "tc_particle must match the reference's struct particle (168 bytes)");
 #else
+_Static_assert(sizeof(tc_particle) == 168, "tc_particle layout");
 _Static_assert(sizeof(tc_master_body) == 128, "tc_master_body layout");
 _Static_assert(sizeof(tc_body) == 152, "tc_body must match the reference Body (152 bytes)");
 _Static_assert(__builtin_offsetof(tc_body, J) == 120 && __builtin_offsetof(tc_body, issink) == 148, "tc_body layout");

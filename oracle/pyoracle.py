@@ -65,6 +65,9 @@ def lib():
         L.orc_shift_from_seed.argtypes = [C.c_long, C.c_int, _dp]
         L.orc_master_direct.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_master_evolve.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                        C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                        C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.orc_master_fmm.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
                                      C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp,
                                      C.POINTER(OrcStats)]
@@ -246,6 +249,26 @@ def ref_eval(p, x, v, m, seed, ncrit=64, theta=0.5, mode=0, nthreads=0):
     return pot, acc.reshape(n, 3), jerk.reshape(n, 3), acco, counts, shift
 
 
+def master_evolve(m, pos, vel, nsteps, dt, p=8, ncrit=300, force_accuracy=2e-7, fmm=False,
+                  pair_mass=False, dt_param=0.025, simtime=0.0):
+    """The oracle's master step loop (do_evolve, GRADIENT) for nsteps steps.
+    Returns a dict of pos, vel, pos_e, timestep, acc, pot, acc_old, simtime."""
+    n = m.size
+    out = dict(pos=np.array(pos, dtype=np.float64).reshape(-1),
+               vel=np.array(vel, dtype=np.float64).reshape(-1), pos_e=np.zeros(3 * n),
+               timestep=np.zeros(n), acc=np.zeros(3 * n), pot=np.zeros(n), acc_old=np.zeros(n))
+    t = C.c_double(simtime)
+    rc = lib().orc_master_evolve(n, np.ascontiguousarray(m, dtype=np.float64), out["pos"],
+                                 out["pos_e"], out["vel"], out["timestep"], out["acc"],
+                                 out["pot"], out["acc_old"], p, ncrit, force_accuracy, int(fmm),
+                                 int(pair_mass), dt_param, nsteps, dt, C.byref(t))
+    assert rc == 0
+    for k in ("pos", "vel", "pos_e", "acc"):
+        out[k] = out[k].reshape(n, 3)
+    out["simtime"] = t.value
+    return out
+
+
 _mrefs = {}
 
 
@@ -266,6 +289,8 @@ def master_reference(p):
     L.refm_eval.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                             C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     L.refm_kick_steps.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+    L.refm_evolve.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                              C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double)]
     L.isa = isa
     _mrefs[p] = L
     return L
@@ -288,6 +313,25 @@ def master_ref_eval(p, x, v, m, issink=None, issource=None, ncrit=300, dt_param=
                      mode, int(get_steps), nthreads, pot, acc, ts, acco)
     assert rc == 0
     return pot, acc.reshape(n, 3), ts, acco
+
+
+def master_ref_evolve(p, m, pos, vel, nsteps, dt, ncrit=300, dt_param=0.025,
+                      force_accuracy=2e-7, simtime=0.0):
+    """nsteps calls of the compiled master do_evolve; outputs by particle id."""
+    L = master_reference(p)
+    n = m.size
+    out = dict(pos=np.array(pos, dtype=np.float64).reshape(-1),
+               vel=np.array(vel, dtype=np.float64).reshape(-1), pos_e=np.zeros(3 * n),
+               pot=np.zeros(n), timestep=np.zeros(n), acc=np.zeros(3 * n))
+    t = C.c_double(simtime)
+    rc = L.refm_evolve(n, np.ascontiguousarray(m, dtype=np.float64), out["pos"], out["vel"],
+                       out["pos_e"], out["pot"], out["timestep"], out["acc"], ncrit, dt_param,
+                       force_accuracy, nsteps, dt, C.byref(t))
+    assert rc == 0
+    for k in ("pos", "vel", "pos_e", "acc"):
+        out[k] = out[k].reshape(n, 3)
+    out["simtime"] = t.value
+    return out
 
 
 def ref_cells(p, ncells_max, nbodies):

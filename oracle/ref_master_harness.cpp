@@ -20,6 +20,9 @@
 //                      491-552: reset, upwardPass, Pn, horizontalPass(.., 1,
 //                      get_steps), downwardPass), so the symmetric walk, M2L
 //                      and P2P can be compared without the pre-pass
+//   refm_evolve        nsteps calls of do_evolve(mainsys, dt) (integrator.h:864-884,
+//                      the GRADIENT scheme evolve_split_omelyan) on a particle
+//                      system, outputs by particle id (split reorders records)
 //   refm_kick_steps    kick_cpu_with_steps(0, s, s, 0, true) (integrator.h:101-180):
 //                      the integrator's own all-pairs time step, min over every
 //                      source of the pair-mass estimate
@@ -139,6 +142,43 @@ int refm_kick_steps(int n, const double *x, const double *v, const double *m, do
   // velocities while it updates the sinks', and here sinks are the sources)
   kick_cpu_with_steps(0, s, s, 0.0, true);
   for (int i = 0; i < n; i++) timestep[i] = part[i].timestep;
+  return 0;
+}
+
+int refm_evolve(int n, const double *m, double *pos, double *vel, double *pos_e, double *pot,
+                double *timestep, double *acc, int ncrit_, double dt_param_,
+                double force_accuracy_, int nsteps, double dt, double *simtime) {
+  ensure_init();
+  ncrit = ncrit_;
+  dt_param = dt_param_;
+  force_accuracy = force_accuracy_;
+  diag = &global_diag;
+  diag->simtime = *simtime;
+  std::vector<particle> part(n);
+  std::memset((void *)part.data(), 0, sizeof(particle) * n);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      part[i].pos[d] = pos[3 * i + d];
+      part[i].vel[d] = vel[3 * i + d];
+    }
+    part[i].mass = m[i];
+    part[i].id = i;
+  }
+  struct sys s = {(size_t)n, part.data(), part.data() + n - 1};
+  for (int k = 0; k < nsteps; k++) do_evolve(s, dt);
+  for (int k = 0; k < n; k++) {
+    const particle &q = part[k];
+    const int i = (int)q.id;
+    for (int d = 0; d < 3; d++) {
+      pos[3 * i + d] = q.pos[d];
+      vel[3 * i + d] = q.vel[d];
+      pos_e[3 * i + d] = q.pos_e[d];
+      acc[3 * i + d] = q.acc[d];
+    }
+    pot[i] = q.pot;
+    timestep[i] = q.timestep;
+  }
+  *simtime = diag->simtime;
   return 0;
 }
 

@@ -1423,6 +1423,163 @@ struct Fmm {
 
 }  // namespace orc
 
+// ------------------------------------------------ master step loop ----
+// do_evolve with GRADIENT (master integrator.h:864-884): kick_naive, then
+// evolve_split_omelyan (:720-812) with kick (:378-457), kick_cpu (:65-98),
+// kick_gradient (:313-376), split (:592-640) and drift with COMPSUM (:642-655,
+// :42).  Subsystems are index lists (split is a stable partition; the
+// reference swaps records, which changes no force or step).  Forces: the
+// master default build (direct over leaf pairs) or, with fmm, the -DFMM
+// build, each on a fresh body set of sinks ++ sources as the reference's
+// kicks build them.
+namespace orc {
+struct MasterLoop {
+  int n;
+  real *pos, *pos_e, *vel, *mass, *ts, *acc, *pot, *acc_old;
+  Params prm;
+  bool fmm, pair_mass;
+  real dtp, simtime;
+
+  void force(const std::vector<int> &si, const std::vector<int> &sj, bool sinks_are_sources,
+             real shift, bool steps, std::vector<real> &F, std::vector<real> &P,
+             std::vector<real> &T, std::vector<real> &AO) {
+    Fmm f(prm);
+    int nu = (int)(si.size() + sj.size());
+    f.B.resize(nu);
+    for (int k = 0; k < nu; k++) {
+      bool sink = k < (int)si.size();
+      int i = sink ? si[k] : sj[k - si.size()];
+      Body &b = f.B[k];
+      for (int d = 0; d < 3; d++) {
+        b.X[d] = shift != 0 ? pos[3 * i + d] + acc[3 * i + d] * shift : pos[3 * i + d];
+        b.V[d] = vel[3 * i + d];
+      }
+      b.q = mass[i];
+      b.index = k;
+      b.issink = sink;
+      b.issource = sink ? sinks_are_sources : true;
+      b.acc_old = 0;
+    }
+    if (fmm)
+      f.master_fmm(steps, pair_mass, dtp);
+    else
+      f.master_direct(steps, pair_mass, dtp);
+    F.assign(3 * nu, 0);
+    P.assign(nu, 0);
+    T.assign(nu, 0);
+    AO.assign(nu, 0);
+    for (auto &b : f.B) {
+      int k = b.index;
+      for (int d = 0; d < 3; d++) F[3 * k + d] = b.F[d];
+      P[k] = b.p;
+      T[k] = fmm ? b.ts : b.acc_old;  // master_direct keeps the sum in acc_old
+      AO[k] = fmm ? b.acc_old : 0;
+    }
+  }
+  void kick(const std::vector<int> &si, const std::vector<int> &sj, real dt, bool sinks_fast) {
+    if (si.empty()) return;
+    std::vector<real> F, P, T, AO;
+    force(si, sj, !sinks_fast, 0, false, F, P, T, AO);
+    for (size_t k = 0; k < si.size(); k++) {
+      for (int d = 0; d < 3; d++) vel[3 * si[k] + d] += F[3 * k + d] * dt;
+      pot[si[k]] = P[k];
+    }
+  }
+  void kick_cpu(const std::vector<int> &s1, const std::vector<int> &s2, real dt) {
+    for (int i : s1) {
+      real a[3] = {0, 0, 0};
+      for (int j : s2) {
+        real dx[3];
+        for (int d = 0; d < 3; d++) dx[d] = pos[3 * i + d] - pos[3 * j + d];
+        real dr2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        if (dr2 > 0) {
+          real dr = std::sqrt(dr2), dr3 = dr * dr2, acci = mass[j] / dr3;
+          for (int d = 0; d < 3; d++) a[d] -= dx[d] * acci;
+        }
+      }
+      for (int d = 0; d < 3; d++) vel[3 * i + d] += dt * a[d];
+    }
+  }
+  void kick_gradient(const std::vector<int> &si, const std::vector<int> &sj, real dt,
+                     bool sinks_fast) {
+    if (si.empty()) return;
+    std::vector<real> F, P, T, AO;
+    force(si, sj, !sinks_fast, dt * dt / 24, false, F, P, T, AO);
+    for (size_t k = 0; k < si.size(); k++)
+      for (int d = 0; d < 3; d++) vel[3 * si[k] + d] += F[3 * k + d] * dt * 2 / 3;
+  }
+  void kick_naive(const std::vector<int> &s) {
+    if (s.empty()) return;
+    std::vector<real> F, P, T, AO;
+    force(s, {}, true, 0, true, F, P, T, AO);
+    real mn = HUGE_VAL;
+    for (size_t k = 0; k < s.size(); k++) {
+      int i = s[k];
+      pot[i] = P[k];
+      for (int d = 0; d < 3; d++) acc[3 * i + d] = F[3 * k + d];
+      acc_old[i] = AO[k];
+      real t = 1 / std::sqrt(std::sqrt(T[k]));
+      if (mn > t) mn = t;
+    }
+    for (int i : s) ts[i] = mn;
+  }
+  void drift(const std::vector<int> &s, real dt) {
+    for (int i : s)
+      for (int d = 0; d < 3; d++) {
+        real a = pos[3 * i + d];
+        pos_e[3 * i + d] += dt * vel[3 * i + d];
+        pos[3 * i + d] = a + pos_e[3 * i + d];
+        pos_e[3 * i + d] += a - pos[3 * i + d];
+      }
+  }
+  void omelyan(const std::vector<int> &s, real stime, real etime, real dt, bool calc) {
+    if (calc) kick_naive(s);
+    std::vector<int> fast, slow;
+    for (int i : s)
+      if (ts[i] < std::fabs(dt)) fast.push_back(i);
+    for (int i : s)
+      if (!(ts[i] < std::fabs(dt))) slow.push_back(i);
+    if (fast.size() == 1) {
+      slow.insert(slow.begin(), fast[0]);
+      fast.clear();
+    }
+    if (fast.empty()) simtime += dt;
+    auto outer = [&]() {
+      if (s.size() > 128) {
+        kick(slow, fast, dt / 6, false);
+        if (!fast.empty()) kick(fast, slow, dt / 6, true);
+      } else if (!slow.empty()) {
+        std::vector<int> joined = fast;
+        joined.insert(joined.end(), slow.begin(), slow.end());
+        kick_cpu(slow, joined, dt / 6);
+        if (!fast.empty()) kick_cpu(fast, slow, dt / 6);
+      }
+    };
+    outer();
+    if (!fast.empty()) omelyan(fast, stime, stime + dt / 2, dt / 2, false);
+    drift(slow, dt / 2);
+    if (!s.empty()) {
+      kick(slow, {}, 0, false);
+      kick_gradient(slow, {}, dt, false);
+      kick(slow, fast, 0, true);
+      kick(fast, slow, 0, true);
+      kick_gradient(slow, fast, dt, true);
+      kick_gradient(fast, slow, dt, true);
+    }
+    drift(slow, dt / 2);
+    if (!fast.empty()) omelyan(fast, stime + dt / 2, etime, dt / 2, true);
+    outer();
+  }
+  void do_evolve(real dt) {
+    if (dt == 0) return;
+    std::vector<int> all(n);
+    for (int i = 0; i < n; i++) all[i] = i;
+    kick_naive(all);
+    omelyan(all, simtime, simtime + dt, dt, true);
+  }
+};
+}  // namespace orc
+
 // ------------------------------------------------------------------ C API ----
 extern "C" {
 
@@ -1551,6 +1708,32 @@ int orc_master_fmm(void *h, int n, const double *x, const double *v, const doubl
     if (acc_old && !direct) acc_old[i] = b.acc_old;
   }
   if (st) std::memcpy(st, &F.st, sizeof(*st));
+  return 0;
+}
+
+// nsteps calls of the master do_evolve on device-free host arrays (in/out).
+int orc_master_evolve(int n, const double *mass, double *pos, double *pos_e, double *vel,
+                      double *ts, double *acc, double *pot, double *acc_old, int p, int ncrit,
+                      double force_accuracy, int fmm, int pair_mass, double dt_param, int nsteps,
+                      double dt, double *simtime) {
+  orc::MasterLoop L;
+  L.n = n;
+  L.mass = const_cast<double *>(mass);
+  L.pos = pos, L.pos_e = pos_e, L.vel = vel, L.ts = ts, L.acc = acc, L.pot = pot;
+  L.acc_old = acc_old;
+  std::memset(&L.prm, 0, sizeof(L.prm));
+  L.prm.p = p;
+  L.prm.ncrit = ncrit;
+  L.prm.theta = 0.5;
+  L.prm.mac = 1;
+  L.prm.force_accuracy = force_accuracy;
+  L.prm.quirk = 1;
+  L.fmm = fmm != 0;
+  L.pair_mass = pair_mass != 0;
+  L.dtp = dt_param;
+  L.simtime = *simtime;
+  for (int k = 0; k < nsteps; k++) L.do_evolve(dt);
+  *simtime = L.simtime;
   return 0;
 }
 

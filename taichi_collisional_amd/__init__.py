@@ -43,6 +43,16 @@ master_body_dtype = np.dtype({
     "itemsize": 128,
 })
 
+# struct particle of the reference (master exafmm.h:73-86), 168 bytes.
+particle_dtype = np.dtype({
+    "names": ["pos", "pos_e", "mass", "vel", "timestep", "acc", "pot", "jerk", "postime",
+              "acc_old", "id"],
+    "formats": [("<f8", 3), ("<f8", 3), "<f8", ("<f8", 3), "<f8", ("<f8", 3), "<f8",
+                ("<f8", 3), "<f8", "<f8", "<u4"],
+    "offsets": [0, 24, 48, 56, 80, 88, 112, 120, 144, 152, 160],
+    "itemsize": 168,
+})
+
 # mode bits of tc_master_eval_* (include/tcfmm.h)
 TC_MASTER_STEPS = 1
 TC_MASTER_FMM = 2
@@ -64,6 +74,11 @@ class TcStats(C.Structure):
                 ("ms_downward", C.c_double), ("ms_total", C.c_double),
                 ("n_m2l_low", C.c_longlong), ("n_p2p_low", C.c_longlong),
                 ("ms_prepass", C.c_double)]
+
+
+class TcParticles(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.c_void_p) for k in (
+        "pos", "pos_e", "vel", "mass", "timestep", "acc", "pot", "acc_old")]
 
 
 class TcCellLayout(C.Structure):
@@ -104,6 +119,10 @@ def lib():
         L.tc_last_error.restype = C.c_char_p
         L.tc_supported_orders.argtypes = [C.POINTER(C.c_int), C.c_int]
         L.tc_master_eval_bodies.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double]
+        L.tc_master_evolve.argtypes = [vp, C.POINTER(TcParticles), C.POINTER(C.c_double),
+                                       C.c_double, C.c_int, C.c_double]
+        L.tc_master_evolve_particles.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double),
+                                                 C.c_double, C.c_int, C.c_double]
         L.tc_master_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_double, vp,
                                          vp, vp, vp]
         _lib = L
@@ -113,7 +132,8 @@ def lib():
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_acc_old",
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
-                    "tc_P2P", "tc_master_eval_bodies", "tc_master_eval_soa", "tc_get_stats",
+                    "tc_P2P", "tc_master_eval_bodies", "tc_master_eval_soa",
+                    "tc_master_evolve", "tc_master_evolve_particles", "tc_get_stats",
                     "tc_last_error", "tc_supported_orders"]
 
 
@@ -268,6 +288,31 @@ class Fmm:
                                              mode, float(dt_param), _ptr(pot), _ptr(acc),
                                              _ptr(ts), _ptr(acc_old)), "tc_master_eval_soa")
         return pot, acc, ts
+
+    def master_evolve(self, state, dt, simtime=0.0, mode=0, dt_param=0.025):
+        """One do_evolve(sys, dt) of the master step loop (GRADIENT scheme) on
+        device tensors: state is a dict with pos, pos_e, vel, acc (n,3), mass,
+        timestep, pot, acc_old (n,) float64 cuda tensors, updated in place.
+        Returns the new simulation time."""
+        n = state["mass"].shape[0]
+        sp = TcParticles(n, *[state[k].data_ptr() for k in (
+            "pos", "pos_e", "vel", "mass", "timestep", "acc", "pot", "acc_old")])
+        t = C.c_double(simtime)
+        self._check(lib().tc_master_evolve(self.h, C.byref(sp), C.byref(t), float(dt), int(mode),
+                                           float(dt_param)), "tc_master_evolve")
+        return t.value
+
+    def master_evolve_particles(self, part, dt, simtime=0.0, mode=0, dt_param=0.025):
+        """The same on a host array of particle_dtype records (the reference's
+        struct particle); returns the new simulation time."""
+        if part.dtype != particle_dtype or not part.flags.c_contiguous:
+            raise TypeError("part must be a C-contiguous array of particle_dtype")
+        t = C.c_double(simtime)
+        self._check(lib().tc_master_evolve_particles(self.h, part.ctypes.data, part.size,
+                                                     C.byref(t), float(dt), int(mode),
+                                                     float(dt_param)),
+                    "tc_master_evolve_particles")
+        return t.value
 
     # ---- per-kernel entry points over the reference's Cell / Body records ----
     def _recs(self, bodies, cells):

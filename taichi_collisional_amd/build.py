@@ -27,7 +27,7 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]
-UNITS = ["tc_m2l.hip", "tc_tree.hip", "tc_upward.hip", "tc_p2p.hip", "tc_downward.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip",
+UNITS = ["tc_m2l.hip", "tc_tree.hip", "tc_upward.hip", "tc_p2p.hip", "tc_downward.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip", "tc_evolve.hip",
          "tc_engine.hip"]
 
 

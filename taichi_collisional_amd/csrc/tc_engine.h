@@ -140,6 +140,13 @@ struct tc_context {
     }                                                                    \
   } while (0)
 
+// The master variant's get_force_and_potential on device arrays (x, v [n][3],
+// m, flags): pot, acc [n][3], tstep (TC_MASTER_STEPS), acc_old (in/out, the
+// TC_MASTER_FMM pre-pass).  tc_engine.hip; used by the step loop too.
+int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
+                       const double *m, const unsigned char *flags, int mode, double dt,
+                       double *pot, double *acc, double *tstep, double *acc_old);
+
 inline BodiesDev bodies_of(tc_context *ctx) {
   return {ctx->bx.as<double>(),   ctx->by.as<double>(),  ctx->bz.as<double>(),
           ctx->bvx.as<double>(),  ctx->bvy.as<double>(), ctx->bvz.as<double>(),

@@ -690,9 +690,9 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
 // get_force_and_potential of the reference's top-level sources (master
 // integrator.h:459-562).  The tree is the master one: getBounds there has no
 // root shift (master build_tree.h), so the shift is zero here.
-static int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
-                              const double *m, const unsigned char *flags, int mode, double dt,
-                              double *pot, double *acc, double *tstep, double *acc_old) {
+int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
+                       const double *m, const unsigned char *flags, int mode, double dt,
+                       double *pot, double *acc, double *tstep, double *acc_old) {
   hipStream_t s = ctx->stream;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->has_acc_old = false;

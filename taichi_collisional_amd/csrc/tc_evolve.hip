@@ -1,0 +1,461 @@
+// The master variant's step loop on the device: do_evolve with the
+// fourth-order force-gradient scheme the master Makefile builds (GRADIENT:
+// evolve_split_omelyan + kick_gradient, master integrator.h:720-812, 313-376,
+// with kick_naive :182-306, kick :378-457, kick_cpu :65-98, split :592-640,
+// drift with compensated summation :642-655, COMPSUM :42).  File and line
+// citations refer to the reference's top-level (master) sources.
+//
+// The particles stay resident in HBM as arrays (positions, their compensated
+// error terms, velocities, masses, time steps, accelerations, potentials,
+// acc_old).  The control flow -- the recursion over time-step levels and the
+// sequence of kicks and drifts -- runs on the host; every kick is one force
+// evaluation of the master variant (tc_engine.hip) on the union of its sinks
+// and sources, gathered by index into the context's staging arrays.
+//
+// split() in the reference swaps particle records to put the fast ones first;
+// here a subsystem is an index list and split is a stable partition of it,
+// so the caller's arrays keep their order (the reference tracks identity by
+// particle id).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "tc_engine.h"
+
+namespace tcfmm {
+
+namespace {
+
+// union = sinks ++ sources: positions (shifted by acc * shift_fac, the
+// kick_gradient position of integrator.h:322-341; 0 for plain kicks),
+// velocities, masses and sink/source flags.
+__global__ void k_gather_union(const int *__restrict__ is, int ns, const int *__restrict__ js,
+                               int nj, bool sinks_are_sources, const double *__restrict__ pos,
+                               const double *__restrict__ vel, const double *__restrict__ mass,
+                               const double *__restrict__ acc, double shift_fac,
+                               double *__restrict__ x, double *__restrict__ v,
+                               double *__restrict__ m, unsigned char *__restrict__ fl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns + nj) return;
+  const bool sink = k < ns;
+  const int i = sink ? is[k] : js[k - ns];
+  for (int d = 0; d < 3; d++) {
+    x[3 * k + d] = shift_fac != 0.0 ? pos[3 * i + d] + acc[3 * i + d] * shift_fac : pos[3 * i + d];
+    v[3 * k + d] = vel[3 * i + d];
+  }
+  m[k] = mass[i];
+  fl[k] = sink ? (kSink | (sinks_are_sources ? kSource : 0)) : kSource;
+}
+
+// vel[sinks] += F * dt * fac (fac: 1 for kicks, 2/3 for gradient kicks, applied
+// as ((F dt) 2) / 3 like integrator.h:350-353); pot[sinks] = p when set_pot.
+__global__ void k_apply_kick(const int *__restrict__ is, int ns, const double *__restrict__ F,
+                             const double *__restrict__ p, double dt, bool gradient,
+                             bool set_pot, double *__restrict__ vel, double *__restrict__ pot) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const int i = is[k];
+  for (int d = 0; d < 3; d++) {
+    const double dv = gradient ? F[3 * k + d] * dt * 2.0 / 3.0 : F[3 * k + d] * dt;
+    vel[3 * i + d] += dv;
+  }
+  if (set_pot) pot[i] = p[k];
+}
+
+// kick_naive's bookkeeping (integrator.h:250-300) for the subsystem's bodies:
+// pot, acc, acc_old copied; the time step 1/sqrt(sqrt(sum)) into tmp for the
+// minimum; vel += F * dt with dt = 0 is skipped (it changes nothing finite).
+__global__ void k_naive_results(const int *__restrict__ is, int ns, const double *__restrict__ F,
+                                const double *__restrict__ p, const double *__restrict__ tsum,
+                                const double *__restrict__ ao, double *__restrict__ acc,
+                                double *__restrict__ pot, double *__restrict__ acc_old,
+                                double *__restrict__ tmp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const int i = is[k];
+  pot[i] = p[k];
+  for (int d = 0; d < 3; d++) acc[3 * i + d] = F[3 * k + d];
+  acc_old[i] = ao ? ao[k] : 0.0;
+  tmp[k] = 1.0 / sqrt(sqrt(tsum[k]));
+}
+
+__global__ void k_set_timestep(const int *__restrict__ is, int ns, const double *__restrict__ mn,
+                               double *__restrict__ ts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ns) ts[is[k]] = *mn;
+}
+
+// One block: minimum of tmp[0..n) into *out (fmin ignores NaN as the
+// reference's `if(min > t)` test does).
+__global__ __launch_bounds__(1024) void k_min(const double *__restrict__ tmp, int n,
+                                              double *__restrict__ out) {
+  __shared__ double red[16];
+  double v = INFINITY;
+  for (int k = threadIdx.x; k < n; k += 1024) v = fmin(v, tmp[k]);
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 16; w++) r = fmin(r, red[w]);
+    *out = r;
+  }
+}
+
+// COMPSUM(s, e, ds): a = s; e += ds; s = a + e; e += a - s (integrator.h:42)
+__global__ void k_drift(const int *__restrict__ is, int ns, double dt,
+                        const double *__restrict__ vel, double *__restrict__ pos,
+                        double *__restrict__ pos_e) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const int i = is[k];
+  for (int d = 0; d < 3; d++) {
+    const double a = pos[3 * i + d];
+    double e = pos_e[3 * i + d] + dt * vel[3 * i + d];
+    const double s = a + e;
+    e += a - s;
+    pos[3 * i + d] = s;
+    pos_e[3 * i + d] = e;
+  }
+}
+
+__global__ void k_split_flags(const int *__restrict__ is, int ns, const double *__restrict__ ts,
+                              double dt, int *__restrict__ fast) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= ns) fast[k] = k < ns ? (ts[is[k]] < dt ? 1 : 0) : 0;
+}
+
+// stable partition: fast ones first (their order kept), then the slow ones
+__global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *__restrict__ fast,
+                                const int *__restrict__ off, int nfast, int *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const int pos = fast[k] ? off[k] : nfast + (k - off[k]);
+  out[pos] = is[k];
+}
+
+inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+// A subsystem: a range of one index buffer (device).
+struct Sub {
+  const int *idx = nullptr;
+  int n = 0;
+};
+
+class Evolver {
+ public:
+  Evolver(tc_context *ctx, const tc_particles &p, int mode, double dt_param)
+      : ctx_(ctx), p_(p), mode_(mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS)), dtp_(dt_param) {}
+  ~Evolver() {
+    hipStreamSynchronize(ctx_->stream);
+    for (DBuf &b : lev_) b.release();
+    for (DBuf *b : {&all_, &ux_, &uv_, &um_, &ufl_, &upot_, &uacc_, &uts_, &uao_, &tmp_,
+                    &flags_, &offs_, &scan_tmp_})
+      b->release();
+  }
+
+  int run(double *simtime, double dt);
+
+ private:
+  tc_context *ctx_;
+  tc_particles p_;
+  int mode_;
+  double dtp_;
+  double simtime_ = 0;
+  // index buffers: one per recursion level holds that level's subsystem
+  // partitioned into [fast | slow]
+  std::vector<DBuf> lev_;
+  DBuf all_, ux_, uv_, um_, ufl_, upot_, uacc_, uts_, uao_, tmp_, flags_, offs_, scan_tmp_;
+
+  int ensure_union(int nu);
+  int force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
+            bool with_acc_old);
+  int kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
+  int kick_cpu(Sub s1, Sub s2, double dt);
+  int kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
+  int kick_naive(Sub s);
+  int drift(Sub s, double dt);
+  int split(int level, Sub s, double dt, Sub &slow, Sub &fast);
+  int omelyan(int level, Sub s, double stime, double etime, double dt, bool calc_timestep);
+};
+
+int Evolver::ensure_union(int nu) {
+  tc_context *ctx = ctx_;
+  const size_t n = (size_t)std::max(nu, 1);
+  TC_CHECK(ux_.ensure(24 * n));
+  TC_CHECK(uv_.ensure(24 * n));
+  TC_CHECK(um_.ensure(8 * n));
+  TC_CHECK(ufl_.ensure(n));
+  TC_CHECK(upot_.ensure(8 * n));
+  TC_CHECK(uacc_.ensure(24 * n));
+  TC_CHECK(uts_.ensure(8 * n));
+  TC_CHECK(uao_.ensure(8 * n));
+  TC_CHECK(tmp_.ensure(8 * n + 8));
+  return 0;
+}
+
+// One get_force_and_potential on sinks ++ sources (kick_naive / kick /
+// kick_gradient set the bodies up this way, integrator.h:190-245, 318-351,
+// 383-424).  Results for the sinks are the first sinks.n entries of upot_ /
+// uacc_ (and uts_ with steps).
+int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
+                   bool with_acc_old) {
+  tc_context *ctx = ctx_;
+  const int nu = sinks.n + sources.n;
+  if (nu == 0) return 0;
+  if (int e = ensure_union(nu)) return e;
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_gather_union, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
+                     sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass, p_.acc,
+                     shift_fac, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
+                     ufl_.as<unsigned char>());
+  double *ao = nullptr;
+  if (with_acc_old && (mode_ & TC_MASTER_FMM)) {
+    // kick_naive builds fresh bodies: the pre-pass starts from 0
+    ao = uao_.as<double>();
+    TC_CHECK(hipMemsetAsync(ao, 0, 8 * (size_t)nu, s));
+  }
+  return master_eval_device(ctx, nu, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
+                            ufl_.as<unsigned char>(), mode_ | (steps ? TC_MASTER_STEPS : 0), dtp_,
+                            upot_.as<double>(), uacc_.as<double>(),
+                            steps ? uts_.as<double>() : nullptr, ao);
+}
+
+// kick (integrator.h:378-457): vel[sinks] += F dt, pot[sinks] = p
+int Evolver::kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast) {
+  if (sinks.n == 0) return 0;
+  if (int e = force(sinks, sources, !sinks_are_fast, 0.0, false, false)) return e;
+  hipLaunchKernelGGL(k_apply_kick, grid_for(sinks.n), dim3(256), 0, ctx_->stream, sinks.idx,
+                     sinks.n, uacc_.as<double>(), upot_.as<double>(), dt, false, true, p_.vel,
+                     p_.pot);
+  return 0;
+}
+
+// kick_cpu (integrator.h:65-98): the all-pairs kick of s1 by s2 the
+// reference uses for subsystems of <= 128 bodies; the potential is not set.
+int Evolver::kick_cpu(Sub s1, Sub s2, double dt) {
+  if (s1.n == 0) return 0;
+  // s2 may contain s1 (join(slow, fast)); r = 0 pairs are skipped either way,
+  // so s1 as sinks that are not sources plus s2 as sources is the same sum
+  if (int e = force(s1, s2, false, 0.0, false, false)) return e;
+  hipLaunchKernelGGL(k_apply_kick, grid_for(s1.n), dim3(256), 0, ctx_->stream, s1.idx, s1.n,
+                     uacc_.as<double>(), upot_.as<double>(), dt, false, false, p_.vel, p_.pot);
+  return 0;
+}
+
+// kick_gradient (integrator.h:313-376): positions shifted by acc dt^2/24,
+// vel[sinks] += F dt 2/3.  (The reference also sets pot from an array it
+// never fills; that is not reproduced.)
+int Evolver::kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast) {
+  if (sinks.n == 0) return 0;
+  const double fac = dt * dt / 24.0;
+  if (int e = force(sinks, sources, !sinks_are_fast, fac, false, false)) return e;
+  hipLaunchKernelGGL(k_apply_kick, grid_for(sinks.n), dim3(256), 0, ctx_->stream, sinks.idx,
+                     sinks.n, uacc_.as<double>(), upot_.as<double>(), dt, true, false, p_.vel,
+                     p_.pot);
+  return 0;
+}
+
+// kick_naive(clevel, s, zerosys, zerosys, 0, true) (integrator.h:182-306)
+// with GRADIENT: every body of s gets the minimum over s of the time step.
+int Evolver::kick_naive(Sub s) {
+  tc_context *ctx = ctx_;
+  if (s.n == 0) return 0;
+  if (int e = force(s, Sub{}, true, 0.0, true, true)) return e;
+  hipStream_t st = ctx->stream;
+  double *tmp = tmp_.as<double>();
+  const double *ao = (mode_ & TC_MASTER_FMM) ? uao_.as<double>() : nullptr;
+  hipLaunchKernelGGL(k_naive_results, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
+                     uacc_.as<double>(), upot_.as<double>(), uts_.as<double>(), ao, p_.acc,
+                     p_.pot, p_.acc_old, tmp);
+  hipLaunchKernelGGL(k_min, dim3(1), dim3(1024), 0, st, tmp, s.n, tmp + s.n);
+  hipLaunchKernelGGL(k_set_timestep, grid_for(s.n), dim3(256), 0, st, s.idx, s.n, tmp + s.n,
+                     p_.timestep);
+  return 0;
+}
+
+int Evolver::drift(Sub s, double dt) {
+  if (s.n == 0) return 0;
+  hipLaunchKernelGGL(k_drift, grid_for(s.n), dim3(256), 0, ctx_->stream, s.idx, s.n, dt, p_.vel,
+                     p_.pos, p_.pos_e);
+  return 0;
+}
+
+// split (integrator.h:592-640): fast = time step < |dt|, first; a single fast
+// body is not split off.
+int Evolver::split(int level, Sub s, double dt, Sub &slow, Sub &fast) {
+  tc_context *ctx = ctx_;
+  hipStream_t st = ctx->stream;
+  if ((int)lev_.size() <= level) lev_.resize(level + 1);
+  DBuf &out = lev_[level];
+  TC_CHECK(out.ensure(sizeof(int) * (size_t)std::max(s.n, 1)));
+  TC_CHECK(flags_.ensure(sizeof(int) * ((size_t)s.n + 1)));
+  TC_CHECK(offs_.ensure(sizeof(int) * ((size_t)s.n + 1)));
+  TC_CHECK(scan_tmp_.ensure(scan_temp_bytes(s.n + 1) + 256));
+  hipLaunchKernelGGL(k_split_flags, grid_for(s.n + 1), dim3(256), 0, st, s.idx, s.n, p_.timestep,
+                     std::fabs(dt), flags_.as<int>());
+  if (launch_scan(flags_.as<int>(), offs_.as<int>(), s.n + 1, scan_tmp_.p, scan_tmp_.bytes, st)) {
+    ctx->err = "scan failed";
+    return -4;
+  }
+  TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, offs_.as<int>() + s.n, sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+  TC_CHECK(hipStreamSynchronize(st));
+  int nfast = ctx->h_pin[12];
+  hipLaunchKernelGGL(k_split_scatter, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
+                     flags_.as<int>(), offs_.as<int>(), nfast, out.as<int>());
+  if (nfast == 1) nfast = 0;
+  fast = Sub{out.as<int>(), nfast};
+  slow = Sub{out.as<int>() + nfast, s.n - nfast};
+  return 0;
+}
+
+// evolve_split_omelyan (integrator.h:720-812)
+int Evolver::omelyan(int level, Sub s, double stime, double etime, double dt,
+                     bool calc_timestep) {
+  tc_context *ctx = ctx_;
+  if (level >= 64) {  // MAXLEVEL (integrator.h:8)
+    ctx->err = "time step too small: more than 64 levels";
+    return -2;
+  }
+  if (calc_timestep)
+    if (int e = kick_naive(s)) return e;
+  Sub slow, fast;
+  if (int e = split(level, s, dt, slow, fast)) return e;
+  if (fast.n == 0) simtime_ += dt;
+  auto outer_kicks = [&]() -> int {
+    if (s.n > 128) {
+      if (int e = kick(slow, fast, dt / 6, false)) return e;
+      if (fast.n > 0)
+        if (int e = kick(fast, slow, dt / 6, true)) return e;
+    } else if (slow.n > 0) {
+      // join(slow, fast) is s itself
+      if (int e = kick_cpu(slow, s, dt / 6)) return e;
+      if (fast.n > 0)
+        if (int e = kick_cpu(fast, slow, dt / 6)) return e;
+    }
+    return 0;
+  };
+  if (int e = outer_kicks()) return e;
+  if (fast.n > 0)
+    if (int e = omelyan(level + 1, fast, stime, stime + dt / 2, dt / 2, false)) return e;
+  if (int e = drift(slow, dt / 2)) return e;
+  if (s.n > 0) {
+    const Sub none;
+    if (int e = kick(slow, none, 0.0, false)) return e;
+    if (int e = kick_gradient(slow, none, dt, false)) return e;
+    if (int e = kick(slow, fast, 0.0, true)) return e;
+    if (int e = kick(fast, slow, 0.0, true)) return e;
+    if (int e = kick_gradient(slow, fast, dt, true)) return e;
+    if (int e = kick_gradient(fast, slow, dt, true)) return e;
+  }
+  if (int e = drift(slow, dt / 2)) return e;
+  if (fast.n > 0)
+    if (int e = omelyan(level + 1, fast, stime + dt / 2, etime, dt / 2, true)) return e;
+  return outer_kicks();
+}
+
+// do_evolve (integrator.h:864-884) with GRADIENT
+int Evolver::run(double *simtime, double dt) {
+  tc_context *ctx = ctx_;
+  simtime_ = *simtime;
+  if (dt == 0) return 0;
+  const int n = p_.n;
+  TC_CHECK(all_.ensure(sizeof(int) * (size_t)n));
+  std::vector<int> id(n);
+  for (int i = 0; i < n; i++) id[i] = i;
+  TC_CHECK(hipMemcpyAsync(all_.p, id.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                          ctx->stream));
+  const Sub all{all_.as<int>(), n};
+  if (int e = kick_naive(all)) return e;
+  if (int e = omelyan(0, all, simtime_, simtime_ + dt, dt, true)) return e;
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  TC_CHECK(hipGetLastError());
+  *simtime = simtime_;
+  return 0;
+}
+
+}  // namespace tcfmm
+
+extern "C" int tc_master_evolve(tc_context *ctx, const tc_particles *p, double *simtime,
+                                double dt, int mode, double dt_param) {
+  if (!ctx) return -1;
+  if (!p || !simtime || p->n <= 0 || !p->pos || !p->pos_e || !p->vel || !p->mass ||
+      !p->timestep || !p->acc || !p->pot || !p->acc_old) {
+    ctx->err = "tc_master_evolve: every particle array is required";
+    return -1;
+  }
+  if (!(dt_param > 0) || !std::isfinite(dt)) {
+    ctx->err = "tc_master_evolve: dt_param > 0 and a finite dt are required";
+    return -1;
+  }
+  tcfmm::Evolver ev(ctx, *p, mode, dt_param);
+  return ev.run(simtime, dt);
+}
+
+extern "C" int tc_master_evolve_particles(tc_context *ctx, tc_particle *part, int n,
+                                          double *simtime, double dt, int mode,
+                                          double dt_param) {
+  if (!ctx) return -1;
+  if (!part || n <= 0 || !simtime) {
+    ctx->err = "tc_master_evolve_particles: particles and simtime are required";
+    return -1;
+  }
+  using tcfmm::DBuf;
+  const size_t N = (size_t)n;
+  // device arrays: pos, pos_e, vel, acc [3n]; mass, timestep, pot, acc_old [n]
+  DBuf buf;
+  TC_CHECK(buf.ensure(sizeof(double) * 16 * N));
+  std::vector<double> h(16 * N);
+  double *hp = h.data(), *he = hp + 3 * N, *hv = he + 3 * N, *ha = hv + 3 * N;
+  double *hm = ha + 3 * N, *ht = hm + N, *hpot = ht + N, *hao = hpot + N;
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      hp[3 * i + d] = part[i].pos[d];
+      he[3 * i + d] = part[i].pos_e[d];
+      hv[3 * i + d] = part[i].vel[d];
+      ha[3 * i + d] = part[i].acc[d];
+    }
+    hm[i] = part[i].mass;
+    ht[i] = part[i].timestep;
+    hpot[i] = part[i].pot;
+    hao[i] = part[i].acc_old;
+  }
+  hipStream_t s = ctx->stream;
+  TC_CHECK(hipMemcpyAsync(buf.p, h.data(), sizeof(double) * 16 * N, hipMemcpyHostToDevice, s));
+  double *d = buf.as<double>();
+  tc_particles p;
+  p.n = n;
+  p.pos = d;
+  p.pos_e = d + 3 * N;
+  p.vel = d + 6 * N;
+  p.acc = d + 9 * N;
+  p.mass = d + 12 * N;
+  p.timestep = d + 13 * N;
+  p.pot = d + 14 * N;
+  p.acc_old = d + 15 * N;
+  const double t0 = *simtime;
+  int e = tc_master_evolve(ctx, &p, simtime, dt, mode, dt_param);
+  if (e == 0) {
+    TC_CHECK(hipMemcpyAsync(h.data(), buf.p, sizeof(double) * 16 * N, hipMemcpyDeviceToHost, s));
+    TC_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) {
+      for (int d3 = 0; d3 < 3; d3++) {
+        part[i].pos[d3] = hp[3 * i + d3];
+        part[i].pos_e[d3] = he[3 * i + d3];
+        part[i].vel[d3] = hv[3 * i + d3];
+        part[i].acc[d3] = ha[3 * i + d3];
+      }
+      part[i].timestep = ht[i];
+      part[i].pot = hpot[i];
+      part[i].acc_old = hao[i];
+      if (dt != 0) part[i].postime = t0 + dt;
+    }
+  }
+  buf.release();
+  return e;
+}
