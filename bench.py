@@ -121,6 +121,33 @@ print(json.dumps({"cpu_reference_seconds": time.time() - t}))
     return out
 
 
+def master_evolve_cpu_baseline(n, dt, seed, threads, timeout):
+    """The master reference's do_evolve (GRADIENT) on the host, one step."""
+    code = r"""
+import sys, time, json, os
+sys.path.insert(0, %r); sys.path.insert(0, os.path.join(%r, "oracle"))
+import numpy as np, pyoracle as po
+from taichi_collisional_amd import ics
+if not po.master_ref_available(8):
+    print(json.dumps({})); sys.exit(0)
+x, v, m = ics.plummer(%d, seed=%d)
+t = time.time()
+po.master_ref_evolve(8, m, x, v, 1, %r, ncrit=300)
+print(json.dumps({"cpu_reference_seconds": time.time() - t}))
+""" % (ROOT, ROOT, n, seed, dt)
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               OMP_WAIT_POLICY="active", OMP_STACKSIZE="100M")
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=timeout, env=env)
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception:  # noqa: BLE001 - the row just lacks the CPU figure
+        return {}
+    out["cpu_cores"] = threads
+    return out
+
+
 def load_traffic(path):
     try:
         d = json.load(open(path))
@@ -273,6 +300,37 @@ def main():
             if row.get("cpu_reference_seconds"):
                 row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (mms * 1e-3)
         extra["master_direct"] = row
+        # the master step loop: one do_evolve (GRADIENT scheme) on N = 4096
+        ne, dte = 4096, 1.0 / 1024
+        xe, ve, me = ics.plummer(ne, seed=args.seed)
+        est = {k: torch.zeros(ne, 3, dtype=torch.float64, device=dev)
+              for k in ("pos_e", "acc")}
+        est.update({k: torch.zeros(ne, dtype=torch.float64, device=dev)
+                   for k in ("timestep", "pot", "acc_old")})
+        est["pos"], est["vel"], est["mass"] = (torch.from_numpy(a).to(dev) for a in (xe, ve, me))
+        fe = tca.Fmm(p=args.p, ncrit=300, direct_below=0)
+        warm = {k: a.clone() for k, a in est.items()}
+        fe.master_evolve(warm, dte)  # warm-up on a copy (first launches, allocations)
+        del warm
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fe.master_evolve(est, dte)
+        torch.cuda.synchronize()
+        ems = (time.perf_counter() - t) * 1e3
+        tau = float(est["timestep"][0])
+        levels = 0
+        while dte / 2 ** levels > tau:
+            levels += 1
+        row = {"what": "master step loop: one do_evolve(sys, dt) of the GRADIENT scheme, "
+                       "Plummer N=4096, dt=1/1024, dt_param 0.025, default-build forces",
+               "ms_per_step": ems, "levels": levels, "substeps": 2 ** levels}
+        del fe
+        if cpu is not None:
+            row.update(master_evolve_cpu_baseline(ne, dte, args.seed, args.cpu_threads,
+                                                  args.cpu_timeout))
+            if row.get("cpu_reference_seconds"):
+                row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (ems * 1e-3)
+        extra["master_evolve"] = row
 
     if args.calibrate_copy_mib > 0:
         src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)

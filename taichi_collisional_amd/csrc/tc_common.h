@@ -45,6 +45,18 @@ __device__ __forceinline__ double rsqrt_fp64(double x) {
   return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
+// 1/x in FP64 without the division sequence: the hardware estimate
+// v_rcp_f64 and two Newton steps y <- y + y (1 - x y), each squaring the
+// relative error, to within an ulp or two of 1/x.  For finite nonzero x
+// inside the estimate's range (|x| between about 1e-300 and 1e300).
+__device__ __forceinline__ double rcp_fp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 // Wave-wide sum of a double, result in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -711,17 +711,31 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     return -1;
   }
   if (int e = ensure_bodies(ctx, n)) return e;
-  const double zero[3] = {0, 0, 0};
-  std::memcpy(ctx->shift, zero, sizeof(zero));
-  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
   TC_CHECK(hipEventRecord(ctx->ev[0], s));
-  if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
   // the -DFMM build runs direct() below ncrit bodies (integrator.h:462-465)
-  if ((mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit)
-    return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
+  const bool fmm = (mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit;
+  if (steps || fmm) {
+    // the master tree has no root shift
+    const double zero[3] = {0, 0, 0};
+    std::memcpy(ctx->shift, zero, sizeof(zero));
+    std::memcpy(ctx->h_pin + 16, zero, sizeof(zero));
+    TC_CHECK(hipMemcpyAsync(ctx->dshift.p, ctx->h_pin + 16, sizeof(zero), hipMemcpyHostToDevice,
+                            s));
+  }
   TC_CHECK(ctx->mlend.ensure((size_t)n));
-  launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
-                   ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
+  if (!steps && !fmm) {
+    // direct() without time steps: the sums over leaf pairs are the all-pairs
+    // sums, and the leaves do not enter them, so no tree is built
+    TC_CHECK(ctx->dtg.ensure(sizeof(int) * (size_t)n));
+    launch_iota(ctx->dtg.as<int>(), n, s);
+    launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, bodies_of(ctx), s);
+    launch_zero_u8(ctx->mlend.as<unsigned char>(), n, s);
+  } else {
+    if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
+    if (fmm) return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
+    launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
+                     ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
+  }
   TC_CHECK(hipEventRecord(ctx->ev[1], s));
   launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, steps, pair_mass, dt,
                        pot, acc, tstep, s);
@@ -739,7 +753,7 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
   ctx->st.ms_tree = ms[0];
   ctx->st.ms_p2p = ms[1];
   ctx->st.ms_total = ms[2];
-  ctx->st.n_p2p_cells = (long long)ctx->nleaves * ctx->nleaves;
+  ctx->st.n_p2p_cells = (!steps && !fmm) ? 1 : (long long)ctx->nleaves * ctx->nleaves;
   ctx->st.n_p2p_pairs = (long long)n * n;
   return 0;
 }

@@ -166,6 +166,8 @@ int launch_l2l_ids(int p, CellsDev c, const int *ids, int nids, hipStream_t s);
 int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids, int nleaves,
                hipStream_t s);
 // master (non-jerk) variant, tc_master.hip
+void launch_iota(int *p, int n, hipStream_t s);
+void launch_zero_u8(unsigned char *p, int n, hipStream_t s);
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
                       int n, unsigned char *lend, hipStream_t s);
 void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,

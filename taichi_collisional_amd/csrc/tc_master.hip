@@ -34,6 +34,11 @@ __global__ void k_fill_u8(unsigned char *p, int n) {
   if (i < n) p[i] = 0;
 }
 
+__global__ void k_iota(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
 // lend[last body of every leaf] = 1
 __global__ void k_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody,
                             const int *cnbody, unsigned char *lend) {
@@ -53,27 +58,34 @@ __global__ void k_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody,
 //    q_i + q_j, the fly-by estimate only for v2 > 0.
 // tau_ff = dt/sqrt(2) r sqrt(r / M), tau_fb = dt r / |dv|, each divided by
 // (1 - dtau/2) with its rate term dtau capped at 1.
+// The vector form is written with reciprocal square roots and reciprocals
+// instead of the divisions and square roots of the source expression
+// (ir = 1/r, r sqrt(r / M) = r^2 / sqrt(r M), mi / (v2 r) = mi ir / v2): the
+// same quantities to an ulp or two each, at a fraction of the instructions.
+// ir is the target-source 1/r that the force uses as well (r2 = 0 pairs use
+// the 1e38 stand-in, whose force is masked).
 template <bool kPairMass>
 __device__ __forceinline__ void pair_tau(double dx, double dy, double dz, double dvx, double dvy,
-                                         double dvz, double r2, double qi, double qj, double dt,
-                                         double &mi, double &ts) {
+                                         double dvz, double r2, double ir, double qi, double qj,
+                                         double dt, double &mi, double &ts) {
   const double dts2 = dt / M_SQRT2;
   if (!kPairMass) {
     const double v2 = dvx * dvx + dvy * dvy + dvz * dvz + 1e-20;
     const bool pos = r2 > 0;
     mi += pos ? qj : 0.0;
     const double rr = pos ? r2 : 1e38;
-    const double vdr = (dx * dvx + dy * dvy + dz * dvz) / rr;
-    const double r = sqrt(rr);
-    double tau = dts2 * r * sqrt(r / mi);
+    const double vdr = (dx * dvx + dy * dvy + dz * dvz) * (ir * ir);
+    const double r = rr * ir;
+    double tau = dts2 * rr * rsqrt_fp64(r * mi);
     double dtau = 3.0 * tau * vdr / 2.0;
     dtau = dtau < 1.0 ? dtau : 1.0;
-    tau /= (1.0 - dtau / 2.0);
+    tau *= rcp_fp64(1.0 - dtau / 2.0);
     ts = fmin(tau, ts);
-    tau = dt * r / sqrt(v2);
-    dtau = tau * vdr * (1.0 + mi / (v2 * r));
+    const double iv = rsqrt_fp64(v2);
+    tau = dt * r * iv;
+    dtau = tau * vdr * (1.0 + mi * ir * (iv * iv));
     dtau = dtau < 1.0 ? dtau : 1.0;
-    tau /= (1.0 - dtau / 2.0);
+    tau *= rcp_fp64(1.0 - dtau / 2.0);
     ts = fmin(tau, ts);
   } else {
     if (!(r2 > 0)) return;
@@ -130,12 +142,12 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
       const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz;
       const unsigned char f = sf[k];
+      const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
       if (kSteps)
-        pair_tau<kPairMass>(dx, dy, dz, svx[k] - vxi, svy[k] - vyi, svz[k] - vzi, r2, qi, sq[k],
-                            dt, mi, ts);
+        pair_tau<kPairMass>(dx, dy, dz, svx[k] - vxi, svy[k] - vyi, svz[k] - vzi, r2, ir, qi,
+                            sq[k], dt, mi, ts);
       // P2P_simple / P2P force: sources only, r2 = 0 masked
       const bool use = (f & 1) && r2 > 0;
-      const double ir = rsqrt_fp64(use ? r2 : 1.0);
       const double w = use ? sq[k] * ir : 0.0;
       pt += w;
       const double w3 = w * ir * ir;
@@ -212,11 +224,11 @@ __global__ __launch_bounds__(64) void k_master_p2p(CellsDev c, BodiesDev b,
     for (int j = j0; j < j0 + nj; j++) {
       const double dx = b.x[j] - xi, dy = b.y[j] - yi, dz = b.z[j] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
       if (kSteps)
-        pair_tau<kPairMass>(dx, dy, dz, b.vx[j] - vxi, b.vy[j] - vyi, b.vz[j] - vzi, r2, qi,
+        pair_tau<kPairMass>(dx, dy, dz, b.vx[j] - vxi, b.vy[j] - vyi, b.vz[j] - vzi, r2, ir, qi,
                             b.q[j], dt, mi, ts);
       const bool use = (b.flags[j] & kSource) && r2 > 0;
-      const double ir = rsqrt_fp64(use ? r2 : 1.0);
       const double w = use ? b.q[j] * ir : 0.0;
       pt += w;
       const double w3 = w * ir * ir;
@@ -267,6 +279,14 @@ void launch_master_p2p(const CellsDev &c, const BodiesDev &b, const int2 *pairs,
 void launch_scatter_ts(const BodiesDev &b, const double *ts_sorted, int n, double *out,
                        hipStream_t s) {
   if (n > 0) k_scatter_ts<<<(n + 255) / 256, 256, 0, s>>>(b, ts_sorted, n, out);
+}
+
+void launch_iota(int *p, int n, hipStream_t s) {
+  if (n > 0) k_iota<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+}
+
+void launch_zero_u8(unsigned char *p, int n, hipStream_t s) {
+  if (n > 0) k_fill_u8<<<(n + 255) / 256, 256, 0, s>>>(p, n);
 }
 
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
