@@ -9,10 +9,10 @@
 //
 // GPU form.  The swap S_n is the same matrix for every pair, so it is applied
 // as a product of constant sparse blocks against each lane's coefficients:
-// one lane = one (target, source) pair, the integer swap entries
-// (k * 2^-(n-1)) are folded into the instruction stream as literal operands of
-// v_fmac_f64, and only the per-pair z-rotations and translation factors live in
-// registers.  Work that the translation cannot see is skipped: after the
+// one lane = one (target, source) pair, the swap entries are folded into the
+// instruction stream as constant operands of v_fma_f64, and only the per-pair
+// z-rotations and translation factors live in registers.  The forward half's
+// scaling by (n+|m|)!(n-|m|)! is folded into its two swaps' constants.  Work that the translation cannot see is skipped: after the
 // second forward swap only components |m| <= min(n, p-n) are formed, and the
 // first backward swap only reads those.  The quirk of the reference's
 // factorial table (factorial_table[2p] == 0, kernel_fast_lean.h:129) removes
@@ -30,10 +30,8 @@
 
 namespace tcfmm {
 
-// Scaling (n+|m|)!(n-|m|)! and its inverse per real-packed term, loaded with
-// scalar loads (uniform across the wave).  Filled by m2l_init_tables().
-__constant__ double c_scale[kMaxNT];
-__constant__ double c_scale_inv[kMaxNT];
+// n! for the translation factors, loaded with scalar loads (uniform across the
+// wave).  Filled by m2l_init_tables().
 __constant__ double c_fact[2 * kMaxP + 2];
 
 // z-rotation of one order by e^{i m alpha} (forward) or e^{-i m alpha}
@@ -51,7 +49,10 @@ __constant__ double c_fact[2 * kMaxP + 2];
 template <int N, bool kBack>
 __device__ __forceinline__ void rotate(double *v, double c1, double s1) {
   asm volatile("" : "+v"(c1), "+v"(s1));
-  double c = c1, s = s1;
+  // Chebyshev form of the power recurrence: cos((m+1)a) = 2 cos a cos(ma) -
+  // cos((m-1)a), and the same for sin, one FMA per component per power
+  const double t = c1 + c1;
+  double cq = 1.0, sq = 0.0, c = c1, s = s1;
 #pragma unroll
   for (int m = 1; m <= N; m++) {
     double im = v[N - m], re = v[N + m];
@@ -63,9 +64,11 @@ __device__ __forceinline__ void rotate(double *v, double c1, double s1) {
       v[N + m] = c * re - s * im;
     }
     if (m < N) {
-      double cn = c * c1 - s * s1;
-      s = c * s1 + s * c1;
+      const double cn = fma(t, c, -cq), sn = fma(t, s, -sq);
+      cq = c;
+      sq = s;
       c = cn;
+      s = sn;
     }
   }
 }
@@ -86,24 +89,27 @@ __device__ __forceinline__ void rotate_tab(double *v, const double *cp, const do
   }
 }
 
-// Powers e^{iNa} from e^{i(N-1)a} (same recurrence and operation order as
-// rotate()'s chain), stored at index N.
+// Powers e^{iNa} from the two below (the Chebyshev recurrence rotate() uses),
+// stored at index N; index 0 holds e^0.
 template <int N>
 __device__ __forceinline__ void extend_powers(double *cp, double *sp, double c1, double s1) {
   if constexpr (N == 1) {
+    cp[0] = 1.0;
+    sp[0] = 0.0;
     cp[1] = c1;
     sp[1] = s1;
   } else if constexpr (N > 1) {
-    const double c = cp[N - 1], s = sp[N - 1];
-    cp[N] = c * c1 - s * s1;
-    sp[N] = c * s1 + s * c1;
+    const double t = c1 + c1;
+    cp[N] = fma(t, cp[N - 1], -cp[N - 2]);
+    sp[N] = fma(t, sp[N - 1], -sp[N - 2]);
   }
 }
 
 // out rows |r-N| <= QR from input columns |c-N| <= QC (everything else 0).
 // S_n only couples components of one parity class (Re/Im part and parity of
 // m); rows are produced class by class so each class's inputs die early.
-template <int N, int QR, int QC>
+// kScaled: D^-1 S_n D with D = diag((n+|m|)!(n-|m|)!) instead of S_n.
+template <int N, int QR, int QC, bool kScaled = false>
 __device__ __forceinline__ void swap_block(const double *in, double *out) {
 #pragma unroll
   for (int cls = 0; cls < 4; cls++) {
@@ -117,7 +123,7 @@ __device__ __forceinline__ void swap_block(const double *in, double *out) {
 #pragma unroll
         for (int e = kSwapRowPtr[N * N + r]; e < kSwapRowPtr[N * N + r + 1]; e++) {
           const int c = kSwapCol[e];
-          if (c - N <= QC && N - c <= QC) acc += kSwapVal[e] * in[c];
+          if (c - N <= QC && N - c <= QC) acc += (kScaled ? kSwapValScaled[e] : kSwapVal[e]) * in[c];
         }
       }
       out[r] = acc;
@@ -193,19 +199,23 @@ __device__ __forceinline__ void m2l_forward_order(const double *__restrict__ src
     load_order<N - kAhead>(src, nb);
     __builtin_amdgcn_sched_barrier(0);
   }
+  // D^-1 S Rx S D Rz with D = diag((n+|m|)!(n-|m|)!) (the reference's scale
+  // and unscale, kernel_fast_lean.h:1175-1181, 1247-1251): the z-rotations
+  // only mix +-m, so they commute with D, and the product is
+  // (D^-1 S D) Rx (D^-1 S D) Rz -- the scaling lives in the swap constants
   double v[K], w[K];
 #pragma unroll
-  for (int r = 0; r < K; r++) v[r] = b0[r] * c_scale[N * N + r];
+  for (int r = 0; r < K; r++) v[r] = b0[r];
   if (N == P && quirk) {
     v[0] = 0.0;
     v[2 * N] = 0.0;
   }
   rotate<N, false>(v, ezr, ezi);
-  swap_block<N, N, N>(v, w);
+  swap_block<N, N, N, true>(v, w);
   rotate<N, false>(w, exr, exi);
-  swap_block<N, Q, N>(w, v);
+  swap_block<N, Q, N, true>(w, v);
 #pragma unroll
-  for (int m = -Q; m <= Q; m++) mp[O + Q + m] = v[N + m] * c_scale_inv[N * N + N + m];
+  for (int m = -Q; m <= Q; m++) mp[O + Q + m] = v[N + m];
   if constexpr (N > 0) {
     if constexpr (kAhead == 1)
       m2l_forward_order<P, N - 1>(src, quirk, ezr, ezi, exr, exi, nb, nullptr, nullptr, nullptr,
@@ -405,16 +415,6 @@ void m2l_init_tables(int p, int quirk) {
   double fact[2 * kMaxP + 2];
   fact[0] = 1.0;
   for (int i = 1; i < 2 * kMaxP + 2; i++) fact[i] = fact[i - 1] * i;
-  double sc[kMaxNT], si[kMaxNT];
-  for (int n = 0; n <= p; n++)
-    for (int m = -n; m <= n; m++) {
-      int a = m < 0 ? -m : m;
-      double s = fact[n + a] * fact[n - a];
-      sc[ixnm(n, m)] = s;
-      si[ixnm(n, m)] = 1.0 / s;
-    }
-  hipMemcpyToSymbol(HIP_SYMBOL(c_scale), sc, sizeof(double) * nterm(p));
-  hipMemcpyToSymbol(HIP_SYMBOL(c_scale_inv), si, sizeof(double) * nterm(p));
   hipMemcpyToSymbol(HIP_SYMBOL(c_fact), fact, sizeof(fact));
   done_p = p;
   done_q = quirk;
