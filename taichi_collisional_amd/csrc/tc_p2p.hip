@@ -140,8 +140,17 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
     nsrc += fill;
     __syncthreads();
     if (act) {
+      // fill is the same on every lane: the first fill / G bodies of every
+      // slice run as a loop with a uniform trip count (scalar control, no
+      // per-pair exec-mask updates); the ragged last round follows
+      const int nfull = __builtin_amdgcn_readfirstlane(fill / G);
+      int k = s;
 #pragma unroll 2
-      for (int k = s; k < fill; k += G) {
+      for (int i = 0; i < nfull; i++, k += G) {
+        const double4 P = sp[k], V = sv[k];
+        pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
+      }
+      if (k < fill) {
         const double4 P = sp[k], V = sv[k];
         pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
       }
