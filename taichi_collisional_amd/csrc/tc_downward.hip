@@ -191,6 +191,34 @@ __global__ void k_scatter_results(BodiesDev b, ResultsDev r, int n, double *__re
   }
 }
 
+// near + far result sets, summed per body in tree order, to the caller's order
+__global__ void k_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n,
+                                      double *__restrict__ pot, double *__restrict__ acc,
+                                      double *__restrict__ jerk) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int i = b.order[k];
+  if (pot) pot[i] = r.pot[k] + f.pot[k];
+  if (acc) {
+    acc[3 * (size_t)i] = r.ax[k] + f.ax[k];
+    acc[3 * (size_t)i + 1] = r.ay[k] + f.ay[k];
+    acc[3 * (size_t)i + 2] = r.az[k] + f.az[k];
+  }
+  if (jerk) {
+    jerk[3 * (size_t)i] = r.jx[k] + f.jx[k];
+    jerk[3 * (size_t)i + 1] = r.jy[k] + f.jy[k];
+    jerk[3 * (size_t)i + 2] = r.jz[k] + f.jz[k];
+  }
+}
+
+int launch_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n, double *pot,
+                               double *acc3, double *jerk3, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_scatter_results_sum, dim3((n + 255) / 256), dim3(256), 0, s, b, r, f, n,
+                     pot, acc3, jerk3);
+  return 0;
+}
+
 int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
                            double *jerk3, hipStream_t s) {
   if (n <= 0) return 0;

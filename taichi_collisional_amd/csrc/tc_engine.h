@@ -128,7 +128,12 @@ struct tc_context {
   // master variant: last-body-of-leaf flags, time-step sums (caller order,
   // tree order), symmetric M2L list and its scan
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
-  hipEvent_t ev[9];
+  // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream
+  hipEvent_t ev[11];
+  // second stream (default priority): P2P runs on it while the downward pass
+  // runs on `stream`; its far-field results go to a second result set
+  hipStream_t near_stream = nullptr;
+  DBuf fpot, fax, fay, faz, fjx, fjy, fjz;
 };
 
 #define TC_CHECK(expr)                                                   \

@@ -27,7 +27,8 @@ static int ensure_bodies(tc_context *ctx, int n) {
   size_t d = sizeof(double) * (size_t)n;
   for (DBuf *b : {&ctx->bx, &ctx->by, &ctx->bz, &ctx->bvx, &ctx->bvy, &ctx->bvz, &ctx->bq,
                   &ctx->bacc, &ctx->rpot, &ctx->rax, &ctx->ray, &ctx->raz, &ctx->rjx, &ctx->rjy,
-                  &ctx->rjz})
+                  &ctx->rjz, &ctx->fpot, &ctx->fax, &ctx->fay, &ctx->faz, &ctx->fjx, &ctx->fjy,
+                  &ctx->fjz})
     TC_CHECK(b->ensure(d));
   TC_CHECK(ctx->lowF.ensure(sizeof(double) * 3 * (size_t)n));
   TC_CHECK(ctx->bflags.ensure(n));
@@ -498,25 +499,37 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   // the GPU runs M2L while the host builds the P2P work items
   if (int e = build_p2p_items(ctx)) return e;
   TC_CHECK(hipEventRecord(ctx->ev[5], s));
-  // ---- near field
+  // ---- near field on the second stream, after M2L (and the work-item upload
+  // queued behind it), alongside the downward pass: the downward pass's top
+  // levels are a handful of blocks each, which P2P's blocks fill around
+  hipStream_t s2 = ctx->near_stream;
+  TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[5], 0));
+  TC_CHECK(hipEventRecord(ctx->ev[9], s2));
   ResultsDev r = results_of(ctx);
   for (double *q : {r.pot, r.ax, r.ay, r.az, r.jx, r.jy, r.jz})
-    TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
+    TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s2));
   TC_CHECK(ctx->p2p_ranges.ensure(sizeof(int2) * (size_t)std::max<long long>(ctx->st.n_p2p_cells, 1)));
   TC_CHECK(ctx->npairs.ensure(sizeof(unsigned long long)));
-  TC_CHECK(hipMemsetAsync(ctx->npairs.p, 0, sizeof(unsigned long long), s));
+  TC_CHECK(hipMemsetAsync(ctx->npairs.p, 0, sizeof(unsigned long long), s2));
   launch_p2p_ranges(cells_of(ctx), ctx->p2p_src.as<int>(), (int)ctx->st.n_p2p_cells,
-                    ctx->p2p_ranges.as<int2>(), s);
+                    ctx->p2p_ranges.as<int2>(), s2);
   launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
              (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
-             ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s);
-  TC_CHECK(hipEventRecord(ctx->ev[6], s));
-  // ---- downward
+             ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2);
+  TC_CHECK(hipEventRecord(ctx->ev[10], s2));
+  // ---- downward into the far-field result set
+  ResultsDev rf = {ctx->fpot.as<double>(), ctx->fax.as<double>(), ctx->fay.as<double>(),
+                   ctx->faz.as<double>(), ctx->fjx.as<double>(), ctx->fjy.as<double>(),
+                   ctx->fjz.as<double>()};
+  for (double *q : {rf.pot, rf.ax, rf.ay, rf.az, rf.jx, rf.jy, rf.jz})
+    TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
   int L = (int)ctx->level_start.size() - 1;
   for (int l = 0; l < L; l++)
     launch_l2l(cfg.p, cells_of(ctx), ctx->level_start[l], ctx->level_start[l + 1], s);
-  launch_l2p(cfg.p, cells_of(ctx), bodies_of(ctx), r, ctx->leaf_ids.as<int>(), ctx->nleaves, s);
-  launch_scatter_results(bodies_of(ctx), r, n, pot, acc, jerk, s);
+  launch_l2p(cfg.p, cells_of(ctx), bodies_of(ctx), rf, ctx->leaf_ids.as<int>(), ctx->nleaves, s);
+  TC_CHECK(hipEventRecord(ctx->ev[6], s));
+  TC_CHECK(hipStreamWaitEvent(s, ctx->ev[10], 0));
+  launch_scatter_results_sum(bodies_of(ctx), r, rf, n, pot, acc, jerk, s);
   TC_CHECK(hipEventRecord(ctx->ev[7], s));
   // body pairs evaluated by P2P (statistics, counted inside the P2P kernel)
   unsigned long long pairs = 0;
@@ -524,7 +537,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   TC_CHECK(hipStreamSynchronize(s));
   ctx->st.n_p2p_pairs = (long long)pairs;
   float ms[8];
-  for (int k = 0; k < 7; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
+  for (int k = 0; k < 5; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
+  hipEventElapsedTime(&ms[5], ctx->ev[9], ctx->ev[10]);  // P2P (second stream)
+  hipEventElapsedTime(&ms[6], ctx->ev[5], ctx->ev[6]);   // downward (alongside P2P)
   hipEventElapsedTime(&ms[7], ctx->ev[0], ctx->ev[7]);
   if (err_mac) {
     // tree phase ends where the pre-pass starts
@@ -797,6 +812,11 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     ctx->own_stream = true;
   }
   for (auto &e : ctx->ev) hipEventCreate(&e);
+  if (hipStreamCreateWithFlags(&ctx->near_stream, hipStreamNonBlocking) != hipSuccess) {
+    ctx->near_stream = nullptr;
+    tc_destroy(ctx);
+    return -3;
+  }
   if (hipHostMalloc((void **)&ctx->h_pin, 64 * sizeof(int)) != hipSuccess) {
     tc_destroy(ctx);
     return -3;
@@ -822,6 +842,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
                  &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts, &ctx->mts_sorted,
                  &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym,
+                 &ctx->fpot, &ctx->fax, &ctx->fay, &ctx->faz, &ctx->fjx, &ctx->fjy, &ctx->fjz,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
@@ -833,6 +854,10 @@ int tc_destroy(tc_context *ctx) {
   ctx->h_stage.release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  if (ctx->near_stream) {
+    hipStreamSynchronize(ctx->near_stream);
+    hipStreamDestroy(ctx->near_stream);
+  }
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;

@@ -188,5 +188,7 @@ void launch_scatter_ts(const BodiesDev &b, const double *ts_sorted, int n, doubl
 
 int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
                            double *jerk3, hipStream_t s);
+int launch_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n, double *pot,
+                               double *acc3, double *jerk3, hipStream_t s);
 
 }  // namespace tcfmm
