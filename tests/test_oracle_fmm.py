@@ -125,3 +125,28 @@ def test_minball_is_minimal(oracle_lib):
             dirn /= np.linalg.norm(dirn)
             c2 = c + dirn * 1e-6 * max(np.sqrt(r2[0]), 1e-3)
             assert ((pts - c2) ** 2).sum(1).max() >= r2[0] * (1 - 1e-10)
+
+
+def test_kroupa_binaries_generator():
+    """C3 initial conditions: deterministic, total mass 1, masses inside the
+    IMF range (up to the normalisation), every pair bound with semi-major
+    axis in [1e-5, 1e-3] and 10% of the bodies in pairs."""
+    x, v, m = ics.kroupa_binaries(20000, seed=3)
+    x2, v2, m2 = ics.kroupa_binaries(20000, seed=3)
+    assert _sha(x, v, m) == _sha(x2, v2, m2)
+    assert abs(m.sum() - 1.0) < 1e-12
+    assert m.max() / m.min() <= 100.0 / 0.08 * (1 + 1e-12)
+    npair = 1000
+    ns = 20000 - 2 * npair
+    a, b = slice(ns, None, 2), slice(ns + 1, None, 2)
+    mt = m[a] + m[b]
+    r = np.linalg.norm(x[b] - x[a], axis=1)
+    vr = np.linalg.norm(v[b] - v[a], axis=1)
+    energy = 0.5 * vr ** 2 - mt / r
+    assert np.all(energy < 0)
+    sma = -mt / (2 * energy)
+    assert sma.min() > 1e-5 * (1 - 1e-9) and sma.max() < 1e-3 * (1 + 1e-9)
+    # the pair's centre of mass is the system's Plummer position
+    com = (m[a, None] * x[a] + m[b, None] * x[b]) / mt[:, None]
+    xs, _, _ = ics.plummer(20000 - npair, seed=3)
+    np.testing.assert_allclose(com, xs[ns:], atol=1e-12)
