@@ -400,6 +400,10 @@ extern "C" int tc_master_evolve(tc_context *ctx, const tc_particles *p, double *
     ctx->err = "tc_master_evolve: dt_param > 0 and a finite dt are required";
     return -1;
   }
+  if (mode & ~(TC_MASTER_FMM | TC_MASTER_PAIR_MASS)) {
+    ctx->err = "tc_master_evolve: unknown mode bits (TC_MASTER_FMM, TC_MASTER_PAIR_MASS)";
+    return -1;
+  }
   tcfmm::Evolver ev(ctx, *p, mode, dt_param);
   return ev.run(simtime, dt);
 }
