@@ -217,6 +217,40 @@ int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v,
                        const unsigned char *flags, int mode, double dt_param, double *pot,
                        double *acc, double *timestep, double *acc_old);
 
+/* Per-kernel entry points over the master Cell (exafmm.h:50-71: M[NTERM],
+ * L[NTERM], Pn[P+1] first, then NCHILD ... has_sink; no Mdot / Ldot) and
+ * tc_master_body records, with the list-order semantics of the tc_P2M ...
+ * tc_P2P family above.  Master kernels (kernel_fast_lean.h):
+ *   tc_master_P2M  <- P2M(C) :676      X, R, min_acc, has_sink; M +=
+ *   tc_master_M2M  <- M2M(C) :784      the same from the children
+ *   tc_master_M2L  <- M2L_rotate(Ci, Cj) :1034, symmetric: entry (ci[k],
+ *                     cj[k]) does nothing unless ci[k] < cj[k] (the
+ *                     reference compares the Cell pointers) and one of the
+ *                     two has a sink; then L of each sink cell of the two +=
+ *                     the other's field.  ci[k] == cj[k] is rejected (-1).
+ *   tc_master_L2L  <- L2L(C) :941      L of the children +=
+ *   tc_master_L2P  <- L2P(C) :989      p, F of the sink bodies += / -=
+ *   tc_master_P2P  <- P2P_simple(Ci, Cj) :268, or with TC_MASTER_STEPS in
+ *                     mode P2P(Ci, Cj) :462, which also adds (min over Cj's
+ *                     bodies of tau)^-4 to each sink body's timestep
+ *                     (TC_MASTER_PAIR_MASS: the scalar form, as for
+ *                     tc_master_eval_bodies).
+ * tc_get_master_cell_layout gives the offsets (Mdot = Ldot = -1): at p = 12,
+ * 2888 bytes, L@1352, Pn@2704, NCHILD@2808. */
+int tc_get_master_cell_layout(int p, tc_cell_layout *out);
+int tc_master_P2M(tc_context *ctx, tc_master_body *bodies, int nbodies, void *cells, int ncells,
+                  const int *ids, int nids);
+int tc_master_M2M(tc_context *ctx, tc_master_body *bodies, int nbodies, void *cells, int ncells,
+                  const int *ids, int nids);
+int tc_master_M2L(tc_context *ctx, void *cells, int ncells, const int *ci, const int *cj,
+                  int npairs);
+int tc_master_L2L(tc_context *ctx, void *cells, int ncells, const int *ids, int nids);
+int tc_master_L2P(tc_context *ctx, tc_master_body *bodies, int nbodies, void *cells, int ncells,
+                  const int *ids, int nids);
+int tc_master_P2P(tc_context *ctx, tc_master_body *bodies, int nbodies, void *cells, int ncells,
+                  const int *targets, const int *sources, int npairs, int mode,
+                  double dt_param);
+
 /* ---- The master variant's step loop ------------------------------------------
  * tc_master_evolve <- do_evolve(sys, dt) (master integrator.h:864-884) as the
  * master Makefile builds it (GRADIENT): kick_naive for the time steps, then

@@ -289,11 +289,25 @@ def master_reference(p):
     L.refm_eval.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                             C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     L.refm_kick_steps.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+    L.refm_layout.argtypes = [C.c_void_p]
     L.refm_evolve.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                               C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double)]
     L.isa = isa
     _mrefs[p] = L
     return L
+
+
+def master_ref_layout(p):
+    """Sizes and byte offsets of the master Body and Cell as the reference
+    compiles them at order p: dict of 'Body.X' ... and 'Cell.M' ... ."""
+    out = np.zeros(32, np.int64)
+    n = master_reference(p).refm_layout(out.ctypes.data)
+    names = ["Body", "Body.X", "Body.q", "Body.F", "Body.p", "Body.V", "Body.acc_old",
+             "Body.Morton", "Body.timestep", "Body.index", "Body.issink", "Body.issource",
+             "Cell", "Cell.M", "Cell.L", "Cell.Pn", "Cell.NCHILD", "Cell.NBODY", "Cell.CHILD",
+             "Cell.BODY", "Cell.X", "Cell.R", "Cell.min_acc", "Cell.cell_mass", "Cell.has_sink"]
+    assert n == len(names)
+    return dict(zip(names, (int(v) for v in out[:n])))
 
 
 def master_ref_eval(p, x, v, m, issink=None, issource=None, ncrit=300, dt_param=0.025,

@@ -29,6 +29,7 @@
 //
 // Bodies are set up as kick_naive does (integrator.h:190-205): value-initialised
 // records, then X, V, q, index, issink, issource, timestep = 0.
+#include <cstddef>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -61,6 +62,27 @@ extern "C" {
 
 int refm_expansion() { return EXPANSION; }
 int refm_body_size() { return (int)sizeof(Body); }
+
+// Master Body and Cell layouts as compiled (sizes and byte offsets), for the
+// test that pins tc_master_body and tc_get_master_cell_layout.
+int refm_layout(long long *out) {
+  const long long v[] = {(long long)sizeof(Body), (long long)offsetof(Body, X),
+                         (long long)offsetof(Body, q), (long long)offsetof(Body, F),
+                         (long long)offsetof(Body, p), (long long)offsetof(Body, V),
+                         (long long)offsetof(Body, acc_old), (long long)offsetof(Body, Morton),
+                         (long long)offsetof(Body, timestep), (long long)offsetof(Body, index),
+                         (long long)offsetof(Body, issink), (long long)offsetof(Body, issource),
+                         (long long)sizeof(Cell), (long long)offsetof(Cell, M),
+                         (long long)offsetof(Cell, L), (long long)offsetof(Cell, Pn),
+                         (long long)offsetof(Cell, NCHILD), (long long)offsetof(Cell, NBODY),
+                         (long long)offsetof(Cell, CHILD), (long long)offsetof(Cell, BODY),
+                         (long long)offsetof(Cell, X), (long long)offsetof(Cell, R),
+                         (long long)offsetof(Cell, min_acc), (long long)offsetof(Cell, cell_mass),
+                         (long long)offsetof(Cell, has_sink)};
+  const int n = (int)(sizeof(v) / sizeof(v[0]));
+  for (int k = 0; k < n; k++) out[k] = v[k];
+  return n;
+}
 
 int refm_eval(int n, const double *x, const double *v, const double *m,
               const unsigned char *issink, const unsigned char *issource, int ncrit_,

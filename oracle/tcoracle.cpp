@@ -1362,8 +1362,9 @@ struct Fmm {
     for (auto &e : m2l) {
       int i = e.first, j = e.second;
       if (i > j || (!C[i].has_sink && !C[j].has_sink)) continue;
-      lm[i].push_back(j);
-      lm[j].push_back(i);
+      // each local is written only when its cell holds a sink (:1276-1294)
+      if (C[i].has_sink) lm[i].push_back(j);
+      if (C[j].has_sink) lm[j].push_back(i);
     }
     for (auto &e : p2p) lp[e.first].push_back(e.second);
     st.n_m2l = (long long)m2l.size();

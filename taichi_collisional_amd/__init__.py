@@ -114,6 +114,13 @@ def lib():
         L.tc_L2L.argtypes = [vp, vp, C.c_int, vp, C.c_int]
         L.tc_M2L.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int]
         L.tc_P2P.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int]
+        L.tc_get_master_cell_layout.argtypes = [C.c_int, C.POINTER(TcCellLayout)]
+        for k in ("tc_master_P2M", "tc_master_M2M", "tc_master_L2P"):
+            getattr(L, k).argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.tc_master_L2L.argtypes = [vp, vp, C.c_int, vp, C.c_int]
+        L.tc_master_M2L.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int]
+        L.tc_master_P2P.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_int,
+                                    C.c_double]
         L.tc_get_stats.argtypes = [vp, C.POINTER(TcStats)]
         L.tc_last_error.argtypes = [vp]
         L.tc_last_error.restype = C.c_char_p
@@ -134,7 +141,9 @@ EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "t
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
                     "tc_P2P", "tc_master_eval_bodies", "tc_master_eval_soa",
                     "tc_master_evolve", "tc_master_evolve_particles", "tc_get_stats",
-                    "tc_last_error", "tc_supported_orders"]
+                    "tc_last_error", "tc_supported_orders", "tc_get_master_cell_layout",
+                    "tc_master_P2M", "tc_master_M2M", "tc_master_M2L", "tc_master_L2L",
+                    "tc_master_L2P", "tc_master_P2P"]
 
 
 def cell_dtype(p):
@@ -151,6 +160,23 @@ def cell_dtype(p):
                      "offsets": [lay.nchild, lay.nbody, lay.child, lay.body, lay.X, lay.R,
                                  lay.min_acc, lay.cell_mass, lay.has_sink, lay.M, lay.L,
                                  lay.Mdot, lay.Ldot, lay.Pn],
+                     "itemsize": lay.size})
+
+
+def master_cell_dtype(p):
+    """numpy dtype of the master sources' struct Cell for order p (master
+    exafmm.h:50-71: M, L, Pn first; no Mdot / Ldot)."""
+    lay = TcCellLayout()
+    if lib().tc_get_master_cell_layout(int(p), C.byref(lay)) != 0:
+        raise ValueError("no cell layout for p=%r" % p)
+    nt = (p + 1) ** 2
+    return np.dtype({"names": ["M", "L", "Pn", "NCHILD", "NBODY", "CHILD", "BODY", "X", "R",
+                               "min_acc", "cell_mass", "has_sink"],
+                     "formats": [("<f8", nt), ("<f8", nt), ("<f8", p + 1), "<i4", "<i4", "<u8",
+                                 "<u8", ("<f8", 3), "<f8", "<f8", "<f8", "u1"],
+                     "offsets": [lay.M, lay.L, lay.Pn, lay.nchild, lay.nbody, lay.child,
+                                 lay.body, lay.X, lay.R, lay.min_acc, lay.cell_mass,
+                                 lay.has_sink],
                      "itemsize": lay.size})
 
 
@@ -362,6 +388,62 @@ class Fmm:
         s, ps = self._ids(sources)
         self._check(lib().tc_P2P(self.h, bodies.ctypes.data, bodies.size, cells.ctypes.data,
                                  cells.size, pt, ps, t.size), "tc_P2P")
+
+    # ---- the same over the master sources' Cell / Body records ----
+    def _mrecs(self, bodies, cells):
+        if cells.dtype != master_cell_dtype(self.p) or not cells.flags.c_contiguous:
+            raise TypeError("cells must be a C-contiguous array of master_cell_dtype(p)")
+        if bodies is not None and (bodies.dtype != master_body_dtype
+                                   or not bodies.flags.c_contiguous):
+            raise TypeError("bodies must be a C-contiguous array of master_body_dtype")
+
+    def master_P2M(self, bodies, cells, ids):
+        self._mrecs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_master_P2M(self.h, bodies.ctypes.data, bodies.size,
+                                        cells.ctypes.data, cells.size, pa, a.size),
+                    "tc_master_P2M")
+
+    def master_M2M(self, bodies, cells, ids):
+        self._mrecs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_master_M2M(self.h, bodies.ctypes.data, bodies.size,
+                                        cells.ctypes.data, cells.size, pa, a.size),
+                    "tc_master_M2M")
+
+    def master_M2L(self, cells, ci, cj):
+        """Symmetric M2L_rotate(Ci, Cj) per entry (only ci < cj acts)."""
+        self._mrecs(None, cells)
+        t, pt = self._ids(ci)
+        s, ps = self._ids(cj)
+        if t.size != s.size:
+            raise ValueError("ci and cj differ in length")
+        self._check(lib().tc_master_M2L(self.h, cells.ctypes.data, cells.size, pt, ps, t.size),
+                    "tc_master_M2L")
+
+    def master_L2L(self, cells, ids):
+        self._mrecs(None, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_master_L2L(self.h, cells.ctypes.data, cells.size, pa, a.size),
+                    "tc_master_L2L")
+
+    def master_L2P(self, bodies, cells, ids):
+        self._mrecs(bodies, cells)
+        a, pa = self._ids(ids)
+        self._check(lib().tc_master_L2P(self.h, bodies.ctypes.data, bodies.size,
+                                        cells.ctypes.data, cells.size, pa, a.size),
+                    "tc_master_L2P")
+
+    def master_P2P(self, bodies, cells, targets, sources, mode=0, dt_param=0.025):
+        """P2P_simple, or with TC_MASTER_STEPS in mode P2P with the time-step sum."""
+        self._mrecs(bodies, cells)
+        t, pt = self._ids(targets)
+        s, ps = self._ids(sources)
+        if t.size != s.size:
+            raise ValueError("targets and sources differ in length")
+        self._check(lib().tc_master_P2P(self.h, bodies.ctypes.data, bodies.size,
+                                        cells.ctypes.data, cells.size, pt, ps, t.size,
+                                        int(mode), float(dt_param)), "tc_master_P2P")
 
     def stats(self):
         st = TcStats()

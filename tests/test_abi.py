@@ -87,3 +87,30 @@ def test_cell_layout_matches_reference_cell():
     assert (off["NCHILD"], off["NBODY"], off["CHILD"], off["BODY"]) == (0, 4, 8, 16)
     assert (off["X"], off["R"], off["min_acc"], off["cell_mass"], off["has_sink"]) == (24, 48, 56, 64, 72)
     assert (off["M"], off["L"], off["Mdot"], off["Ldot"], off["Pn"]) == (80, 1432, 2784, 4136, 5488)
+
+
+def test_master_cell_layout_matches_reference_cell():
+    """The master struct Cell (master exafmm.h:50-71: M, L, Pn first, no
+    Mdot / Ldot): at p = 12, 2888 bytes, L@1352, Pn@2704, NCHILD@2808; and
+    the layout the compiled reference reports at p = 8 and 12."""
+    import taichi_collisional_amd as tca
+    dt = tca.master_cell_dtype(12)
+    off = {k: dt.fields[k][1] for k in dt.names}
+    assert dt.itemsize == 2888
+    assert (off["M"], off["L"], off["Pn"], off["NCHILD"], off["has_sink"]) == (0, 1352, 2704, 2808,
+                                                                            2880)
+    lay = tca.TcCellLayout()
+    assert tca.lib().tc_get_master_cell_layout(12, ctypes.byref(lay)) == 0
+    assert lay.Mdot == -1 and lay.Ldot == -1
+    import pyoracle
+    if not pyoracle.master_ref_available(12):
+        pytest.skip("master reference harness not built")
+    for p in (8, 12):
+        ref = pyoracle.master_ref_layout(p)
+        dt = tca.master_cell_dtype(p)
+        assert dt.itemsize == ref["Cell"]
+        for k in dt.names:
+            assert dt.fields[k][1] == ref["Cell." + k], (p, k)
+        assert tca.master_body_dtype.itemsize == ref["Body"]
+        for k in tca.master_body_dtype.names:
+            assert tca.master_body_dtype.fields[k][1] == ref["Body." + k], k

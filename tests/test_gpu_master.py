@@ -233,3 +233,139 @@ def test_gpu_master_fmm_body_records(torch_cuda):
     b2 = b.copy()
     f.master_get_force_and_potential(b2, get_steps=False)
     assert np.array_equal(b2["acc_old"], b["acc_old"])
+
+
+# ---------------------------------------------------------------------------
+# Per-kernel entry points over the master Cell / Body records: the oracle's
+# master tree and walk lists are replayed kernel by kernel through the C ABI.
+
+def _master_records(tca, cl, x, v, m, sink, src, acc_old, p):
+    order = cl["order"]
+    n = order.size
+    bodies = np.zeros(n, tca.master_body_dtype)
+    bodies["X"], bodies["V"], bodies["q"] = x[order], v[order], m[order]
+    bodies["issink"], bodies["issource"], bodies["index"] = sink[order], src[order], order
+    bodies["acc_old"] = acc_old[order]
+    nc = cl["level"].size
+    cells = np.zeros(nc, tca.master_cell_dtype(p))
+    cells["NCHILD"], cells["NBODY"] = cl["nchild"], cl["nbody"]
+    csz = np.uint64(cells.dtype.itemsize)
+    cells["CHILD"] = np.where(cl["nchild"] > 0,
+                              np.uint64(cells.ctypes.data) + cl["child"].astype(np.uint64) * csz, 0)
+    cells["BODY"] = np.uint64(bodies.ctypes.data) + cl["body"].astype(np.uint64) * np.uint64(128)
+    cells["X"], cells["R"] = cl["Xg"], cl["Rg"]
+    return bodies, cells
+
+
+def test_gpu_master_per_kernel_replay_oracle(torch_cuda, oracle_lib):
+    """P2M, M2M, symmetric M2L, L2L, L2P and P2P (with the time-step sum) on
+    the master records reproduce the oracle's -DFMM evaluation: coefficients
+    to 1e-12 / 1e-11, potentials, forces and time steps to the module's bars.
+    The walk with steps accepts no far pairs and lists the same P2P pairs, so
+    time steps come from that run and forces from the run without steps."""
+    import taichi_collisional_amd as tca
+    n, p = 20000, 8
+    x, v, m = mmg.ics.plummer(n, seed=34)
+    rng = np.random.default_rng(34)
+    sink = (rng.random(n) < 0.7).astype(np.uint8)
+    src = (rng.random(n) < 0.9).astype(np.uint8)
+    o = oracle_lib.Oracle(p, ncrit=64, mac=1, force_accuracy=2e-7)
+    _, _, ot, _ = o.master_fmm(x, v, m, sink, src, steps=True)
+    tp1, sp1 = o.lists(1)
+    assert o.lists(0)[0].size == 0
+    op, oa, _, oao = o.master_fmm(x, v, m, sink, src, steps=False)
+    cl = o.cells()
+    nc = cl["level"].size
+    OM, _, OL, _ = o.coefs(nc)
+    tm, sm = o.lists(0)
+    tp, sp = o.lists(1)
+    assert np.array_equal(tp, tp1) and np.array_equal(sp, sp1)
+    assert tm.size > 1000 and np.any(tm > sm) and np.any(tm < sm)
+    bodies, cells = _master_records(tca, cl, x, v, m, sink, src, oao, p)
+    f = tca.Fmm(p=p, ncrit=64, mac=tca.TC_MAC_ERROR, force_accuracy=2e-7, direct_below=0)
+    leaves = np.where(cl["nchild"] == 0)[0]
+    internal = np.where(cl["nchild"] > 0)[0]
+    up = internal[np.argsort(-cl["level"][internal], kind="stable")]
+    down = internal[np.argsort(cl["level"][internal], kind="stable")]
+
+    def rel(a, b):
+        return np.abs(a - b).max() / np.abs(b).max()
+
+    f.master_P2M(bodies, cells, leaves)
+    f.master_M2M(bodies, cells, up)
+    assert np.abs(cells["X"] - cl["X"]).max() < 1e-12
+    assert np.abs(cells["R"] - cl["R"]).max() < 1e-12
+    assert rel(cells["M"], OM) < 1e-12
+    # has_sink and min_acc over each cell's bodies
+    cs = np.concatenate([[0], np.cumsum(bodies["issink"].astype(np.int64))])
+    want_sink = cs[cl["body"] + cl["nbody"]] > cs[cl["body"]]
+    assert np.array_equal(cells["has_sink"].astype(bool), want_sink)
+    want_min = np.array([bodies["acc_old"][b:b + k].min() for b, k in zip(cl["body"], cl["nbody"])])
+    np.testing.assert_array_equal(cells["min_acc"], want_min)
+    f.master_M2L(cells, tm, sm)
+    assert not cells["L"][~want_sink].any()  # locals of cells without sinks stay unwritten
+    f.master_L2L(cells, down)
+    assert rel(cells["L"], OL) < 1e-11
+    f.master_L2P(bodies, cells, leaves)
+    f.master_P2P(bodies, cells, tp, sp, mode=tca.TC_MASTER_STEPS, dt_param=0.025)
+    order = cl["order"]
+    live = sink[order].astype(bool)
+    _check((bodies["p"], bodies["F"], bodies["timestep"]), (op[order], oa[order], ot[order]),
+           live, True, "master per-kernel replay")
+    assert not bodies["F"][~live].any() and not bodies["timestep"][~live].any()
+
+
+def test_gpu_master_per_kernel_semantics(torch_cuda, oracle_lib):
+    """List-order semantics of the symmetric M2L and the P2P modes: an entry
+    with ci > cj does nothing, a repeated entry adds twice, (ci, cj) fills
+    both locals; P2P without TC_MASTER_STEPS leaves timestep alone and the
+    pair-mass form differs from the vector form; bad input is rejected."""
+    import taichi_collisional_amd as tca
+    n, p = 8000, 4
+    x, v, m = mmg.ics.plummer(n, seed=35)
+    ones = np.ones(n, np.uint8)
+    o = oracle_lib.Oracle(p, ncrit=64, mac=1, force_accuracy=2e-7)
+    _, _, _, oao = o.master_fmm(x, v, m, ones, ones, steps=False)
+    cl = o.cells()
+    tm, sm = o.lists(0)
+    bodies, cells = _master_records(tca, cl, x, v, m, ones, ones, oao, p)
+    f = tca.Fmm(p=p, ncrit=64, mac=tca.TC_MAC_ERROR, force_accuracy=2e-7, direct_below=0)
+    leaves = np.where(cl["nchild"] == 0)[0]
+    internal = np.where(cl["nchild"] > 0)[0]
+    f.master_P2M(bodies, cells, leaves)
+    f.master_M2M(bodies, cells, internal[np.argsort(-cl["level"][internal], kind="stable")])
+    k = int(np.where(tm < sm)[0][0])
+    i, j = int(tm[k]), int(sm[k])
+    base = cells.copy()
+    f.master_M2L(cells, [j], [i])                 # the pair in the other order: nothing
+    assert np.array_equal(cells["L"], base["L"])
+    f.master_M2L(cells, [i], [j])
+    one = cells["L"].copy()
+    assert one[i].any() and one[j].any()
+    assert not np.delete(one, [i, j], axis=0).any()
+    f.master_M2L(cells, [i, i], [j, j])
+    np.testing.assert_allclose(cells["L"][[i, j]], 3 * one[[i, j]], rtol=1e-13, atol=0)
+    with pytest.raises(RuntimeError, match="itself"):
+        f.master_M2L(cells, [i], [i])
+    tp, sp = o.lists(1)
+
+    def p2p(mode):
+        # the cells' BODY pointers address `bodies`: run on it, from zeroed outputs
+        for k in ("p", "F", "timestep"):
+            bodies[k] = 0
+        f.master_P2P(bodies, cells, tp, sp, mode=mode)
+        return bodies.copy()
+
+    b0 = p2p(0)
+    assert not b0["timestep"].any() and b0["p"].any()
+    b1 = p2p(tca.TC_MASTER_STEPS)
+    b2 = p2p(tca.TC_MASTER_STEPS | tca.TC_MASTER_PAIR_MASS)
+    np.testing.assert_allclose(b1["p"], b0["p"], rtol=1e-13, atol=0)
+    assert (b1["timestep"] > 0).all() and (b2["timestep"] > 0).all()
+    assert not np.allclose(b1["timestep"], b2["timestep"], rtol=1e-6)
+    with pytest.raises(RuntimeError, match="mode"):
+        f.master_P2P(bodies, cells, tp, sp, mode=tca.TC_MASTER_FMM)
+    bad = cells.copy()
+    bad["BODY"][leaves[0]] += np.uint64(8)
+    with pytest.raises(RuntimeError):
+        f.master_P2M(bodies, bad, leaves[:1])
