@@ -20,7 +20,10 @@ struct DBuf {
   // grow to at least b bytes; keep `keep` bytes of old content
   hipError_t ensure(size_t b, size_t keep = 0, hipStream_t s = nullptr) {
     if (b <= bytes) return hipSuccess;
-    size_t nb = std::max(b, bytes + bytes / 2);
+    // a quarter of headroom on the first allocation and 1.5x growth after it:
+    // sizes that vary from call to call (a new root shift, a new tree) settle
+    // after a call or two instead of reallocating (and synchronizing) again
+    size_t nb = std::max(b + b / 4, bytes + bytes / 2);
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, nb);
     if (e != hipSuccess) return e;
@@ -54,7 +57,7 @@ struct HBuf {
   size_t n = 0;
   hipError_t ensure(size_t m) {
     if (m <= n) return hipSuccess;
-    size_t nn = std::max(m, n + n / 2);
+    size_t nn = std::max(m + m / 4, n + n / 2);  // headroom as DBuf::ensure
     T *q = nullptr;
     hipError_t e = hipHostMalloc((void **)&q, sizeof(T) * nn, hipHostMallocDefault);
     if (e != hipSuccess) return e;
@@ -104,10 +107,9 @@ struct tc_context {
   DBuf split_offs, split_nch, split_scan, scan_tmp;
   std::vector<int> level_start;
   DBuf leaf_ids;
-  HBuf<int> h_leaves, h_nch, h_nbd, h_pcnt;
+  HBuf<int> h_leaves, h_nbd, h_pcnt;
   HBuf<int4> h_items;
   size_t n_items = 0;
-  std::vector<int4> h_groups;
   DBuf p2p_items, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists
@@ -128,8 +130,9 @@ struct tc_context {
   // master variant: last-body-of-leaf flags, time-step sums (caller order,
   // tree order), symmetric M2L list and its scan
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
-  // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream
-  hipEvent_t ev[11];
+  // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
+  // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd
+  hipEvent_t ev[12];
   // second stream (default priority): P2P runs on it while the downward pass
   // runs on `stream`; its far-field results go to a second result set
   hipStream_t near_stream = nullptr;

@@ -48,7 +48,7 @@ static int ensure_bodies(tc_context *ctx, int n) {
 
 static int ensure_cells(tc_context *ctx, int nc, int keep) {
   if (nc <= ctx->cap_c) return 0;
-  int cap = std::max(nc, ctx->cap_c + ctx->cap_c / 2);
+  int cap = std::max(nc + nc / 4, ctx->cap_c + ctx->cap_c / 2);
   size_t NT = (size_t)nterm(ctx->cfg.p);
   size_t ki = sizeof(int) * (size_t)keep, kd = sizeof(double) * (size_t)keep;
   hipStream_t s = ctx->stream;
@@ -71,11 +71,8 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
   for (DBuf *b : {&ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur, &ctx->m2l_coff, &ctx->p2p_cnt,
                   &ctx->p2p_off, &ctx->p2p_cur})
     TC_CHECK(b->ensure(sizeof(int) * (cap + 1)));
-  TC_CHECK(ctx->split_offs.ensure(sizeof(int) * 8 * cap));
-  TC_CHECK(ctx->split_nch.ensure(sizeof(int) * cap));
-  TC_CHECK(ctx->split_scan.ensure(sizeof(int) * cap));
   TC_CHECK(ctx->scan_tmp.ensure(scan_temp_bytes(cap + 1) + 256));
-  TC_CHECK(ctx->leaf_ids.ensure(sizeof(int) * cap));
+  TC_CHECK(ctx->leaf_ids.ensure(sizeof(int) * cap, ki, s));  // leaves so far <= keep
   ctx->cap_c = cap;
   return 0;
 }
@@ -90,24 +87,33 @@ static int build_tree(tc_context *ctx, int n) {
   TC_CHECK(hipStreamSynchronize(s));
   if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64), 0)) return e;
   launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s);
-  int depth = 0;
+  int depth = 0, nleaf = 0;
   while (le > ls) {
     ctx->level_start.push_back(le);
     int nl = le - ls;
-    // Reserve for the worst case (8 children per cell) before splitting, so no
-    // buffer the split writes is reallocated before the children are made.
-    if (int e = ensure_cells(ctx, le + 8 * nl, le)) return e;
+    // per-level split results (8 body offsets per cell, then child counts and
+    // leaf flags, and their scans); they are not kept across levels, and the
+    // cell arrays below grow without touching them
+    TC_CHECK(ctx->split_offs.ensure(sizeof(int) * 8 * (size_t)nl));
+    TC_CHECK(ctx->split_nch.ensure(sizeof(int) * 2 * (size_t)nl));
+    TC_CHECK(ctx->split_scan.ensure(sizeof(int) * 2 * (size_t)nl));
     CellsDev c = cells_of(ctx);
     launch_split(c, b, ls, le, ncrit, ctx->split_offs.as<int>(), ctx->split_nch.as<int>(), s);
-    launch_scan_small(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, 0, 1,
-                      ctx->tcnt.as<int>() + 7, s);
-    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 8, ctx->tcnt.as<int>() + 7, sizeof(int),
+    launch_scan_small(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, nl, 2,
+                      ctx->tcnt.as<int>() + 6, s);
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 8, ctx->tcnt.as<int>() + 6, 2 * sizeof(int),
                             hipMemcpyDeviceToHost, s));  // pinned: a true asynchronous copy
     TC_CHECK(hipStreamSynchronize(s));
-    int total = ctx->h_pin[8];
-    if (total == 0) break;
+    const int total = ctx->h_pin[8], nlev_leaves = ctx->h_pin[9];
+    // grow to the exact need: a worst-case reservation (8 children per cell)
+    // would move the arrays whenever a new root shift deepens the tree
+    if (total > 0)
+      if (int e = ensure_cells(ctx, le + total, le)) return e;
+    // the children, and this level's leaves appended to leaf_ids
     launch_children(cells_of(ctx), ls, le, ctx->split_offs.as<int>(), ctx->split_scan.as<int>(),
-                    ctx->tp.as<TreeParams>(), s);
+                    ctx->tp.as<TreeParams>(), ctx->leaf_ids.as<int>(), nleaf, s);
+    nleaf += nlev_leaves;
+    if (total == 0) break;
     ls = le;
     le = le + total;
     depth++;
@@ -117,32 +123,16 @@ static int build_tree(tc_context *ctx, int n) {
   ctx->st.ncells = nc;
   ctx->st.depth = depth;
   ctx->st.nlevels = (int)ctx->level_start.size() - 1;
-  // leaf list (host side, one copy of the child counts)
-  TC_CHECK(ctx->h_nch.ensure(nc));
-  TC_CHECK(ctx->h_nbd.ensure(nc));
-  TC_CHECK(ctx->h_leaves.ensure(nc));
-  const int *nch = ctx->h_nch.p, *nbd = ctx->h_nbd.p;
-  int *leaves = ctx->h_leaves.p;  // outlives the asynchronous upload
-  std::vector<int4> &groups = ctx->h_groups;
-  groups.clear();
-  TC_CHECK(hipMemcpyAsync(ctx->h_nch.p, ctx->cnchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipMemcpyAsync(ctx->h_nbd.p, ctx->cnbody.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));
-  int nleaf = 0;
-  for (int i = 0; i < nc; i++)
-    if (nch[i] == 0) {
-      leaves[nleaf++] = i;
-      // near-field target groups of <= 32 bodies, balanced within the leaf
-      int nb = nbd[i], ng = (nb + 31) / 32, o = 0;
-      for (int g = 0; g < ng; g++) {
-        int k = (nb - o) / (ng - g);
-        groups.push_back(make_int4(i, o, k, 0));
-        o += k;
-      }
-    }
   ctx->nleaves = nleaf;
-  ctx->st.nleaves = ctx->nleaves;
-  TC_CHECK(hipMemcpyAsync(ctx->leaf_ids.p, leaves, sizeof(int) * nleaf, hipMemcpyHostToDevice, s));
+  ctx->st.nleaves = nleaf;
+  // leaf ids and body counts for the host-built P2P work items: copied now,
+  // read after ev[11] (build_p2p_items, while the GPU runs M2L)
+  TC_CHECK(ctx->h_nbd.ensure(nc));
+  TC_CHECK(ctx->h_leaves.ensure(std::max(nleaf, 1)));
+  TC_CHECK(hipMemcpyAsync(ctx->h_leaves.p, ctx->leaf_ids.p, sizeof(int) * nleaf,
+                          hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipMemcpyAsync(ctx->h_nbd.p, ctx->cnbody.p, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipEventRecord(ctx->ev[11], s));
   return 0;
 }
 
@@ -176,8 +166,8 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   TC_CHECK(ctx->front_a.ensure(sizeof(int2) * 4096));
   TC_CHECK(ctx->m2l.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 24 * n)));
   TC_CHECK(ctx->p2p.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 8 * n)));
-  int2 root = make_int2(0, 0);
-  TC_CHECK(hipMemcpyAsync(ctx->front_a.p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
+  h[32] = h[33] = 0;  // the root pair (0, 0), from pinned memory
+  TC_CHECK(hipMemcpyAsync(ctx->front_a.p, h + 32, sizeof(int2), hipMemcpyHostToDevice, s));
   int nin = 1, nm2l = 0, np2p = 0;
   while (nin > 0) {
     const int nblk = (nin + 255) / 256, stride = nblk + 1;
@@ -280,16 +270,26 @@ static int build_lists(tc_context *ctx, int &nchunks) {
 // while the GPU is busy with M2L; the upload is queued behind it.
 static int build_p2p_items(tc_context *ctx) {
   constexpr int kItemList = 128;
-  const int *pcnt = ctx->h_pcnt.p;
+  TC_CHECK(hipEventSynchronize(ctx->ev[11]));  // leaf ids and body counts (build_tree)
+  const int *pcnt = ctx->h_pcnt.p, *leaves = ctx->h_leaves.p, *nbd = ctx->h_nbd.p;
   size_t nitem = 0;
-  for (const int4 &g : ctx->h_groups) nitem += (pcnt[g.x] + kItemList - 1) / kItemList;
+  for (int l = 0; l < ctx->nleaves; l++) {
+    const int i = leaves[l];
+    nitem += (size_t)((nbd[i] + 31) / 32) * ((pcnt[i] + kItemList - 1) / kItemList);
+  }
   TC_CHECK(ctx->h_items.ensure(std::max<size_t>(nitem, 1)));
   int4 *items = ctx->h_items.p;
   size_t k = 0;
-  for (const int4 &g : ctx->h_groups) {
-    const int cnt = pcnt[g.x];
-    for (int e0 = 0; e0 < cnt; e0 += kItemList)
-      items[k++] = make_int4(g.x, g.y, e0, std::min(kItemList, cnt - e0) | (g.z << 16));
+  for (int l = 0; l < ctx->nleaves; l++) {
+    // near-field target groups of <= 32 bodies, balanced within the leaf
+    const int i = leaves[l], nb = nbd[i], ng = (nb + 31) / 32, cnt = pcnt[i];
+    int o = 0;
+    for (int g = 0; g < ng; g++) {
+      const int gs = (nb - o) / (ng - g);
+      for (int e0 = 0; e0 < cnt; e0 += kItemList)
+        items[k++] = make_int4(i, o, e0, std::min(kItemList, cnt - e0) | (gs << 16));
+      o += gs;
+    }
   }
   ctx->n_items = nitem;
   TC_CHECK(ctx->p2p_items.ensure(sizeof(int4) * std::max<size_t>(nitem, 1)));
@@ -423,7 +423,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
     for (int d = 0; d < 3; d++) u[d] = ctx->rng.next() - 0.5;
   }
   std::memcpy(ctx->shift, u, sizeof(u));
-  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, u, sizeof(u), hipMemcpyHostToDevice, s));
+  // through pinned memory: a copy from pageable memory is staged and holds the host
+  std::memcpy(ctx->h_pin + 24, u, sizeof(u));
+  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, ctx->h_pin + 24, sizeof(u), hipMemcpyHostToDevice, s));
   TC_CHECK(hipEventRecord(ctx->ev[0], s));
 
   if (cfg.direct_below > 0 && n < cfg.direct_below) {
@@ -847,7 +849,6 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->lowF, &ctx->cPn, &ctx->accout};
   for (DBuf *b : all) b->release();
   ctx->h_leaves.release();
-  ctx->h_nch.release();
   ctx->h_nbd.release();
   ctx->h_pcnt.release();
   ctx->h_items.release();

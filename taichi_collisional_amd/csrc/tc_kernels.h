@@ -64,10 +64,14 @@ int launch_gather(const double *x3, const double *v3, const double *q,
                   const unsigned char *flags_in, const int *perm, int n, BodiesDev b,
                   hipStream_t s);
 int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s);
+// split: per level cell k, octant offsets offs[8k..8k+7], child count nch[k]
+// and leaf flag nch[(le - ls) + k]
 int launch_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit, int *offs, int *nch,
                  hipStream_t s);
-int launch_children(CellsDev c, int ls, int le, const int *offs, const int *nch_scan,
-                    const TreeParams *d_tp, hipStream_t s);
+// children of the level's cells (scan[k]: exclusive scan of the child counts)
+// and the level's leaves to leaf_ids[leaf_base + scan[(le - ls) + k]]
+int launch_children(CellsDev c, int ls, int le, const int *offs, const int *scan,
+                    const TreeParams *d_tp, int *leaf_ids, int leaf_base, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 // narrays exclusive scans of n ints (arrays stride apart), totals to total[k]

@@ -231,13 +231,15 @@ __device__ __forceinline__ int octant_of(const BodiesDev &b, int i, double X0, d
   return (b.x[i] >= X0) + ((b.y[i] >= X1) << 1) + ((b.z[i] >= X2) << 2);
 }
 
-// offs: 8 ints per cell of the level (octant starts), nch: children per cell
+// offs: 8 ints per cell of the level (octant starts), nch: children per
+// cell, then (at nch + nl) 1 for a leaf
 __global__ __launch_bounds__(256) void k_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit,
                                                int *__restrict__ offs, int *__restrict__ nch) {
   int k = blockIdx.x * 256 + threadIdx.x;
   int ci = ls + k;
   if (ci >= le) return;
   int nb = c.nbody[ci];
+  nch[(le - ls) + k] = nb <= ncrit;
   if (nb <= ncrit) {
     nch[k] = 0;
     c.nchild[ci] = 0;
@@ -291,15 +293,20 @@ int launch_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit, int *offs, 
 
 __global__ __launch_bounds__(256) void k_children(CellsDev c, int ls, int le,
                                                   const int *__restrict__ offs,
-                                                  const int *__restrict__ nch_scan,
-                                                  const TreeParams *__restrict__ tp) {
+                                                  const int *__restrict__ scan,
+                                                  const TreeParams *__restrict__ tp,
+                                                  int *__restrict__ leaf_ids, int leaf_base) {
 #pragma clang fp contract(off)
   int k = blockIdx.x * 256 + threadIdx.x;
   int ci = ls + k;
   if (ci >= le) return;
   int nc = c.nchild[ci];
-  if (nc == 0) return;
-  int first = le + nch_scan[k];
+  if (nc == 0) {
+    // leaves in cell order, level by level
+    leaf_ids[leaf_base + scan[(le - ls) + k]] = ci;
+    return;
+  }
+  int first = le + scan[k];
   c.child[ci] = first;
   int level = c.level[ci];
   int begin = c.body[ci], end = begin + c.nbody[ci];
@@ -323,12 +330,12 @@ __global__ __launch_bounds__(256) void k_children(CellsDev c, int ls, int le,
   }
 }
 
-int launch_children(CellsDev c, int ls, int le, const int *offs, const int *nch_scan,
-                    const TreeParams *d_tp, hipStream_t s) {
+int launch_children(CellsDev c, int ls, int le, const int *offs, const int *scan,
+                    const TreeParams *d_tp, int *leaf_ids, int leaf_base, hipStream_t s) {
   int n = le - ls;
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_children, dim3((n + 255) / 256), dim3(256), 0, s, c, ls, le, offs,
-                     nch_scan, d_tp);
+  hipLaunchKernelGGL(k_children, dim3((n + 255) / 256), dim3(256), 0, s, c, ls, le, offs, scan,
+                     d_tp, leaf_ids, leaf_base);
   return 0;
 }
 
