@@ -131,8 +131,9 @@ struct tc_context {
   // tree order), symmetric M2L list and its scan
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
-  // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd
-  hipEvent_t ev[12];
+  // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],
+  // [13] end of the upward geometry (stream) and moments (near_stream)
+  hipEvent_t ev[14];
   // second stream (default priority): P2P runs on it while the downward pass
   // runs on `stream`; its far-field results go to a second result set
   hipStream_t near_stream = nullptr;

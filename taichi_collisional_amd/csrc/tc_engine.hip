@@ -136,7 +136,10 @@ static int build_tree(tc_context *ctx, int n) {
   return 0;
 }
 
-static int upward(tc_context *ctx) {
+// Upward pass.  split: the geometry (balls, min_acc, has_sink: all the theta
+// walk reads) on `stream`, then the moments on near_stream after it; ev[13]
+// marks their end, and the caller makes `stream` wait for it before M is read.
+static int upward(tc_context *ctx, bool split = false) {
   hipStream_t s = ctx->stream;
   int p = ctx->cfg.p;
   int nc = ctx->st.ncells;
@@ -148,13 +151,23 @@ static int upward(tc_context *ctx) {
   TC_CHECK(hipMemsetAsync(ctx->csink.p, 0, nc, s));
   CellsDev c = cells_of(ctx);
   BodiesDev b = bodies_of(ctx);
-  if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s)) {
+  const int L = (int)ctx->level_start.size() - 1;
+  const int first = split ? kUpGeom : kUpGeom | kUpMoments;
+  if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s, first)) {
     ctx->err = "expansion order not compiled";
     return -2;
   }
-  int L = (int)ctx->level_start.size() - 1;
   for (int l = L - 1; l >= 0; l--)
-    launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s);
+    launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s, first);
+  if (split) {
+    hipStream_t s2 = ctx->near_stream;
+    TC_CHECK(hipEventRecord(ctx->ev[12], s));
+    TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[12], 0));
+    launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s2, kUpMoments);
+    for (int l = L - 1; l >= 0; l--)
+      launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s2, kUpMoments);
+    TC_CHECK(hipEventRecord(ctx->ev[13], s2));
+  }
   return 0;
 }
 
@@ -470,8 +483,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
     ctx->has_acc_old = true;
   }
   TC_CHECK(hipEventRecord(ctx->ev[1], s));
-  // ---- upward
-  if (int e = upward(ctx)) return e;
+  // ---- upward: with the theta MAC the walk needs only the geometry, so the
+  // moments are formed on the second stream while the walk and the lists run
+  if (int e = upward(ctx, !err_mac)) return e;
   if (err_mac)
     launch_pn(ctx->cM.as<double>(), ctx->st.ncells, cfg.p, fact_table(cfg), ctx->cPn.as<double>(),
               s);
@@ -481,6 +495,7 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   TC_CHECK(hipEventRecord(ctx->ev[3], s));
   int nchunks = 0;
   if (int e = build_lists(ctx, nchunks)) return e;
+  if (!err_mac) TC_CHECK(hipStreamWaitEvent(s, ctx->ev[13], 0));  // the moments
   TC_CHECK(hipEventRecord(ctx->ev[4], s));
   // ---- far field
   M2LArgs ma;

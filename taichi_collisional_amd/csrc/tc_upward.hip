@@ -81,7 +81,10 @@ __device__ __forceinline__ void p2m_orders(double dx, double dy, double dz, doub
   }
 }
 
-template <int P>
+// kWhat: kUpGeom (ball, min_acc, has_sink), kUpMoments (M, Mdot about the
+// ball centre already in c.X), or both.  Split, the traversal can start on
+// the geometry while the moments are formed on another stream.
+template <int P, int kWhat>
 __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
                                             const int *__restrict__ leaf_ids) {
   constexpr int NT = nterm(P);
@@ -92,6 +95,10 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
   const int lane = threadIdx.x;
   const int ci = leaf_ids[blockIdx.x];
   const int b0 = c.body[ci], nb = c.nbody[ci];
+  double X[3];
+  if constexpr (!(kWhat & kUpGeom)) {
+    for (int d = 0; d < 3; d++) X[d] = c.X[3 * ci + d];
+  } else {
   const bool staged = nb <= kStage;
   if (staged)
     for (int i = lane; i < nb; i += kWave) {
@@ -100,7 +107,7 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
       s_q[2][i] = b.z[b0 + i];
     }
   __syncthreads();
-  double X[3], R;
+  double R;
   if (nb > 0) {
     DBall ball = wave_minball(nb, [&](int i, double *q) {
       if (staged) {
@@ -136,6 +143,8 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
     c.min_acc[ci] = mina;
     c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
   }
+  }
+  if constexpr (!(kWhat & kUpMoments)) return;
   double *Mc = c.M + (size_t)ci * NT;
   double *Mdc = c.Md + (size_t)ci * NT;
   for (int base = 0; base < nb || base == 0; base += kWave) {
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
   }
 }
 
-template <int P>
+template <int P, int kWhat>
 __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int *__restrict__ ids,
                                              int ls) {
 #pragma clang fp contract(off)
@@ -183,6 +192,10 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
+  double X[3];
+  if constexpr (!(kWhat & kUpGeom)) {
+    for (int d = 0; d < 3; d++) X[d] = c.X[3 * ci + d];
+  } else {
   if (tid < 192) s_td[tid] = kTDesign[tid];
   if (tid == 0) {
     int pre = 0, nbs = 0, ng = 0;
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
         }
       },
       s_e, s_i);
-  const double X[3] = {ball.c[0], ball.c[1], ball.c[2]};
+  for (int d = 0; d < 3; d++) X[d] = ball.c[d];
   if (tid == 0) {
     double mina = 1e30;
     int sink = 0;
@@ -281,6 +294,8 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
     c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
   }
   __syncthreads();  // the ball's buffers are reused below
+  }
+  if constexpr (!(kWhat & kUpMoments)) return;
   if constexpr (kPar) {
     // translation, children in parallel: lane = (task group g, child k) with
     // k fastest, each lane a strided share of the output terms of child k;
@@ -390,21 +405,36 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
 }
 
 template <int P>
-static void p2m_launch(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s) {
-  hipLaunchKernelGGL(k_p2m<P>, dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids);
+static void p2m_launch(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, int what,
+                       hipStream_t s) {
+  if (what == kUpGeom)
+    hipLaunchKernelGGL((k_p2m<P, kUpGeom>), dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids);
+  else if (what == kUpMoments)
+    hipLaunchKernelGGL((k_p2m<P, kUpMoments>), dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids);
+  else
+    hipLaunchKernelGGL((k_p2m<P, kUpGeom | kUpMoments>), dim3(nleaves), dim3(64), 0, s, c, b,
+                       leaf_ids);
 }
 
 template <int P>
-static void m2m_launch(CellsDev c, BodiesDev b, const int *ids, int ls, int ncell, hipStream_t s) {
-  hipLaunchKernelGGL(k_m2m<P>, dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
+static void m2m_launch(CellsDev c, BodiesDev b, const int *ids, int ls, int ncell, int what,
+                       hipStream_t s) {
+  if (what == kUpGeom)
+    hipLaunchKernelGGL((k_m2m<P, kUpGeom>), dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
+  else if (what == kUpMoments)
+    hipLaunchKernelGGL((k_m2m<P, kUpMoments>), dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
+  else
+    hipLaunchKernelGGL((k_m2m<P, kUpGeom | kUpMoments>), dim3(ncell), dim3(256), 0, s, c, b,
+                       ids, ls);
 }
 
-int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s) {
+int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s,
+               int what) {
   if (nleaves <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                            \
-  case P:                                     \
-    p2m_launch<P>(c, b, leaf_ids, nleaves, s); \
+#define TC_CASE(P)                                   \
+  case P:                                            \
+    p2m_launch<P>(c, b, leaf_ids, nleaves, what, s); \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
@@ -413,12 +443,12 @@ int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
   }
 }
 
-int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
+int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s, int what) {
   if (le <= ls) return 0;
   switch (p) {
-#define TC_CASE(P)                                 \
-  case P:                                          \
-    m2m_launch<P>(c, b, nullptr, ls, le - ls, s);  \
+#define TC_CASE(P)                                      \
+  case P:                                               \
+    m2m_launch<P>(c, b, nullptr, ls, le - ls, what, s); \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
@@ -430,9 +460,9 @@ int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s) {
 int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hipStream_t s) {
   if (nids <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                             \
-  case P:                                      \
-    m2m_launch<P>(c, b, ids, 0, nids, s);      \
+#define TC_CASE(P)                                                 \
+  case P:                                                          \
+    m2m_launch<P>(c, b, ids, 0, nids, kUpGeom | kUpMoments, s);    \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
