@@ -5,9 +5,11 @@
 //
 //   L2L (kernel_fast_lean.h:1001-1058): for each child,
 //       L_child[n,m] += sum_{k<=p-n} sum_{|l|<=k} L[n+k, m+l] conj Y_k^l(X - X_child)
-//       one block of 256 threads per parent cell: 32 threads per child, each
-//       thread a strided share of the child's (output term, L or Ldot) tasks;
-//       the eight harmonic tables and the parent's expansion sit in LDS.
+//       two blocks of 128 threads per parent cell, four children each: 32
+//       threads per child, each thread a strided share of the child's (output
+//       term, L or Ldot) tasks; the four harmonic tables and the parent's
+//       expansion sit in LDS (16 KB at p = 12, so a block finds room on a CU
+//       that the P2P kernel running alongside has otherwise filled).
 //   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
 //       dY/dt with velocity -v_b, streamed order by order (three harmonic
 //       rows in registers); only n <= 1 of the local field is formed:
@@ -20,17 +22,18 @@
 namespace tcfmm {
 
 template <int P>
-__global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
+__global__ __launch_bounds__(128) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
   constexpr int kGroup = 32;  // threads per child
   static_assert(P + 1 <= kGroup, "one thread per harmonic column");
-  __shared__ cplx G[8][NT], cl[NT], cld[NT];
+  __shared__ cplx G[4][NT], cl[NT], cld[NT];
   const int tid = threadIdx.x;
-  const int ci = ids ? ids[blockIdx.x] : ls + blockIdx.x;
-  const int nch = c.nchild[ci];
-  if (nch == 0) return;
-  const int ch0 = c.child[ci];
+  const int cell = blockIdx.x >> 1, half = blockIdx.x & 1;
+  const int ci = ids ? ids[cell] : ls + cell;
+  const int nch = c.nchild[ci] - 4 * half;  // children of this block's half
+  if (nch <= 0) return;
+  const int ch0 = c.child[ci] + 4 * half;
   const double *Lp = c.L + (size_t)ci * NT;
   const double *Ldp = c.Ld + (size_t)ci * NT;
   for (int t = tid; t < NT; t += blockDim.x) {
@@ -229,7 +232,7 @@ int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double
 
 template <int P>
 static void l2l_go(CellsDev c, const int *ids, int ls, int ncell, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2l<P>, dim3(ncell), dim3(256), 0, s, c, ids, ls);
+  hipLaunchKernelGGL(k_l2l<P>, dim3(2 * ncell), dim3(128), 0, s, c, ids, ls);
 }
 template <int P>
 static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {
