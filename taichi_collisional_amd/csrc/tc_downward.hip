@@ -5,11 +5,14 @@
 //
 //   L2L (kernel_fast_lean.h:1001-1058): for each child,
 //       L_child[n,m] += sum_{k<=p-n} sum_{|l|<=k} L[n+k, m+l] conj Y_k^l(X - X_child)
-//       two blocks of 128 threads per parent cell, four children each: 32
+//       two blocks of 256 threads per parent cell, four children each: 64
 //       threads per child, each thread a strided share of the child's (output
 //       term, L or Ldot) tasks; the four harmonic tables and the parent's
 //       expansion sit in LDS (16 KB at p = 12, so a block finds room on a CU
-//       that the P2P kernel running alongside has otherwise filled).
+//       that the P2P kernel running alongside has otherwise filled).  A
+//       task's sum runs over contiguous runs of both tables, two terms at a
+//       time into two partial sums: the top levels are a few blocks each, and
+//       their time is that of one thread's chain of LDS reads and adds.
 //   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
 //       dY/dt with velocity -v_b, streamed order by order (three harmonic
 //       rows in registers); only n <= 1 of the local field is formed:
@@ -22,10 +25,10 @@
 namespace tcfmm {
 
 template <int P>
-__global__ __launch_bounds__(128) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
+__global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
   constexpr int NT = nterm(P);
   constexpr int NO = tri(P, P) + 1;
-  constexpr int kGroup = 32;  // threads per child
+  constexpr int kGroup = 64;  // threads per child
   static_assert(P + 1 <= kGroup, "one thread per harmonic column");
   __shared__ cplx G[4][NT], cl[NT], cld[NT];
   const int tid = threadIdx.x;
@@ -60,10 +63,19 @@ __global__ __launch_bounds__(128) void k_l2l(CellsDev c, const int *__restrict__
     while (tri(n + 1, 0) <= o) n++;
     const int m = o - tri(n, 0);
     const cplx *src = which ? cld : cl;
-    cplx a = {0, 0};
-    for (int kk = 0; kk <= P - n; kk++)
-      for (int l = -kk; l <= kk; l++)
-        a = cadd(a, cmul(src[ixnm(n + kk, m + l)], cconj(Gk[ixnm(kk, l)])));
+    cplx a = {0, 0}, a2 = {0, 0};
+    for (int kk = 0; kk <= P - n; kk++) {
+      // l = -kk..kk: src[ixnm(n + kk, m + l)] and Gk[ixnm(kk, l)] run contiguously
+      const cplx *sr = src + ixnm(n + kk, m - kk), *gr = Gk + ixnm(kk, -kk);
+      const int len = 2 * kk + 1;
+      int t = 0;
+      for (; t + 1 < len; t += 2) {
+        a = cadd(a, cmul(sr[t], cconj(gr[t])));
+        a2 = cadd(a2, cmul(sr[t + 1], cconj(gr[t + 1])));
+      }
+      if (t < len) a = cadd(a, cmul(sr[t], cconj(gr[t])));
+    }
+    a = cadd(a, a2);
     double *dst = which ? Ldc : Lc;
     const int s = n * n + n;
     dst[s + m] += a.re;
@@ -232,7 +244,7 @@ int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double
 
 template <int P>
 static void l2l_go(CellsDev c, const int *ids, int ls, int ncell, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2l<P>, dim3(2 * ncell), dim3(128), 0, s, c, ids, ls);
+  hipLaunchKernelGGL(k_l2l<P>, dim3(2 * ncell), dim3(256), 0, s, c, ids, ls);
 }
 template <int P>
 static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {
