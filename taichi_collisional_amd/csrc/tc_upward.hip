@@ -325,14 +325,23 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
         int n = 0;
         while (tri(n + 1, 0) <= o && n < P) n++;
         const int m = o - tri(n, 0);
-        cplx acc = {0, 0};
+        cplx acc = {0, 0}, acc2 = {0, 0};
         if (haveK)
           for (int kk = 0; kk <= n; kk++) {
             const int lo = -kk > m - n + kk ? -kk : m - n + kk;
             const int hi = kk < m + n - kk ? kk : m + n - kk;
-            for (int l = lo; l <= hi; l++)
-              acc = cadd(acc, cmul(src[ixnm(n - kk, m - l)], Gk[ixnm(kk, l)]));
+            // src[ixnm(n - kk, m - l)] runs down and Gk[ixnm(kk, l)] up with
+            // l; two partial sums halve the chain of dependent adds
+            const cplx *sr = src + ixnm(n - kk, m - lo), *gr = Gk + ixnm(kk, lo);
+            const int len = hi - lo + 1;
+            int t = 0;
+            for (; t + 1 < len; t += 2) {
+              acc = cadd(acc, cmul(sr[-t], gr[t]));
+              acc2 = cadd(acc2, cmul(sr[-t - 1], gr[t + 1]));
+            }
+            if (t < len) acc = cadd(acc, cmul(sr[-t], gr[t]));
           }
+        acc = cadd(acc, acc2);
 #pragma unroll
         for (int x = 1; x < 8; x <<= 1) {
           acc.re += __shfl_xor(acc.re, x, kWave);
