@@ -113,7 +113,8 @@ struct tc_context {
   DBuf p2p_items, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists
-  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan;
+  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan, tstate;
+  int trav_steps[4] = {0, 0, 0, 0};  // steps of the last walk per MAC mode (batch size)
   DBuf gtg_a, gtg_b, gsrc_a, gsort_tmp;
   int *h_pin = nullptr;  // small pinned host buffer for per-step counts
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;

@@ -171,53 +171,75 @@ static int upward(tc_context *ctx, bool split = false) {
   return 0;
 }
 
-// Breadth-first dual traversal from (root, root), one launch pair per step.
+// Breadth-first dual traversal from (root, root).  Steps are queued in
+// batches without reading counts back (TravState holds them on the device);
+// after a batch one small copy says whether the frontier is empty.  A step
+// whose output would not fit its buffer ends the walk with a flag and what it
+// needed: the buffers then grow and the walk starts again (in practice only
+// on the first evaluation of a workload).
 static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   hipStream_t s = ctx->stream;
-  int *h = ctx->h_pin;
-  TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 8));
-  TC_CHECK(ctx->front_a.ensure(sizeof(int2) * 4096));
+  TravState *hst = reinterpret_cast<TravState *>(ctx->h_pin + 40);
+  static_assert(sizeof(TravState) <= 24 * sizeof(int), "pinned slots 40..63");
+  TC_CHECK(ctx->tstate.ensure(sizeof(TravState)));
+  TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * kTravBlocks));
+  TC_CHECK(ctx->front_a.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 4 * n)));
+  TC_CHECK(ctx->front_b.ensure(ctx->front_a.bytes));
   TC_CHECK(ctx->m2l.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 24 * n)));
   TC_CHECK(ctx->p2p.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 8 * n)));
-  h[32] = h[33] = 0;  // the root pair (0, 0), from pinned memory
-  TC_CHECK(hipMemcpyAsync(ctx->front_a.p, h + 32, sizeof(int2), hipMemcpyHostToDevice, s));
-  int nin = 1, nm2l = 0, np2p = 0;
-  while (nin > 0) {
-    const int nblk = (nin + 255) / 256, stride = nblk + 1;
-    TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * (size_t)stride));
-    int *blk = ctx->tblk.as<int>();
-    CellsDev c = cells_of(ctx);
-    launch_trav_count(c, ctx->front_a.as<int2>(), nin, mac, blk, blk + stride,
-                      blk + 2 * stride, s);
-    launch_scan_small(blk, blk + 3 * stride, nblk, stride, 3, ctx->tcnt.as<int>() + 4, s);
-    TC_CHECK(hipMemcpyAsync(h + 4, ctx->tcnt.as<int>() + 4, 3 * sizeof(int),
-                            hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipStreamSynchronize(s));
-    const int tm = h[4], tp = h[5], ts = h[6];
-    // capacity for this step's output before anything is written
-    TC_CHECK(ctx->m2l.ensure(sizeof(int2) * ((size_t)nm2l + tm), sizeof(int2) * (size_t)nm2l, s));
-    TC_CHECK(ctx->p2p.ensure(sizeof(int2) * ((size_t)np2p + tp), sizeof(int2) * (size_t)np2p, s));
-    TC_CHECK(ctx->front_b.ensure(sizeof(int2) * (size_t)std::max(ts, 1)));
-    h[0] = 0;
-    h[1] = nm2l;
-    h[2] = np2p;
-    TC_CHECK(hipMemcpyAsync(ctx->tcnt.p, h, 3 * sizeof(int), hipMemcpyHostToDevice, s));
-    TravArgs a;
-    a.fin = ctx->front_a.as<int2>();
-    a.nin = nin;
-    a.fout = ctx->front_b.as<int2>();
-    a.cnt = ctx->tcnt.as<int>();
-    a.m2l = ctx->m2l.as<int2>();
-    a.p2p = ctx->p2p.as<int2>();
-    a.mac = mac;
-    launch_trav_write(cells_of(ctx), a, blk + 3 * stride, blk + 4 * stride, blk + 5 * stride, s);
-    nm2l += tm;
-    np2p += tp;
-    nin = ts;
-    std::swap(ctx->front_a, ctx->front_b);
+  int &known = ctx->trav_steps[mac.mode & 3];
+  for (int attempt = 0;; attempt++) {
+    if (attempt > 16) {
+      ctx->err = "traversal: list buffers keep overflowing";
+      return -3;
+    }
+    auto cap = [](const DBuf &d) {
+      return (long long)std::min<size_t>(d.bytes / sizeof(int2), 0x7fffffff);
+    };
+    const long long cm = cap(ctx->m2l), cp = cap(ctx->p2p),
+                    cf = std::min(cap(ctx->front_a), cap(ctx->front_b));
+    std::memset(hst, 0, sizeof(TravState));
+    hst->nin[0] = 1;
+    ctx->h_pin[32] = ctx->h_pin[33] = 0;  // the root pair (0, 0)
+    TC_CHECK(hipMemcpyAsync(ctx->tstate.p, hst, sizeof(TravState), hipMemcpyHostToDevice, s));
+    TC_CHECK(hipMemcpyAsync(ctx->front_a.p, ctx->h_pin + 32, sizeof(int2), hipMemcpyHostToDevice,
+                            s));
+    int k = 0, batch = std::max(8, known + 1);
+    for (;;) {
+      for (int i = 0; i < batch; i++, k++) {
+        const bool odd = k & 1;
+        TravArgs a;
+        a.fin = (odd ? ctx->front_b : ctx->front_a).as<int2>();
+        a.fout = (odd ? ctx->front_a : ctx->front_b).as<int2>();
+        a.m2l = ctx->m2l.as<int2>();
+        a.p2p = ctx->p2p.as<int2>();
+        a.mac = mac;
+        launch_trav_step(cells_of(ctx), a, ctx->tstate.as<TravState>(), (int)odd,
+                         ctx->tblk.as<int>(), cm, cp, cf, s);
+      }
+      TC_CHECK(hipMemcpyAsync(hst, ctx->tstate.p, sizeof(TravState), hipMemcpyDeviceToHost, s));
+      TC_CHECK(hipStreamSynchronize(s));
+      if (hst->overflow || hst->nin[k & 1] == 0) break;
+      if (k > 100000) {
+        ctx->err = "traversal does not end";
+        return -3;
+      }
+      batch = 4;
+    }
+    if (!hst->overflow) break;
+    // grow what overflowed (a quarter more than the failing step needed) and walk again
+    const size_t gm = (size_t)hst->need_m2l, gp = (size_t)hst->need_p2p,
+                 gf = (size_t)hst->need_front;
+    if ((long long)gm > cm) TC_CHECK(ctx->m2l.ensure(sizeof(int2) * (gm + gm / 4)));
+    if ((long long)gp > cp) TC_CHECK(ctx->p2p.ensure(sizeof(int2) * (gp + gp / 4)));
+    if ((long long)gf > cf) {
+      TC_CHECK(ctx->front_a.ensure(sizeof(int2) * (gf + gf / 4)));
+      TC_CHECK(ctx->front_b.ensure(ctx->front_a.bytes));
+    }
   }
-  ctx->st.n_m2l = nm2l;
-  ctx->st.n_p2p_cells = np2p;
+  known = hst->steps;
+  ctx->st.n_m2l = hst->nm2l;
+  ctx->st.n_p2p_cells = hst->np2p;
   return 0;
 }
 
@@ -853,7 +875,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->cnbody, &ctx->cXg, &ctx->cRg, &ctx->cX, &ctx->cR, &ctx->cmin, &ctx->csink,
                  &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
                  &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
-                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan,
+                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate,
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,

@@ -98,17 +98,25 @@ struct FactTable {
   double fact[2 * kMaxP + 1];  // the reference's factorial_table (quirk applied)
 };
 struct TravArgs {
-  const int2 *fin;
-  int nin;
-  int2 *fout;
-  int *cnt;  // [1] M2L pairs so far, [2] P2P pairs so far (device copy)
-  int2 *m2l, *p2p;
+  const int2 *fin;  // this step's frontier
+  int2 *fout;       // the next step's
+  int2 *m2l, *p2p;  // the lists, appended to
   MacArgs mac;
 };
-int launch_trav_count(CellsDev c, const int2 *fin, int nin, const MacArgs &mac, int *bm2l,
-                      int *bp2p, int *bsplit, hipStream_t s);
-int launch_trav_write(CellsDev c, const TravArgs &a, const int *om2l, const int *op2p,
-                      const int *osplit, hipStream_t s);
+// Device-side state of a walk (ints): frontier sizes nin[step parity], list
+// lengths so far, this step's list offsets, whether pass 2 writes, steps
+// taken, and on overflow what the failing step needed.
+struct TravState {
+  int nin[2];
+  int nm2l, np2p, base_m2l, base_p2p;
+  int write, overflow, steps;
+  int need_m2l, need_p2p, need_front;
+};
+constexpr int kTravBlocks = 2048;  // blocks of a walk step; blk holds 6 x kTravBlocks ints
+// One step of the walk (count, offsets, write) on the frontier of parity par;
+// caps are the capacities of the lists and of the next frontier (pairs).
+int launch_trav_step(CellsDev c, const TravArgs &a, TravState *st, int par, int *blk,
+                     long long cap_m2l, long long cap_p2p, long long cap_front, hipStream_t s);
 size_t pair_sort_temp_bytes(int n);
 int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *tg_a, int *tg_b,
                        int *src_a, int *src, int *start, int *count, void *temp,

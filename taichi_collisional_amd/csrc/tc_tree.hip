@@ -478,15 +478,17 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
   return 4;
 }
 
-__device__ __forceinline__ int3 block_excl_scan3(int3 v, int3 *tot) {
-  __shared__ int3 wsum[4];
+// Exclusive scan of three ints over a block of NW waves; totals to *tot.
+template <int NW>
+__device__ __forceinline__ int3 block_excl_scan3_n(int3 v, int3 *tot) {
+  __shared__ int3 wsum[NW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int tx, ty, tz;
   int ex = wave_excl_scan(v.x, &tx), ey = wave_excl_scan(v.y, &ty), ez = wave_excl_scan(v.z, &tz);
   if (lane == 0) wsum[w] = make_int3(tx, ty, tz);
   __syncthreads();
   int3 base = make_int3(0, 0, 0), all = make_int3(0, 0, 0);
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < NW; k++) {
     int3 q = wsum[k];
     if (k < w) base = make_int3(base.x + q.x, base.y + q.y, base.z + q.z);
     all = make_int3(all.x + q.x, all.y + q.y, all.z + q.z);
@@ -496,66 +498,140 @@ __device__ __forceinline__ int3 block_excl_scan3(int3 v, int3 *tot) {
   return make_int3(base.x + ex, base.y + ey, base.z + ez);
 }
 
+// Steps of the walk queued without host reads: the frontier size, the list
+// lengths and the step's offsets live in TravState.  Block b of the
+// kTravBlocks blocks owns frontier pairs [b R, (b + 1) R), R = ceil(nin /
+// kTravBlocks), walked 256 at a time, so the outputs keep frontier order.
+__device__ __forceinline__ void trav_range(int nin, int &lo, int &hi) {
+  const int R = (nin + kTravBlocks - 1) / kTravBlocks;
+  const long long l = (long long)blockIdx.x * R;
+  lo = l < nin ? (int)l : nin;
+  hi = nin - lo < R ? nin : lo + R;
+}
+
+// pass 1: per-block totals of M2L pairs, P2P pairs and child pairs
 __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__restrict__ fin,
-                                                    int nin, MacArgs mac,
-                                                    int *__restrict__ bm2l,
-                                                    int *__restrict__ bp2p,
-                                                    int *__restrict__ bsplit) {
-  int t = blockIdx.x * 256 + threadIdx.x;
-  int3 v = make_int3(0, 0, 0);
-  if (t < nin) {
-    int2 pr = fin[t];
-    int ns;
-    int kind = trav_classify(c, mac, pr.x, pr.y, &ns);
-    v = make_int3(kind == 1, kind == 2, ns);
+                                                    const TravState *__restrict__ st, int par,
+                                                    MacArgs mac, int *__restrict__ blk) {
+  int lo, hi;
+  trav_range(st->nin[par], lo, hi);
+  int3 acc = make_int3(0, 0, 0);
+  for (int base = lo; base < hi; base += 256) {
+    const int t = base + (int)threadIdx.x;
+    int3 v = make_int3(0, 0, 0);
+    if (t < hi) {
+      const int2 pr = fin[t];
+      int ns;
+      const int kind = trav_classify(c, mac, pr.x, pr.y, &ns);
+      v = make_int3(kind == 1, kind == 2, ns);
+    }
+    int3 tot;
+    block_excl_scan3_n<4>(v, &tot);
+    acc = make_int3(acc.x + tot.x, acc.y + tot.y, acc.z + tot.z);
   }
-  int3 tot;
-  block_excl_scan3(v, &tot);
   if (threadIdx.x == 0) {
-    bm2l[blockIdx.x] = tot.x;
-    bp2p[blockIdx.x] = tot.y;
-    bsplit[blockIdx.x] = tot.z;
+    blk[blockIdx.x] = acc.x;
+    blk[kTravBlocks + blockIdx.x] = acc.y;
+    blk[2 * kTravBlocks + blockIdx.x] = acc.z;
   }
 }
 
-__global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
-                                                    const int *__restrict__ om2l,
-                                                    const int *__restrict__ op2p,
-                                                    const int *__restrict__ osplit, int nblk) {
-  int t = blockIdx.x * 256 + threadIdx.x;
-  int kind = 0, ns = 0;
-  int2 pr = make_int2(0, 0);
-  if (t < a.nin) {
-    pr = a.fin[t];
-    kind = trav_classify(c, a.mac, pr.x, pr.y, &ns);
+// the block offsets (exclusive scans of pass 1's totals, to blk[3 G ..]) and
+// the state for pass 2 and the next step; a step whose output would not fit
+// sets the overflow flag, records what it needed and ends the walk
+__global__ __launch_bounds__(1024) void k_trav_scan(int *__restrict__ blk, TravState *st, int par,
+                                                   long long cap_m2l, long long cap_p2p,
+                                                   long long cap_front) {
+  constexpr int G = kTravBlocks, PER = G / 1024;
+  static_assert(G % 1024 == 0, "whole entries per thread");
+  const int t = threadIdx.x;
+  int3 a[PER];
+  int3 sum = make_int3(0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = t * PER + k;
+    a[k] = make_int3(blk[i], blk[G + i], blk[2 * G + i]);
+    sum = make_int3(sum.x + a[k].x, sum.y + a[k].y, sum.z + a[k].z);
   }
   int3 tot;
-  int3 ex = block_excl_scan3(make_int3(kind == 1, kind == 2, ns), &tot);
-  const int b = blockIdx.x;
-  if (kind == 1) a.m2l[a.cnt[1] + om2l[b] + ex.x] = pr;
-  if (kind == 2) a.p2p[a.cnt[2] + op2p[b] + ex.y] = pr;
-  if (kind >= 3) {
-    int base = osplit[b] + ex.z;
-    int c0 = c.child[kind == 3 ? pr.x : pr.y];
-    for (int k = 0; k < ns; k++)
-      a.fout[base + k] = kind == 3 ? make_int2(c0 + k, pr.y) : make_int2(pr.x, c0 + k);
+  int3 run = block_excl_scan3_n<16>(sum, &tot);
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = t * PER + k;
+    blk[3 * G + i] = run.x;
+    blk[4 * G + i] = run.y;
+    blk[5 * G + i] = run.z;
+    run = make_int3(run.x + a[k].x, run.y + a[k].y, run.z + a[k].z);
   }
-  (void)nblk;
+  if (t == 0) {
+    const int nin = st->nin[par];
+    const long long nm = st->nm2l, np = st->np2p;
+    const bool fits = !st->overflow && nm + tot.x <= cap_m2l && np + tot.y <= cap_p2p &&
+                      tot.z <= cap_front;
+    if (fits) {
+      st->base_m2l = (int)nm;
+      st->base_p2p = (int)np;
+      st->nm2l = (int)(nm + tot.x);
+      st->np2p = (int)(np + tot.y);
+      st->nin[par ^ 1] = tot.z;
+      st->write = 1;
+      if (nin > 0) st->steps++;
+    } else {
+      if (!st->overflow) {
+        const long long big = 0x7fffffff;
+        st->need_m2l = (int)(nm + tot.x < big ? nm + tot.x : big);
+        st->need_p2p = (int)(np + tot.y < big ? np + tot.y : big);
+        st->need_front = tot.z;
+      }
+      st->overflow = 1;
+      st->write = 0;
+      st->nin[par ^ 1] = 0;
+    }
+  }
 }
 
-int launch_trav_count(CellsDev c, const int2 *fin, int nin, const MacArgs &mac, int *bm2l,
-                      int *bp2p, int *bsplit, hipStream_t s) {
-  if (nin <= 0) return 0;
-  hipLaunchKernelGGL(k_trav_count, dim3((nin + 255) / 256), dim3(256), 0, s, c, fin, nin, mac,
-                     bm2l, bp2p, bsplit);
-  return 0;
+// pass 2: every output at its place (block offset + running count within the
+// block + rank within the 256-pair slice)
+__global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
+                                                    const TravState *__restrict__ st, int par,
+                                                    const int *__restrict__ blk) {
+  if (!st->write) return;
+  int lo, hi;
+  trav_range(st->nin[par], lo, hi);
+  const int b = blockIdx.x;
+  int3 run = make_int3(blk[3 * kTravBlocks + b], blk[4 * kTravBlocks + b],
+                       blk[5 * kTravBlocks + b]);
+  const int bm = st->base_m2l, bp = st->base_p2p;
+  for (int base = lo; base < hi; base += 256) {
+    const int t = base + (int)threadIdx.x;
+    int kind = 0, ns = 0;
+    int2 pr = make_int2(0, 0);
+    if (t < hi) {
+      pr = a.fin[t];
+      kind = trav_classify(c, a.mac, pr.x, pr.y, &ns);
+    }
+    int3 tot;
+    const int3 ex = block_excl_scan3_n<4>(make_int3(kind == 1, kind == 2, ns), &tot);
+    if (kind == 1) a.m2l[bm + run.x + ex.x] = pr;
+    if (kind == 2) a.p2p[bp + run.y + ex.y] = pr;
+    if (kind >= 3) {
+      const int o = run.z + ex.z;
+      const int c0 = c.child[kind == 3 ? pr.x : pr.y];
+      for (int k = 0; k < ns; k++)
+        a.fout[o + k] = kind == 3 ? make_int2(c0 + k, pr.y) : make_int2(pr.x, c0 + k);
+    }
+    run = make_int3(run.x + tot.x, run.y + tot.y, run.z + tot.z);
+  }
 }
 
-int launch_trav_write(CellsDev c, const TravArgs &a, const int *om2l, const int *op2p,
-                      const int *osplit, hipStream_t s) {
-  if (a.nin <= 0) return 0;
-  int nblk = (a.nin + 255) / 256;
-  hipLaunchKernelGGL(k_trav_write, dim3(nblk), dim3(256), 0, s, c, a, om2l, op2p, osplit, nblk);
+int launch_trav_step(CellsDev c, const TravArgs &a, TravState *st, int par, int *blk,
+                     long long cap_m2l, long long cap_p2p, long long cap_front, hipStream_t s) {
+  hipLaunchKernelGGL(k_trav_count, dim3(kTravBlocks), dim3(256), 0, s, c, a.fin,
+                     (const TravState *)st, par, a.mac, blk);
+  hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(1024), 0, s, blk, st, par, cap_m2l, cap_p2p,
+                     cap_front);
+  hipLaunchKernelGGL(k_trav_write, dim3(kTravBlocks), dim3(256), 0, s, c, a,
+                     (const TravState *)st, par, (const int *)blk);
   return 0;
 }
 
