@@ -113,7 +113,7 @@ struct tc_context {
   DBuf p2p_items, p2p_ranges, npairs;
   int nleaves = 0;
   // traversal / lists
-  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan, tstate;
+  DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan, tstate, tcls;
   int trav_steps[4] = {0, 0, 0, 0};  // steps of the last walk per MAC mode (batch size)
   DBuf gtg_a, gtg_b, gsrc_a, gsort_tmp;
   int *h_pin = nullptr;  // small pinned host buffer for per-step counts

@@ -185,6 +185,7 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * kTravBlocks));
   TC_CHECK(ctx->front_a.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 4 * n)));
   TC_CHECK(ctx->front_b.ensure(ctx->front_a.bytes));
+  TC_CHECK(ctx->tcls.ensure(ctx->front_a.bytes / sizeof(int2)));
   TC_CHECK(ctx->m2l.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 24 * n)));
   TC_CHECK(ctx->p2p.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 8 * n)));
   int &known = ctx->trav_steps[mac.mode & 3];
@@ -197,7 +198,8 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
       return (long long)std::min<size_t>(d.bytes / sizeof(int2), 0x7fffffff);
     };
     const long long cm = cap(ctx->m2l), cp = cap(ctx->p2p),
-                    cf = std::min(cap(ctx->front_a), cap(ctx->front_b));
+                    cf = std::min({cap(ctx->front_a), cap(ctx->front_b),
+                                   (long long)std::min<size_t>(ctx->tcls.bytes, 0x7fffffff)});
     std::memset(hst, 0, sizeof(TravState));
     hst->nin[0] = 1;
     ctx->h_pin[32] = ctx->h_pin[33] = 0;  // the root pair (0, 0)
@@ -213,6 +215,7 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
         a.fout = (odd ? ctx->front_a : ctx->front_b).as<int2>();
         a.m2l = ctx->m2l.as<int2>();
         a.p2p = ctx->p2p.as<int2>();
+        a.cls = ctx->tcls.as<unsigned char>();
         a.mac = mac;
         launch_trav_step(cells_of(ctx), a, ctx->tstate.as<TravState>(), (int)odd,
                          ctx->tblk.as<int>(), cm, cp, cf, s);
@@ -235,6 +238,7 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
     if ((long long)gf > cf) {
       TC_CHECK(ctx->front_a.ensure(sizeof(int2) * (gf + gf / 4)));
       TC_CHECK(ctx->front_b.ensure(ctx->front_a.bytes));
+      TC_CHECK(ctx->tcls.ensure(ctx->front_a.bytes / sizeof(int2)));
     }
   }
   known = hst->steps;
@@ -875,7 +879,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->cnbody, &ctx->cXg, &ctx->cRg, &ctx->cX, &ctx->cR, &ctx->cmin, &ctx->csink,
                  &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
                  &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
-                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate,
+                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate, &ctx->tcls,
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,

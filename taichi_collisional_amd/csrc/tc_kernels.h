@@ -101,6 +101,7 @@ struct TravArgs {
   const int2 *fin;  // this step's frontier
   int2 *fout;       // the next step's
   int2 *m2l, *p2p;  // the lists, appended to
+  unsigned char *cls;  // per frontier pair: kind << 4 | child pairs (pass 1 -> pass 2)
   MacArgs mac;
 };
 // Device-side state of a walk (ints): frontier sizes nin[step parity], list

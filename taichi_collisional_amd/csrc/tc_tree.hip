@@ -509,10 +509,12 @@ __device__ __forceinline__ void trav_range(int nin, int &lo, int &hi) {
   hi = nin - lo < R ? nin : lo + R;
 }
 
-// pass 1: per-block totals of M2L pairs, P2P pairs and child pairs
+// pass 1: per-block totals of M2L pairs, P2P pairs and child pairs; each
+// pair's class (kind << 4 | child pairs) is kept for pass 2
 __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__restrict__ fin,
                                                     const TravState *__restrict__ st, int par,
-                                                    MacArgs mac, int *__restrict__ blk) {
+                                                    MacArgs mac, int *__restrict__ blk,
+                                                    unsigned char *__restrict__ cls) {
   int lo, hi;
   trav_range(st->nin[par], lo, hi);
   int3 acc = make_int3(0, 0, 0);
@@ -524,6 +526,7 @@ __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__re
       int ns;
       const int kind = trav_classify(c, mac, pr.x, pr.y, &ns);
       v = make_int3(kind == 1, kind == 2, ns);
+      cls[t] = (unsigned char)(kind << 4 | ns);
     }
     int3 tot;
     block_excl_scan3_n<4>(v, &tot);
@@ -608,7 +611,9 @@ __global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
     int2 pr = make_int2(0, 0);
     if (t < hi) {
       pr = a.fin[t];
-      kind = trav_classify(c, a.mac, pr.x, pr.y, &ns);
+      const int k = a.cls[t];
+      kind = k >> 4;
+      ns = k & 15;
     }
     int3 tot;
     const int3 ex = block_excl_scan3_n<4>(make_int3(kind == 1, kind == 2, ns), &tot);
@@ -627,7 +632,7 @@ __global__ __launch_bounds__(256) void k_trav_write(CellsDev c, TravArgs a,
 int launch_trav_step(CellsDev c, const TravArgs &a, TravState *st, int par, int *blk,
                      long long cap_m2l, long long cap_p2p, long long cap_front, hipStream_t s) {
   hipLaunchKernelGGL(k_trav_count, dim3(kTravBlocks), dim3(256), 0, s, c, a.fin,
-                     (const TravState *)st, par, a.mac, blk);
+                     (const TravState *)st, par, a.mac, blk, a.cls);
   hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(1024), 0, s, blk, st, par, cap_m2l, cap_p2p,
                      cap_front);
   hipLaunchKernelGGL(k_trav_write, dim3(kTravBlocks), dim3(256), 0, s, c, a,
