@@ -85,15 +85,23 @@ __global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__
 
 // Orders K..P of L2P for one body: row K of Y(X_c - x_b) (compile-time K, no
 // case tests) and its time derivative, contracted with L / Ldot of orders K
-// and K+1 for the n <= 1 part of the local field; l = +m and l = -m use
-// Y_k^{-m} = (-1)^m conj(Y_k^m).  A scheduling barrier per order keeps the
-// later orders' coefficient reads from being hoisted (which multiplies the
-// live registers).
+// and K+1 for the n <= 1 part of the local field.  l = -m uses
+// Y_k^{-m} = (-1)^m conj(Y_k^m), and the same of L, Ldot and dY/dt, so the
+// terms of the outputs of order m = 0 that are read as real parts (Phi_0^0,
+// Phi_1^0 and its time derivative) come in equal pairs: only l >= 0 is
+// formed, the l > 0 sums (h00, h10, e10) counting twice at the end.
+// Phi_1^1 and its derivative keep both l = +m and l = -m.  A scheduling
+// barrier per order keeps the later orders' coefficient reads from being
+// hoisted (which multiplies the live registers).
+struct L2PAcc {
+  double p00, h00, p10, h10, d10, e10;  // Re parts: l = 0 terms, l > 0 terms
+  cplx p11, d11;
+};
+
 template <int P, int K>
 __device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, double r2, double vx,
                                            double vy, double vz, const cplx *cl, const cplx *cld,
-                                           cplx *y1, cplx *y2, cplx &p00, cplx &p10, cplx &p11,
-                                           cplx &d10, cplx &d11) {
+                                           cplx *y1, cplx *y2, L2PAcc &a) {
   __builtin_amdgcn_sched_barrier(0);
   cplx yk[P + 1];
   harm_row_c<K>(dx, dy, dz, r2, y1, y2, yk);
@@ -101,6 +109,7 @@ __device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, doub
   const cplx *L1 = cl + (K + 1) * (K + 1) + (K + 1);
   const cplx *D1 = cld + (K + 1) * (K + 1) + (K + 1);
   constexpr bool up = K < P;
+  auto redot = [](cplx u, cplx v) { return u.re * v.re + u.im * v.im; };  // Re(u conj v)
 #pragma unroll
   for (int m = 0; m <= K; m++) {
     cplx dy_m = {0.0, 0.0};
@@ -115,25 +124,29 @@ __device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, doub
         dy_m = cadd(dy_m, cmul({-c1.re, -c1.im}, xi));
       }
     }
-    {
-      const cplx g = cconj(yk[m]), gd = cconj(dy_m);
-      p00 = cadd(p00, cmul(L0[m], g));
+    const cplx y = yk[m];
+    if (m == 0) {
+      a.p00 += redot(L0[0], y);
       if (up) {
-        p10 = cadd(p10, cmul(L1[m], g));
-        p11 = cadd(p11, cmul(L1[m + 1], g));
-        d10 = cadd(d10, cadd(cmul(L1[m], gd), cmul(D1[m], g)));
-        d11 = cadd(d11, cadd(cmul(L1[m + 1], gd), cmul(D1[m + 1], g)));
+        a.p10 += redot(L1[0], y);
+        a.d10 += redot(L1[0], dy_m) + redot(D1[0], y);
+      }
+    } else {
+      a.h00 += redot(L0[m], y);
+      if (up) {
+        a.h10 += redot(L1[m], y);
+        a.e10 += redot(L1[m], dy_m) + redot(D1[m], y);
       }
     }
-    if (m > 0) {
-      const double sg = (m & 1) ? -1.0 : 1.0;
-      const cplx g = cscale(sg, yk[m]), gd = cscale(sg, dy_m);
-      p00 = cadd(p00, cmul(L0[-m], g));
-      if (up) {
-        p10 = cadd(p10, cmul(L1[-m], g));
-        p11 = cadd(p11, cmul(L1[1 - m], g));
-        d10 = cadd(d10, cadd(cmul(L1[-m], gd), cmul(D1[-m], g)));
-        d11 = cadd(d11, cadd(cmul(L1[1 - m], gd), cmul(D1[1 - m], g)));
+    if (up) {
+      const cplx g = cconj(y), gd = cconj(dy_m);
+      a.p11 = cadd(a.p11, cmul(L1[m + 1], g));
+      a.d11 = cadd(a.d11, cadd(cmul(L1[m + 1], gd), cmul(D1[m + 1], g)));
+      if (m > 0) {
+        const double sg = (m & 1) ? -1.0 : 1.0;
+        const cplx g2 = cscale(sg, y), gd2 = cscale(sg, dy_m);
+        a.p11 = cadd(a.p11, cmul(L1[1 - m], g2));
+        a.d11 = cadd(a.d11, cadd(cmul(L1[1 - m], gd2), cmul(D1[1 - m], g2)));
       }
     }
   }
@@ -143,7 +156,7 @@ __device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, doub
       y2[m] = y1[m];
       y1[m] = yk[m];
     }
-    l2p_orders<P, K + 1>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, p00, p10, p11, d10, d11);
+    l2p_orders<P, K + 1>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, a);
   }
 }
 
@@ -175,16 +188,15 @@ __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev 
     const double dx = X0 - b.x[i], dy = X1 - b.y[i], dz = X2 - b.z[i];
     const double r2 = dx * dx + dy * dy + dz * dz;
     cplx y1[P + 1], y2[P + 1];
-    cplx p00 = {0, 0}, p10 = {0, 0}, p11 = {0, 0};
-    cplx d10 = {0, 0}, d11 = {0, 0};
-    l2p_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, p00, p10, p11, d10, d11);
-    r.pot[i] += p00.re;
-    r.ax[i] -= p11.re;
-    r.ay[i] -= p11.im;
-    r.az[i] -= p10.re;
-    r.jx[i] -= d11.re;
-    r.jy[i] -= d11.im;
-    r.jz[i] -= d10.re;
+    L2PAcc a = {0, 0, 0, 0, 0, 0, {0, 0}, {0, 0}};
+    l2p_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, a);
+    r.pot[i] += a.p00 + 2.0 * a.h00;
+    r.ax[i] -= a.p11.re;
+    r.ay[i] -= a.p11.im;
+    r.az[i] -= a.p10 + 2.0 * a.h10;
+    r.jx[i] -= a.d11.re;
+    r.jy[i] -= a.d11.im;
+    r.jz[i] -= a.d10 + 2.0 * a.e10;
   }
 }
 
