@@ -232,62 +232,74 @@ __device__ __forceinline__ int octant_of(const BodiesDev &b, int i, double X0, d
 }
 
 // offs: 8 ints per cell of the level (octant starts), nch: children per
-// cell, then (at nch + nl) 1 for a leaf
+// cell, then (at nch + nl) 1 for a leaf.  Eight lanes per cell, lane j for
+// octant j: above 32 bodies lane j > 0 runs the binary search for the start
+// of octants >= j (the search never tests the cell's first body, as the
+// reference's does not, so its start is l + 1); the eight searches run side
+// by side instead of one after another (the top levels' cells hold up to all
+// N bodies, and their splits are chains of dependent loads).  Up to 32
+// bodies lane j counts octant j's bodies.
 __global__ __launch_bounds__(256) void k_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit,
                                                int *__restrict__ offs, int *__restrict__ nch) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  int ci = ls + k;
-  if (ci >= le) return;
-  int nb = c.nbody[ci];
-  nch[(le - ls) + k] = nb <= ncrit;
+  const int k = (blockIdx.x * 256 + (int)threadIdx.x) >> 3, j = threadIdx.x & 7;
+  const int ci = ls + k;
+  if (ci >= le) return;  // whole groups of eight
+  const int nb = c.nbody[ci];
   if (nb <= ncrit) {
-    nch[k] = 0;
-    c.nchild[ci] = 0;
-    c.child[ci] = -1;
+    if (j == 0) {
+      nch[(le - ls) + k] = 1;
+      nch[k] = 0;
+      c.nchild[ci] = 0;
+      c.child[ci] = -1;
+    }
     return;
   }
-  int begin = c.body[ci], end = begin + nb;
-  double X0 = c.Xg[3 * ci], X1 = c.Xg[3 * ci + 1], X2 = c.Xg[3 * ci + 2];
-  int o[8], size[8];
-  if (nb <= 32) {
-    for (int i = 0; i < 8; i++) size[i] = 0;
-    for (int i = begin; i < end; i++) size[octant_of(b, i, X0, X1, X2)]++;
-    int acc = begin;
-    for (int i = 0; i < 8; i++) {
-      o[i] = acc;
-      acc += size[i];
+  const int begin = c.body[ci], end = begin + nb;
+  const double X0 = c.Xg[3 * ci], X1 = c.Xg[3 * ci + 1], X2 = c.Xg[3 * ci + 2];
+  const bool small = nb <= 32;
+  int o = begin, sz = 0;
+  if (small) {
+    for (int i = begin; i < end; i++) sz += octant_of(b, i, X0, X1, X2) == j;
+  } else if (j > 0) {
+    int l = begin, r = end;
+    while (l < r - 1) {
+      const int mid = (l + r) / 2;
+      if (octant_of(b, mid, X0, X1, X2) >= j)
+        r = mid;
+      else
+        l = mid;
     }
-  } else {
-    o[0] = begin;
-    for (int i = 1; i < 8; i++) {
-      int l = begin, r = end;
-      while (l < r - 1) {
-        int mid = (l + r) / 2;
-        if (octant_of(b, mid, X0, X1, X2) >= i)
-          r = mid;
-        else
-          l = mid;
-      }
-      o[i] = l + 1;
-    }
-    for (int i = 0; i < 7; i++) size[i] = o[i + 1] - o[i];
-    size[7] = end - o[7];
+    o = l + 1;
   }
-  int cnt = 0;
-  for (int i = 0; i < 8; i++) {
-    offs[8 * k + i] = o[i];
-    if (size[i]) cnt++;
+  // within the group of eight: starts from counts, or counts from starts
+  const int lane = threadIdx.x & (kWave - 1);
+  int incl = sz;
+#pragma unroll
+  for (int d = 1; d < 8; d <<= 1) {
+    const int t = __shfl(incl, lane - d, kWave);
+    if (j >= d) incl += t;
   }
-  nch[k] = cnt;
-  c.nchild[ci] = cnt;
+  const int onext = __shfl(o, lane + 1, kWave);
+  if (small)
+    o = begin + incl - sz;
+  else
+    sz = (j < 7 ? onext : end) - o;
+  offs[8 * k + j] = o;
+  const unsigned long long nz = __ballot(sz != 0);
+  const int cnt = __popcll((nz >> (lane & ~7)) & 0xffull);
+  if (j == 0) {
+    nch[(le - ls) + k] = 0;
+    nch[k] = cnt;
+    c.nchild[ci] = cnt;
+  }
 }
 
 int launch_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit, int *offs, int *nch,
                  hipStream_t s) {
   int n = le - ls;
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_split, dim3((n + 255) / 256), dim3(256), 0, s, c, b, ls, le, ncrit, offs,
-                     nch);
+  hipLaunchKernelGGL(k_split, dim3((8 * n + 255) / 256), dim3(256), 0, s, c, b, ls, le, ncrit,
+                     offs, nch);
   return 0;
 }
 
