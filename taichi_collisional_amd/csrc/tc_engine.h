@@ -107,7 +107,9 @@ struct tc_context {
   DBuf split_offs, split_nch, split_scan, scan_tmp;
   std::vector<int> level_start;
   DBuf leaf_ids;
-  HBuf<int> h_leaves, h_nbd, h_pcnt;
+  HBuf<int> h_leaves, h_nbd, h_pcnt, h_tree;
+  DBuf tree_state;      // TreeState of the level-by-level build
+  int tree_levels = 0;  // levels of the last tree (batch size of the next build)
   HBuf<int4> h_items;
   size_t n_items = 0;
   DBuf p2p_items, p2p_ranges, npairs;

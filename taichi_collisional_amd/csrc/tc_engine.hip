@@ -77,47 +77,62 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
   return 0;
 }
 
+// Tree levels are queued in batches without host reads (TreeState on the
+// device holds the level range and counts); after a batch one small copy says
+// whether the last level had no children.  A level whose children do not fit
+// the cell arrays ends the build with a flag and what it needed: the arrays
+// grow and the build starts again from the root (a new workload's first
+// evaluation, or a tree that a new root shift makes larger).
 static int build_tree(tc_context *ctx, int n) {
   hipStream_t s = ctx->stream;
   BodiesDev b = bodies_of(ctx);
-  int ncrit = ctx->cfg.ncrit;
-  TC_CHECK(ctx->tcnt.ensure(sizeof(int) * 8));  // [7]: the children of one level
-  ctx->level_start.assign(1, 0);
-  int ls = 0, le = 1;
+  const int ncrit = ctx->cfg.ncrit;
   TC_CHECK(hipStreamSynchronize(s));
   if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64), 0)) return e;
-  launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s);
-  int depth = 0, nleaf = 0;
-  while (le > ls) {
-    ctx->level_start.push_back(le);
-    int nl = le - ls;
-    // per-level split results (8 body offsets per cell, then child counts and
-    // leaf flags, and their scans); they are not kept across levels, and the
-    // cell arrays below grow without touching them
-    TC_CHECK(ctx->split_offs.ensure(sizeof(int) * 8 * (size_t)nl));
-    TC_CHECK(ctx->split_nch.ensure(sizeof(int) * 2 * (size_t)nl));
-    TC_CHECK(ctx->split_scan.ensure(sizeof(int) * 2 * (size_t)nl));
-    CellsDev c = cells_of(ctx);
-    launch_split(c, b, ls, le, ncrit, ctx->split_offs.as<int>(), ctx->split_nch.as<int>(), s);
-    launch_scan_small(ctx->split_nch.as<int>(), ctx->split_scan.as<int>(), nl, nl, 2,
-                      ctx->tcnt.as<int>() + 6, s);
-    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 8, ctx->tcnt.as<int>() + 6, 2 * sizeof(int),
-                            hipMemcpyDeviceToHost, s));  // pinned: a true asynchronous copy
-    TC_CHECK(hipStreamSynchronize(s));
-    const int total = ctx->h_pin[8], nlev_leaves = ctx->h_pin[9];
-    // grow to the exact need: a worst-case reservation (8 children per cell)
-    // would move the arrays whenever a new root shift deepens the tree
-    if (total > 0)
-      if (int e = ensure_cells(ctx, le + total, le)) return e;
-    // the children, and this level's leaves appended to leaf_ids
-    launch_children(cells_of(ctx), ls, le, ctx->split_offs.as<int>(), ctx->split_scan.as<int>(),
-                    ctx->tp.as<TreeParams>(), ctx->leaf_ids.as<int>(), nleaf, s);
-    nleaf += nlev_leaves;
-    if (total == 0) break;
-    ls = le;
-    le = le + total;
-    depth++;
+  TC_CHECK(ctx->tree_state.ensure(sizeof(TreeState)));
+  TC_CHECK(ctx->h_tree.ensure(sizeof(TreeState) / sizeof(int)));
+  TreeState *hst = reinterpret_cast<TreeState *>(ctx->h_tree.p);
+  for (int attempt = 0;; attempt++) {
+    if (attempt > 16) {
+      ctx->err = "tree build: cell arrays keep overflowing";
+      return -3;
+    }
+    const int cap = ctx->cap_c;
+    TC_CHECK(ctx->split_offs.ensure(sizeof(int) * 8 * (size_t)cap));
+    TC_CHECK(ctx->split_nch.ensure(sizeof(int) * 2 * (size_t)cap));
+    TC_CHECK(ctx->split_scan.ensure(sizeof(int) * 2 * (size_t)cap));
+    std::memset(hst, 0, sizeof(TreeState));
+    hst->ls = 0;
+    hst->le = 1;
+    TC_CHECK(hipMemcpyAsync(ctx->tree_state.p, hst, sizeof(TreeState), hipMemcpyHostToDevice, s));
+    launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s);
+    int queued = 0, batch = std::max(4, ctx->tree_levels + 1);
+    for (;;) {
+      for (int i = 0; i < batch; i++, queued++)
+        launch_tree_level(cells_of(ctx), b, ctx->tree_state.as<TreeState>(), ncrit, cap,
+                          ctx->split_offs.as<int>(), ctx->split_nch.as<int>(),
+                          ctx->split_scan.as<int>(), ctx->tp.as<TreeParams>(),
+                          ctx->leaf_ids.as<int>(), s);
+      TC_CHECK(hipMemcpyAsync(hst, ctx->tree_state.p, sizeof(TreeState), hipMemcpyDeviceToHost,
+                              s));
+      TC_CHECK(hipStreamSynchronize(s));
+      if (hst->done || hst->overflow) break;
+      if (queued > 4 * kMaxTreeLevels) {
+        ctx->err = "tree build does not end";
+        return -3;
+      }
+      batch = 4;
+    }
+    if (hst->overflow == 2) {
+      ctx->err = "tree deeper than 64 levels (more than ncrit coincident bodies?)";
+      return -1;
+    }
+    if (!hst->overflow) break;
+    if (int e = ensure_cells(ctx, hst->need, 0)) return e;
   }
+  ctx->tree_levels = hst->nlev;
+  ctx->level_start.assign(hst->level_start, hst->level_start + hst->nlev + 1);
+  const int depth = hst->nlev - 1, nleaf = hst->nleaf;
   // level_start holds the start of every level and, last, the total cell count
   int nc = ctx->level_start.back();
   ctx->st.ncells = nc;
@@ -879,7 +894,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->cnbody, &ctx->cXg, &ctx->cRg, &ctx->cX, &ctx->cR, &ctx->cmin, &ctx->csink,
                  &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
                  &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
-                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate, &ctx->tcls,
+                 &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate, &ctx->tcls, &ctx->tree_state,
                  &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
@@ -891,6 +906,7 @@ int tc_destroy(tc_context *ctx) {
   for (DBuf *b : all) b->release();
   ctx->h_leaves.release();
   ctx->h_nbd.release();
+  ctx->h_tree.release();
   ctx->h_pcnt.release();
   ctx->h_items.release();
   ctx->h_stage.release();

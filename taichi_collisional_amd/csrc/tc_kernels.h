@@ -64,19 +64,23 @@ int launch_gather(const double *x3, const double *v3, const double *q,
                   const unsigned char *flags_in, const int *perm, int n, BodiesDev b,
                   hipStream_t s);
 int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s);
-// split: per level cell k, octant offsets offs[8k..8k+7], child count nch[k]
-// and leaf flag nch[(le - ls) + k]
-int launch_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit, int *offs, int *nch,
-                 hipStream_t s);
-// children of the level's cells (scan[k]: exclusive scan of the child counts)
-// and the level's leaves to leaf_ids[leaf_base + scan[(le - ls) + k]]
-int launch_children(CellsDev c, int ls, int le, const int *offs, const int *scan,
-                    const TreeParams *d_tp, int *leaf_ids, int leaf_base, hipStream_t s);
+// Device-side state of a tree build (ints): the level being split [ls, le),
+// leaves and levels so far, the level being placed (cls, cle, its first leaf
+// slot cbase, go), done, overflow (1: the cell arrays are too small, need
+// cells; 2: deeper than kMaxTreeLevels) and level_start[0..nlev].
+constexpr int kMaxTreeLevels = 64;
+struct TreeState {
+  int ls, le, nleaf, nlev;
+  int cls, cle, cbase, go;
+  int done, overflow, need, pad;
+  int level_start[kMaxTreeLevels + 4];
+};
+// One level: split (offs, nch), scans (scan), children and leaves.  Arrays of
+// per-level cells hold cap entries (nch and scan: 2 x cap).
+int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap, int *offs,
+                      int *nch, int *scan, const TreeParams *d_tp, int *leaf_ids, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
-// narrays exclusive scans of n ints (arrays stride apart), totals to total[k]
-int launch_scan_small(const int *in, int *out, int n, int stride, int narrays, int *total,
-                      hipStream_t s);
 
 // ---- traversal (tc_tree.hip)
 // Multipole acceptance: kMacTheta  |dX|^2 theta^2 > (Ri+Rj)^2 on the balls

@@ -231,123 +231,206 @@ __device__ __forceinline__ int octant_of(const BodiesDev &b, int i, double X0, d
   return (b.x[i] >= X0) + ((b.y[i] >= X1) << 1) + ((b.z[i] >= X2) << 2);
 }
 
-// offs: 8 ints per cell of the level (octant starts), nch: children per
-// cell, then (at nch + nl) 1 for a leaf.  Eight lanes per cell, lane j for
-// octant j: above 32 bodies lane j > 0 runs the binary search for the start
-// of octants >= j (the search never tests the cell's first body, as the
-// reference's does not, so its start is l + 1); the eight searches run side
-// by side instead of one after another (the top levels' cells hold up to all
-// N bodies, and their splits are chains of dependent loads).  Up to 32
-// bodies lane j counts octant j's bodies.
-__global__ __launch_bounds__(256) void k_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit,
-                                               int *__restrict__ offs, int *__restrict__ nch) {
-  const int k = (blockIdx.x * 256 + (int)threadIdx.x) >> 3, j = threadIdx.x & 7;
-  const int ci = ls + k;
-  if (ci >= le) return;  // whole groups of eight
-  const int nb = c.nbody[ci];
-  if (nb <= ncrit) {
-    if (j == 0) {
-      nch[(le - ls) + k] = 1;
-      nch[k] = 0;
-      c.nchild[ci] = 0;
-      c.child[ci] = -1;
+// One level of the tree build, queued without host reads: the level's range
+// and the running counts live in TreeState.  Arrays of the level's cells
+// (offs: 8 octant starts per cell; nch: child counts, and at nch + cap the
+// leaf flags; scan: their exclusive scans, the same layout) hold up to cap
+// cells.
+//
+// Split: eight lanes per cell, lane j for octant j; above 32 bodies lane j > 0
+// runs the binary search for the start of octants >= j (the search never
+// tests the cell's first body, as the reference's does not, so its start is
+// l + 1); the seven searches run side by side instead of one after another
+// (the top levels' cells hold up to all N bodies, and their splits are chains
+// of dependent loads).  Up to 32 bodies lane j counts octant j's bodies.
+__global__ __launch_bounds__(256) void k_split(CellsDev c, BodiesDev b,
+                                               const TreeState *__restrict__ st, int ncrit,
+                                               int cap, int *__restrict__ offs,
+                                               int *__restrict__ nch) {
+  if (st->done || st->overflow) return;
+  const int ls = st->ls, nl = st->le - ls;
+  const int j = threadIdx.x & 7, lane = threadIdx.x & (kWave - 1);
+  for (int k = (blockIdx.x * 256 + (int)threadIdx.x) >> 3; k < nl; k += gridDim.x * 32) {
+    const int ci = ls + k;
+    const int nb = c.nbody[ci];
+    if (nb <= ncrit) {
+      if (j == 0) {
+        nch[cap + k] = 1;
+        nch[k] = 0;
+        c.nchild[ci] = 0;
+        c.child[ci] = -1;
+      }
+      continue;
     }
+    const int begin = c.body[ci], end = begin + nb;
+    const double X0 = c.Xg[3 * ci], X1 = c.Xg[3 * ci + 1], X2 = c.Xg[3 * ci + 2];
+    const bool small = nb <= 32;
+    int o = begin, sz = 0;
+    if (small) {
+      for (int i = begin; i < end; i++) sz += octant_of(b, i, X0, X1, X2) == j;
+    } else if (j > 0) {
+      int l = begin, r = end;
+      while (l < r - 1) {
+        const int mid = (l + r) / 2;
+        if (octant_of(b, mid, X0, X1, X2) >= j)
+          r = mid;
+        else
+          l = mid;
+      }
+      o = l + 1;
+    }
+    // within the group of eight: starts from counts, or counts from starts
+    int incl = sz;
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+      const int t = __shfl(incl, lane - d, kWave);
+      if (j >= d) incl += t;
+    }
+    const int onext = __shfl(o, lane + 1, kWave);
+    if (small)
+      o = begin + incl - sz;
+    else
+      sz = (j < 7 ? onext : end) - o;
+    offs[8 * k + j] = o;
+    const unsigned long long nz = __ballot(sz != 0);
+    const int cnt = __popcll((nz >> (lane & ~7)) & 0xffull);
+    if (j == 0) {
+      nch[cap + k] = 0;
+      nch[k] = cnt;
+      c.nchild[ci] = cnt;
+    }
+  }
+}
+
+// Exclusive scans of the level's child counts and leaf flags (one block), and
+// the state: the level the children kernel places (cls, cle, cbase, go), the
+// next level, the leaves and levels so far.  A level whose children would not
+// fit the cell arrays sets overflow = 1 and what it needed; a level past
+// kMaxTreeLevels sets overflow = 2.
+__global__ __launch_bounds__(1024) void k_level_scan(const int *__restrict__ nch,
+                                                     int *__restrict__ scan, TreeState *st,
+                                                     int cap) {
+  __shared__ int wsum[2][16];
+  __shared__ int carry[2];
+  if (st->done || st->overflow) {
+    if (threadIdx.x == 0) st->go = 0;
     return;
   }
-  const int begin = c.body[ci], end = begin + nb;
-  const double X0 = c.Xg[3 * ci], X1 = c.Xg[3 * ci + 1], X2 = c.Xg[3 * ci + 2];
-  const bool small = nb <= 32;
-  int o = begin, sz = 0;
-  if (small) {
-    for (int i = begin; i < end; i++) sz += octant_of(b, i, X0, X1, X2) == j;
-  } else if (j > 0) {
-    int l = begin, r = end;
-    while (l < r - 1) {
-      const int mid = (l + r) / 2;
-      if (octant_of(b, mid, X0, X1, X2) >= j)
-        r = mid;
-      else
-        l = mid;
-    }
-    o = l + 1;
-  }
-  // within the group of eight: starts from counts, or counts from starts
-  const int lane = threadIdx.x & (kWave - 1);
-  int incl = sz;
+  const int ls = st->ls, le = st->le, nl = le - ls;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  if (t < 2) carry[t] = 0;
+  __syncthreads();
+  constexpr int PER = 8, TILE = 1024 * PER;
+  for (int base = 0; base < nl; base += TILE) {
+    for (int a = 0; a < 2; a++) {
+      const int *in = nch + (size_t)a * cap;
+      int *out = scan + (size_t)a * cap;
+      int v[PER], sum = 0;
 #pragma unroll
-  for (int d = 1; d < 8; d <<= 1) {
-    const int t = __shfl(incl, lane - d, kWave);
-    if (j >= d) incl += t;
+      for (int q = 0; q < PER; q++) {
+        const int i = base + t * PER + q;
+        v[q] = i < nl ? in[i] : 0;
+        sum += v[q];
+      }
+      int wt;
+      const int ex = wave_excl_scan(sum, &wt);
+      if (lane == 0) wsum[a][w] = wt;
+      __syncthreads();
+      int before = carry[a], all = 0;
+      for (int k = 0; k < 16; k++) {
+        if (k < w) before += wsum[a][k];
+        all += wsum[a][k];
+      }
+      int run = before + ex;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int i = base + t * PER + q;
+        if (i < nl) out[i] = run;
+        run += v[q];
+      }
+      __syncthreads();
+      if (t == 0) carry[a] += all;
+      __syncthreads();
+    }
   }
-  const int onext = __shfl(o, lane + 1, kWave);
-  if (small)
-    o = begin + incl - sz;
-  else
-    sz = (j < 7 ? onext : end) - o;
-  offs[8 * k + j] = o;
-  const unsigned long long nz = __ballot(sz != 0);
-  const int cnt = __popcll((nz >> (lane & ~7)) & 0xffull);
-  if (j == 0) {
-    nch[(le - ls) + k] = 0;
-    nch[k] = cnt;
-    c.nchild[ci] = cnt;
+  if (t == 0) {
+    const int total = carry[0], leaves = carry[1];
+    st->cls = ls;
+    st->cle = le;
+    st->cbase = st->nleaf;
+    if ((long long)le + total > cap) {
+      st->overflow = 1;
+      st->need = (int)((long long)le + total < 0x7fffffff ? le + total : 0x7fffffff);
+      st->go = 0;
+    } else if (st->nlev >= kMaxTreeLevels) {
+      st->overflow = 2;
+      st->go = 0;
+    } else {
+      st->go = 1;
+      st->nleaf += leaves;
+      st->nlev += 1;
+      st->level_start[st->nlev] = le;
+      if (total == 0) {
+        st->done = 1;
+      } else {
+        st->ls = le;
+        st->le = le + total;
+      }
+    }
   }
 }
 
-int launch_split(CellsDev c, BodiesDev b, int ls, int le, int ncrit, int *offs, int *nch,
-                 hipStream_t s) {
-  int n = le - ls;
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_split, dim3((8 * n + 255) / 256), dim3(256), 0, s, c, b, ls, le, ncrit,
-                     offs, nch);
-  return 0;
-}
-
-__global__ __launch_bounds__(256) void k_children(CellsDev c, int ls, int le,
-                                                  const int *__restrict__ offs,
+// The children of the level the scan kernel set up (cls, cle), and its leaves
+// appended to leaf_ids from cbase, in cell order.
+__global__ __launch_bounds__(256) void k_children(CellsDev c, const TreeState *__restrict__ st,
+                                                  int cap, const int *__restrict__ offs,
                                                   const int *__restrict__ scan,
                                                   const TreeParams *__restrict__ tp,
-                                                  int *__restrict__ leaf_ids, int leaf_base) {
+                                                  int *__restrict__ leaf_ids) {
 #pragma clang fp contract(off)
-  int k = blockIdx.x * 256 + threadIdx.x;
-  int ci = ls + k;
-  if (ci >= le) return;
-  int nc = c.nchild[ci];
-  if (nc == 0) {
-    // leaves in cell order, level by level
-    leaf_ids[leaf_base + scan[(le - ls) + k]] = ci;
-    return;
-  }
-  int first = le + scan[k];
-  c.child[ci] = first;
-  int level = c.level[ci];
-  int begin = c.body[ci], end = begin + c.nbody[ci];
-  double R0 = tp->R0;
-  double r = ldexp(R0, -(level + 1));
-  int j = 0;
-  for (int i = 0; i < 8; i++) {
-    int o = offs[8 * k + i];
-    int e = (i < 7) ? offs[8 * k + i + 1] : end;
-    int size = e - o;
-    if (!size) continue;
-    int cc = first + j;
-    for (int d = 0; d < 3; d++) c.Xg[3 * cc + d] = c.Xg[3 * ci + d] + r * (((i >> d) & 1) * 2 - 1);
-    c.Rg[cc] = r;
-    c.level[cc] = level + 1;
-    c.body[cc] = o;
-    c.nbody[cc] = size;
-    c.nchild[cc] = 0;
-    c.child[cc] = -1;
-    j++;
+  if (!st->go) return;
+  const int ls = st->cls, le = st->cle, leaf_base = st->cbase, nl = le - ls;
+  for (int k = blockIdx.x * 256 + (int)threadIdx.x; k < nl; k += gridDim.x * 256) {
+    const int ci = ls + k;
+    const int nc = c.nchild[ci];
+    if (nc == 0) {
+      leaf_ids[leaf_base + scan[cap + k]] = ci;
+      continue;
+    }
+    const int first = le + scan[k];
+    c.child[ci] = first;
+    const int level = c.level[ci];
+    const int end = c.body[ci] + c.nbody[ci];
+    const double r = ldexp(tp->R0, -(level + 1));
+    int jc = 0;
+    for (int i = 0; i < 8; i++) {
+      const int o = offs[8 * k + i];
+      const int e = (i < 7) ? offs[8 * k + i + 1] : end;
+      const int size = e - o;
+      if (!size) continue;
+      const int cc = first + jc;
+      for (int d = 0; d < 3; d++)
+        c.Xg[3 * cc + d] = c.Xg[3 * ci + d] + r * (((i >> d) & 1) * 2 - 1);
+      c.Rg[cc] = r;
+      c.level[cc] = level + 1;
+      c.body[cc] = o;
+      c.nbody[cc] = size;
+      c.nchild[cc] = 0;
+      c.child[cc] = -1;
+      jc++;
+    }
   }
 }
 
-int launch_children(CellsDev c, int ls, int le, const int *offs, const int *scan,
-                    const TreeParams *d_tp, int *leaf_ids, int leaf_base, hipStream_t s) {
-  int n = le - ls;
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_children, dim3((n + 255) / 256), dim3(256), 0, s, c, ls, le, offs, scan,
-                     d_tp, leaf_ids, leaf_base);
+int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap, int *offs,
+                      int *nch, int *scan, const TreeParams *d_tp, int *leaf_ids, hipStream_t s) {
+  // grids cover the widest levels in a few sweeps; narrow levels leave most
+  // blocks idle for a few microseconds
+  hipLaunchKernelGGL(k_split, dim3(512), dim3(256), 0, s, c, b, (const TreeState *)st, ncrit, cap,
+                     offs, nch);
+  hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1024), 0, s, (const int *)nch, scan, st, cap);
+  hipLaunchKernelGGL(k_children, dim3(256), dim3(256), 0, s, c, (const TreeState *)st, cap,
+                     (const int *)offs, (const int *)scan, d_tp, leaf_ids);
   return 0;
 }
 
@@ -362,51 +445,12 @@ int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, h
   return hipcub::DeviceScan::ExclusiveSum(temp, tb, in, out, n, s) == hipSuccess ? 0 : -1;
 }
 
-// Exclusive prefix sums of small int arrays (the cells of one level, the
-// blocks of one traversal step), one 1024-thread block per array (blockIdx.x
-// selects array k at in + k * stride, out + k * stride): each thread sums a
-// contiguous chunk, the chunk sums are scanned across the block, then every
-// thread writes its chunk.  The array's total goes to total[k].  One launch,
-// no temporary storage and none of a library call's host-side dispatch.
-__global__ __launch_bounds__(1024) void k_scan_small(const int *__restrict__ in,
-                                                     int *__restrict__ out, int n, int stride,
-                                                     int *__restrict__ total) {
-  __shared__ int wsum[16];
-  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  const int *a = in + (size_t)blockIdx.x * stride;
-  int *o = out + (size_t)blockIdx.x * stride;
-  const int per = (n + 1023) / 1024;
-  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
-  int sum = 0;
-  for (int i = b0; i < b1; i++) sum += a[i];
-  int wt;
-  const int ex = wave_excl_scan(sum, &wt);
-  if (lane == 0) wsum[w] = wt;
-  __syncthreads();
-  int base = 0;
-  for (int k = 0; k < w; k++) base += wsum[k];
-  int run = base + ex;
-  for (int i = b0; i < b1; i++) {
-    const int v = a[i];
-    o[i] = run;
-    run += v;
-  }
-  if (t == 1023) total[blockIdx.x] = run;
-}
-
-int launch_scan_small(const int *in, int *out, int n, int stride, int narrays, int *total,
-                      hipStream_t s) {
-  if (n <= 0 || narrays <= 0) return 0;
-  hipLaunchKernelGGL(k_scan_small, dim3(narrays), dim3(1024), 0, s, in, out, n, stride, total);
-  return 0;
-}
-
 // ------------------------------------------------------------ traversal -----
 // One breadth-first step, in two passes so that no shared counter is hit by
-// every wave: pass 1 classifies each frontier pair and writes per-block
-// totals (M2L, P2P, child pairs); the totals are scanned over blocks; pass 2
-// classifies again and writes every output at its scanned position, so the
-// lists come out in frontier order (deterministic).
+// every wave: pass 1 classifies each frontier pair, keeps the class and writes
+// per-block totals (M2L, P2P, child pairs); the totals are scanned over
+// blocks; pass 2 writes every output at its scanned position, so the lists
+// come out in frontier order (deterministic).
 __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a, int i, int j,
                                              int *nsplit) {
 #pragma clang fp contract(off)
