@@ -385,3 +385,34 @@ def test_gpu_per_kernel_accumulates_and_orders(torch_cuda, oracle_lib):
     bad["BODY"][leaves[0]] += np.uint64(8)
     with pytest.raises(RuntimeError):
         f.P2M(bodies, bad, leaves[:1])
+
+
+def test_gpu_context_reuse_across_sizes(torch_cuda):
+    """One context through small, large, clustered and small inputs: the tree
+    build and the walk grow their arrays when a level or a step does not fit
+    (and restart), so every call must equal a fresh context's on the same
+    input (same shift; sums in another order, so to round-off)."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    cases = [ics.plummer(3000, seed=41), ics.plummer(120000, seed=42),
+             ics.plummer(20000, seed=43), ics.plummer(3000, seed=41)]
+    # a clustered case: most bodies in a tight core deepen the tree
+    x, v, m = ics.plummer(60000, seed=44)
+    x[:40000] *= 1e-3
+    cases.insert(2, (x, v, m))
+    shift = (0.1, -0.2, 0.05)
+    f = tca.Fmm(p=6, ncrit=32, theta=0.5, direct_below=0)
+    f.set_shift(shift)
+    for k, (x, v, m) in enumerate(cases):
+        dx, dv, dm = _dev(torch_cuda, x, v, m)
+        pot, acc, jerk = f.eval(dx, dv, dm)
+        st = f.stats()
+        g = tca.Fmm(p=6, ncrit=32, theta=0.5, direct_below=0)
+        g.set_shift(shift)
+        p2, a2, j2 = g.eval(dx, dv, dm)
+        st2 = g.stats()
+        torch_cuda.cuda.synchronize()
+        for key in ("ncells", "nleaves", "n_m2l", "n_p2p_cells", "depth"):
+            assert st[key] == st2[key], (k, key, st[key], st2[key])
+        _check((pot.cpu().numpy(), acc.cpu().numpy(), jerk.cpu().numpy()),
+               (p2.cpu().numpy(), a2.cpu().numpy(), j2.cpu().numpy()), "reuse case %d" % k)
