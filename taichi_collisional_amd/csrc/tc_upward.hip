@@ -197,18 +197,27 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
     for (int d = 0; d < 3; d++) X[d] = c.X[3 * ci + d];
   } else {
   if (tid < 192) s_td[tid] = kTDesign[tid];
+  // the children's records, one thread each (their loads overlap), then the
+  // prefix over at most eight children from LDS
+  __shared__ int s_nc2[8], s_nb[8], s_g0[8];
+  if (tid < nch) {
+    const int ch = ch0 + tid;
+    s_nc2[tid] = c.nchild[ch];
+    s_nb[tid] = c.nbody[ch];
+    s_body[tid] = c.body[ch];
+    s_g0[tid] = c.child[ch];
+  }
+  __syncthreads();
   if (tid == 0) {
     int pre = 0, nbs = 0, ng = 0;
     for (int k = 0; k < nch; k++) {
-      const int ch = ch0 + k;
-      const int nc2 = c.nchild[ch];
+      const int nc2 = s_nc2[k];
       s_pre[k] = pre;
       s_leaf[k] = nc2 == 0;
-      s_body[k] = c.body[ch];
       if (nc2 == 0) {
         s_base[k] = nbs;
-        nbs += c.nbody[ch];
-        pre += c.nbody[ch];
+        nbs += s_nb[k];
+        pre += s_nb[k];
       } else {
         s_base[k] = ng;
         ng += nc2;
@@ -220,28 +229,28 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
   }
   __syncthreads();
   const bool staged = s_nstage <= kStage;
-  for (int k = 0; k < nch; k++) {
-    const int ch = ch0 + k;
-    if (s_leaf[k]) {
-      if (staged) {
-        const int nbk = s_pre[k + 1] - s_pre[k], o = s_base[k], bb = s_body[k];
-        for (int i = tid; i < nbk; i += blockDim.x) {
-          s_p[0][o + i] = b.x[bb + i];
-          s_p[1][o + i] = b.y[bb + i];
-          s_p[2][o + i] = b.z[bb + i];
-        }
-      }
-    } else {
-      const int g0 = c.child[ch], ngk = (s_pre[k + 1] - s_pre[k]) >> 6;
-      if (tid < ngk) {
-        const int g = g0 + tid;
-        s_g[s_base[k] + tid][0] = c.X[3 * g];
-        s_g[s_base[k] + tid][1] = c.X[3 * g + 1];
-        s_g[s_base[k] + tid][2] = c.X[3 * g + 2];
-        s_g[s_base[k] + tid][3] = c.R[g];
-      }
+  // grandchildren's balls of the internal children: thread 8 k + j takes
+  // grandchild j of child k
+  if (tid < 64) {
+    const int k = tid >> 3, j = tid & 7;
+    if (k < nch && !s_leaf[k] && j < s_nc2[k]) {
+      const int g = s_g0[k] + j;
+      s_g[s_base[k] + j][0] = c.X[3 * g];
+      s_g[s_base[k] + j][1] = c.X[3 * g + 1];
+      s_g[s_base[k] + j][2] = c.X[3 * g + 2];
+      s_g[s_base[k] + j][3] = c.R[g];
     }
   }
+  // bodies of the leaf children, all of them strided over the block
+  if (staged)
+    for (int i = tid; i < s_nstage; i += blockDim.x) {
+      int k = 0;  // the leaf child whose staging range holds slot i
+      while (k < nch - 1 && !(s_leaf[k] && i >= s_base[k] && i < s_base[k] + s_nb[k])) k++;
+      const int bi = s_body[k] + (i - s_base[k]);
+      s_p[0][i] = b.x[bi];
+      s_p[1][i] = b.y[bi];
+      s_p[2][i] = b.z[bi];
+    }
   __syncthreads();
   // point `local` of child k, in the order the reference enumerates them
   auto point = [&](int k, int local, double *q) {
