@@ -117,7 +117,7 @@ struct tc_context {
   // traversal / lists
   DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan, tstate, tcls;
   int trav_steps[4] = {0, 0, 0, 0};  // steps of the last walk per MAC mode (batch size)
-  DBuf gtg_a, gtg_b, gsrc_a, gsort_tmp;
+  DBuf gsort1[4], gsort2[4];  // list grouping scratch per stream: keys in, out, sources, temp
   int *h_pin = nullptr;  // small pinned host buffer for per-step counts
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
@@ -135,8 +135,10 @@ struct tc_context {
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
   // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],
-  // [13] end of the upward geometry (stream) and moments (near_stream)
-  hipEvent_t ev[14];
+  // [13] end of the upward geometry (stream) and moments (near_stream); [14]
+  // the walk's lists written (stream), [15] the P2P list grouped and its
+  // lengths in h_pcnt (near_stream)
+  hipEvent_t ev[16];
   // second stream (default priority): P2P runs on it while the downward pass
   // runs on `stream`; its far-field results go to a second result set
   hipStream_t near_stream = nullptr;

@@ -263,23 +263,26 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
 }
 
 // group (target, source) pairs by target: stable radix sort + run bounds
+// (second: on near_stream with its own sort scratch, so that two lists are
+// grouped at once)
 static int build_csr(tc_context *ctx, const int2 *pairs, int npairs, DBuf &cnt, DBuf &off,
-                     DBuf &src) {
-  hipStream_t s = ctx->stream;
+                     DBuf &src, bool second = false) {
+  hipStream_t s = second ? ctx->near_stream : ctx->stream;
+  DBuf *sc = second ? ctx->gsort2 : ctx->gsort1;  // target keys in/out, sources in, temp
   int nc = ctx->st.ncells;
   TC_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * (nc + 1), s));
   TC_CHECK(hipMemsetAsync(off.p, 0, sizeof(int) * (nc + 1), s));
   size_t m = (size_t)std::max(npairs, 1);
   TC_CHECK(src.ensure(sizeof(int) * m));
-  TC_CHECK(ctx->gtg_a.ensure(sizeof(int) * m));
-  TC_CHECK(ctx->gtg_b.ensure(sizeof(int) * m));
-  TC_CHECK(ctx->gsrc_a.ensure(sizeof(int) * m));
-  TC_CHECK(ctx->gsort_tmp.ensure(pair_sort_temp_bytes(npairs) + 256));
+  TC_CHECK(sc[0].ensure(sizeof(int) * m));
+  TC_CHECK(sc[1].ensure(sizeof(int) * m));
+  TC_CHECK(sc[2].ensure(sizeof(int) * m));
+  TC_CHECK(sc[3].ensure(pair_sort_temp_bytes(npairs) + 256));
   int bits = 1;
   while ((1 << bits) < nc) bits++;
-  if (launch_group_pairs(pairs, npairs, bits, nc, ctx->gtg_a.as<int>(), ctx->gtg_b.as<int>(),
-                         ctx->gsrc_a.as<int>(), src.as<int>(), off.as<int>(), cnt.as<int>(),
-                         ctx->gsort_tmp.p, ctx->gsort_tmp.bytes, s))
+  if (launch_group_pairs(pairs, npairs, bits, nc, sc[0].as<int>(), sc[1].as<int>(),
+                         sc[2].as<int>(), src.as<int>(), off.as<int>(), cnt.as<int>(), sc[3].p,
+                         sc[3].bytes, s))
     return -4;
   return 0;
 }
@@ -292,16 +295,20 @@ __global__ void k_chunk_counts(const int *__restrict__ cnt, int n, int *__restri
 static int build_lists(tc_context *ctx, int &nchunks) {
   hipStream_t s = ctx->stream;
   int nc = ctx->st.ncells;
+  // the P2P list on near_stream meanwhile (after the walk that wrote it), with
+  // the per-leaf list lengths for the host-built work items; ev[15] marks both
+  TC_CHECK(hipEventRecord(ctx->ev[14], s));
+  TC_CHECK(hipStreamWaitEvent(ctx->near_stream, ctx->ev[14], 0));
+  if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
+                        ctx->p2p_off, ctx->p2p_src, true))
+    return e;
+  TC_CHECK(ctx->h_pcnt.ensure(nc));
+  TC_CHECK(hipMemcpyAsync(ctx->h_pcnt.p, ctx->p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost,
+                          ctx->near_stream));
+  TC_CHECK(hipEventRecord(ctx->ev[15], ctx->near_stream));
   if (int e = build_csr(ctx, ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->m2l_cnt, ctx->m2l_off,
                         ctx->m2l_src))
     return e;
-  if (int e = build_csr(ctx, ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->p2p_cnt,
-                        ctx->p2p_off, ctx->p2p_src))
-    return e;
-  // per-leaf P2P list lengths for the host-built work items (see build_p2p_items)
-  TC_CHECK(ctx->h_pcnt.ensure(nc));
-  TC_CHECK(hipMemcpyAsync(ctx->h_pcnt.p, ctx->p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost,
-                          s));
   // chunks of <= 64 sources per target for M2L
   hipLaunchKernelGGL(k_chunk_counts, dim3((nc + 1 + 255) / 256), dim3(256), 0, s,
                      ctx->m2l_cnt.as<int>(), nc, ctx->m2l_cur.as<int>());
@@ -310,7 +317,7 @@ static int build_lists(tc_context *ctx, int &nchunks) {
     return -4;
   TC_CHECK(hipMemcpyAsync(ctx->h_pin + 10, ctx->m2l_coff.as<int>() + nc, sizeof(int),
                           hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));  // also completes the list-length copy above
+  TC_CHECK(hipStreamSynchronize(s));
   nchunks = ctx->h_pin[10];
   TC_CHECK(ctx->m2l_chunks.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
   launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
@@ -325,6 +332,7 @@ static int build_lists(tc_context *ctx, int &nchunks) {
 static int build_p2p_items(tc_context *ctx) {
   constexpr int kItemList = 128;
   TC_CHECK(hipEventSynchronize(ctx->ev[11]));  // leaf ids and body counts (build_tree)
+  TC_CHECK(hipEventSynchronize(ctx->ev[15]));  // list lengths (build_lists)
   const int *pcnt = ctx->h_pcnt.p, *leaves = ctx->h_leaves.p, *nbd = ctx->h_nbd.p;
   size_t nitem = 0;
   for (int l = 0; l < ctx->nleaves; l++) {
@@ -695,6 +703,7 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
   TC_CHECK(hipEventRecord(ctx->ev[3], s));
   int nchunks = 0;
   if (int e = build_lists(ctx, nchunks)) return e;
+  TC_CHECK(hipStreamWaitEvent(s, ctx->ev[15], 0));  // the P2P list's grouping (near_stream)
   TC_CHECK(hipEventRecord(ctx->ev[4], s));
   // ---- far field
   if (nchunks > 0) {
@@ -895,7 +904,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->cM, &ctx->cMd, &ctx->cL, &ctx->cLd, &ctx->split_offs, &ctx->split_nch,
                  &ctx->split_scan, &ctx->scan_tmp, &ctx->leaf_ids, &ctx->front_a, &ctx->front_b,
                  &ctx->m2l, &ctx->p2p, &ctx->tcnt, &ctx->tblk, &ctx->tscan, &ctx->tstate, &ctx->tcls, &ctx->tree_state,
-                 &ctx->gtg_a, &ctx->gtg_b, &ctx->gsrc_a, &ctx->gsort_tmp, &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
+                 &ctx->gsort1[0], &ctx->gsort1[1], &ctx->gsort1[2], &ctx->gsort1[3], &ctx->gsort2[0], &ctx->gsort2[1], &ctx->gsort2[2], &ctx->gsort2[3], &ctx->m2l_cnt, &ctx->m2l_off, &ctx->m2l_cur,
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
                  &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts, &ctx->mts_sorted,
