@@ -139,6 +139,7 @@ struct tc_context {
   // the walk's lists written (stream), [15] the P2P list grouped and its
   // lengths in h_pcnt (near_stream)
   hipEvent_t ev[16];
+  hipEvent_t ev_wait = nullptr;  // spin_wait's event (no timing)
   // second stream (default priority): P2P runs on it while the downward pass
   // runs on `stream`; its far-field results go to a second result set
   hipStream_t near_stream = nullptr;

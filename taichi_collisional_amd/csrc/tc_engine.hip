@@ -77,6 +77,21 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
   return 0;
 }
 
+// Wait for the work queued on s by polling an event (spinning) instead of
+// hipStreamSynchronize, for the short waits inside an evaluation where the
+// host needs a few counts to go on: a blocking wait may sleep until an
+// interrupt, and that wake-up has been measured at hundreds of microseconds
+// on some runs, per wait.
+static hipError_t spin_wait(tc_context *ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->ev_wait, s);
+  if (e != hipSuccess) return e;
+  for (;;) {
+    e = hipEventQuery(ctx->ev_wait);
+    if (e != hipErrorNotReady) return e;
+    __builtin_ia32_pause();
+  }
+}
+
 // Tree levels are queued in batches without host reads (TreeState on the
 // device holds the level range and counts); after a batch one small copy says
 // whether the last level had no children.  A level whose children do not fit
@@ -87,7 +102,8 @@ static int build_tree(tc_context *ctx, int n) {
   hipStream_t s = ctx->stream;
   BodiesDev b = bodies_of(ctx);
   const int ncrit = ctx->cfg.ncrit;
-  TC_CHECK(hipStreamSynchronize(s));
+  // no wait for the sort queued before: the cell arrays are not in its use,
+  // and a reallocation's hipFree synchronizes by itself
   if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64), 0)) return e;
   TC_CHECK(ctx->tree_state.ensure(sizeof(TreeState)));
   TC_CHECK(ctx->h_tree.ensure(sizeof(TreeState) / sizeof(int)));
@@ -115,7 +131,7 @@ static int build_tree(tc_context *ctx, int n) {
                           ctx->leaf_ids.as<int>(), s);
       TC_CHECK(hipMemcpyAsync(hst, ctx->tree_state.p, sizeof(TreeState), hipMemcpyDeviceToHost,
                               s));
-      TC_CHECK(hipStreamSynchronize(s));
+      TC_CHECK(spin_wait(ctx, s));
       if (hst->done || hst->overflow) break;
       if (queued > 4 * kMaxTreeLevels) {
         ctx->err = "tree build does not end";
@@ -236,7 +252,7 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
                          ctx->tblk.as<int>(), cm, cp, cf, s);
       }
       TC_CHECK(hipMemcpyAsync(hst, ctx->tstate.p, sizeof(TravState), hipMemcpyDeviceToHost, s));
-      TC_CHECK(hipStreamSynchronize(s));
+      TC_CHECK(spin_wait(ctx, s));
       if (hst->overflow || hst->nin[k & 1] == 0) break;
       if (k > 100000) {
         ctx->err = "traversal does not end";
@@ -317,7 +333,7 @@ static int build_lists(tc_context *ctx, int &nchunks) {
     return -4;
   TC_CHECK(hipMemcpyAsync(ctx->h_pin + 10, ctx->m2l_coff.as<int>() + nc, sizeof(int),
                           hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(spin_wait(ctx, s));
   nchunks = ctx->h_pin[10];
   TC_CHECK(ctx->m2l_chunks.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
   launch_make_chunks(ctx->m2l_off.as<int>(), ctx->m2l_cnt.as<int>(), ctx->m2l_coff.as<int>(), nc,
@@ -691,7 +707,7 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
     }
     TC_CHECK(hipMemcpyAsync(ctx->h_pin + 11, ctx->msym_off.as<int>() + na, sizeof(int),
                             hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipStreamSynchronize(s));
+    TC_CHECK(spin_wait(ctx, s));
     const int nsym = ctx->h_pin[11];
     TC_CHECK(ctx->m2l_sym.ensure(sizeof(int2) * (size_t)std::max(nsym, 1)));
     launch_sym_write(ctx->m2l.as<int2>(), na, ctx->csink.as<unsigned char>(),
@@ -879,6 +895,7 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     ctx->own_stream = true;
   }
   for (auto &e : ctx->ev) hipEventCreate(&e);
+  hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming);
   if (hipStreamCreateWithFlags(&ctx->near_stream, hipStreamNonBlocking) != hipSuccess) {
     ctx->near_stream = nullptr;
     tc_destroy(ctx);
@@ -920,6 +937,7 @@ int tc_destroy(tc_context *ctx) {
   ctx->h_items.release();
   ctx->h_stage.release();
   for (auto &e : ctx->ev) hipEventDestroy(e);
+  hipEventDestroy(ctx->ev_wait);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->near_stream) {
     hipStreamSynchronize(ctx->near_stream);
