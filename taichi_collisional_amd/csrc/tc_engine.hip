@@ -82,14 +82,18 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
 // host needs a few counts to go on: a blocking wait may sleep until an
 // interrupt, and that wake-up has been measured at hundreds of microseconds
 // on some runs, per wait.
-static hipError_t spin_wait(tc_context *ctx, hipStream_t s) {
-  hipError_t e = hipEventRecord(ctx->ev_wait, s);
-  if (e != hipSuccess) return e;
+static hipError_t spin_event(hipEvent_t ev) {
   for (;;) {
-    e = hipEventQuery(ctx->ev_wait);
+    const hipError_t e = hipEventQuery(ev);
     if (e != hipErrorNotReady) return e;
     __builtin_ia32_pause();
   }
+}
+
+static hipError_t spin_wait(tc_context *ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->ev_wait, s);
+  if (e != hipSuccess) return e;
+  return spin_event(ctx->ev_wait);
 }
 
 // Tree levels are queued in batches without host reads (TreeState on the
@@ -211,7 +215,7 @@ static int upward(tc_context *ctx, bool split = false) {
 static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   hipStream_t s = ctx->stream;
   TravState *hst = reinterpret_cast<TravState *>(ctx->h_pin + 40);
-  static_assert(sizeof(TravState) <= 24 * sizeof(int), "pinned slots 40..63");
+  static_assert(sizeof(TravState) <= 12 * sizeof(int), "pinned slots 40..51 (52, 53: P2P pairs)");
   TC_CHECK(ctx->tstate.ensure(sizeof(TravState)));
   TC_CHECK(ctx->tblk.ensure(sizeof(int) * 6 * kTravBlocks));
   TC_CHECK(ctx->front_a.ensure(sizeof(int2) * (size_t)std::max(1 << 16, 4 * n)));
@@ -347,8 +351,8 @@ static int build_lists(tc_context *ctx, int &nchunks) {
 // while the GPU is busy with M2L; the upload is queued behind it.
 static int build_p2p_items(tc_context *ctx) {
   constexpr int kItemList = 128;
-  TC_CHECK(hipEventSynchronize(ctx->ev[11]));  // leaf ids and body counts (build_tree)
-  TC_CHECK(hipEventSynchronize(ctx->ev[15]));  // list lengths (build_lists)
+  TC_CHECK(spin_event(ctx->ev[11]));  // leaf ids and body counts (build_tree)
+  TC_CHECK(spin_event(ctx->ev[15]));  // list lengths (build_lists)
   const int *pcnt = ctx->h_pcnt.p, *leaves = ctx->h_leaves.p, *nbd = ctx->h_nbd.p;
   size_t nitem = 0;
   for (int l = 0; l < ctx->nleaves; l++) {
@@ -614,10 +618,10 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   launch_scatter_results_sum(bodies_of(ctx), r, rf, n, pot, acc, jerk, s);
   TC_CHECK(hipEventRecord(ctx->ev[7], s));
   // body pairs evaluated by P2P (statistics, counted inside the P2P kernel)
-  unsigned long long pairs = 0;
-  TC_CHECK(hipMemcpyAsync(&pairs, ctx->npairs.p, sizeof(pairs), hipMemcpyDeviceToHost, s));
-  TC_CHECK(hipStreamSynchronize(s));
-  ctx->st.n_p2p_pairs = (long long)pairs;
+  unsigned long long *pairs = reinterpret_cast<unsigned long long *>(ctx->h_pin + 52);
+  TC_CHECK(hipMemcpyAsync(pairs, ctx->npairs.p, sizeof(*pairs), hipMemcpyDeviceToHost, s));
+  TC_CHECK(spin_wait(ctx, s));
+  ctx->st.n_p2p_pairs = (long long)*pairs;
   float ms[8];
   for (int k = 0; k < 5; k++) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
   hipEventElapsedTime(&ms[5], ctx->ev[9], ctx->ev[10]);  // P2P (second stream)
