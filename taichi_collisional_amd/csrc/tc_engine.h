@@ -122,7 +122,7 @@ struct tc_context {
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
   // error-controlled MAC: p = 2 pre-pass, multipole norms, acc_old output
-  DBuf lowM, lowL, lowF, cPn, accout;
+  DBuf lowM, lowL, lowF, cPn, accout, lowP;  // lowP: float records for P2P_low
   bool has_acc_old = false;  // the last evaluation ran the pre-pass
   int last_n = 0;
   // direct

@@ -414,6 +414,7 @@ static int low_pass(tc_context *ctx, int n) {
   TC_CHECK(hipMemsetAsync(ctx->lowM.p, 0, sizeof(double) * 9 * nc, s));
   TC_CHECK(hipMemsetAsync(ctx->lowL.p, 0, sizeof(double) * 9 * nc, s));
   TC_CHECK(hipMemsetAsync(ctx->lowF.p, 0, sizeof(double) * 3 * (size_t)n, s));
+  TC_CHECK(ctx->lowP.ensure(sizeof(float4) * (size_t)n));
   launch_low_upward(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
                     ctx->lowM.as<double>(), s);
   if (int e = traverse(ctx, n, mac_args(ctx, kMacLow))) return e;
@@ -428,7 +429,8 @@ static int low_pass(tc_context *ctx, int n) {
   launch_low_horizontal(c, b, nc, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->m2l_off.as<int>(),
                         ctx->m2l_cnt.as<int>(), ctx->m2l_src.as<int>(), ctx->p2p_off.as<int>(),
                         ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowM.as<double>(),
-                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), s);
+                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), n,
+                        ctx->lowP.as<float4>(), s);
   launch_low_downward(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
                       ctx->lowL.as<double>(), ctx->lowF.as<double>(), n, s);
   return 0;
@@ -681,7 +683,7 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
   launch_low_horizontal(c, b, nc, ctx->leaf_ids.as<int>(), 0, ctx->m2l_off.as<int>(),
                         ctx->m2l_cnt.as<int>(), ctx->m2l_src.as<int>(), ctx->p2p_off.as<int>(),
                         ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowM.as<double>(),
-                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), s);
+                        ctx->lowL.as<double>(), ctx->lowF.as<double>(), n, nullptr, s);
   launch_low_p2p_master(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->p2p_off.as<int>(),
                         ctx->p2p_cnt.as<int>(), ctx->p2p_src.as<int>(), ctx->lowF.as<double>(), s);
   launch_low_downward_master(c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->level_start,
@@ -932,7 +934,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym,
                  &ctx->fpot, &ctx->fax, &ctx->fay, &ctx->faz, &ctx->fjx, &ctx->fjy, &ctx->fjz,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
-                 &ctx->lowF, &ctx->cPn, &ctx->accout};
+                 &ctx->lowF, &ctx->cPn, &ctx->accout, &ctx->lowP};
   for (DBuf *b : all) b->release();
   ctx->h_leaves.release();
   ctx->h_nbd.release();

@@ -166,7 +166,8 @@ int launch_low_upward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
 int launch_low_horizontal(CellsDev c, BodiesDev b, int nc, const int *leaf_ids, int nleaves,
                           const int *m2l_off, const int *m2l_cnt, const int *m2l_src,
                           const int *p2p_off, const int *p2p_cnt, const int *p2p_src,
-                          const double *lowM, double *lowL, double *F, hipStream_t s);
+                          const double *lowM, double *lowL, double *F, int n, float4 *P,
+                          hipStream_t s);  // P: n float records for P2P_low (nleaves > 0)
 int launch_low_downward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
                         const std::vector<int> &level_start, double *lowL, double *F, int n,
                         hipStream_t s);

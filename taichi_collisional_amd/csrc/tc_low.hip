@@ -171,13 +171,21 @@ __global__ void k_low_m2l(CellsDev c, int nc, const int *__restrict__ off,
   for (int t = 0; t < LNT; t++) lowL[LNT * i + t] += acc[t];
 }
 
+// The bodies as P2P_low reads them: (float) x, y, z, q, one 16-byte record each.
+__global__ void k_low_stage(BodiesDev b, int n, float4 *__restrict__ P) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) P[i] = make_float4((float)b.x[i], (float)b.y[i], (float)b.z[i], (float)b.q[i]);
+}
+
 // One block of 64 per target leaf, lane = target body; the source bodies of
-// each listed leaf are read as floats (every lane reads the same body).
+// each listed leaf come from the float records (every lane reads the same
+// one: one load per source body instead of four and their conversions).
 __global__ __launch_bounds__(64) void k_low_p2p(CellsDev c, BodiesDev b,
                                                 const int *__restrict__ leaf_ids,
                                                 const int *__restrict__ off,
                                                 const int *__restrict__ cnt,
                                                 const int *__restrict__ src,
+                                                const float4 *__restrict__ P,
                                                 double *__restrict__ F) {
   const int ci = leaf_ids[blockIdx.x];
   const int b0 = c.body[ci], nb = c.nbody[ci];
@@ -193,8 +201,9 @@ __global__ __launch_bounds__(64) void k_low_p2p(CellsDev c, BodiesDev b,
       const int j0 = c.body[j], nj = c.nbody[j];
       float ax = 0, ay = 0, az = 0;
       for (int y = j0; y < j0 + nj; y++) {
-        const float mj = (float)b.q[y];
-        const float dx = xi + (float)b.x[y], dy = yi + (float)b.y[y], dz = zi + (float)b.z[y];
+        const float4 q = P[y];
+        const float mj = q.w;
+        const float dx = xi + q.x, dy = yi + q.y, dz = zi + q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 > 0)) continue;
         const float rr = sqrtf(r2);
@@ -402,13 +411,16 @@ int launch_low_upward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
 int launch_low_horizontal(CellsDev c, BodiesDev b, int nc, const int *leaf_ids, int nleaves,
                           const int *m2l_off, const int *m2l_cnt, const int *m2l_src,
                           const int *p2p_off, const int *p2p_cnt, const int *p2p_src,
-                          const double *lowM, double *lowL, double *F, hipStream_t s) {
+                          const double *lowM, double *lowL, double *F, int n, float4 *P,
+                          hipStream_t s) {
   if (nc > 0)
     hipLaunchKernelGGL(k_low_m2l, blocks_for(nc, 128), dim3(128), 0, s, c, nc, m2l_off, m2l_cnt,
                        m2l_src, lowM, lowL);
-  if (nleaves > 0)
+  if (nleaves > 0) {
+    hipLaunchKernelGGL(k_low_stage, dim3((n + 255) / 256), dim3(256), 0, s, b, n, P);
     hipLaunchKernelGGL(k_low_p2p, dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids, p2p_off,
-                       p2p_cnt, p2p_src, F);
+                       p2p_cnt, p2p_src, (const float4 *)P, F);
+  }
   return 0;
 }
 
