@@ -25,6 +25,12 @@
 
 namespace tcfmm {
 
+__host__ __device__ constexpr int pow2_floor(int v) {
+  int p = 1;
+  while (2 * p <= v) p *= 2;
+  return p;
+}
+
 // Orders N..P of P2M for one lane's body (w = its mass, 0 when idle): row N of
 // the harmonics from rows N-1, N-2 (compile-time N, so no case tests), then
 // q Y_N^m and q dY_N^m/dt summed over the bodies through the LDS transpose
@@ -62,10 +68,19 @@ __device__ __forceinline__ void p2m_orders(double dx, double dy, double dz, doub
       red[lane][2 * m + 1] = w * v.im;
     }
     __syncthreads();
-    const int m = lane >> 1, im = lane & 1;
-    if (lane < 2 * (N + 1) && !(m == 0 && im)) {
-      double acc = 0;
-      for (int r = 0; r < cnt; r++) acc += red[r][lane];
+    // column c = 2m + (imaginary) summed by G lanes (G a power of two, C G <=
+    // 64), each over every G-th body row, then a butterfly inside the group:
+    // 64 / G dependent LDS reads per lane instead of 64
+    constexpr int C = 2 * (N + 1);
+    constexpr int G = pow2_floor(kWave / C);
+    const int col = lane / G, part = lane % G;
+    double acc = 0;
+    if (col < C)
+      for (int r = part; r < cnt; r += G) acc += red[r][col];
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    const int m = col >> 1, im = col & 1;
+    if (col < C && part == 0 && !(m == 0 && im)) {
       double *dst = (pass == 0 ? Mc : Mdc) + (im ? s - m : s + m);
       *dst += acc;
     }
