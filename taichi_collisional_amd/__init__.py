@@ -208,8 +208,10 @@ class Fmm:
         self._last_n = 0
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().tc_destroy(self.h)
+        # at interpreter exit the module globals may already be cleared to None
+        L = globals().get("_lib")
+        if getattr(self, "h", None) and L is not None:
+            L.tc_destroy(self.h)
             self.h = None
 
     __del__ = close

@@ -171,38 +171,75 @@ static int build_tree(tc_context *ctx, int n) {
   return 0;
 }
 
-// Upward pass.  split: the geometry (balls, min_acc, has_sink: all the theta
-// walk reads) on `stream`, then the moments on near_stream after it; ev[13]
-// marks their end, and the caller makes `stream` wait for it before M is read.
-static int upward(tc_context *ctx, bool split = false) {
-  hipStream_t s = ctx->stream;
+// Upward pass, in one of three arrangements:
+//   kUpwardWhole   all of it on `stream`;
+//   kUpwardTheta   the geometry (balls, min_acc, has_sink: all the theta walk
+//                  reads) on `stream`, then the moments on near_stream after
+//                  it, while the walk and the lists run;
+//   kUpwardErrMac  geometry and moments on near_stream, after the tree,
+//                  while `stream` runs the pre-pass (they do not read
+//                  acc_old); upward_min() then forms min_acc on `stream`.
+// In the last two ev[13] marks the end of the near_stream part, which
+// `stream` waits for before it reads M.
+enum { kUpwardWhole, kUpwardTheta, kUpwardErrMac };
+
+static int upward(tc_context *ctx, int how = kUpwardWhole) {
+  hipStream_t s = ctx->stream, s2 = ctx->near_stream;
   int p = ctx->cfg.p;
   int nc = ctx->st.ncells;
   size_t NT = (size_t)nterm(p);
-  TC_CHECK(hipMemsetAsync(ctx->cM.p, 0, sizeof(double) * NT * nc, s));
-  TC_CHECK(hipMemsetAsync(ctx->cMd.p, 0, sizeof(double) * NT * nc, s));
-  TC_CHECK(hipMemsetAsync(ctx->cL.p, 0, sizeof(double) * NT * nc, s));
-  TC_CHECK(hipMemsetAsync(ctx->cLd.p, 0, sizeof(double) * NT * nc, s));
-  TC_CHECK(hipMemsetAsync(ctx->csink.p, 0, nc, s));
+  hipStream_t sz = s;  // where the moments are cleared
+  if (how == kUpwardErrMac) {
+    TC_CHECK(hipEventRecord(ctx->ev[12], s));  // the tree
+    TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[12], 0));
+    sz = s2;
+  }
+  TC_CHECK(hipMemsetAsync(ctx->cM.p, 0, sizeof(double) * NT * nc, sz));
+  TC_CHECK(hipMemsetAsync(ctx->cMd.p, 0, sizeof(double) * NT * nc, sz));
+  TC_CHECK(hipMemsetAsync(ctx->cL.p, 0, sizeof(double) * NT * nc, sz));
+  TC_CHECK(hipMemsetAsync(ctx->cLd.p, 0, sizeof(double) * NT * nc, sz));
+  TC_CHECK(hipMemsetAsync(ctx->csink.p, 0, nc, sz));
   CellsDev c = cells_of(ctx);
   BodiesDev b = bodies_of(ctx);
   const int L = (int)ctx->level_start.size() - 1;
-  const int first = split ? kUpGeom : kUpGeom | kUpMoments;
-  if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s, first)) {
+  auto pass = [&](int what, hipStream_t st) {
+    if (launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, st, what)) return -2;
+    for (int l = L - 1; l >= 0; l--)
+      launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], st, what);
+    return 0;
+  };
+  int e = 0;
+  if (how == kUpwardWhole) {
+    e = pass(kUpAll, s);
+  } else if (how == kUpwardTheta) {
+    e = pass(kUpGeom | kUpMin, s);
+    if (!e) {
+      TC_CHECK(hipEventRecord(ctx->ev[12], s));
+      TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[12], 0));
+      e = pass(kUpMoments, s2);
+    }
+  } else {
+    e = pass(kUpGeom | kUpMoments, s2);
+  }
+  if (e) {
     ctx->err = "expansion order not compiled";
     return -2;
   }
+  if (how != kUpwardWhole) TC_CHECK(hipEventRecord(ctx->ev[13], s2));
+  return 0;
+}
+
+// min_acc of every cell from acc_old (the error MAC's), on `stream`, after the
+// near_stream part of a kUpwardErrMac upward pass.
+static int upward_min(tc_context *ctx) {
+  hipStream_t s = ctx->stream;
+  TC_CHECK(hipStreamWaitEvent(s, ctx->ev[13], 0));
+  CellsDev c = cells_of(ctx);
+  BodiesDev b = bodies_of(ctx);
+  const int p = ctx->cfg.p, L = (int)ctx->level_start.size() - 1;
+  launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s, kUpMin);
   for (int l = L - 1; l >= 0; l--)
-    launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s, first);
-  if (split) {
-    hipStream_t s2 = ctx->near_stream;
-    TC_CHECK(hipEventRecord(ctx->ev[12], s));
-    TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[12], 0));
-    launch_p2m(p, c, b, ctx->leaf_ids.as<int>(), ctx->nleaves, s2, kUpMoments);
-    for (int l = L - 1; l >= 0; l--)
-      launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s2, kUpMoments);
-    TC_CHECK(hipEventRecord(ctx->ev[13], s2));
-  }
+    launch_m2m(p, c, b, ctx->level_start[l], ctx->level_start[l + 1], s, kUpMin);
   return 0;
 }
 
@@ -549,6 +586,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
   const bool err_mac = cfg.mac == TC_MAC_ERROR;
   if (err_mac) {
+    // the high-order geometry and moments do not read acc_old: they run on
+    // the second stream alongside the pre-pass
+    if (int e = upward(ctx, kUpwardErrMac)) return e;
     TC_CHECK(hipEventRecord(ctx->ev[8], s));
     if (int e = low_pass(ctx, n)) return e;
     ctx->has_acc_old = true;
@@ -556,10 +596,13 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   TC_CHECK(hipEventRecord(ctx->ev[1], s));
   // ---- upward: with the theta MAC the walk needs only the geometry, so the
   // moments are formed on the second stream while the walk and the lists run
-  if (int e = upward(ctx, !err_mac)) return e;
-  if (err_mac)
+  if (err_mac) {
+    if (int e = upward_min(ctx)) return e;
     launch_pn(ctx->cM.as<double>(), ctx->st.ncells, cfg.p, fact_table(cfg), ctx->cPn.as<double>(),
               s);
+  } else {
+    if (int e = upward(ctx, kUpwardTheta)) return e;
+  }
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
   // ---- traversal and lists
   if (int e = traverse(ctx, n, mac_args(ctx, err_mac ? kMacError : kMacTheta))) return e;

@@ -130,13 +130,14 @@ int launch_make_chunks(const int *offs, const int *count, const int *chunk_offs,
                        int4 *chunks, hipStream_t s);
 
 // ---- upward (tc_upward.hip)
-// what: kUpGeom (ball X, R, min_acc, has_sink), kUpMoments (M, Mdot about the
-// X already there), or both (default)
-enum { kUpGeom = 1, kUpMoments = 2 };
+// what: kUpGeom (ball X, R, has_sink), kUpMin (min_acc from acc_old),
+// kUpMoments (M, Mdot about the X already there); all by default.  Launched
+// combinations: all, kUpGeom | kUpMin, kUpMoments, kUpGeom | kUpMoments, kUpMin.
+enum { kUpGeom = 1, kUpMoments = 2, kUpMin = 4, kUpAll = 7 };
 int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s,
-               int what = kUpGeom | kUpMoments);
+               int what = kUpAll);
 int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s,
-               int what = kUpGeom | kUpMoments);
+               int what = kUpAll);
 int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hipStream_t s);
 
 // ---- M2L (tc_m2l.hip)

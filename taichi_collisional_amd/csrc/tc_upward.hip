@@ -96,9 +96,11 @@ __device__ __forceinline__ void p2m_orders(double dx, double dy, double dz, doub
   }
 }
 
-// kWhat: kUpGeom (ball, min_acc, has_sink), kUpMoments (M, Mdot about the
-// ball centre already in c.X), or both.  Split, the traversal can start on
-// the geometry while the moments are formed on another stream.
+// kWhat: kUpGeom (ball, has_sink), kUpMin (min_acc), kUpMoments (M, Mdot
+// about the ball centre already in c.X), or a combination.  Split, the theta
+// walk starts on the geometry while the moments are formed on another stream,
+// and with the error MAC geometry and moments run alongside the pre-pass that
+// min_acc waits for.
 template <int P, int kWhat>
 __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
                                             const int *__restrict__ leaf_ids) {
@@ -110,54 +112,63 @@ __global__ __launch_bounds__(64) void k_p2m(CellsDev c, BodiesDev b,
   const int lane = threadIdx.x;
   const int ci = leaf_ids[blockIdx.x];
   const int b0 = c.body[ci], nb = c.nbody[ci];
+  if constexpr (kWhat == kUpMin) {
+    // min_acc alone (the error MAC's, once the pre-pass has set acc_old)
+    double mina = 1e30;
+    for (int i = lane; i < nb; i += kWave) mina = fmin(b.acc_old[b0 + i], mina);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mina = fmin(mina, __shfl_xor(mina, o, kWave));
+    if (lane == 0) c.min_acc[ci] = mina;
+    return;
+  }
   double X[3];
   if constexpr (!(kWhat & kUpGeom)) {
     for (int d = 0; d < 3; d++) X[d] = c.X[3 * ci + d];
   } else {
-  const bool staged = nb <= kStage;
-  if (staged)
-    for (int i = lane; i < nb; i += kWave) {
-      s_q[0][i] = b.x[b0 + i];
-      s_q[1][i] = b.y[b0 + i];
-      s_q[2][i] = b.z[b0 + i];
-    }
-  __syncthreads();
-  double R;
-  if (nb > 0) {
-    DBall ball = wave_minball(nb, [&](int i, double *q) {
-      if (staged) {
-        q[0] = s_q[0][i];
-        q[1] = s_q[1][i];
-        q[2] = s_q[2][i];
-      } else {
-        q[0] = b.x[b0 + i];
-        q[1] = b.y[b0 + i];
-        q[2] = b.z[b0 + i];
+    const bool staged = nb <= kStage;
+    if (staged)
+      for (int i = lane; i < nb; i += kWave) {
+        s_q[0][i] = b.x[b0 + i];
+        s_q[1][i] = b.y[b0 + i];
+        s_q[2][i] = b.z[b0 + i];
       }
-    });
-    for (int d = 0; d < 3; d++) X[d] = ball.c[d];
-    R = sqrt(ball.r2) + 1e-6;
-  } else {
-    for (int d = 0; d < 3; d++) X[d] = c.Xg[3 * ci + d];
-    R = 1e-6;
-  }
-  double mina = 1e30;
-  int sink = 0;
-  for (int i = lane; i < nb; i += kWave) {
-    mina = fmin(b.acc_old[b0 + i], mina);
-    sink |= (b.flags[b0 + i] & kSink) ? 1 : 0;
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    mina = fmin(mina, __shfl_xor(mina, o, kWave));
-    sink |= __shfl_xor(sink, o, kWave);
-  }
-  if (lane == 0) {
-    for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
-    c.R[ci] = R;
-    c.min_acc[ci] = mina;
-    c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
-  }
+    __syncthreads();
+    double R;
+    if (nb > 0) {
+      DBall ball = wave_minball(nb, [&](int i, double *q) {
+        if (staged) {
+          q[0] = s_q[0][i];
+          q[1] = s_q[1][i];
+          q[2] = s_q[2][i];
+        } else {
+          q[0] = b.x[b0 + i];
+          q[1] = b.y[b0 + i];
+          q[2] = b.z[b0 + i];
+        }
+      });
+      for (int d = 0; d < 3; d++) X[d] = ball.c[d];
+      R = sqrt(ball.r2) + 1e-6;
+    } else {
+      for (int d = 0; d < 3; d++) X[d] = c.Xg[3 * ci + d];
+      R = 1e-6;
+    }
+    double mina = 1e30;
+    int sink = 0;
+    for (int i = lane; i < nb; i += kWave) {
+      if (kWhat & kUpMin) mina = fmin(b.acc_old[b0 + i], mina);
+      sink |= (b.flags[b0 + i] & kSink) ? 1 : 0;
+    }
+  #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mina = fmin(mina, __shfl_xor(mina, o, kWave));
+      sink |= __shfl_xor(sink, o, kWave);
+    }
+    if (lane == 0) {
+      for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
+      c.R[ci] = R;
+      if (kWhat & kUpMin) c.min_acc[ci] = mina;
+      c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
+    }
   }
   if constexpr (!(kWhat & kUpMoments)) return;
   double *Mc = c.M + (size_t)ci * NT;
@@ -207,117 +218,125 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
   const int nch = c.nchild[ci];
   if (nch == 0) return;
   const int ch0 = c.child[ci];
+  if constexpr (kWhat == kUpMin) {
+    if (tid == 0) {
+      double mina = 1e30;
+      for (int k = 0; k < nch; k++) mina = fmin(c.min_acc[ch0 + k], mina);
+      c.min_acc[ci] = mina;
+    }
+    return;
+  }
   double X[3];
   if constexpr (!(kWhat & kUpGeom)) {
     for (int d = 0; d < 3; d++) X[d] = c.X[3 * ci + d];
   } else {
-  if (tid < 192) s_td[tid] = kTDesign[tid];
-  // the children's records, one thread each (their loads overlap), then the
-  // prefix over at most eight children from LDS
-  __shared__ int s_nc2[8], s_nb[8], s_g0[8];
-  if (tid < nch) {
-    const int ch = ch0 + tid;
-    s_nc2[tid] = c.nchild[ch];
-    s_nb[tid] = c.nbody[ch];
-    s_body[tid] = c.body[ch];
-    s_g0[tid] = c.child[ch];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int pre = 0, nbs = 0, ng = 0;
-    for (int k = 0; k < nch; k++) {
-      const int nc2 = s_nc2[k];
-      s_pre[k] = pre;
-      s_leaf[k] = nc2 == 0;
-      if (nc2 == 0) {
-        s_base[k] = nbs;
-        nbs += s_nb[k];
-        pre += s_nb[k];
-      } else {
-        s_base[k] = ng;
-        ng += nc2;
-        pre += 64 * nc2;
-      }
+    if (tid < 192) s_td[tid] = kTDesign[tid];
+    // the children's records, one thread each (their loads overlap), then the
+    // prefix over at most eight children from LDS
+    __shared__ int s_nc2[8], s_nb[8], s_g0[8];
+    if (tid < nch) {
+      const int ch = ch0 + tid;
+      s_nc2[tid] = c.nchild[ch];
+      s_nb[tid] = c.nbody[ch];
+      s_body[tid] = c.body[ch];
+      s_g0[tid] = c.child[ch];
     }
-    s_pre[nch] = pre;
-    s_nstage = nbs;
-  }
-  __syncthreads();
-  const bool staged = s_nstage <= kStage;
-  // grandchildren's balls of the internal children: thread 8 k + j takes
-  // grandchild j of child k
-  if (tid < 64) {
-    const int k = tid >> 3, j = tid & 7;
-    if (k < nch && !s_leaf[k] && j < s_nc2[k]) {
-      const int g = s_g0[k] + j;
-      s_g[s_base[k] + j][0] = c.X[3 * g];
-      s_g[s_base[k] + j][1] = c.X[3 * g + 1];
-      s_g[s_base[k] + j][2] = c.X[3 * g + 2];
-      s_g[s_base[k] + j][3] = c.R[g];
-    }
-  }
-  // bodies of the leaf children, all of them strided over the block
-  if (staged)
-    for (int i = tid; i < s_nstage; i += blockDim.x) {
-      int k = 0;  // the leaf child whose staging range holds slot i
-      while (k < nch - 1 && !(s_leaf[k] && i >= s_base[k] && i < s_base[k] + s_nb[k])) k++;
-      const int bi = s_body[k] + (i - s_base[k]);
-      s_p[0][i] = b.x[bi];
-      s_p[1][i] = b.y[bi];
-      s_p[2][i] = b.z[bi];
-    }
-  __syncthreads();
-  // point `local` of child k, in the order the reference enumerates them
-  auto point = [&](int k, int local, double *q) {
-    if (s_leaf[k]) {
-      if (staged) {
-        const int o = s_base[k] + local;
-        q[0] = s_p[0][o];
-        q[1] = s_p[1][o];
-        q[2] = s_p[2][o];
-      } else {
-        const int bi = s_body[k] + local;
-        q[0] = b.x[bi];
-        q[1] = b.y[bi];
-        q[2] = b.z[bi];
-      }
-    } else {
-      const int g = s_base[k] + (local >> 6), t = local & 63;
-      const double Rg = s_g[g][3];
-      for (int d = 0; d < 3; d++) q[d] = s_g[g][d] + Rg * s_td[3 * t + d];
-    }
-  };
-  DBall ball = block_minball<NW>(
-      [&](int i, double *q) {
-        int k = 0;
-        while (k < nch - 1 && i >= s_pre[k + 1]) k++;
-        point(k, i - s_pre[k], q);
-      },
-      [&](auto f) {
-        for (int k = 0; k < nch; k++) {
-          const int p0 = s_pre[k], cnt = s_pre[k + 1] - p0;
-          for (int l = tid; l < cnt; l += NW * kWave) {
-            double q[3];
-            point(k, l, q);
-            f(p0 + l, q);
-          }
+    __syncthreads();
+    if (tid == 0) {
+      int pre = 0, nbs = 0, ng = 0;
+      for (int k = 0; k < nch; k++) {
+        const int nc2 = s_nc2[k];
+        s_pre[k] = pre;
+        s_leaf[k] = nc2 == 0;
+        if (nc2 == 0) {
+          s_base[k] = nbs;
+          nbs += s_nb[k];
+          pre += s_nb[k];
+        } else {
+          s_base[k] = ng;
+          ng += nc2;
+          pre += 64 * nc2;
         }
-      },
-      s_e, s_i);
-  for (int d = 0; d < 3; d++) X[d] = ball.c[d];
-  if (tid == 0) {
-    double mina = 1e30;
-    int sink = 0;
-    for (int k = 0; k < nch; k++) {
-      mina = fmin(c.min_acc[ch0 + k], mina);
-      sink |= c.has_sink[ch0 + k];
+      }
+      s_pre[nch] = pre;
+      s_nstage = nbs;
     }
-    for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
-    c.R[ci] = sqrt(ball.r2) + 1e-6;
-    c.min_acc[ci] = mina;
-    c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
-  }
-  __syncthreads();  // the ball's buffers are reused below
+    __syncthreads();
+    const bool staged = s_nstage <= kStage;
+    // grandchildren's balls of the internal children: thread 8 k + j takes
+    // grandchild j of child k
+    if (tid < 64) {
+      const int k = tid >> 3, j = tid & 7;
+      if (k < nch && !s_leaf[k] && j < s_nc2[k]) {
+        const int g = s_g0[k] + j;
+        s_g[s_base[k] + j][0] = c.X[3 * g];
+        s_g[s_base[k] + j][1] = c.X[3 * g + 1];
+        s_g[s_base[k] + j][2] = c.X[3 * g + 2];
+        s_g[s_base[k] + j][3] = c.R[g];
+      }
+    }
+    // bodies of the leaf children, all of them strided over the block
+    if (staged)
+      for (int i = tid; i < s_nstage; i += blockDim.x) {
+        int k = 0;  // the leaf child whose staging range holds slot i
+        while (k < nch - 1 && !(s_leaf[k] && i >= s_base[k] && i < s_base[k] + s_nb[k])) k++;
+        const int bi = s_body[k] + (i - s_base[k]);
+        s_p[0][i] = b.x[bi];
+        s_p[1][i] = b.y[bi];
+        s_p[2][i] = b.z[bi];
+      }
+    __syncthreads();
+    // point `local` of child k, in the order the reference enumerates them
+    auto point = [&](int k, int local, double *q) {
+      if (s_leaf[k]) {
+        if (staged) {
+          const int o = s_base[k] + local;
+          q[0] = s_p[0][o];
+          q[1] = s_p[1][o];
+          q[2] = s_p[2][o];
+        } else {
+          const int bi = s_body[k] + local;
+          q[0] = b.x[bi];
+          q[1] = b.y[bi];
+          q[2] = b.z[bi];
+        }
+      } else {
+        const int g = s_base[k] + (local >> 6), t = local & 63;
+        const double Rg = s_g[g][3];
+        for (int d = 0; d < 3; d++) q[d] = s_g[g][d] + Rg * s_td[3 * t + d];
+      }
+    };
+    DBall ball = block_minball<NW>(
+        [&](int i, double *q) {
+          int k = 0;
+          while (k < nch - 1 && i >= s_pre[k + 1]) k++;
+          point(k, i - s_pre[k], q);
+        },
+        [&](auto f) {
+          for (int k = 0; k < nch; k++) {
+            const int p0 = s_pre[k], cnt = s_pre[k + 1] - p0;
+            for (int l = tid; l < cnt; l += NW * kWave) {
+              double q[3];
+              point(k, l, q);
+              f(p0 + l, q);
+            }
+          }
+        },
+        s_e, s_i);
+    for (int d = 0; d < 3; d++) X[d] = ball.c[d];
+    if (tid == 0) {
+      double mina = 1e30;
+      int sink = 0;
+      for (int k = 0; k < nch; k++) {
+        if (kWhat & kUpMin) mina = fmin(c.min_acc[ch0 + k], mina);
+        sink |= c.has_sink[ch0 + k];
+      }
+      for (int d = 0; d < 3; d++) c.X[3 * ci + d] = X[d];
+      c.R[ci] = sqrt(ball.r2) + 1e-6;
+      if (kWhat & kUpMin) c.min_acc[ci] = mina;
+      c.has_sink[ci] = (unsigned char)(c.has_sink[ci] | sink);
+    }
+    __syncthreads();  // the ball's buffers are reused below
   }
   if constexpr (!(kWhat & kUpMoments)) return;
   if constexpr (kPar) {
@@ -437,28 +456,50 @@ __global__ __launch_bounds__(256) void k_m2m(CellsDev c, BodiesDev b, const int 
   }
 }
 
+// the combinations the engine uses: all, geometry (+ min_acc), moments,
+// geometry + moments without min_acc, min_acc alone
 template <int P>
 static void p2m_launch(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, int what,
                        hipStream_t s) {
-  if (what == kUpGeom)
-    hipLaunchKernelGGL((k_p2m<P, kUpGeom>), dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids);
-  else if (what == kUpMoments)
-    hipLaunchKernelGGL((k_p2m<P, kUpMoments>), dim3(nleaves), dim3(64), 0, s, c, b, leaf_ids);
-  else
-    hipLaunchKernelGGL((k_p2m<P, kUpGeom | kUpMoments>), dim3(nleaves), dim3(64), 0, s, c, b,
-                       leaf_ids);
+  const dim3 g(nleaves), t(64);
+  switch (what) {
+    case kUpGeom | kUpMin:
+      hipLaunchKernelGGL((k_p2m<P, kUpGeom | kUpMin>), g, t, 0, s, c, b, leaf_ids);
+      break;
+    case kUpMoments:
+      hipLaunchKernelGGL((k_p2m<P, kUpMoments>), g, t, 0, s, c, b, leaf_ids);
+      break;
+    case kUpGeom | kUpMoments:
+      hipLaunchKernelGGL((k_p2m<P, kUpGeom | kUpMoments>), g, t, 0, s, c, b, leaf_ids);
+      break;
+    case kUpMin:
+      hipLaunchKernelGGL((k_p2m<P, kUpMin>), g, t, 0, s, c, b, leaf_ids);
+      break;
+    default:
+      hipLaunchKernelGGL((k_p2m<P, kUpAll>), g, t, 0, s, c, b, leaf_ids);
+  }
 }
 
 template <int P>
 static void m2m_launch(CellsDev c, BodiesDev b, const int *ids, int ls, int ncell, int what,
                        hipStream_t s) {
-  if (what == kUpGeom)
-    hipLaunchKernelGGL((k_m2m<P, kUpGeom>), dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
-  else if (what == kUpMoments)
-    hipLaunchKernelGGL((k_m2m<P, kUpMoments>), dim3(ncell), dim3(256), 0, s, c, b, ids, ls);
-  else
-    hipLaunchKernelGGL((k_m2m<P, kUpGeom | kUpMoments>), dim3(ncell), dim3(256), 0, s, c, b,
-                       ids, ls);
+  const dim3 g(ncell), t(256);
+  switch (what) {
+    case kUpGeom | kUpMin:
+      hipLaunchKernelGGL((k_m2m<P, kUpGeom | kUpMin>), g, t, 0, s, c, b, ids, ls);
+      break;
+    case kUpMoments:
+      hipLaunchKernelGGL((k_m2m<P, kUpMoments>), g, t, 0, s, c, b, ids, ls);
+      break;
+    case kUpGeom | kUpMoments:
+      hipLaunchKernelGGL((k_m2m<P, kUpGeom | kUpMoments>), g, t, 0, s, c, b, ids, ls);
+      break;
+    case kUpMin:
+      hipLaunchKernelGGL((k_m2m<P, kUpMin>), g, t, 0, s, c, b, ids, ls);
+      break;
+    default:
+      hipLaunchKernelGGL((k_m2m<P, kUpAll>), g, t, 0, s, c, b, ids, ls);
+  }
 }
 
 int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, hipStream_t s,
@@ -495,7 +536,7 @@ int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hip
   switch (p) {
 #define TC_CASE(P)                                                 \
   case P:                                                          \
-    m2m_launch<P>(c, b, ids, 0, nids, kUpGeom | kUpMoments, s);    \
+    m2m_launch<P>(c, b, ids, 0, nids, kUpAll, s);    \
     return 0;
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
