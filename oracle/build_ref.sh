@@ -50,5 +50,22 @@ if [ -f "$MREF/kernel_fast_lean.h" ]; then
     done
   done
 fi
+# The snapshot I/O (io.h) over libhdf5, when it is installed: the jerk build
+# with the jerk Makefile's output flags (Makefile:2-3,9), the same without
+# them, and the master io.h with the master Makefile's (Makefile:5-6,12).
+# EXPANSION only sizes Cell arrays that io.h does not touch.
+H5=${TC_HDF5_DIR:-/opt/conda}
+if [ -f "$H5/include/hdf5.h" ]; then
+  IOFLAGS="-O2 -fPIC -shared -w -DEXPANSION=4 -DH5_USE_16_API -DHUGE=3.40282347e+38F \
+    -I$H5/include -L$H5/lib -lhdf5 -Wl,-rpath,$H5/lib"
+  OUTS="-DOUTPUTPOT -DOUTPUTACC -DDEBUG"
+  for spec in "jerk:$REF:$OUTS" "jerk_plain:$REF:" "master:$MREF:$OUTS"; do
+    IFS=: read -r name dir defs <<< "$spec"
+    tgt="$OUT/libtcrefio_${name}.so"
+    if [ -f "$tgt" ] && [ "$tgt" -nt "$HERE/ref_io_harness.cpp" ]; then continue; fi
+    g++ $defs -I"$dir" "$HERE/ref_io_harness.cpp" $IOFLAGS -o "$tgt.tmp" && mv "$tgt.tmp" "$tgt" &
+    pids+=($!)
+  done
+fi
 for p in "${pids[@]}"; do wait "$p"; done
 echo "build_ref: libraries in $OUT"

@@ -3,7 +3,9 @@
     python -m taichi_collisional_amd.build [--jobs N]
 
 Each csrc/*.hip translation unit is compiled with hipcc for gfx950 into
-build/*.o, then linked into taichi_collisional_amd/libtcfmm.so.  Packed-FP32
+build/*.o, then linked into taichi_collisional_amd/libtcfmm.so.  The HDF5 snapshot library
+(csrc/tc_io.cpp, host code) is built with g++ into libtcio.so when libhdf5 is
+present.  Packed-FP32
 instructions are disabled for all device code.  Always recompiles every unit
 (the driver's build check expects that); object files go to
 taichi_collisional_amd/build/ (git-ignored).
@@ -42,6 +44,32 @@ def _compile(unit):
     return unit, time.time() - t
 
 
+HDF5_DIR = os.environ.get("TC_HDF5_DIR", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libtcio.so")
+
+
+def build_io(verbose=True):
+    """Build libtcio.so (the HDF5 snapshot format, host C++) with g++ against
+    libhdf5 under HDF5_DIR.  Returns None, with a message, when HDF5 is absent."""
+    inc = os.path.join(HDF5_DIR, "include")
+    libdir = os.path.join(HDF5_DIR, "lib")
+    if not os.path.isfile(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print("  libtcio: no hdf5.h under %s; snapshot library not built" % inc, flush=True)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc,
+           os.path.join(CSRC, "tc_io.cpp"), "-o", tmp, "-L" + libdir, "-lhdf5",
+           "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for tc_io.cpp:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print("  linked", IO_LIB, flush=True)
+    return IO_LIB
+
+
 def build(jobs=None, verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
@@ -58,6 +86,7 @@ def build(jobs=None, verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("  linked", LIB, flush=True)
+    build_io(verbose)
     return LIB
 
 
