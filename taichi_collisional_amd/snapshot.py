@@ -125,3 +125,18 @@ def snapshot_number(path):
 def snapshot_name(num):
     """The file name write_snapshot uses for snapshot `num` (io.h:112)."""
     return "snapshot_%d.hdf5" % num
+
+
+def error_cdf(path, ref_path, quantity="acc"):
+    """Relative errors of one snapshot against a reference snapshot of the same
+    particles, as plot_jerk_error.py (taichi_with_jerks/plot_jerk_error.py:13-45)
+    plots them: |d a| / |a_ref| per particle (quantity "acc", from Acc*; "jerk"
+    from Jerk*), sorted, and the fraction of particles above each, 1 - i/n.
+    Both files hold the particles in id order, so the rows correspond."""
+    stem = {"acc": "Acc", "jerk": "Jerk"}[quantity]
+    if not np.array_equal(read_dataset(path, "ID", True), read_dataset(ref_path, "ID", True)):
+        raise ValueError("the snapshots hold different particle ids")
+    a = np.stack([read_dataset(path, stem + c) for c in "xyz"], axis=1)
+    r = np.stack([read_dataset(ref_path, stem + c) for c in "xyz"], axis=1)
+    err = np.sort(np.linalg.norm(a - r, axis=1) / np.linalg.norm(r, axis=1))
+    return err, 1.0 - np.arange(err.size) / err.size

@@ -173,6 +173,27 @@ def test_errors(tmp_path):
         assert snap.snapshot_number(name) == k, name
 
 
+def test_error_cdf(tmp_path):
+    """plot_jerk_error.py's CDF: zero against itself; a known relative
+    perturbation of one particle shows up as that particle's error only."""
+    part = _particles(1000, seed=6)
+    a, b = tmp_path / "snapshot_0_ref.hdf5", tmp_path / "snapshot_0.hdf5"
+    snap.write_snapshot(a, part, 0.0)
+    err, frac = snap.error_cdf(a, a)
+    assert not np.any(err) and frac[0] == 1.0 and abs(frac[-1] - 1e-3) < 1e-15
+    k = int(np.argmax(part["id"]))  # the last row of the files
+    q = part.copy()
+    q["jerk"][k] *= 1.0 + 1e-6
+    snap.write_snapshot(b, q, 0.0)
+    assert not np.any(snap.error_cdf(b, a, "acc")[0])
+    ej = snap.error_cdf(b, a, "jerk")[0]
+    assert np.count_nonzero(ej) == 1 and abs(ej[-1] / 1e-6 - 1) < 1e-9
+    q["id"][0] += 10**6
+    snap.write_snapshot(b, q, 0.0)
+    with pytest.raises(ValueError, match="different particle ids"):
+        snap.error_cdf(b, a)
+
+
 def test_header_declares_exports():
     """libtcio exports every function include/tcio.h declares."""
     hdr = open(os.path.join(ROOT, "include", "tcio.h")).read()
