@@ -133,6 +133,7 @@ struct tc_context {
   // master variant: last-body-of-leaf flags, time-step sums (caller order,
   // tree order), symmetric M2L list and its scan
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
+  DBuf mdpart;  // partial sums of the source slabs of the master direct sum
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
   // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],
   // [13] end of the upward geometry (stream) and moments (near_stream); [14]
@@ -161,6 +162,25 @@ struct tc_context {
 int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        const double *m, const unsigned char *flags, int mode, double dt,
                        double *pot, double *acc, double *tstep, double *acc_old);
+
+// Wait for the work queued on s by polling an event (spinning) instead of
+// hipStreamSynchronize, for the short waits inside an evaluation where the
+// host needs a few counts to go on: a blocking wait may sleep until an
+// interrupt, and that wake-up has been measured at hundreds of microseconds
+// on some runs, per wait.
+inline hipError_t spin_event(hipEvent_t ev) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+    __builtin_ia32_pause();
+  }
+}
+
+inline hipError_t spin_wait(tc_context *ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->ev_wait, s);
+  if (e != hipSuccess) return e;
+  return spin_event(ctx->ev_wait);
+}
 
 inline BodiesDev bodies_of(tc_context *ctx) {
   return {ctx->bx.as<double>(),   ctx->by.as<double>(),  ctx->bz.as<double>(),

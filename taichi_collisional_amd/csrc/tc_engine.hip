@@ -77,25 +77,6 @@ static int ensure_cells(tc_context *ctx, int nc, int keep) {
   return 0;
 }
 
-// Wait for the work queued on s by polling an event (spinning) instead of
-// hipStreamSynchronize, for the short waits inside an evaluation where the
-// host needs a few counts to go on: a blocking wait may sleep until an
-// interrupt, and that wake-up has been measured at hundreds of microseconds
-// on some runs, per wait.
-static hipError_t spin_event(hipEvent_t ev) {
-  for (;;) {
-    const hipError_t e = hipEventQuery(ev);
-    if (e != hipErrorNotReady) return e;
-    __builtin_ia32_pause();
-  }
-}
-
-static hipError_t spin_wait(tc_context *ctx, hipStream_t s) {
-  hipError_t e = hipEventRecord(ctx->ev_wait, s);
-  if (e != hipSuccess) return e;
-  return spin_event(ctx->ev_wait);
-}
-
 // Tree levels are queued in batches without host reads (TreeState on the
 // device holds the level range and counts); after a batch one small copy says
 // whether the last level had no children.  A level whose children do not fit
@@ -807,7 +788,7 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
   if (steps) launch_scatter_ts(b, ctx->mts_sorted.as<double>(), n, tstep, s);
   if (acc_old) launch_scatter_acc_old(b, n, acc_old, s);
   TC_CHECK(hipEventRecord(ctx->ev[7], s));
-  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(spin_wait(ctx, s));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
@@ -883,11 +864,13 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
                      ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
   }
+  const int nslab = master_direct_slabs(n);
+  if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 5 * (size_t)nslab * n));
   TC_CHECK(hipEventRecord(ctx->ev[1], s));
   launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, steps, pair_mass, dt,
-                       pot, acc, tstep, s);
+                       pot, acc, tstep, nslab, ctx->mdpart.as<double>(), s);
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
-  TC_CHECK(hipStreamSynchronize(s));
+  TC_CHECK(spin_wait(ctx, s));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
@@ -974,7 +957,7 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->m2l_src, &ctx->m2l_coff, &ctx->m2l_chunks, &ctx->p2p_cnt, &ctx->p2p_off,
                  &ctx->p2p_cur, &ctx->p2p_src, &ctx->dtg, &ctx->dpot, &ctx->dacc, &ctx->djerk,
                  &ctx->dx3, &ctx->dv3, &ctx->dm, &ctx->dfl, &ctx->hop, &ctx->hoa, &ctx->hoj, &ctx->mlend, &ctx->mts, &ctx->mts_sorted,
-                 &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym,
+                 &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym, &ctx->mdpart,
                  &ctx->fpot, &ctx->fax, &ctx->fay, &ctx->faz, &ctx->fjx, &ctx->fjy, &ctx->fjz,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
                  &ctx->lowF, &ctx->cPn, &ctx->accout, &ctx->lowP};

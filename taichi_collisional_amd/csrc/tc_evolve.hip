@@ -311,7 +311,7 @@ int Evolver::split(int level, Sub s, double dt, Sub &slow, Sub &fast) {
   }
   TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, offs_.as<int>() + s.n, sizeof(int),
                           hipMemcpyDeviceToHost, st));
-  TC_CHECK(hipStreamSynchronize(st));
+  TC_CHECK(spin_wait(ctx, st));
   int nfast = ctx->h_pin[12];
   hipLaunchKernelGGL(k_split_scatter, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
                      flags_.as<int>(), offs_.as<int>(), nfast, out.as<int>());

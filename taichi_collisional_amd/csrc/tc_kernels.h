@@ -194,9 +194,12 @@ void launch_iota(int *p, int n, hipStream_t s);
 void launch_zero_u8(unsigned char *p, int n, hipStream_t s);
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
                       int n, unsigned char *lend, hipStream_t s);
+// Source slabs for launch_master_direct at n bodies (1: no partial sums).
+int master_direct_slabs(int n);
+// part: 5 * nslab * n doubles of partial sums when nslab > 1
 void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
                           bool pair_mass, double dt, double *pot, double *acc, double *tstep,
-                          hipStream_t s);
+                          int nslab, double *part, hipStream_t s);
 // symmetric M2L list: for every accepted (i, j) with i < j and a sink in
 // either cell, (i, j) and (j, i); cnt/off hold npairs + 1 ints
 void launch_sym_count(const int2 *pairs, int npairs, const unsigned char *has_sink, int *cnt,

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 
 #include "tc_common.h"
+#include <algorithm>
+
 #include "tc_kernels.h"
 
 namespace tcfmm {
@@ -106,13 +108,40 @@ __device__ __forceinline__ void pair_tau(double dx, double dy, double dz, double
   }
 }
 
+// Sources may be split into gridDim.y slabs of about `slab` bodies each, so
+// that a small system still fills the GPU.  A slab starts and ends where a
+// source leaf does (lend marks each leaf's last body): the running minimum
+// of the time-step estimate belongs to one leaf, so a leaf is never cut.
+// Each slab writes its partial sums, in sorted order, to part[slab][5][n]
+// (potential, acceleration, time-step sum); k_master_direct_sum adds the
+// slabs in order.  With one slab the results are written directly.
 template <bool kSteps, bool kPairMass>
 __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsigned char *lend,
-                                                       int n, double dt, double *pot, double *acc,
-                                                       double *tstep) {
+                                                       int n, int slab, double dt, double *pot,
+                                                       double *acc, double *tstep,
+                                                       double *__restrict__ part) {
   __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT], svx[kMT], svy[kMT], svz[kMT];
   __shared__ unsigned char sf[kMT];
+  __shared__ int s_range[2];
   const int t = threadIdx.x;
+  int j0 = 0, j1 = n;
+  if (gridDim.y > 1) {
+    if (t == 0) {
+      // both ends move forward to the next leaf start (or n), by the same
+      // rule, so neighbouring slabs meet
+      int a = (int)min((long long)n, (long long)blockIdx.y * slab);
+      int e = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
+      if (kSteps) {
+        while (a < n && a > 0 && !lend[a - 1]) a++;
+        while (e < n && !lend[e - 1]) e++;
+      }
+      s_range[0] = a;
+      s_range[1] = e;
+    }
+    __syncthreads();
+    j0 = s_range[0];
+    j1 = s_range[1];
+  }
   const int i = blockIdx.x * kMT + t;
   const bool live = i < n;
   // lanes past n run the loop on a harmless stand-in body and write nothing
@@ -121,10 +150,10 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
                vzi = live ? b.vz[i] : 0.0;
   const double qi = live ? b.q[i] : 1.0;
   double ax = 0, ay = 0, az = 0, pt = 0, tsum = 0, mi = qi, ts = 1e38;
-  for (int base = 0; base < n; base += kMT) {
+  for (int base = j0; base < j1; base += kMT) {
     __syncthreads();
     const int j = base + t;
-    if (j < n) {
+    if (j < j1) {
       sx[t] = b.x[j];
       sy[t] = b.y[j];
       sz[t] = b.z[j];
@@ -137,7 +166,7 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
       sf[t] = ((b.flags[j] & kSource) ? 1 : 0) | (lend[j] ? 2 : 0);
     }
     __syncthreads();
-    const int cnt = min(kMT, n - base);
+    const int cnt = min(kMT, j1 - base);
     for (int k = 0; k < cnt; k++) {
       const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz;
@@ -164,6 +193,15 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
     }
   }
   if (!live) return;
+  if (gridDim.y > 1) {
+    double *q = part + (size_t)blockIdx.y * 5 * n + i;
+    q[0] = pt;
+    q[(size_t)n] = ax;
+    q[2 * (size_t)n] = ay;
+    q[3 * (size_t)n] = az;
+    q[4 * (size_t)n] = tsum;
+    return;
+  }
   const bool sink = b.flags[i] & kSink;
   const int o = b.order[i];
   pot[o] = sink ? pt : 0.0;
@@ -296,16 +334,51 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
     k_leaf_ends<<<(nleaves + 255) / 256, 256, 0, s>>>(leaf_ids, nleaves, cbody, cnbody, lend);
 }
 
+// The slabs' partial sums of k_master_direct, added in slab order, written
+// to caller order for sinks (zero for the others).
+__global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double *__restrict__ part,
+                                    bool steps, double *pot, double *acc, double *tstep) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int q = 0; q < nslab; q++) {
+    const double *r = part + (size_t)q * 5 * n + i;
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] += r[(size_t)k * n];
+  }
+  const bool sink = b.flags[i] & kSink;
+  const int o = b.order[i];
+  pot[o] = sink ? v[0] : 0.0;
+  acc[3 * o] = sink ? v[1] : 0.0;
+  acc[3 * o + 1] = sink ? v[2] : 0.0;
+  acc[3 * o + 2] = sink ? v[3] : 0.0;
+  if (steps) tstep[o] = sink ? v[4] : 0.0;
+}
+
+int master_direct_slabs(int n) {
+  // about 1024 blocks (four per CU) when n is small, at least 64 sources a
+  // slab, at most 64 slabs; one slab (no partial sums) from 256 K bodies on
+  const int nb = (n + kMT - 1) / kMT;
+  const int want = (1024 + nb - 1) / nb;
+  return std::max(1, std::min({want, (n + 63) / 64, 64}));
+}
+
 void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
                           bool pair_mass, double dt, double *pot, double *acc, double *tstep,
-                          hipStream_t s) {
+                          int nslab, double *part, hipStream_t s) {
   const int nb = (n + kMT - 1) / kMT;
+  if (nslab < 1 || !part) nslab = 1;
+  const int slab = (n + nslab - 1) / nslab;
+  const dim3 grid(nb, nslab);
   if (!steps)
-    k_master_direct<false, false><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+    k_master_direct<false, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
   else if (!pair_mass)
-    k_master_direct<true, false><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+    k_master_direct<true, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
   else
-    k_master_direct<true, true><<<nb, kMT, 0, s>>>(b, lend, n, dt, pot, acc, tstep);
+    k_master_direct<true, true><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
+  if (nslab > 1)
+    k_master_direct_sum<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, steps, pot, acc,
+                                                        tstep);
 }
 
 }  // namespace tcfmm
