@@ -24,6 +24,7 @@
 
 #include "tc_common.h"
 #include <algorithm>
+#include <climits>
 
 #include "tc_kernels.h"
 
@@ -108,6 +109,24 @@ __device__ __forceinline__ void pair_tau(double dx, double dy, double dz, double
   }
 }
 
+// The smallest k >= j that starts a source leaf (k == 0, or lend[k - 1] set),
+// or n.  The whole block searches kMT candidates per round (a leaf spans at
+// most ncrit bodies, so a round or two); every thread returns the same value.
+__device__ int next_leaf_start(const unsigned char *__restrict__ lend, int n, int j) {
+  __shared__ int s_min;
+  const int t = threadIdx.x;
+  for (int base = j;; base += kMT) {
+    if (t == 0) s_min = INT_MAX;
+    __syncthreads();
+    const int k = base + t;
+    if (k >= n || k == 0 || lend[k - 1]) atomicMin(&s_min, k);
+    __syncthreads();
+    const int r = s_min;
+    __syncthreads();  // everyone has read s_min before the next round resets it
+    if (r != INT_MAX) return r;
+  }
+}
+
 // Sources may be split into gridDim.y slabs of about `slab` bodies each, so
 // that a small system still fills the GPU.  A slab starts and ends where a
 // source leaf does (lend marks each leaf's last body): the running minimum
@@ -122,25 +141,17 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
                                                        double *__restrict__ part) {
   __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT], svx[kMT], svy[kMT], svz[kMT];
   __shared__ unsigned char sf[kMT];
-  __shared__ int s_range[2];
   const int t = threadIdx.x;
   int j0 = 0, j1 = n;
   if (gridDim.y > 1) {
-    if (t == 0) {
-      // both ends move forward to the next leaf start (or n), by the same
-      // rule, so neighbouring slabs meet
-      int a = (int)min((long long)n, (long long)blockIdx.y * slab);
-      int e = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
-      if (kSteps) {
-        while (a < n && a > 0 && !lend[a - 1]) a++;
-        while (e < n && !lend[e - 1]) e++;
-      }
-      s_range[0] = a;
-      s_range[1] = e;
+    // both ends move forward to the next leaf start (or n), by the same rule,
+    // so neighbouring slabs meet
+    j0 = (int)min((long long)n, (long long)blockIdx.y * slab);
+    j1 = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
+    if (kSteps) {
+      j0 = next_leaf_start(lend, n, j0);
+      j1 = next_leaf_start(lend, n, j1);
     }
-    __syncthreads();
-    j0 = s_range[0];
-    j1 = s_range[1];
   }
   const int i = blockIdx.x * kMT + t;
   const bool live = i < n;
@@ -356,10 +367,10 @@ __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double 
 }
 
 int master_direct_slabs(int n) {
-  // about 1024 blocks (four per CU) when n is small, at least 64 sources a
-  // slab, at most 64 slabs; one slab (no partial sums) from 256 K bodies on
+  // about 2048 blocks (eight per CU) when n is small, at least 64 sources a
+  // slab, at most 64 slabs; one slab (no partial sums) from 512 K bodies on
   const int nb = (n + kMT - 1) / kMT;
-  const int want = (1024 + nb - 1) / nb;
+  const int want = (2048 + nb - 1) / nb;
   return std::max(1, std::min({want, (n + 63) / 64, 64}));
 }
 
