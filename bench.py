@@ -156,6 +156,34 @@ def load_traffic(path):
         return None
 
 
+def master_evolve_row(tca, ics, torch, dev, p, seed):
+    """One timed do_evolve of the master step loop (GRADIENT scheme) on a
+    Plummer N = 4096, dt = 1/1024, after one untimed warm-up on a copy."""
+    ne, dte = 4096, 1.0 / 1024
+    xe, ve, me = ics.plummer(ne, seed=seed)
+    est = {k: torch.zeros(ne, 3, dtype=torch.float64, device=dev) for k in ("pos_e", "acc")}
+    est.update({k: torch.zeros(ne, dtype=torch.float64, device=dev)
+                for k in ("timestep", "pot", "acc_old")})
+    est["pos"], est["vel"], est["mass"] = (torch.from_numpy(a).to(dev) for a in (xe, ve, me))
+    fe = tca.Fmm(p=p, ncrit=300, direct_below=0)
+    warm = {k: a.clone() for k, a in est.items()}
+    fe.master_evolve(warm, dte)  # warm-up on a copy (first launches, allocations)
+    del warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fe.master_evolve(est, dte)
+    torch.cuda.synchronize()
+    ems = (time.perf_counter() - t) * 1e3
+    tau = float(est["timestep"][0])
+    levels = 0
+    while dte / 2 ** levels > tau:
+        levels += 1
+    fe.close()
+    return {"what": "master step loop: one do_evolve(sys, dt) of the GRADIENT scheme, "
+                    "Plummer N=4096, dt=1/1024, dt_param 0.025, default-build forces",
+            "ms_per_step": ems, "levels": levels, "substeps": 2 ** levels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,12 +204,22 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once (a plain copy kernel "
                          "of known size for scaling HBM byte counters)")
+    ap.add_argument("--only-master-evolve", action="store_true",
+                    help="time only the master step loop row (for profiling it) and print it")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.only_master_evolve:
+        import torch
+        import taichi_collisional_amd as tca
+        from taichi_collisional_amd import ics
+        print(json.dumps({"master_evolve": master_evolve_row(
+            tca, ics, torch, torch.device("cuda:0"), args.p, args.seed)}), flush=True)
+        return
 
     cpu = None
     if not args.no_cpu:
@@ -302,35 +340,12 @@ def main():
                 row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (mms * 1e-3)
         extra["master_direct"] = row
         # the master step loop: one do_evolve (GRADIENT scheme) on N = 4096
-        ne, dte = 4096, 1.0 / 1024
-        xe, ve, me = ics.plummer(ne, seed=args.seed)
-        est = {k: torch.zeros(ne, 3, dtype=torch.float64, device=dev)
-              for k in ("pos_e", "acc")}
-        est.update({k: torch.zeros(ne, dtype=torch.float64, device=dev)
-                   for k in ("timestep", "pot", "acc_old")})
-        est["pos"], est["vel"], est["mass"] = (torch.from_numpy(a).to(dev) for a in (xe, ve, me))
-        fe = tca.Fmm(p=args.p, ncrit=300, direct_below=0)
-        warm = {k: a.clone() for k, a in est.items()}
-        fe.master_evolve(warm, dte)  # warm-up on a copy (first launches, allocations)
-        del warm
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        fe.master_evolve(est, dte)
-        torch.cuda.synchronize()
-        ems = (time.perf_counter() - t) * 1e3
-        tau = float(est["timestep"][0])
-        levels = 0
-        while dte / 2 ** levels > tau:
-            levels += 1
-        row = {"what": "master step loop: one do_evolve(sys, dt) of the GRADIENT scheme, "
-                       "Plummer N=4096, dt=1/1024, dt_param 0.025, default-build forces",
-               "ms_per_step": ems, "levels": levels, "substeps": 2 ** levels}
-        del fe
+        row = master_evolve_row(tca, ics, torch, dev, args.p, args.seed)
         if cpu is not None:
-            row.update(master_evolve_cpu_baseline(ne, dte, args.seed, args.cpu_threads,
+            row.update(master_evolve_cpu_baseline(4096, 1.0 / 1024, args.seed, args.cpu_threads,
                                                   args.cpu_timeout))
             if row.get("cpu_reference_seconds"):
-                row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (ems * 1e-3)
+                row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (row["ms_per_step"] * 1e-3)
         extra["master_evolve"] = row
 
     if args.calibrate_copy_mib > 0:
