@@ -149,11 +149,13 @@ print(json.dumps({"cpu_reference_seconds": time.time() - t}))
 
 
 def load_traffic(path):
+    """HBM bytes and executed FP64 flops per k_m2l launch, from the committed
+    rocprofv3 counter summary (None where absent)."""
     try:
         d = json.load(open(path))
-        return d.get("hbm_bytes_per_launch")
+        return d.get("hbm_bytes_per_launch"), d.get("hw_fp64_flops_per_launch")
     except (OSError, ValueError):
-        return None
+        return None, None
 
 
 def master_evolve_row(tca, ics, torch, dev, p, seed):
@@ -354,7 +356,7 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-    traffic = load_traffic(args.traffic_file)
+    traffic, hw_flops = load_traffic(args.traffic_file)
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -365,11 +367,17 @@ def main():
                                                              args.theta, args.ncrit),
                    "n": args.n, "p": args.p, "theta": args.theta, "ncrit": args.ncrit,
                    "mac": "theta", "precision": "fp64"},
-        "roofline": {"bound": "mfma", "achieved": m2l_tflops, "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "fp64-valu", "achieved": m2l_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": m2l_tflops / PEAK_FP64_TFLOPS,
                      "traffic": traffic, "kernel": "k_m2l (M2L, FP64)",
                      "flops_per_launch": fl["M2L"], "ms_per_launch": m2l_ms,
-                     "note": "FP64 peak 78.6 TFLOP/s is the same for vector and matrix units"},
+                     "hw_fp64_flops_per_launch": hw_flops,
+                     "hw_frac": (hw_flops / (m2l_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+                                 if hw_flops else None),
+                     "note": "k_m2l issues no MFMA: it is bound by FP64 VALU issue "
+                             "(v_fma_f64; 78.6 TFLOP/s vector peak). frac counts the "
+                             "reference's algorithmic flops; hw_frac the FP64 flops the "
+                             "kernel executes (SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 counters)"},
         "cpu_baseline": cpu,
         "error_vs_direct": err,
         "phases_ms": phases,

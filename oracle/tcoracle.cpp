@@ -36,6 +36,20 @@ typedef std::complex<double> cplx;
 
 static inline int ix(int n, int m) { return n * n + n + m; }
 
+// The walks' squared distance norm(dX) (kernel_fast_lean.h:113-116, called by
+// every MAC in traverse_eager.h) as the reference's x86-64 build evaluates it:
+// GCC at -O3 contracts x*x + y*y + z*z into fma(z, z, fma(x, x, y*y)).  The
+// p = 2 pre-pass compares it with 3(Ri+Rj)^2 on grid-aligned cells, where
+// diagonal neighbours tie in exact arithmetic and this rounding decides them.
+static inline real mac_norm(const real *d) {
+  return std::fma(d[2], d[2], std::fma(d[0], d[0], d[1] * d[1]));
+}
+// One term of the error-MAC sums (traverse_eager.h:80-88; master :82-90), as
+// the same build contracts it: E = fma(Pn * power, comb, E).
+static inline real mac_term(real E, real pn, real pw, real comb) {
+  return std::fma(pn * pw, comb, E);
+}
+
 // ---------------------------------------------------------------- tables ----
 // initKernel (kernel_fast_lean.h:124-162).  With quirk=1 the factorial table
 // is filled only below index 2p, as the reference does (:129), so fact[2p]=0.
@@ -634,7 +648,7 @@ struct Fmm {
   void trav_theta(int i, int j) {
     Cell &ci = C[i], &cj = C[j];
     real dx[3] = {ci.X[0] - cj.X[0], ci.X[1] - cj.X[1], ci.X[2] - cj.X[2]};
-    real R2 = (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]) * prm.theta * prm.theta;
+    real R2 = mac_norm(dx) * prm.theta * prm.theta;
     if (R2 > (ci.R + cj.R) * (ci.R + cj.R)) {
       m2l.push_back({i, j});
     } else if (ci.NCHILD == 0 && cj.NCHILD == 0) {
@@ -651,7 +665,7 @@ struct Fmm {
     Cell &ci = C[i], &cj = C[j];
     int P = prm.p;
     real dx[3] = {ci.X[0] - cj.X[0], ci.X[1] - cj.X[1], ci.X[2] - cj.X[2]};
-    real R2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    real R2 = mac_norm(dx);
     real f2 = 10;
     real Ra = ci.R, Rb = cj.R, s = Ra + Rb;
     if (R2 * 0.4 > s * s) {
@@ -659,7 +673,7 @@ struct Fmm {
       real Rp = std::pow(R, -(P + 1));
       real pa = 1, E = 0;
       for (int k = 0; k <= P; k++) {
-        E += cj.Pn[P - k] * pa * tab.comb[k];
+        E = mac_term(E, cj.Pn[P - k], pa, tab.comb[k]);
         pa *= Ra;
       }
       E *= 8 * std::fmax(Ra, Rb) * Rp;
@@ -1015,11 +1029,9 @@ struct Fmm {
       Cell &a = C[pr.first], &b = C[pr.second];
       real dx[3] = {a.Xg[0] - b.Xg[0], a.Xg[1] - b.Xg[1], a.Xg[2] - b.Xg[2]};
       // Cells of the tree sit on a grid, so diagonal neighbours tie with this
-      // threshold in exact arithmetic and the rounding of R2 decides them; the
-      // reference's compiled rounding is not reproduced bit for bit, so a few
-      // percent of pre-pass pairs may be split differently (acc_old then agrees
-      // to the pre-pass accuracy, not to round-off).
-      real R2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+      // threshold in exact arithmetic and the rounding of R2 decides them:
+      // mac_norm reproduces the reference build's FMA contraction.
+      real R2 = mac_norm(dx);
       if (R2 > (a.Rg + b.Rg) * (a.Rg + b.Rg) * 3) {
         lm.push_back(pr);
       } else if (a.NCHILD == 0 && b.NCHILD == 0) {
@@ -1267,14 +1279,14 @@ struct Fmm {
     Cell &ci = C[i], &cj = C[j];
     int P = prm.p;
     real dx[3] = {ci.X[0] - cj.X[0], ci.X[1] - cj.X[1], ci.X[2] - cj.X[2]};
-    real R2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    real R2 = mac_norm(dx);
     real f1 = 10, f2 = 10;
     real Ra = ci.R, Rb = cj.R, s = Ra + Rb;
     if (R2 > s * s) {
       real R = std::sqrt(R2), Rp = R2, pa = 1, pb = 1, Eab = 0, Eba = 0;
       for (int k = 0; k <= P; k++) {
-        Eab += ci.Pn[P - k] * pb * tab.comb[k];
-        Eba += cj.Pn[P - k] * pa * tab.comb[k];
+        Eab = mac_term(Eab, ci.Pn[P - k], pb, tab.comb[k]);
+        Eba = mac_term(Eba, cj.Pn[P - k], pa, tab.comb[k]);
         pa *= Ra;
         pb *= Rb;
         Rp *= R;

@@ -190,6 +190,36 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def check_tensors(specs, n, device_type="cuda"):
+    """Validate the device tensors an entry point hands to the kernels as raw
+    pointers.  specs: (name, tensor, dtype, trailing shape) with shape (3,) for
+    an (n, 3) array and () for an (n,) array; tensor None is skipped.  Every
+    tensor must be a contiguous `device_type` tensor of that dtype and shape,
+    all on one device.  Raises TypeError (type, dtype, device, layout) or
+    ValueError (shape)."""
+    import torch
+    dev = None
+    for name, t, dtype, tail in specs:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("%s must be a torch tensor" % name)
+        if t.device.type != device_type:
+            raise TypeError("%s must be a %s tensor (got %s)" % (name, device_type, t.device))
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise TypeError("%s is on %s, the other arrays on %s" % (name, t.device, dev))
+        if t.dtype != dtype:
+            raise TypeError("%s must be %s (got %s)" % (name, dtype, t.dtype))
+        if not t.is_contiguous():
+            raise TypeError("%s must be contiguous" % name)
+        if tuple(t.shape) != (n,) + tuple(tail):
+            raise ValueError("%s must have shape %s (got %s)" % (name, (n,) + tuple(tail),
+                                                                 tuple(t.shape)))
+    return dev
+
+
 class Fmm:
     """One FMM configuration on the current GPU (see include/tcfmm.h)."""
 
@@ -245,10 +275,10 @@ class Fmm:
         """Device tensors: x, v (n,3) float64, m (n,) float64, flags (n,) uint8."""
         import torch
         n = m.shape[0]
-        dev = x.device
-        for t in (x, v, m):
-            if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
-                raise TypeError("x, v, m must be contiguous float64 cuda tensors")
+        f8, u1 = torch.float64, torch.uint8
+        dev = check_tensors([("x", x, f8, (3,)), ("v", v, f8, (3,)), ("m", m, f8, ()),
+                             ("flags", flags, u1, ()), ("pot", pot, f8, ()),
+                             ("acc", acc, f8, (3,)), ("jerk", jerk, f8, (3,))], n)
         if pot is None:
             pot = torch.empty(n, dtype=torch.float64, device=dev)
         if acc is None:
@@ -275,7 +305,9 @@ class Fmm:
         """FP64 all-pairs potential/acceleration/jerk for the given target indices."""
         import torch
         nt = targets.shape[0]
-        dev = x.device
+        f8 = torch.float64
+        dev = check_tensors([("x", x, f8, (3,)), ("v", v, f8, (3,)), ("m", m, f8, ()),
+                             ("flags", flags, torch.uint8, ())], m.shape[0])
         tg = targets.to(device=dev, dtype=torch.int32).contiguous()
         pot = torch.empty(nt, dtype=torch.float64, device=dev)
         acc = torch.empty(nt, 3, dtype=torch.float64, device=dev)
@@ -303,10 +335,9 @@ class Fmm:
         acc_old (n,) cuda float64, TC_MASTER_FMM only: read, then overwritten."""
         import torch
         n = m.shape[0]
-        dev = x.device
-        for t in (x, v, m):
-            if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
-                raise TypeError("x, v, m must be contiguous float64 cuda tensors")
+        f8 = torch.float64
+        dev = check_tensors([("x", x, f8, (3,)), ("v", v, f8, (3,)), ("m", m, f8, ()),
+                             ("flags", flags, torch.uint8, ()), ("acc_old", acc_old, f8, ())], n)
         mode = int(mode) | (TC_MASTER_STEPS if get_steps else 0)
         pot = torch.empty(n, dtype=torch.float64, device=dev)
         acc = torch.empty(n, 3, dtype=torch.float64, device=dev)
@@ -322,9 +353,15 @@ class Fmm:
         device tensors: state is a dict with pos, pos_e, vel, acc (n,3), mass,
         timestep, pot, acc_old (n,) float64 cuda tensors, updated in place.
         Returns the new simulation time."""
+        import torch
+        keys = ("pos", "pos_e", "vel", "mass", "timestep", "acc", "pot", "acc_old")
+        missing = [k for k in keys if state.get(k) is None]
+        if missing:
+            raise TypeError("state lacks %s" % ", ".join(missing))
         n = state["mass"].shape[0]
-        sp = TcParticles(n, *[state[k].data_ptr() for k in (
-            "pos", "pos_e", "vel", "mass", "timestep", "acc", "pot", "acc_old")])
+        vec = ("pos", "pos_e", "vel", "acc")
+        check_tensors([(k, state[k], torch.float64, (3,) if k in vec else ()) for k in keys], n)
+        sp = TcParticles(n, *[state[k].data_ptr() for k in keys])
         t = C.c_double(simtime)
         self._check(lib().tc_master_evolve(self.h, C.byref(sp), C.byref(t), float(dt), int(mode),
                                            float(dt_param)), "tc_master_evolve")

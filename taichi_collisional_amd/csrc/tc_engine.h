@@ -145,6 +145,14 @@ struct tc_context {
   // runs on `stream`; its far-field results go to a second result set
   hipStream_t near_stream = nullptr;
   DBuf fpot, fax, fay, faz, fjx, fjy, fjz;
+  // the step loop (tc_evolve.hip): one index buffer per time-step level, the
+  // identity list, the gathered union of sinks and sources and its results,
+  // and the split's flags and scan.  Kept between calls: released by
+  // tc_destroy.
+  std::vector<DBuf> ev_lev;
+  DBuf ev_all, ev_ux, ev_uv, ev_um, ev_ufl, ev_upot, ev_uacc, ev_uts, ev_uao, ev_tmp,
+      ev_flags, ev_offs, ev_scan_tmp;
+  int ev_all_n = 0;  // entries of the identity list written so far
 };
 
 #define TC_CHECK(expr)                                                   \

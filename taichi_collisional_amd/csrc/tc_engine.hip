@@ -960,8 +960,12 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->msym_cnt, &ctx->msym_off, &ctx->msym_tmp, &ctx->m2l_sym, &ctx->mdpart,
                  &ctx->fpot, &ctx->fax, &ctx->fay, &ctx->faz, &ctx->fjx, &ctx->fjy, &ctx->fjz,
                  &ctx->p2p_ranges, &ctx->npairs, &ctx->p2p_items, &ctx->lowM, &ctx->lowL,
-                 &ctx->lowF, &ctx->cPn, &ctx->accout, &ctx->lowP};
+                 &ctx->lowF, &ctx->cPn, &ctx->accout, &ctx->lowP, &ctx->ev_all, &ctx->ev_ux,
+                 &ctx->ev_uv, &ctx->ev_um, &ctx->ev_ufl, &ctx->ev_upot, &ctx->ev_uacc,
+                 &ctx->ev_uts, &ctx->ev_uao, &ctx->ev_tmp, &ctx->ev_flags, &ctx->ev_offs,
+                 &ctx->ev_scan_tmp};
   for (DBuf *b : all) b->release();
+  for (DBuf &b : ctx->ev_lev) b.release();
   ctx->h_leaves.release();
   ctx->h_nbd.release();
   ctx->h_tree.release();

@@ -136,6 +136,11 @@ __global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *_
   out[pos] = is[k];
 }
 
+__global__ void k_iota(int *__restrict__ out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = k;
+}
+
 inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
@@ -146,17 +151,16 @@ struct Sub {
   int n = 0;
 };
 
+// The scratch buffers live in the context (grown on demand, released by
+// tc_destroy), so repeated calls allocate nothing once they have grown.
 class Evolver {
  public:
   Evolver(tc_context *ctx, const tc_particles &p, int mode, double dt_param)
-      : ctx_(ctx), p_(p), mode_(mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS)), dtp_(dt_param) {}
-  ~Evolver() {
-    hipStreamSynchronize(ctx_->stream);
-    for (DBuf &b : lev_) b.release();
-    for (DBuf *b : {&all_, &ux_, &uv_, &um_, &ufl_, &upot_, &uacc_, &uts_, &uao_, &tmp_,
-                    &flags_, &offs_, &scan_tmp_})
-      b->release();
-  }
+      : ctx_(ctx), p_(p), mode_(mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS)), dtp_(dt_param),
+        lev_(ctx->ev_lev), all_(ctx->ev_all), ux_(ctx->ev_ux), uv_(ctx->ev_uv), um_(ctx->ev_um),
+        ufl_(ctx->ev_ufl), upot_(ctx->ev_upot), uacc_(ctx->ev_uacc), uts_(ctx->ev_uts),
+        uao_(ctx->ev_uao), tmp_(ctx->ev_tmp), flags_(ctx->ev_flags), offs_(ctx->ev_offs),
+        scan_tmp_(ctx->ev_scan_tmp) {}
 
   int run(double *simtime, double dt);
 
@@ -168,12 +172,14 @@ class Evolver {
   double simtime_ = 0;
   // index buffers: one per recursion level holds that level's subsystem
   // partitioned into [fast | slow]
-  std::vector<DBuf> lev_;
-  DBuf all_, ux_, uv_, um_, ufl_, upot_, uacc_, uts_, uao_, tmp_, flags_, offs_, scan_tmp_;
+  std::vector<DBuf> &lev_;
+  DBuf &all_, &ux_, &uv_, &um_, &ufl_, &upot_, &uacc_, &uts_, &uao_, &tmp_, &flags_, &offs_,
+      &scan_tmp_;
 
   int ensure_union(int nu);
+  // exact: drop TC_MASTER_FMM, so that the sum is the all-pairs one
   int force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
-            bool with_acc_old);
+            bool with_acc_old, bool exact = false);
   int kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
   int kick_cpu(Sub s1, Sub s2, double dt);
   int kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
@@ -203,7 +209,7 @@ int Evolver::ensure_union(int nu) {
 // 383-424).  Results for the sinks are the first sinks.n entries of upot_ /
 // uacc_ (and uts_ with steps).
 int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
-                   bool with_acc_old) {
+                   bool with_acc_old, bool exact) {
   tc_context *ctx = ctx_;
   const int nu = sinks.n + sources.n;
   if (nu == 0) return 0;
@@ -220,14 +226,15 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
                      sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass, p_.acc,
                      shift_fac, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
                      ufl_.as<unsigned char>());
+  const int mode = exact ? (mode_ & ~TC_MASTER_FMM) : mode_;
   double *ao = nullptr;
-  if (with_acc_old && (mode_ & TC_MASTER_FMM)) {
+  if (with_acc_old && (mode & TC_MASTER_FMM)) {
     // kick_naive builds fresh bodies: the pre-pass starts from 0
     ao = uao_.as<double>();
     TC_CHECK(hipMemsetAsync(ao, 0, 8 * (size_t)nu, s));
   }
   return master_eval_device(ctx, nu, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
-                            ufl_.as<unsigned char>(), mode_ | (steps ? TC_MASTER_STEPS : 0), dtp_,
+                            ufl_.as<unsigned char>(), mode | (steps ? TC_MASTER_STEPS : 0), dtp_,
                             upot_.as<double>(), uacc_.as<double>(),
                             steps ? uts_.as<double>() : nullptr, ao);
 }
@@ -244,11 +251,13 @@ int Evolver::kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast) {
 
 // kick_cpu (integrator.h:65-98): the all-pairs kick of s1 by s2 the
 // reference uses for subsystems of <= 128 bodies; the potential is not set.
+// It is an exact sum in every build, so the force call drops TC_MASTER_FMM
+// (s1.n + s2.n may reach 256 bodies, above ncrit).
 int Evolver::kick_cpu(Sub s1, Sub s2, double dt) {
   if (s1.n == 0) return 0;
   // s2 may contain s1 (join(slow, fast)); r = 0 pairs are skipped either way,
   // so s1 as sinks that are not sources plus s2 as sources is the same sum
-  if (int e = force(s1, s2, false, 0.0, false, false)) return e;
+  if (int e = force(s1, s2, false, 0.0, false, false, true)) return e;
   hipLaunchKernelGGL(k_apply_kick, grid_for(s1.n), dim3(256), 0, ctx_->stream, s1.idx, s1.n,
                      uacc_.as<double>(), upot_.as<double>(), dt, false, false, p_.vel, p_.pot);
   return 0;
@@ -372,11 +381,11 @@ int Evolver::run(double *simtime, double dt) {
   simtime_ = *simtime;
   if (dt == 0) return 0;
   const int n = p_.n;
-  TC_CHECK(all_.ensure(sizeof(int) * (size_t)n));
-  std::vector<int> id(n);
-  for (int i = 0; i < n; i++) id[i] = i;
-  TC_CHECK(hipMemcpyAsync(all_.p, id.data(), sizeof(int) * n, hipMemcpyHostToDevice,
-                          ctx->stream));
+  if (n > ctx->ev_all_n) {
+    TC_CHECK(all_.ensure(sizeof(int) * (size_t)n));
+    hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, ctx->stream, all_.as<int>(), n);
+    ctx->ev_all_n = n;
+  }
   const Sub all{all_.as<int>(), n};
   if (int e = kick_naive(all)) return e;
   if (int e = omelyan(0, all, simtime_, simtime_ + dt, dt, true)) return e;

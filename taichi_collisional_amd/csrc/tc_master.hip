@@ -131,9 +131,10 @@ __device__ int next_leaf_start(const unsigned char *__restrict__ lend, int n, in
 // that a small system still fills the GPU.  A slab starts and ends where a
 // source leaf does (lend marks each leaf's last body): the running minimum
 // of the time-step estimate belongs to one leaf, so a leaf is never cut.
-// Each slab writes its partial sums, in sorted order, to part[slab][5][n]
-// (potential, acceleration, time-step sum); k_master_direct_sum adds the
-// slabs in order.  With one slab the results are written directly.
+// Each slab writes its partial sums, in sorted order, to part[slab][nc][n]
+// (potential, acceleration and, with steps, the time-step sum: nc = 4 or 5);
+// k_master_direct_sum adds the slabs in order.  With one slab the results
+// are written directly.
 template <bool kSteps, bool kPairMass>
 __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsigned char *lend,
                                                        int n, int slab, double dt, double *pot,
@@ -205,12 +206,13 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
   }
   if (!live) return;
   if (gridDim.y > 1) {
-    double *q = part + (size_t)blockIdx.y * 5 * n + i;
+    constexpr int nc = kSteps ? 5 : 4;
+    double *q = part + (size_t)blockIdx.y * nc * n + i;
     q[0] = pt;
     q[(size_t)n] = ax;
     q[2 * (size_t)n] = ay;
     q[3 * (size_t)n] = az;
-    q[4 * (size_t)n] = tsum;
+    if (kSteps) q[4 * (size_t)n] = tsum;
     return;
   }
   const bool sink = b.flags[i] & kSink;
@@ -347,15 +349,19 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
 
 // The slabs' partial sums of k_master_direct, added in slab order, written
 // to caller order for sinks (zero for the others).
+template <bool kSteps>
 __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double *__restrict__ part,
-                                    bool steps, double *pot, double *acc, double *tstep) {
+                                    double *pot, double *acc, double *tstep) {
+  constexpr int nc = kSteps ? 5 : 4;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double v[5] = {0, 0, 0, 0, 0};
-  for (int q = 0; q < nslab; q++) {
-    const double *r = part + (size_t)q * 5 * n + i;
+  double v[nc];
 #pragma unroll
-    for (int k = 0; k < 5; k++) v[k] += r[(size_t)k * n];
+  for (int k = 0; k < nc; k++) v[k] = 0;
+  for (int q = 0; q < nslab; q++) {
+    const double *r = part + (size_t)q * nc * n + i;
+#pragma unroll
+    for (int k = 0; k < nc; k++) v[k] += r[(size_t)k * n];
   }
   const bool sink = b.flags[i] & kSink;
   const int o = b.order[i];
@@ -363,15 +369,17 @@ __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double 
   acc[3 * o] = sink ? v[1] : 0.0;
   acc[3 * o + 1] = sink ? v[2] : 0.0;
   acc[3 * o + 2] = sink ? v[3] : 0.0;
-  if (steps) tstep[o] = sink ? v[4] : 0.0;
+  if (kSteps) tstep[o] = sink ? v[nc - 1] : 0.0;
 }
 
 int master_direct_slabs(int n) {
-  // about 2048 blocks (eight per CU) when n is small, at least 64 sources a
-  // slab, at most 64 slabs; one slab (no partial sums) from 512 K bodies on
+  // about 1024 blocks (four per CU) when n is small, at least 64 sources a
+  // slab, at most 32 slabs, so that the partial sums (32 x 5 x n doubles
+  // written and read back) stay well below the force kernel's own time; one
+  // slab (no partial sums) from 256 K bodies on
   const int nb = (n + kMT - 1) / kMT;
-  const int want = (2048 + nb - 1) / nb;
-  return std::max(1, std::min({want, (n + 63) / 64, 64}));
+  const int want = (1024 + nb - 1) / nb;
+  return std::max(1, std::min({want, (n + 63) / 64, 32}));
 }
 
 void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
@@ -387,9 +395,12 @@ void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, 
     k_master_direct<true, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
   else
     k_master_direct<true, true><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
-  if (nslab > 1)
-    k_master_direct_sum<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, steps, pot, acc,
-                                                        tstep);
+  if (nslab > 1) {
+    if (steps)
+      k_master_direct_sum<true><<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc, tstep);
+    else
+      k_master_direct_sum<false><<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc, tstep);
+  }
 }
 
 }  // namespace tcfmm

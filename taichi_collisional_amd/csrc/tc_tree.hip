@@ -451,6 +451,20 @@ int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, h
 // per-block totals (M2L, P2P, child pairs); the totals are scanned over
 // blocks; pass 2 writes every output at its scanned position, so the lists
 // come out in frontier order (deterministic).
+// The walks' squared distance norm(dX) (kernel_fast_lean.h:113-116) and one
+// error-MAC term (traverse_eager.h:80-88, master :82-90) rounded as the
+// reference's x86-64 build evaluates them: GCC -O3 contracts the sum of
+// squares into fma(z, z, fma(x, x, y*y)) and the sums into
+// E = fma(Pn * power, comb, E).  The p = 2 pre-pass compares the distance
+// with 3(Ri+Rj)^2 on grid-aligned cells, where diagonal neighbours tie in
+// exact arithmetic and this rounding decides them.
+__device__ __forceinline__ double mac_norm(double dx, double dy, double dz) {
+  return __builtin_fma(dz, dz, __builtin_fma(dx, dx, dy * dy));
+}
+__device__ __forceinline__ double mac_term(double E, double pn, double pw, double comb) {
+  return __builtin_fma(pn * pw, comb, E);
+}
+
 __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a, int i, int j,
                                              int *nsplit) {
 #pragma clang fp contract(off)
@@ -461,7 +475,7 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
     const double dx = c.Xg[3 * i] - c.Xg[3 * j];
     const double dy = c.Xg[3 * i + 1] - c.Xg[3 * j + 1];
     const double dz = c.Xg[3 * i + 2] - c.Xg[3 * j + 2];
-    const double R2 = dx * dx + dy * dy + dz * dz;
+    const double R2 = mac_norm(dx, dy, dz);
     Ri = c.Rg[i];
     Rj = c.Rg[j];
     accept = R2 > (Ri + Rj) * (Ri + Rj) * 3;
@@ -472,10 +486,10 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
     Ri = c.R[i];
     Rj = c.R[j];
     if (a.mode == kMacTheta) {
-      const double R2 = (dx * dx + dy * dy + dz * dz) * a.theta * a.theta;
+      const double R2 = mac_norm(dx, dy, dz) * a.theta * a.theta;
       accept = R2 > (Ri + Rj) * (Ri + Rj);
     } else if (a.mode == kMacMaster) {
-      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double R2 = mac_norm(dx, dy, dz);
       const double s = Ri + Rj;
       double f1 = 10, f2 = 10;
       if (R2 > s * s) {
@@ -484,8 +498,8 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
         const double *Pi = a.Pn + (size_t)i * (P + 1), *Pj = a.Pn + (size_t)j * (P + 1);
         double Rp = R2, pa = 1, pb = 1, Eab = 0, Eba = 0;
         for (int k = 0; k <= P; k++) {
-          Eab += Pi[P - k] * pb * a.comb[k];
-          Eba += Pj[P - k] * pa * a.comb[k];
+          Eab = mac_term(Eab, Pi[P - k], pb, a.comb[k]);
+          Eba = mac_term(Eba, Pj[P - k], pa, a.comb[k]);
           pa *= Ri;
           pb *= Rj;
           Rp *= R;
@@ -504,7 +518,7 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
       }
       accept = false;
     } else {
-      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double R2 = mac_norm(dx, dy, dz);
       const double s = Ri + Rj;
       double f2 = 10;
       if (R2 * 0.4 > s * s) {
@@ -514,7 +528,7 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
         const double *Pn = a.Pn + (size_t)j * (P + 1);
         double pa = 1, E = 0;
         for (int k = 0; k <= P; k++) {
-          E += Pn[P - k] * pa * a.comb[k];
+          E = mac_term(E, Pn[P - k], pa, a.comb[k]);
           pa *= Ri;
         }
         E *= 8 * fmax(Ri, Rj) * Rp;

@@ -100,6 +100,26 @@ def test_gpu_step_loop_fmm_forces(torch_cuda, oracle_lib):
     assert np.all(out["acc_old"] > 0)
 
 
+def test_gpu_step_loop_fmm_kick_cpu_above_ncrit(torch_cuda, oracle_lib):
+    """kick_cpu (integrator.h:65-98) is an exact all-pairs sum in every build.
+    With ncrit = 16 and N = 150, the subsystems of <= 128 bodies take the
+    kick_cpu branch with up to 256 bodies in one force call, above ncrit: the
+    FMM build must still use the exact sum there, so the loop agrees with the
+    oracle's (whose kick_cpu is the exact sum) to round-off."""
+    import taichi_collisional_amd as tca
+    x, v, m = mmg.ics.plummer(150, seed=84)
+    out, t = _run(torch_cuda, x, v, m, 2, 1.0 / 64, ncrit=16, mode=tca.TC_MASTER_FMM)
+    o = oracle_lib.master_evolve(m, x, v, 2, 1.0 / 64, ncrit=16, fmm=True)
+    dp, dv = np.abs(out["pos"] - o["pos"]).max(), np.abs(out["vel"] - o["vel"]).max()
+    print("kick_cpu above ncrit: dpos %.2e dvel %.2e levels %s" %
+          (dp, dv, np.unique(np.round(np.log2(1.0 / 64 / out["timestep"])))))
+    assert t == o["simtime"]
+    np.testing.assert_allclose(out["timestep"], o["timestep"], rtol=1e-11)
+    # some bodies must sit below the top level, or kick_cpu never runs
+    assert np.any(out["timestep"] < 1.0 / 64)
+    assert dp < 1e-13 and dv < 1e-12
+
+
 def test_gpu_step_loop_edges(torch_cuda):
     import taichi_collisional_amd as tca
     x, v, m = mmg.ics.plummer(50, seed=83)

@@ -190,8 +190,9 @@ def test_gpu_master_fmm_matches_oracle(torch_cuda, oracle_lib, k):
     """The whole -DFMM path (pre-pass from the caller's acc_old included)
     against the oracle on the same inputs: the same tie decisions in the
     pre-pass and the walk, so results agree to round-off; the case with
-    the mode-1 reference golden is also checked against it (pre-pass ties
-    decided by the reference's rounding: see tests/test_master_oracle.py)."""
+    the mode-1 reference golden is also checked against it: the pre-pass ties
+    are decided by the reference build's rounding (FMA-contracted norm), so
+    acc_old and the forces agree with the reference to round-off."""
     g = golden("master_direct.npz")
     c = mmg.FMM_CASES[k]
     x, v, m, sink, src, ao, idx = mmg.fmm_inputs(c)
@@ -205,11 +206,12 @@ def test_gpu_master_fmm_matches_oracle(torch_cuda, oracle_lib, k):
     _check((gp, ga, gt), (op, oa, ot), live, c["steps"], "fmm case %d vs oracle" % k)
     if c["mode"] == 1:
         e = np.abs(gao[idx] - g["fmm%d_acc_old" % k]) / g["fmm%d_acc_old" % k]
-        assert np.median(e) < 1e-14 and np.mean(e < 1e-12) > 0.8
+        assert e.max() < 1e-12
         lv = live[idx]
         ea = (np.linalg.norm(ga[idx][lv] - g["fmm%d_acc" % k][lv], axis=1) /
               np.linalg.norm(g["fmm%d_acc" % k][lv], axis=1))
-        assert np.sqrt(np.mean(ea ** 2)) < 3e-8 and ea.max() < 3e-6
+        print("fmm case %d vs reference: acc_old max %.2e acc max %.2e" % (k, e.max(), ea.max()))
+        assert ea.max() < 1e-10
 
 
 def test_gpu_master_fmm_body_records(torch_cuda):
