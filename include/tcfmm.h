@@ -117,6 +117,20 @@ int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, cons
                   const unsigned char *flags, int nt, const int *targets, double *pot,
                   double *acc, double *jerk);
 
+/* tc_direct_soa without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  Same arguments and results as
+ * tc_direct_soa, for n <= 262144 bodies (larger n returns -1 with an error
+ * message).  The bodies are staged in a fixed device array of the library
+ * (~19 MiB, held on each device used until the process ends) instead of the
+ * context's buffers, and the outputs are cleared by a kernel.  The work is
+ * queued on the context's stream and the call returns without waiting.  All
+ * calls on one device share the fixed array: calls that overlap on one
+ * device (other streams, other threads) race. */
+int tc_direct_soa_noalloc(tc_context *ctx, int n, const double *x, const double *v,
+                          const double *m, const unsigned char *flags, int nt,
+                          const int *targets, double *pot, double *acc, double *jerk);
+
 /* ---- Per-kernel entry points over the reference's Cell records -----------
  * The reference's kernels take Cell pointers (kernel_fast_lean.h): P2M(C)
  * :701, M2M(C) :826, M2L_rotate(Ci, Cj) :1130, L2L(C) :1001, L2P(C) :1060,

@@ -107,6 +107,8 @@ def lib():
         L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.tc_direct_soa_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp,
+                                            vp]
         L.tc_get_acc_old.argtypes = [vp, vp]
         L.tc_get_cell_layout.argtypes = [C.c_int, C.POINTER(TcCellLayout)]
         for k in ("tc_P2M", "tc_M2M", "tc_L2P"):
@@ -137,7 +139,8 @@ def lib():
 
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
-                    "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_get_acc_old",
+                    "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_direct_soa_noalloc",
+                    "tc_get_acc_old",
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
                     "tc_P2P", "tc_master_eval_bodies", "tc_master_eval_soa",
                     "tc_master_evolve", "tc_master_evolve_particles", "tc_get_stats",
@@ -316,6 +319,26 @@ class Fmm:
                                         _ptr(flags), nt, _ptr(tg), _ptr(pot), _ptr(acc),
                                         _ptr(jerk)), "tc_direct_soa")
         return pot, acc, jerk
+
+    def direct_noalloc(self, x, v, m, targets, pot, acc, jerk, flags=None):
+        """tc_direct_soa_noalloc: the same sum into caller-given outputs (pot (nt,),
+        acc and jerk (nt, 3), float64 on the bodies' device), staged in the
+        library's fixed scratch; queued on the context's stream, not waited for."""
+        import torch
+        nt = targets.shape[0]
+        f8 = torch.float64
+        dev = check_tensors([("x", x, f8, (3,)), ("v", v, f8, (3,)), ("m", m, f8, ()),
+                             ("flags", flags, torch.uint8, ())], m.shape[0])
+        odev = check_tensors([("pot", pot, f8, ()), ("acc", acc, f8, (3,)),
+                              ("jerk", jerk, f8, (3,))], nt)
+        if odev != dev:
+            raise TypeError("the outputs are on %s, the bodies on %s" % (odev, dev))
+        if targets.dtype != torch.int32 or not targets.is_contiguous() or targets.device != dev:
+            raise TypeError("targets must be a contiguous int32 tensor on %s" % dev)
+        self._check(lib().tc_direct_soa_noalloc(self.h, m.shape[0], _ptr(x), _ptr(v), _ptr(m),
+                                                _ptr(flags), nt, _ptr(targets), _ptr(pot),
+                                                _ptr(acc), _ptr(jerk)),
+                    "tc_direct_soa_noalloc")
 
     # ---- master (non-jerk) variant ----------------------------------------
     def master_get_force_and_potential(self, bodies, get_steps=False, dt_param=0.025, mode=0):

@@ -30,7 +30,7 @@ CFLAGS = [
     "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]
 UNITS = ["tc_m2l.hip", "tc_tree.hip", "tc_upward.hip", "tc_p2p.hip", "tc_downward.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip", "tc_evolve.hip",
-         "tc_engine.hip"]
+         "tc_noalloc.hip", "tc_engine.hip"]
 
 
 def _compile(unit):

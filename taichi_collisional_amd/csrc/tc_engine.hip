@@ -937,6 +937,12 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     tc_destroy(ctx);
     return -3;
   }
+  // the no-allocation entry point's fixed scratch: its address on this device
+  unsigned char *na = nullptr;
+  if (noalloc_scratch(&na) != hipSuccess) {
+    tc_destroy(ctx);
+    return -3;
+  }
   *out = ctx;
   return 0;
 }

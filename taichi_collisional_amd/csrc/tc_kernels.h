@@ -160,6 +160,9 @@ int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ng
 int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s);
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s);
+int launch_direct_sum(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
+                      double *jerk, hipStream_t s);
+void launch_zero_f64(double *p, size_t n, hipStream_t s);
 
 // ---- p = 2 pre-pass and multipole norms (tc_low.hip)
 int launch_low_upward(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
@@ -192,6 +195,9 @@ int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids
 // master (non-jerk) variant, tc_master.hip
 void launch_iota(int *p, int n, hipStream_t s);
 void launch_zero_u8(unsigned char *p, int n, hipStream_t s);
+// ---- fixed scratch of tc_direct_soa_noalloc (tc_noalloc.hip)
+hipError_t noalloc_scratch(unsigned char **out);
+int noalloc_max_bodies();
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
                       int n, unsigned char *lend, hipStream_t s);
 // Source slabs for launch_master_direct at n bodies (1: no partial sums).

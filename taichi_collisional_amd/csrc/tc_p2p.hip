@@ -248,12 +248,28 @@ __global__ __launch_bounds__(256) void k_direct(BodiesDev b, int n, const int *_
   }
 }
 
+__global__ void k_zero_f64(double *__restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+void launch_zero_f64(double *p, size_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s) {
   if (nt <= 0) return 0;
   hipMemsetAsync(pot, 0, sizeof(double) * nt, s);
   hipMemsetAsync(acc, 0, sizeof(double) * 3 * nt, s);
   hipMemsetAsync(jerk, 0, sizeof(double) * 3 * nt, s);
+  return launch_direct_sum(b, n, targets, nt, pot, acc, jerk, s);
+}
+
+// The all-pairs sum alone: the outputs must already be zero.
+int launch_direct_sum(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
+                      double *jerk, hipStream_t s) {
+  if (nt <= 0) return 0;
   int bx = (nt + 255) / 256;
   int slabs = (2048 + bx - 1) / bx;  // aim for >= 2048 blocks
   int slab = (n + slabs - 1) / slabs;
