@@ -53,7 +53,7 @@ typedef struct tc_body {
 enum { TC_MAC_THETA = 0, TC_MAC_ERROR = 1 };
 
 typedef struct tc_config {
-  int p;                 /* expansion order (EXPANSION); compiled: 2,4,...,16       */
+  int p;                 /* expansion order (EXPANSION); compiled: every p in 2..24 */
   int ncrit;             /* max bodies per leaf (ncrit, exafmm.h:115)               */
   double theta;          /* opening angle of the theta-MAC (exafmm.h:116)          */
   int mac;               /* TC_MAC_THETA (horizontalPass) or TC_MAC_ERROR           */
@@ -90,6 +90,19 @@ int tc_set_seed(tc_context *ctx, long seed);
 int tc_set_shift(tc_context *ctx, const double frac[3]);
 /* The three values used by the most recent evaluation. */
 int tc_last_shift(const tc_context *ctx, double frac[3]);
+
+/* Precision of the near field (P2P) of the following evaluations.
+ * TC_P2P_FP64 (default): the reference's double-precision P2P
+ * (-DDOUBLE_P2P, kernel_fast_lean.h:541-548).  TC_P2P_MIXED: body pairs in
+ * FP32 on coordinates taken relative to the target group's first body, FP32
+ * sums within an LDS tile and FP64 sums across tiles, and pairs closer than
+ * close_radius in FP64 (the reference's own FP32 P2P is its Vec16f branch,
+ * kernel_fast_lean.h:549-607, without a close-pair rule).
+ * TC_P2P_MIXED_HILO: as MIXED, with the relative coordinates held as float
+ * hi + lo pairs so the FP32 separations are exact to FP32 rounding.
+ * Returns -1 for an unknown mode or a negative radius. */
+enum { TC_P2P_FP64 = 0, TC_P2P_MIXED = 1, TC_P2P_MIXED_HILO = 2 };
+int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius);
 
 /* Drop-in for get_force_and_potential on the reference's host Body array:
  * F, J, p are overwritten (the reference driver zeroes them before the

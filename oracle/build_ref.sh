@@ -14,7 +14,7 @@ set -euo pipefail
 REF=${TC_REFERENCE_DIR:-/root/reference/taichi_with_jerks}
 HERE="$(cd "$(dirname "$0")" && pwd)"
 OUT="$HERE/_ref"
-ORDERS=${TC_REF_ORDERS:-"4 8 10 12"}
+ORDERS=${TC_REF_ORDERS:-"4 5 8 10 12 16 20 24"}
 if [ ! -d "$REF" ]; then
   echo "build_ref: reference not present at $REF; skipping"
   exit 0

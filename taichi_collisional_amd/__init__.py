@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libtcfmm.so")
 
 TC_MAC_THETA = 0
 TC_MAC_ERROR = 1
+TC_P2P_FP64, TC_P2P_MIXED, TC_P2P_MIXED_HILO = 0, 1, 2
 
 # struct Body of the reference (taichi_with_jerks/exafmm.h:22-46), 152 bytes.
 body_dtype = np.dtype({
@@ -104,6 +105,7 @@ def lib():
         L.tc_set_seed.argtypes = [vp, C.c_long]
         L.tc_set_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_last_shift.argtypes = [vp, C.POINTER(C.c_double)]
+        L.tc_set_precision.argtypes = [vp, C.c_int, C.c_double]
         L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
@@ -139,6 +141,7 @@ def lib():
 
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
+                    "tc_set_precision",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_direct_soa_noalloc",
                     "tc_get_acc_old",
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
@@ -253,6 +256,13 @@ class Fmm:
         if rc != 0:
             raise RuntimeError("%s failed (%d): %s" % (what, rc,
                                                        lib().tc_last_error(self.h).decode()))
+
+    def set_precision(self, p2p_mode=TC_P2P_FP64, close_radius=1e-3):
+        """Near-field precision of the following evaluations (tc_set_precision):
+        TC_P2P_FP64, TC_P2P_MIXED or TC_P2P_MIXED_HILO with FP64 pairs below
+        close_radius."""
+        self._check(lib().tc_set_precision(self.h, int(p2p_mode), float(close_radius)),
+                    "tc_set_precision")
 
     def set_seed(self, seed):
         self._check(lib().tc_set_seed(self.h, int(seed)), "tc_set_seed")

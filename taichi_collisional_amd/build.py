@@ -29,19 +29,32 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]
-UNITS = ["tc_m2l.hip", "tc_tree.hip", "tc_upward.hip", "tc_p2p.hip", "tc_downward.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip", "tc_evolve.hip",
-         "tc_noalloc.hip", "tc_engine.hip"]
+# Units whose kernels are instantiated for every expansion order are compiled
+# once per order shard (-DTC_SHARD=k, see TC_NSHARDS in csrc/tc_kernels.h), so
+# the copies build in parallel.
+NSHARDS = 3
+SHARDED = ["tc_m2l.hip", "tc_upward.hip", "tc_downward.hip"]
+PLAIN = ["tc_tree.hip", "tc_p2p.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip",
+         "tc_evolve.hip", "tc_noalloc.hip", "tc_dispatch.hip", "tc_engine.hip"]
+UNITS = [(u, k) for u in SHARDED for k in range(NSHARDS)] + [(u, None) for u in PLAIN]
+
+
+def _obj(unit):
+    u, k = unit
+    return os.path.join(BUILD, u.replace(".hip", "" if k is None else "_s%d" % k) + ".o")
 
 
 def _compile(unit):
-    src = os.path.join(CSRC, unit)
-    obj = os.path.join(BUILD, unit.replace(".hip", ".o"))
+    u, k = unit
+    src = os.path.join(CSRC, u)
+    obj = _obj(unit)
     t = time.time()
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    defs = [] if k is None else ["-DTC_SHARD=%d" % k, "-DTC_NSHARDS=%d" % NSHARDS]
+    cmd = [HIPCC] + CFLAGS + defs + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (unit, " ".join(cmd), r.stderr[-6000:]))
-    return unit, time.time() - t
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (u, " ".join(cmd), r.stderr[-6000:]))
+    return os.path.basename(obj), time.time() - t
 
 
 HDF5_DIR = os.environ.get("TC_HDF5_DIR", "/opt/conda")
@@ -73,11 +86,13 @@ def build_io(verbose=True):
 def build(jobs=None, verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
+    # slowest first, so the long shards start at once
+    order = sorted(UNITS, key=lambda u: u[1] is None)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for unit, dt in ex.map(_compile, UNITS):
+        for unit, dt in ex.map(_compile, order):
             if verbose:
-                print("  hipcc %-18s %6.1fs" % (unit, dt), flush=True)
-    objs = [os.path.join(BUILD, u.replace(".hip", ".o")) for u in UNITS]
+                print("  hipcc %-22s %6.1fs" % (unit, dt), flush=True)
+    objs = [_obj(u) for u in UNITS]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

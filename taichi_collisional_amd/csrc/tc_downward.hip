@@ -22,7 +22,12 @@
 #include "tc_harm.h"
 #include "tc_kernels.h"
 
+#ifndef TC_SHARD
+#error "compiled once per order shard, with -DTC_SHARD=k (see TC_NSHARDS in tc_kernels.h)"
+#endif
+
 namespace tcfmm {
+namespace TC_SHARD_NS(TC_SHARD) {
 
 template <int P>
 __global__ __launch_bounds__(256) void k_l2l(CellsDev c, const int *__restrict__ ids, int ls) {
@@ -200,60 +205,6 @@ __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev 
   }
 }
 
-__global__ void k_scatter_results(BodiesDev b, ResultsDev r, int n, double *__restrict__ pot,
-                                  double *__restrict__ acc, double *__restrict__ jerk) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  int i = b.order[k];
-  if (pot) pot[i] = r.pot[k];
-  if (acc) {
-    acc[3 * (size_t)i] = r.ax[k];
-    acc[3 * (size_t)i + 1] = r.ay[k];
-    acc[3 * (size_t)i + 2] = r.az[k];
-  }
-  if (jerk) {
-    jerk[3 * (size_t)i] = r.jx[k];
-    jerk[3 * (size_t)i + 1] = r.jy[k];
-    jerk[3 * (size_t)i + 2] = r.jz[k];
-  }
-}
-
-// near + far result sets, summed per body in tree order, to the caller's order
-__global__ void k_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n,
-                                      double *__restrict__ pot, double *__restrict__ acc,
-                                      double *__restrict__ jerk) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  int i = b.order[k];
-  if (pot) pot[i] = r.pot[k] + f.pot[k];
-  if (acc) {
-    acc[3 * (size_t)i] = r.ax[k] + f.ax[k];
-    acc[3 * (size_t)i + 1] = r.ay[k] + f.ay[k];
-    acc[3 * (size_t)i + 2] = r.az[k] + f.az[k];
-  }
-  if (jerk) {
-    jerk[3 * (size_t)i] = r.jx[k] + f.jx[k];
-    jerk[3 * (size_t)i + 1] = r.jy[k] + f.jy[k];
-    jerk[3 * (size_t)i + 2] = r.jz[k] + f.jz[k];
-  }
-}
-
-int launch_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n, double *pot,
-                               double *acc3, double *jerk3, hipStream_t s) {
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_scatter_results_sum, dim3((n + 255) / 256), dim3(256), 0, s, b, r, f, n,
-                     pot, acc3, jerk3);
-  return 0;
-}
-
-int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
-                           double *jerk3, hipStream_t s) {
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_scatter_results, dim3((n + 255) / 256), dim3(256), 0, s, b, r, n, pot,
-                     acc3, jerk3);
-  return 0;
-}
-
 template <int P>
 static void l2l_go(CellsDev c, const int *ids, int ls, int ncell, hipStream_t s) {
   hipLaunchKernelGGL(k_l2l<P>, dim3(2 * ncell), dim3(256), 0, s, c, ids, ls);
@@ -266,10 +217,14 @@ static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n,
 int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s) {
   if (le <= ls) return 0;
   switch (p) {
-#define TC_CASE(P)                            \
-  case P:                                     \
-    l2l_go<P>(c, nullptr, ls, le - ls, s);    \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      l2l_go<P>(c, nullptr, ls, le - ls, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -280,10 +235,14 @@ int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s) {
 int launch_l2l_ids(int p, CellsDev c, const int *ids, int nids, hipStream_t s) {
   if (nids <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                          \
-  case P:                                   \
-    l2l_go<P>(c, ids, 0, nids, s);          \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      l2l_go<P>(c, ids, 0, nids, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -295,10 +254,14 @@ int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids
                hipStream_t s) {
   if (nleaves <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                               \
-  case P:                                        \
-    l2p_go<P>(c, b, r, leaf_ids, nleaves, s);    \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      l2p_go<P>(c, b, r, leaf_ids, nleaves, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -306,4 +269,5 @@ int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids
   }
 }
 
+}  // namespace TC_SHARD_NS(TC_SHARD)
 }  // namespace tcfmm

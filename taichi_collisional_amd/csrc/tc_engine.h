@@ -94,6 +94,8 @@ struct tc_context {
   tc_stats st{};
   Lcg48 rng;
   bool fixed_shift = false;
+  int p2p_mode = 0;           // TC_P2P_* (tc_set_precision)
+  double close_radius = 0.0;  // mixed P2P: pairs closer than this in FP64
   double shift[3] = {0, 0, 0};
   // bodies
   int cap_n = 0;

@@ -627,7 +627,8 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
                     ctx->p2p_ranges.as<int2>(), s2);
   launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
              (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
-             ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2);
+             ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2,
+             ctx->p2p_mode, ctx->close_radius);
   TC_CHECK(hipEventRecord(ctx->ev[10], s2));
   // ---- downward into the far-field result set
   ResultsDev rf = {ctx->fpot.as<double>(), ctx->fax.as<double>(), ctx->fay.as<double>(),
@@ -1004,6 +1005,21 @@ int tc_set_shift(tc_context *ctx, const double frac[3]) {
     ctx->shift[d] = frac[d];
   }
   ctx->fixed_shift = true;
+  return 0;
+}
+
+int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius) {
+  if (!ctx) return -1;
+  if (p2p_mode != TC_P2P_FP64 && p2p_mode != TC_P2P_MIXED && p2p_mode != TC_P2P_MIXED_HILO) {
+    ctx->err = "unknown P2P precision mode";
+    return -1;
+  }
+  if (p2p_mode != TC_P2P_FP64 && !(close_radius >= 0 && close_radius < 1e18)) {
+    ctx->err = "close_radius must be >= 0";
+    return -1;
+  }
+  ctx->p2p_mode = p2p_mode;
+  ctx->close_radius = p2p_mode == TC_P2P_FP64 ? 0.0 : close_radius;
   return 0;
 }
 

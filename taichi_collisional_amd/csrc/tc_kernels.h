@@ -12,11 +12,26 @@
 
 namespace tcfmm {
 
-constexpr int kMaxP = 20;
+constexpr int kMaxP = 24;
 constexpr int kMaxNT = (kMaxP + 1) * (kMaxP + 1);
 
 // Expansion orders with compiled kernels.
-#define TC_FOR_EACH_P(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
+#define TC_FOR_EACH_P(X)                                                                     \
+  X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) \
+      X(19) X(20) X(21) X(22) X(23) X(24)
+
+// Order shards.  tc_m2l.hip, tc_upward.hip and tc_downward.hip instantiate
+// their kernels for every order; each is compiled TC_NSHARDS times, with
+// -DTC_SHARD=k, and copy k holds the orders p with p % TC_NSHARDS == k inside
+// namespace tcfmm::sh<k> (the copies build in parallel).  tc_dispatch.hip
+// routes the launch_* calls of those units (declared below) to the copy that
+// holds p.
+#ifndef TC_NSHARDS
+#define TC_NSHARDS 3
+#endif
+#define TC_SHARD_NS_(k) sh##k
+#define TC_SHARD_NS(k) TC_SHARD_NS_(k)
+constexpr bool in_shard(int p, int k) { return p % TC_NSHARDS == k; }
 
 template <int... Ns>
 struct IntSeq {};
@@ -149,14 +164,20 @@ struct M2LArgs {
   const unsigned char *has_sink;
   const double *M, *Md;
   double *L, *Ld;
-  int quirk;
+  int quirk;       // drop M_p^{+-p} (the reference's factorial_table[2p] == 0)
+  int stride = 0;  // doubles per cell row of M, Md, L, Ld; 0: (p+1)^2
 };
+// p: the truncation order of the listed pairs (<= the order of the rows)
 int launch_m2l(int p, const M2LArgs &a, hipStream_t s);
 
 // ---- P2P (tc_p2p.hip)
+// mode: 0 FP64 (the reference's DOUBLE_P2P form); 1 mixed: FP32 pairs on
+// offsets from the target group's first body, FP64 for pairs closer than
+// close_radius; 2 mixed with the offsets held as float hi + lo pairs
 int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ngroups,
                const int *offs, const int *count, const int2 *ranges,
-               unsigned long long *npairs, hipStream_t s);
+               unsigned long long *npairs, hipStream_t s, int mode = 0,
+               double close_radius = 0.0);
 int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s);
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s);

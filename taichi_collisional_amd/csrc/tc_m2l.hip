@@ -18,6 +18,11 @@
 // factorial table (factorial_table[2p] == 0, kernel_fast_lean.h:129) removes
 // M_p^{+-p} from the sum; with quirk=1 those inputs are zeroed.
 //
+// The order P of a launch is the truncation order of its pairs: the cells'
+// coefficient rows may be longer (stride >= (P+1)^2 doubles, an expansion of
+// a higher order), and only the terms up to order P are read and added, so
+// one set of coefficient arrays serves pairs of different orders.
+//
 // Work decomposition: a block = one wave = one chunk of <= 64 sources of a
 // single target cell (the chunk list is grouped by target) for one array
 // (M -> L or Mdot -> Ldot); each XCD takes a contiguous range of the
@@ -28,7 +33,12 @@
 #include "tc_kernels.h"
 #include "swap_tables.h"
 
+#ifndef TC_SHARD
+#error "compiled once per order shard, with -DTC_SHARD=k (see TC_NSHARDS in tc_kernels.h)"
+#endif
+
 namespace tcfmm {
+namespace TC_SHARD_NS(TC_SHARD) {
 
 // n! for the translation factors, loaded with scalar loads (uniform across the
 // wave).  Filled by m2l_init_tables().
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
                                             const double *__restrict__ M,
                                             const double *__restrict__ Md,
                                             double *__restrict__ L, double *__restrict__ Ld,
-                                            int quirk) {
+                                            int quirk, int stride) {
   constexpr int NT = nterm(P);
   __shared__ double acc[NT + kWave];  // + one spare slot per lane
   __shared__ __attribute__((aligned(16))) double red[RedBuf<P>::count * RedBuf<P>::size];
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
     ezi = dx * q;
   }
   double mp[M2LRegs<P>::total];
-  const double *src = Msrc + (size_t)j * NT;
+  const double *src = Msrc + (size_t)j * stride;
   // Orders run from p down to 0: the large orders (biggest working set) run
   // while few rotated moments are live.
   // orders P .. P-kAhead+1 are gathered up front
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(64) void k_m2l(const int4 *__restrict__ chunks,
   m2l_backward_all<P>(mp, rp0, ri, ezr, ezi, exr, exi, active, acc, red, lane,
                       MakeIntSeq<P + 1>{});
   __syncthreads();
-  double *dst = Lout + (size_t)tgt * NT;
+  double *dst = Lout + (size_t)tgt * stride;
   for (int t = lane; t < NT; t += kWave) atomicAdd(dst + t, acc[t]);
 }
 
@@ -425,16 +435,20 @@ static void launch(const M2LArgs &a, hipStream_t s) {
   if (a.nchunks <= 0) return;
   dim3 grid(a.nchunks, 2);
   hipLaunchKernelGGL(k_m2l<P>, grid, dim3(64), 0, s, a.chunks, a.src, a.cX, a.has_sink, a.M,
-                     a.Md, a.L, a.Ld, a.quirk);
+                     a.Md, a.L, a.Ld, a.quirk, a.stride > 0 ? a.stride : nterm(P));
 }
 
 int launch_m2l(int p, const M2LArgs &a, hipStream_t s) {
   m2l_init_tables(p, a.quirk);
   switch (p) {
-#define TC_CASE(P) \
-  case P:          \
-    launch<P>(a, s); \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      launch<P>(a, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -442,4 +456,5 @@ int launch_m2l(int p, const M2LArgs &a, hipStream_t s) {
   }
 }
 
+}  // namespace TC_SHARD_NS(TC_SHARD)
 }  // namespace tcfmm

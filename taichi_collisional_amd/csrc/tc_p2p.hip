@@ -54,6 +54,68 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
 constexpr int kTile = TC_P2P_TILE;
 static_assert(kTile >= 64 && 2 * kTile * 32 >= 7 * kWave * 8, "tile holds the reduction");
 
+// Mixed-precision pair (FP32): source and target coordinates are offsets
+// from a reference point of the target group (its first body), so they are
+// small numbers and the FP32 differences keep ~7 digits of the separation.
+// kHiLo: each offset is held as a float pair (hi + lo) and the difference is
+// (hi_j - hi_i) + (lo_j - lo_i), accurate to FP32 rounding of the separation
+// itself; otherwise the offsets are rounded to one float each.  Pairs with
+// r^2 below the close radius squared (including r = 0) contribute nothing
+// here and set `close`; the caller evaluates them in FP64.  The jerk's second
+// term is accumulated without the factor -3 (applied once at the end).
+struct Acc10f {
+  float p, ax, ay, az, jx, jy, jz, kx, ky, kz;
+};
+
+template <bool kHiLo>
+__device__ __forceinline__ void pair_fp32(float xh, float yh, float zh, float xl, float yl,
+                                          float zl, float vx, float vy, float vz,
+                                          const float4 &S, const float4 &Sl, const float4 &V,
+                                          float rc2, Acc10f &a, bool &close) {
+  float dx = S.x - xh, dy = S.y - yh, dz = S.z - zh;
+  if (kHiLo) {
+    dx += Sl.x - xl;
+    dy += Sl.y - yl;
+    dz += Sl.z - zl;
+  }
+  const float dvx = V.x - vx, dvy = V.y - vy, dvz = V.z - vz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const bool c = r2 < rc2;
+  close |= c;
+  const float ri = c ? 0.0f : __builtin_amdgcn_rsqf(r2);
+  const float ri2 = ri * ri;
+  float f = S.w * ri;
+  a.p += f;
+  f *= ri2;
+  a.ax = fmaf(dx, f, a.ax);
+  a.ay = fmaf(dy, f, a.ay);
+  a.az = fmaf(dz, f, a.az);
+  a.jx = fmaf(dvx, f, a.jx);
+  a.jy = fmaf(dvy, f, a.jy);
+  a.jz = fmaf(dvz, f, a.jz);
+  const float g = f * ri2 * fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+  a.kx = fmaf(dx, g, a.kx);
+  a.ky = fmaf(dy, g, a.ky);
+  a.kz = fmaf(dz, g, a.kz);
+}
+
+// The FP32 partial sums of one tile into the FP64 accumulators (blocked
+// summation: FP32 runs are at most one tile long per lane).
+__device__ __forceinline__ void flush_fp32(Acc10f &f, Acc7 &a, double &kx, double &ky,
+                                           double &kz) {
+  a.p += (double)f.p;
+  a.ax += (double)f.ax;
+  a.ay += (double)f.ay;
+  a.az += (double)f.az;
+  a.jx += (double)f.jx;
+  a.jy += (double)f.jy;
+  a.jz += (double)f.jz;
+  kx += (double)f.kx;
+  ky += (double)f.ky;
+  kz += (double)f.kz;
+  f = Acc10f{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+}
+
 // One block (one wave) per work item: a target group (<= 32 consecutive bodies
 // of one leaf) and a slice of at most kItemList entries of the leaf's P2P list
 // (long lists -- a sparse halo leaf can touch most of the system -- are split
@@ -65,15 +127,27 @@ static_assert(kTile >= 64 && 2 * kTile * 32 >= 7 * kWave * 8, "tile holds the re
 // slots in parallel.  Lane (slice s, target t) takes every G-th tile body,
 // G = floor(64 / group size).  A single range longer than the tile (only when
 // ncrit > kTile) is streamed through the tile in pieces.
+//
+// kMode: kP2PFp64 (the reference's double-precision P2P), or the mixed forms
+// kP2PMixed / kP2PMixedHiLo: sources are staged as FP32 offsets from the
+// group's first body (three float4 per body), pairs are evaluated in FP32 and
+// summed in FP32 within a tile, tile sums in FP64; pairs closer than the
+// close radius (r^2 < rc2) are redone in FP64 from the FP64 bodies.
+enum { kP2PFp64 = 0, kP2PMixed = 1, kP2PMixedHiLo = 2 };
+
+template <int kMode>
 __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev r,
                                             const int4 *__restrict__ groups,
                                             const int *__restrict__ offs,
                                             const int *__restrict__ count,
                                             const int2 *__restrict__ ranges,
-                                            unsigned long long *__restrict__ npairs) {
+                                            unsigned long long *__restrict__ npairs,
+                                            float rc2) {
   __shared__ double4 sbuf[2 * kTile];
   __shared__ int rstart[kWave], rexcl[kWave];
   double4 *sp = sbuf, *sv = sbuf + kTile;
+  // mixed forms: three float4 per body in the same buffer
+  float4 *fs = reinterpret_cast<float4 *>(sbuf), *fl = fs + kTile, *fv = fs + 2 * kTile;
   double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(sbuf);  // after the loop
   const int lane = threadIdx.x;
   // work item: target leaf, first target, list slice start, slice length | group size << 16
@@ -90,6 +164,41 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
   const double xi = b.x[ib], yi = b.y[ib], zi = b.z[ib];
   const double vxi = b.vx[ib], vyi = b.vy[ib], vzi = b.vz[ib];
   Acc7 a = {0, 0, 0, 0, 0, 0, 0};
+  // mixed forms: the group's reference point and the target's offsets
+  const int i0 = c.body[ci] + gw.y;
+  const double x0 = b.x[i0], y0 = b.y[i0], z0 = b.z[i0];
+  const double u0 = b.vx[i0], v0 = b.vy[i0], w0 = b.vz[i0];
+  float txh = 0, tyh = 0, tzh = 0, txl = 0, tyl = 0, tzl = 0, tvx = 0, tvy = 0, tvz = 0;
+  double kx = 0, ky = 0, kz = 0;
+  Acc10f af = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (kMode != kP2PFp64) {
+    const double ox = xi - x0, oy = yi - y0, oz = zi - z0;
+    txh = (float)ox;
+    tyh = (float)oy;
+    tzh = (float)oz;
+    txl = (float)(ox - (double)txh);
+    tyl = (float)(oy - (double)tyh);
+    tzl = (float)(oz - (double)tzh);
+    tvx = (float)(vxi - u0);
+    tvy = (float)(vyi - v0);
+    tvz = (float)(vzi - w0);
+  }
+  // stage source body bq in tile slot q
+  auto stage = [&](int q, int bq) {
+    const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
+    if (kMode == kP2PFp64) {
+      sp[q] = make_double4(b.x[bq], b.y[bq], b.z[bq], mq);
+      sv[q] = make_double4(b.vx[bq], b.vy[bq], b.vz[bq], 0.0);
+    } else {
+      const double ox = b.x[bq] - x0, oy = b.y[bq] - y0, oz = b.z[bq] - z0;
+      const float hx = (float)ox, hy = (float)oy, hz = (float)oz;
+      fs[q] = make_float4(hx, hy, hz, (float)mq);
+      fl[q] = make_float4((float)(ox - (double)hx), (float)(oy - (double)hy),
+                          (float)(oz - (double)hz), __int_as_float(bq));
+      fv[q] = make_float4((float)(b.vx[bq] - u0), (float)(b.vy[bq] - v0),
+                          (float)(b.vz[bq] - w0), 0.0f);
+    }
+  };
   const int l0 = offs[ci] + gw.z, ln = gw.w & 0xffff;
   long long nsrc = 0;
   int e = 0, off = 0;
@@ -115,22 +224,14 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
           else
             hi = mid - 1;
         }
-        const int bq = rstart[lo] + q - rexcl[lo];
-        const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
-        sp[q] = make_double4(b.x[bq], b.y[bq], b.z[bq], mq);
-        sv[q] = make_double4(b.vx[bq], b.vy[bq], b.vz[bq], 0.0);
+        stage(q, rstart[lo] + q - rexcl[lo]);
       }
       e += nfit;
     } else {
       // first range longer than the tile: stream it piece by piece
       const int base = __shfl(rg.x, 0, kWave), len = __shfl(rg.y, 0, kWave);
       fill = min(kTile, len - off);
-      for (int q = lane; q < fill; q += kWave) {
-        const int bq = base + off + q;
-        const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
-        sp[q] = make_double4(b.x[bq], b.y[bq], b.z[bq], mq);
-        sv[q] = make_double4(b.vx[bq], b.vy[bq], b.vz[bq], 0.0);
-      }
+      for (int q = lane; q < fill; q += kWave) stage(q, base + off + q);
       off += fill;
       if (off >= len) {
         off = 0;
@@ -145,17 +246,51 @@ __global__ __launch_bounds__(64) void k_p2p(CellsDev c, BodiesDev b, ResultsDev 
       // per-pair exec-mask updates); the ragged last round follows
       const int nfull = __builtin_amdgcn_readfirstlane(fill / G);
       int k = s;
+      if (kMode == kP2PFp64) {
 #pragma unroll 2
-      for (int i = 0; i < nfull; i++, k += G) {
-        const double4 P = sp[k], V = sv[k];
-        pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
-      }
-      if (k < fill) {
-        const double4 P = sp[k], V = sv[k];
-        pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
+        for (int i = 0; i < nfull; i++, k += G) {
+          const double4 P = sp[k], V = sv[k];
+          pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
+        }
+        if (k < fill) {
+          const double4 P = sp[k], V = sv[k];
+          pair_fp64(xi, yi, zi, vxi, vyi, vzi, P.x, P.y, P.z, V.x, V.y, V.z, P.w, a);
+        }
+      } else {
+        constexpr bool hl = kMode == kP2PMixedHiLo;
+        bool close = false;
+#pragma unroll 2
+        for (int i = 0; i < nfull; i++, k += G)
+          pair_fp32<hl>(txh, tyh, tzh, txl, tyl, tzl, tvx, tvy, tvz, fs[k], fl[k], fv[k], rc2,
+                        af, close);
+        if (k < fill)
+          pair_fp32<hl>(txh, tyh, tzh, txl, tyl, tzl, tvx, tvy, tvz, fs[k], fl[k], fv[k], rc2,
+                        af, close);
+        flush_fp32(af, a, kx, ky, kz);
+        if (close) {
+          // the few pairs inside the close radius (or at r = 0), in FP64 from
+          // the bodies themselves; the same FP32 test picks them
+          for (int q = s; q < fill; q += G) {
+            Acc10f dummy = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            bool cq = false;
+            pair_fp32<hl>(txh, tyh, tzh, txl, tyl, tzl, tvx, tvy, tvz, fs[q], fl[q], fv[q],
+                          rc2, dummy, cq);
+            if (cq) {
+              const int bq = __float_as_int(fl[q].w);
+              const double mq = (b.flags[bq] & kSource) ? b.q[bq] : 0.0;
+              pair_fp64(xi, yi, zi, vxi, vyi, vzi, b.x[bq], b.y[bq], b.z[bq], b.vx[bq],
+                        b.vy[bq], b.vz[bq], mq, a);
+            }
+          }
+        }
       }
     }
     __syncthreads();
+  }
+  if (kMode != kP2PFp64) {
+    a.jx -= 3.0 * kx;
+    a.jy -= 3.0 * ky;
+    a.jz -= 3.0 * kz;
   }
   if (lane == 0) atomicAdd(npairs, (unsigned long long)(nsrc * nt));
   red[0][lane] = a.p;
@@ -201,10 +336,18 @@ int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream
 
 int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ngroups,
                const int *offs, const int *count, const int2 *ranges,
-               unsigned long long *npairs, hipStream_t s) {
+               unsigned long long *npairs, hipStream_t s, int mode, double close_radius) {
   if (ngroups <= 0) return 0;
-  hipLaunchKernelGGL(k_p2p, dim3(ngroups), dim3(64), 0, s, c, b, r, groups, offs, count, ranges,
-                     npairs);
+  const float rc2 = (float)(close_radius * close_radius);
+  if (mode == kP2PMixed)
+    hipLaunchKernelGGL(k_p2p<kP2PMixed>, dim3(ngroups), dim3(64), 0, s, c, b, r, groups, offs,
+                       count, ranges, npairs, rc2);
+  else if (mode == kP2PMixedHiLo)
+    hipLaunchKernelGGL(k_p2p<kP2PMixedHiLo>, dim3(ngroups), dim3(64), 0, s, c, b, r, groups,
+                       offs, count, ranges, npairs, rc2);
+  else
+    hipLaunchKernelGGL(k_p2p<kP2PFp64>, dim3(ngroups), dim3(64), 0, s, c, b, r, groups, offs,
+                       count, ranges, npairs, 0.0f);
   return 0;
 }
 
@@ -277,6 +420,61 @@ int launch_direct_sum(BodiesDev b, int n, const int *targets, int nt, double *po
   slabs = (n + slab - 1) / slab;
   hipLaunchKernelGGL(k_direct, dim3(bx, slabs), dim3(256), 0, s, b, n, targets, nt, slab, pot,
                      acc, jerk);
+  return 0;
+}
+
+// ---- results back to the caller's body order (after the downward pass)
+__global__ void k_scatter_results(BodiesDev b, ResultsDev r, int n, double *__restrict__ pot,
+                                  double *__restrict__ acc, double *__restrict__ jerk) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int i = b.order[k];
+  if (pot) pot[i] = r.pot[k];
+  if (acc) {
+    acc[3 * (size_t)i] = r.ax[k];
+    acc[3 * (size_t)i + 1] = r.ay[k];
+    acc[3 * (size_t)i + 2] = r.az[k];
+  }
+  if (jerk) {
+    jerk[3 * (size_t)i] = r.jx[k];
+    jerk[3 * (size_t)i + 1] = r.jy[k];
+    jerk[3 * (size_t)i + 2] = r.jz[k];
+  }
+}
+
+// near + far result sets, summed per body in tree order, to the caller's order
+__global__ void k_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n,
+                                      double *__restrict__ pot, double *__restrict__ acc,
+                                      double *__restrict__ jerk) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int i = b.order[k];
+  if (pot) pot[i] = r.pot[k] + f.pot[k];
+  if (acc) {
+    acc[3 * (size_t)i] = r.ax[k] + f.ax[k];
+    acc[3 * (size_t)i + 1] = r.ay[k] + f.ay[k];
+    acc[3 * (size_t)i + 2] = r.az[k] + f.az[k];
+  }
+  if (jerk) {
+    jerk[3 * (size_t)i] = r.jx[k] + f.jx[k];
+    jerk[3 * (size_t)i + 1] = r.jy[k] + f.jy[k];
+    jerk[3 * (size_t)i + 2] = r.jz[k] + f.jz[k];
+  }
+}
+
+int launch_scatter_results_sum(BodiesDev b, ResultsDev r, ResultsDev f, int n, double *pot,
+                               double *acc3, double *jerk3, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_scatter_results_sum, dim3((n + 255) / 256), dim3(256), 0, s, b, r, f, n,
+                     pot, acc3, jerk3);
+  return 0;
+}
+
+int launch_scatter_results(BodiesDev b, ResultsDev r, int n, double *pot, double *acc3,
+                           double *jerk3, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_scatter_results, dim3((n + 255) / 256), dim3(256), 0, s, b, r, n, pot,
+                     acc3, jerk3);
   return 0;
 }
 

@@ -23,7 +23,12 @@
 #include "tc_kernels.h"
 #include "tdesign.h"
 
+#ifndef TC_SHARD
+#error "compiled once per order shard, with -DTC_SHARD=k (see TC_NSHARDS in tc_kernels.h)"
+#endif
+
 namespace tcfmm {
+namespace TC_SHARD_NS(TC_SHARD) {
 
 __host__ __device__ constexpr int pow2_floor(int v) {
   int p = 1;
@@ -506,10 +511,14 @@ int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
                int what) {
   if (nleaves <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                                   \
-  case P:                                            \
-    p2m_launch<P>(c, b, leaf_ids, nleaves, what, s); \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      p2m_launch<P>(c, b, leaf_ids, nleaves, what, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -520,10 +529,14 @@ int launch_p2m(int p, CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
 int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s, int what) {
   if (le <= ls) return 0;
   switch (p) {
-#define TC_CASE(P)                                      \
-  case P:                                               \
-    m2m_launch<P>(c, b, nullptr, ls, le - ls, what, s); \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      m2m_launch<P>(c, b, nullptr, ls, le - ls, what, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -534,10 +547,14 @@ int launch_m2m(int p, CellsDev c, BodiesDev b, int ls, int le, hipStream_t s, in
 int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hipStream_t s) {
   if (nids <= 0) return 0;
   switch (p) {
-#define TC_CASE(P)                                                 \
-  case P:                                                          \
-    m2m_launch<P>(c, b, ids, 0, nids, kUpAll, s);    \
-    return 0;
+#define TC_CASE(P)\
+  case P:\
+    if constexpr (in_shard(P, TC_SHARD)) {\
+      m2m_launch<P>(c, b, ids, 0, nids, kUpAll, s);\
+      return 0;\
+    } else {\
+      return -1;\
+    }
     TC_FOR_EACH_P(TC_CASE)
 #undef TC_CASE
     default:
@@ -545,4 +562,5 @@ int launch_m2m_ids(int p, CellsDev c, BodiesDev b, const int *ids, int nids, hip
   }
 }
 
+}  // namespace TC_SHARD_NS(TC_SHARD)
 }  // namespace tcfmm

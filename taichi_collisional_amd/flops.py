@@ -125,4 +125,4 @@ def eval_flops(p, n, n_pp, n_m2l, n_edges):
 
 
 if __name__ == "__main__":
-    print(json.dumps([constants(p) for p in (4, 8, 10, 12, 14, 16)], indent=1))
+    print(json.dumps([constants(p) for p in range(2, 25)], indent=1))

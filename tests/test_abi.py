@@ -114,3 +114,10 @@ def test_master_cell_layout_matches_reference_cell():
         assert tca.master_body_dtype.itemsize == ref["Body"]
         for k in tca.master_body_dtype.names:
             assert tca.master_body_dtype.fields[k][1] == ref["Body." + k], k
+
+
+def test_every_expansion_order_is_compiled():
+    """The reference builds EXPANSION=24 and 20 (taichi_with_jerks/Makefile:7,
+    10); every order 2..24 has kernels."""
+    import taichi_collisional_amd as tca
+    assert tca.supported_orders() == list(range(2, 25))

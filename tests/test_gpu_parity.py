@@ -60,6 +60,8 @@ def _check(a, b, what=""):
 @pytest.mark.parametrize("n,p,theta,ncrit", [
     (3000, 12, 0.5, 64), (20000, 4, 0.5, 64), (20000, 8, 0.4, 64), (65536, 10, 0.5, 64),
     (30000, 6, 0.6, 16), (30000, 8, 0.5, 200), (20000, 16, 0.5, 64), (20000, 14, 0.4, 64),
+    (8000, 3, 0.5, 64), (12000, 7, 0.5, 64), (12000, 11, 0.45, 32), (10000, 18, 0.5, 64),
+    (10000, 21, 0.6, 64), (10000, 24, 0.5, 64),
 ])
 def test_gpu_matches_oracle(torch_cuda, oracle_lib, n, p, theta, ncrit):
     from taichi_collisional_amd import ics
@@ -75,11 +77,13 @@ def test_gpu_matches_oracle(torch_cuda, oracle_lib, n, p, theta, ncrit):
     _check((gp, ga, gj), (op, oa, oj), "n=%d p=%d" % (n, p))
 
 
-@pytest.mark.parametrize("name", ["e2e_dat10.npz", "e2e_c1.npz", "e2e_c2s.npz", "e2e_c2.npz"])
+@pytest.mark.parametrize("name", ["e2e_dat10.npz", "e2e_c1.npz", "e2e_c2s.npz", "e2e_c2.npz",
+                                  "e2e_p5.npz", "e2e_p16.npz", "e2e_p20.npz", "e2e_p24.npz"])
 def test_gpu_matches_reference_goldens(torch_cuda, name):
     """e2e_c2.npz is the headline workload itself (Plummer N = 2^20, seed 1,
     p = 12, theta = 0.4, srand48(1)): identical pair counts and round-off
-    agreement on a seeded 4096-body subset."""
+    agreement on a seeded 4096-body subset.  e2e_p20 / e2e_p24 are the
+    orders the reference's own Makefiles build (EXPANSION=20, 24)."""
     from taichi_collisional_amd import ics
     g = golden(name)
     if "x" in g:
@@ -422,3 +426,33 @@ def test_gpu_context_reuse_across_sizes(torch_cuda):
             assert st[key] == st2[key], (k, key, st[key], st2[key])
         _check((pot.cpu().numpy(), acc.cpu().numpy(), jerk.cpu().numpy()),
                (p2.cpu().numpy(), a2.cpu().numpy(), j2.cpu().numpy()), "reuse case %d" % k)
+
+
+def test_gpu_every_order_converges(torch_cuda):
+    """Every compiled order 2..24 runs, and the force error against the FP64
+    direct sum falls with p (theta = 0.5: roughly theta^(p+1) per pair) until
+    it reaches round-off; no order is missing from the build."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    assert tca.supported_orders() == list(range(2, 25))
+    n = 4096
+    x, v, m = ics.plummer(n, seed=21)
+    dx, dv, dm = _dev(torch_cuda, x, v, m)
+    f = tca.Fmm(p=4, ncrit=32, theta=0.5, direct_below=0)
+    tg = torch_cuda.arange(n, dtype=torch_cuda.int32, device=dx.device)
+    _, a0, _ = f.direct(dx, dv, dm, tg)
+    a0 = a0.cpu().numpy()
+    errs = []
+    for p in range(2, 25):
+        g = tca.Fmm(p=p, ncrit=32, theta=0.5, direct_below=0)
+        g.set_shift([0.3, -0.2, 0.1])
+        _, acc, _ = g.eval(dx, dv, dm)
+        torch_cuda.cuda.synchronize()
+        errs.append(relerr_vec(acc.cpu().numpy(), a0)[0])
+        g.close()
+    print("rms force error by order:", " ".join("%d:%.1e" % (p, e) for p, e in zip(range(2, 25), errs)))
+    assert errs[0] < 1e-1 and errs[-1] < 1e-10
+    for p in range(2, 20):
+        # at least a factor 1.5 per two orders while far above round-off
+        if errs[p - 2] > 1e-12:
+            assert errs[p] < errs[p - 2] / 1.5, (p, errs[p - 2], errs[p])

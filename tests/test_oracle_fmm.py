@@ -24,7 +24,8 @@ def _inputs(g):
     return x, v, m
 
 
-@pytest.mark.parametrize("name", ["e2e_dat10.npz", "e2e_c1.npz", "e2e_c2s.npz"])
+@pytest.mark.parametrize("name", ["e2e_dat10.npz", "e2e_c1.npz", "e2e_c2s.npz", "e2e_p5.npz",
+                                  "e2e_p16.npz", "e2e_p20.npz", "e2e_p24.npz"])
 def test_oracle_matches_reference_theta_mac(oracle_lib, name):
     g = golden(name)
     x, v, m = _inputs(g)
