@@ -186,6 +186,85 @@ def master_evolve_row(tca, ics, torch, dev, p, seed):
             "ms_per_step": ems, "levels": levels, "substeps": 2 ** levels}
 
 
+def timed_row(fmm, torch, dx, dv, dm, out, idx, da, dj, reps=5, warm=2):
+    """ms per evaluation of a configured context (after warm-up), its phase
+    times and the force / jerk error against the direct sums da, dj of the
+    bodies idx (not timed)."""
+    pot, acc, jerk = out
+    for _ in range(warm):
+        fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sts = []
+    for _ in range(reps):
+        fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+        sts.append(fmm.stats())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / reps
+    ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
+    ej = torch.linalg.norm(jerk[idx] - dj, dim=1) / torch.linalg.norm(dj, dim=1)
+    return {"ms_per_eval": ms, "particles_per_s": dx.shape[0] / (ms * 1e-3),
+            "ms_p2p": sum(s_["ms_p2p"] for s_ in sts) / reps,
+            "ms_m2l": sum(s_["ms_m2l"] for s_ in sts) / reps,
+            "force_rms": float(torch.sqrt(torch.mean(ea ** 2))), "force_max": float(ea.max()),
+            "jerk_rms": float(torch.sqrt(torch.mean(ej ** 2))), "jerk_max": float(ej.max()),
+            "m2l_pairs": sts[-1]["n_m2l"], "p2p_body_pairs": sts[-1]["n_p2p_pairs"]}
+
+
+def pareto(args):
+    """Cost and accuracy of the ways to trade one for the other on the workload
+    (Plummer N, seed): fixed (p, theta) on a grid, the error-controlled MAC at
+    several force_accuracy values, and the learned per-leaf choice at several
+    targets; force error against the FP64 direct sum on a 4096-body subset."""
+    import numpy as np
+    import torch
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import flops, ics, learn
+    x, v, m = ics.plummer(args.n, seed=args.seed)
+    dev = torch.device("cuda:0")
+    dx, dv, dm = (torch.from_numpy(a).to(dev) for a in (x, v, m))
+    out = (torch.empty(args.n, dtype=torch.float64, device=dev),
+           torch.empty(args.n, 3, dtype=torch.float64, device=dev),
+           torch.empty(args.n, 3, dtype=torch.float64, device=dev))
+    rng = np.random.default_rng(12345)
+    tg = torch.from_numpy(rng.choice(args.n, 4096, replace=False).astype(np.int32)).to(dev)
+    idx = tg.long()
+    f0 = tca.Fmm(p=4, ncrit=args.ncrit, direct_below=0)
+    _, da, dj = f0.direct(dx, dv, dm, tg)
+    f0.close()
+
+    def emit(kind, f, **kw):
+        r = timed_row(f, torch, dx, dv, dm, out, idx, da, dj, reps=3, warm=1)
+        c = f.m2l_order_counts()
+        r["m2l_flops"] = float(sum(c[q] * flops.c_m2l(q) for q in range(len(c))))
+        r["p2p_flops"] = 44.0 * r["p2p_body_pairs"]
+        r.update(kind=kind, **kw)
+        r["meets_1e-7"] = r["force_rms"] <= 1e-7
+        print(json.dumps(r), flush=True)
+
+    for p in (6, 8, 10, 12, 14):
+        for th in (0.3, 0.4, 0.5, 0.6, 0.7):
+            f = tca.Fmm(p=p, ncrit=args.ncrit, theta=th, direct_below=0)
+            f.set_seed(args.seed)
+            emit("fixed", f, p=p, theta=th)
+            f.close()
+    for facc in (1e-7, 3e-8, 1e-8, 3e-9):
+        for p in (10, 12):
+            f = tca.Fmm(p=p, ncrit=args.ncrit, mac=tca.TC_MAC_ERROR, force_accuracy=facc,
+                        direct_below=0)
+            f.set_seed(args.seed)
+            emit("error_mac", f, p=p, force_accuracy=facc)
+            f.close()
+    if os.path.exists(learn.MODEL_PATH):
+        model = learn.load_model()
+        for target in (3e-7, 2e-7, 1e-7, 5e-8):
+            f = tca.Fmm(p=learn.P_MAX, ncrit=args.ncrit, direct_below=0)
+            f.set_seed(args.seed)
+            learn.apply_model(f, model, target_rms=target)
+            emit("learned", f, target_rms=target)
+            f.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +287,10 @@ def main():
                          "of known size for scaling HBM byte counters)")
     ap.add_argument("--only-master-evolve", action="store_true",
                     help="time only the master step loop row (for profiling it) and print it")
+    ap.add_argument("--pareto", action="store_true",
+                    help="instead of the benchmark line: time and error of a grid of fixed "
+                         "(p, theta), of the error-controlled MAC and of the learned choice "
+                         "on the workload; one JSON line per configuration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -221,6 +304,10 @@ def main():
         from taichi_collisional_amd import ics
         print(json.dumps({"master_evolve": master_evolve_row(
             tca, ics, torch, torch.device("cuda:0"), args.p, args.seed)}), flush=True)
+        return
+
+    if args.pareto:
+        pareto(args)
         return
 
     cpu = None
@@ -304,6 +391,20 @@ def main():
             extra["default_path"]["cpu_reference_seconds"] = cpu["default_path_seconds"]
             extra["default_path"]["gpu_over_cpu"] = cpu["default_path_seconds"] / (dms * 1e-3)
         del fd
+        # the mixed-precision near field: FP32 body pairs on group-relative
+        # coordinates (one float each, or hi + lo), FP64 inside r < 1e-3
+        rows = {}
+        for name, mode in (("fp32_offsets", tca.TC_P2P_MIXED),
+                           ("fp32_hilo_offsets", tca.TC_P2P_MIXED_HILO)):
+            fx = tca.Fmm(p=args.p, ncrit=args.ncrit, theta=args.theta, direct_below=0)
+            fx.set_seed(args.seed)
+            fx.set_precision(mode, close_radius=1e-3)
+            rows[name] = timed_row(fx, torch, dx, dv, dm, (pot, acc, jerk), idx, da, dj)
+            fx.close()
+        extra["mixed_p2p"] = {
+            "what": "same workload with the P2P near field in FP32 (pairs closer than r=1e-3 "
+                    "in FP64, FP32 sums per LDS tile, FP64 across tiles); M2L/L2L/M2M in FP64",
+            "fp64_ms_p2p": phases["p2p"], **rows}
         # host Body records in and out (AoS <-> device SoA, PCIe both ways)
         bodies = np.zeros(args.n, tca.body_dtype)
         bodies["X"], bodies["V"], bodies["q"] = x, v, m

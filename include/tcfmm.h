@@ -104,6 +104,53 @@ int tc_last_shift(const tc_context *ctx, double frac[3]);
 enum { TC_P2P_FP64 = 0, TC_P2P_MIXED = 1, TC_P2P_MIXED_HILO = 2 };
 int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius);
 
+/* ---- Per-cell expansion order and opening angle (no reference counterpart;
+ * the reference's analogue is its error-controlled walk,
+ * traverse_eager.h:59-122, which sets the MAC from a formula) ------------
+ * With per-cell orders, every leaf carries its own (q, theta): a pair
+ * (Ci, Cj) is accepted when |dX|^2 theta_i^2 > (Ri + Rj)^2 and its M2L is
+ * truncated at order q_i; internal cells take the largest q and smallest
+ * theta of the leaves below them.  Moments and locals are formed at the
+ * context's order p (q <= p).  A probe walk at probe_theta supplies each
+ * leaf's list lengths (features).  theta MAC only.
+ *   mode 0: off (every pair at cfg.p / cfg.theta);
+ *   mode 1: every leaf at (q, theta) through the per-cell path;
+ *   mode 2: the learned choice (needs tc_set_order_model). */
+int tc_set_adaptive(tc_context *ctx, int mode, int q, double theta, double probe_theta);
+/* The MLP (tc_mlp_param_count() floats: W1[64][10] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[64] b4, tanh hidden units) maps a leaf's
+ * features and a candidate (q / 20, theta) to log10 of the leaf's rms
+ * relative force error; each leaf takes the cheapest of the ncand candidates
+ * (relative costs cost[c]) predicted at or below target_rms, else the most
+ * accurate.  params: host memory. */
+int tc_set_order_model(tc_context *ctx, const float *params, int nparams, const int *q,
+                       const float *theta, const float *cost, int ncand, double target_rms);
+/* After an evaluation with per-cell orders: the number of leaves; per leaf
+ * the 8 features (level/16, log10 R/R_root, log2(bodies)/6, log10 mass/mean
+ * body mass, log10 of the order-2 and order-4 multipole norms over M_0 R^n,
+ * log10(1 + M2L / P2P list length in the probe walk)), the leaf index of
+ * every body (caller order) and, in mode 2, each leaf's predicted log10
+ * error.  Device pointers; any may be NULL. */
+int tc_leaf_count(const tc_context *ctx);
+int tc_leaf_data(tc_context *ctx, float *features, int *leaf_of_body, float *pred);
+/* log10 of each leaf's rms relative force error of acc against acc_ref
+ * ([n][3] device arrays, caller order) on the last evaluation's tree: the
+ * measurement the MLP is trained on. */
+int tc_leaf_errors(tc_context *ctx, const double *acc, const double *acc_ref, float *out);
+/* M2L cell pairs of the last evaluation per truncation order, counts[0..maxp]. */
+int tc_m2l_order_counts(const tc_context *ctx, long long *counts, int maxp);
+/* Training kernels, FP32, device pointers.  X: [B][10] inputs, T: [B]
+ * targets; tc_mlp_grad writes the gradient of mean((Y - T)^2) into G
+ * (nparams floats) and the loss into *loss; tc_mlp_adam applies one Adam
+ * step (beta 0.9 / 0.999, eps 1e-8, step >= 1) to params with moments m, v. */
+int tc_mlp_param_count(void);
+int tc_mlp_input_count(void);
+int tc_mlp_forward(tc_context *ctx, const float *params, const float *X, int B, float *Y);
+int tc_mlp_grad(tc_context *ctx, const float *params, const float *X, const float *T, int B,
+                float *G, float *loss);
+int tc_mlp_adam(tc_context *ctx, float *params, const float *G, float *m, float *v, float lr,
+                int step);
+
 /* Drop-in for get_force_and_potential on the reference's host Body array:
  * F, J, p are overwritten (the reference driver zeroes them before the
  * call, integrator.h:350-366), acc_old is written under TC_MAC_ERROR, the
@@ -116,6 +163,20 @@ int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n);
  * is queued on the context's stream and the call returns after it finishes. */
 int tc_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                 const unsigned char *flags, double *pot, double *acc, double *jerk);
+
+/* Many independent systems in one evaluation (a forest): cluster k is the
+ * bodies [starts[k], starts[k+1]) of the device arrays (starts: host, ncl + 1
+ * increasing ints from 0, at least one body each).  Every cluster gets its
+ * own root (own bounds and root shift: three drand48() values per cluster in
+ * cluster order, or the fixed tc_set_shift values), no pair of cells of two
+ * clusters interacts, and each phase (tree levels, walk steps, M2L, P2P,
+ * L2L, L2P) runs once for all clusters.  The result equals evaluating every
+ * cluster on its own context (per cluster, the reference's
+ * get_force_and_potential, integrator.h:308-432), to summation round-off.
+ * No all-pairs branch; not with per-cell orders. */
+int tc_eval_clusters(tc_context *ctx, int ncl, const int *starts, const double *x,
+                     const double *v, const double *m, const unsigned char *flags, double *pot,
+                     double *acc, double *jerk);
 
 /* |a| estimates of the last TC_MAC_ERROR evaluation's pre-pass (the values
  * the reference stores in Body.acc_old, integrator.h:350-356), in the
@@ -305,6 +366,13 @@ typedef struct tc_particles {
   double *acc_old;   /* [n]    */
 } tc_particles;
 
+/* TC_EVOLVE_JERK_FMM (mode bit): every force of the loop is instead the jerk
+ * build's get_force_and_potential (tc_eval_soa: the context's theta-MAC or
+ * error-MAC FMM, or its per-cell orders), which has no time-step estimate:
+ * every body keeps time step +inf, so nothing is split off and the loop is
+ * the single-level fourth-order scheme (five force evaluations per call).
+ * Not with TC_MASTER_FMM / TC_MASTER_PAIR_MASS. */
+enum { TC_EVOLVE_JERK_FMM = 8 };
 int tc_master_evolve(tc_context *ctx, const tc_particles *p, double *simtime, double dt,
                      int mode, double dt_param);
 

@@ -58,6 +58,7 @@ particle_dtype = np.dtype({
 TC_MASTER_STEPS = 1
 TC_MASTER_FMM = 2
 TC_MASTER_PAIR_MASS = 4
+TC_EVOLVE_JERK_FMM = 8
 
 
 class TcConfig(C.Structure):
@@ -106,6 +107,16 @@ def lib():
         L.tc_set_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_last_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_set_precision.argtypes = [vp, C.c_int, C.c_double]
+        L.tc_set_adaptive.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.tc_eval_clusters.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.tc_set_order_model.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_double]
+        L.tc_leaf_count.argtypes = [vp]
+        L.tc_leaf_data.argtypes = [vp, vp, vp, vp]
+        L.tc_leaf_errors.argtypes = [vp, vp, vp, vp]
+        L.tc_m2l_order_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
+        L.tc_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.tc_mlp_grad.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp]
+        L.tc_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_int]
         L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.tc_direct_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
@@ -141,7 +152,9 @@ def lib():
 
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
-                    "tc_set_precision",
+                    "tc_set_precision", "tc_eval_clusters", "tc_set_adaptive", "tc_set_order_model", "tc_leaf_count",
+                    "tc_leaf_data", "tc_leaf_errors", "tc_m2l_order_counts", "tc_mlp_param_count",
+                    "tc_mlp_input_count", "tc_mlp_forward", "tc_mlp_grad", "tc_mlp_adam",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_direct_soa_noalloc",
                     "tc_get_acc_old",
                     "tc_get_cell_layout", "tc_P2M", "tc_M2M", "tc_M2L", "tc_L2L", "tc_L2P",
@@ -226,6 +239,22 @@ def check_tensors(specs, n, device_type="cuda"):
     return dev
 
 
+def mlp_param_count():
+    return lib().tc_mlp_param_count()
+
+
+def _mlp_check(params, X):
+    import torch
+    if params.dtype != torch.float32 or params.shape != (mlp_param_count(),) or \
+            not params.is_contiguous() or params.device.type != "cuda":
+        raise TypeError("params must be a contiguous float32 CUDA tensor of %d values"
+                        % mlp_param_count())
+    if X.dtype != torch.float32 or X.dim() != 2 or X.shape[1] != lib().tc_mlp_input_count() \
+            or not X.is_contiguous() or X.device != params.device:
+        raise TypeError("X must be a contiguous float32 (B, %d) tensor on the params' device"
+                        % lib().tc_mlp_input_count())
+
+
 class Fmm:
     """One FMM configuration on the current GPU (see include/tcfmm.h)."""
 
@@ -301,6 +330,29 @@ class Fmm:
         self._last_n = n
         self._check(lib().tc_eval_soa(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(flags),
                                       _ptr(pot), _ptr(acc), _ptr(jerk)), "tc_eval_soa")
+        return pot, acc, jerk
+
+    def eval_clusters(self, starts, x, v, m, flags=None, pot=None, acc=None, jerk=None):
+        """tc_eval_clusters: independent clusters [starts[k], starts[k+1]) of the
+        device arrays in one evaluation (starts: host ints, starts[0] = 0,
+        starts[-1] = n)."""
+        import numpy as np
+        import torch
+        st = np.ascontiguousarray(starts, dtype=np.int32)
+        n = m.shape[0]
+        if st.ndim != 1 or st.size < 2 or int(st[-1]) != n:
+            raise ValueError("starts must hold ncl + 1 offsets ending at n = %d" % n)
+        f8, u1 = torch.float64, torch.uint8
+        dev = check_tensors([("x", x, f8, (3,)), ("v", v, f8, (3,)), ("m", m, f8, ()),
+                             ("flags", flags, u1, ()), ("pot", pot, f8, ()),
+                             ("acc", acc, f8, (3,)), ("jerk", jerk, f8, (3,))], n)
+        pot = torch.empty(n, dtype=f8, device=dev) if pot is None else pot
+        acc = torch.empty(n, 3, dtype=f8, device=dev) if acc is None else acc
+        jerk = torch.empty(n, 3, dtype=f8, device=dev) if jerk is None else jerk
+        self._last_n = n
+        self._check(lib().tc_eval_clusters(self.h, st.size - 1, st.ctypes.data, _ptr(x), _ptr(v),
+                                           _ptr(m), _ptr(flags), _ptr(pot), _ptr(acc),
+                                           _ptr(jerk)), "tc_eval_clusters")
         return pot, acc, jerk
 
     def acc_old(self):
@@ -516,6 +568,78 @@ class Fmm:
         self._check(lib().tc_master_P2P(self.h, bodies.ctypes.data, bodies.size,
                                         cells.ctypes.data, cells.size, pt, ps, t.size,
                                         int(mode), float(dt_param)), "tc_master_P2P")
+
+    # ---- per-cell order and angle (learned choice) --------------------------
+    def set_adaptive(self, mode, q=0, theta=0.0, probe_theta=0.7):
+        """tc_set_adaptive: 0 off, 1 every leaf at (q, theta), 2 the learned choice."""
+        self._check(lib().tc_set_adaptive(self.h, int(mode), int(q), float(theta),
+                                          float(probe_theta)), "tc_set_adaptive")
+
+    def set_order_model(self, params, q, theta, cost, target_rms):
+        """The MLP parameters (float32, mlp_param_count() values) and the candidate
+        grid: orders q, angles theta, relative costs (one each per candidate)."""
+        import numpy as np
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        qa = np.ascontiguousarray(q, dtype=np.int32)
+        ta = np.ascontiguousarray(theta, dtype=np.float32)
+        ca = np.ascontiguousarray(cost, dtype=np.float32)
+        if not (qa.size == ta.size == ca.size):
+            raise ValueError("q, theta and cost need one entry per candidate")
+        self._check(lib().tc_set_order_model(self.h, P.ctypes.data, P.size, qa.ctypes.data,
+                                             ta.ctypes.data, ca.ctypes.data, qa.size,
+                                             float(target_rms)), "tc_set_order_model")
+
+    def leaf_data(self):
+        """After an evaluation with per-cell orders: (features [nleaves, 8],
+        leaf index of every body [n], predicted log10 error [nleaves] or None)."""
+        import torch
+        nl = lib().tc_leaf_count(self.h)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        f = torch.empty(nl, 8, dtype=torch.float32, device=dev)
+        lob = torch.empty(self._last_n, dtype=torch.int32, device=dev)
+        pred = torch.empty(nl, dtype=torch.float32, device=dev)
+        self._check(lib().tc_leaf_data(self.h, _ptr(f), _ptr(lob), _ptr(pred)), "tc_leaf_data")
+        return f, lob, pred
+
+    def leaf_errors(self, acc, acc_ref):
+        """log10 rms relative force error per leaf of the last evaluation's tree."""
+        import torch
+        dev = check_tensors([("acc", acc, torch.float64, (3,)),
+                             ("acc_ref", acc_ref, torch.float64, (3,))], self._last_n)
+        out = torch.empty(lib().tc_leaf_count(self.h), dtype=torch.float32, device=dev)
+        self._check(lib().tc_leaf_errors(self.h, _ptr(acc), _ptr(acc_ref), _ptr(out)),
+                    "tc_leaf_errors")
+        return out
+
+    def m2l_order_counts(self):
+        """M2L pairs of the last evaluation per truncation order (index = order)."""
+        buf = (C.c_longlong * 25)()
+        lib().tc_m2l_order_counts(self.h, buf, 24)
+        return list(buf)
+
+    def mlp_forward(self, params, X):
+        import torch
+        Y = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+        _mlp_check(params, X)
+        self._check(lib().tc_mlp_forward(self.h, _ptr(params), _ptr(X), X.shape[0], _ptr(Y)),
+                    "tc_mlp_forward")
+        return Y
+
+    def mlp_grad(self, params, X, T):
+        """(gradient of mean((Y - T)^2) w.r.t. params, loss)."""
+        import torch
+        _mlp_check(params, X)
+        if T.dtype != torch.float32 or T.shape != (X.shape[0],) or not T.is_contiguous():
+            raise TypeError("T must be a contiguous float32 tensor of shape (B,)")
+        G = torch.empty_like(params)
+        loss = torch.empty(1, dtype=torch.float32, device=X.device)
+        self._check(lib().tc_mlp_grad(self.h, _ptr(params), _ptr(X), _ptr(T), X.shape[0],
+                                      _ptr(G), _ptr(loss)), "tc_mlp_grad")
+        return G, float(loss.item())
+
+    def mlp_adam(self, params, G, m, v, lr, step):
+        self._check(lib().tc_mlp_adam(self.h, _ptr(params), _ptr(G), _ptr(m), _ptr(v),
+                                      float(lr), int(step)), "tc_mlp_adam")
 
     def stats(self):
         st = TcStats()

@@ -35,7 +35,7 @@ CFLAGS = [
 NSHARDS = 3
 SHARDED = ["tc_m2l.hip", "tc_upward.hip", "tc_downward.hip"]
 PLAIN = ["tc_tree.hip", "tc_p2p.hip", "tc_low.hip", "tc_batch.hip", "tc_master.hip",
-         "tc_evolve.hip", "tc_noalloc.hip", "tc_dispatch.hip", "tc_engine.hip"]
+         "tc_evolve.hip", "tc_noalloc.hip", "tc_dispatch.hip", "tc_learn.hip", "tc_engine.hip"]
 UNITS = [(u, k) for u in SHARDED for k in range(NSHARDS)] + [(u, None) for u in PLAIN]
 
 

@@ -64,6 +64,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
 // Wave-wide argmax of (e, idx): larger e wins, ties -> smaller idx.
 __device__ __forceinline__ void wave_argmax(double &e, int &idx) {
 #pragma unroll

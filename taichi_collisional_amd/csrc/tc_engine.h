@@ -120,7 +120,7 @@ struct tc_context {
   DBuf front_a, front_b, m2l, p2p, tcnt, tblk, tscan, tstate, tcls;
   int trav_steps[4] = {0, 0, 0, 0};  // steps of the last walk per MAC mode (batch size)
   DBuf gsort1[4], gsort2[4];  // list grouping scratch per stream: keys in, out, sources, temp
-  int *h_pin = nullptr;  // small pinned host buffer for per-step counts
+  int *h_pin = nullptr;  // small pinned host buffer for per-step counts (256 ints)
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
   // error-controlled MAC: p = 2 pre-pass, multipole norms, acc_old output
@@ -155,6 +155,24 @@ struct tc_context {
   DBuf ev_all, ev_ux, ev_uv, ev_um, ev_ufl, ev_upot, ev_uacc, ev_uts, ev_uao, ev_tmp,
       ev_flags, ev_offs, ev_scan_tmp;
   int ev_all_n = 0;  // entries of the identity list written so far
+  // per-cell order and angle (tc_learn.hip): adaptive 0 = off (every pair at
+  // cfg.p, cfg.theta), 1 = every leaf at (ad_q, ad_theta) through the
+  // per-cell path, 2 = the learned choice.  The probe walk at probe_theta
+  // gives the list-length features.
+  // forest of independent clusters (tc_eval_clusters): ncl roots, cluster k
+  // = bodies [start[k], start[k+1]) of the input; ncl = 1 otherwise
+  int ncl = 1;
+  DBuf cl_start, cl_id;
+  int adaptive = 0, ad_q = 0;
+  double ad_theta = 0.0, probe_theta = 0.7;
+  float log_target = -7.0f;
+  OrderGrid grid{};
+  bool have_model = false;
+  DBuf mlp_params, cq, ctheta, probe_m2l, probe_p2p, leaf_feat, leaf_pred;
+  DBuf chunk_part, chunk_hist;
+  long long m2l_by_order[kMaxP + 1] = {};  // M2L pairs per truncation order
+  DBuf mlp_Y, mlp_H, mlp_D, mlp_dY;         // tc_mlp_grad scratch
+  int mlp_cap = 0;
 };
 
 #define TC_CHECK(expr)                                                   \
@@ -172,6 +190,10 @@ struct tc_context {
 int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        const double *m, const unsigned char *flags, int mode, double dt,
                        double *pot, double *acc, double *tstep, double *acc_old);
+// The jerk build's get_force_and_potential on device arrays (tc_eval_soa);
+// used by the step loop's TC_EVOLVE_JERK_FMM mode.
+int eval_soa_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                    const unsigned char *flags, double *pot, double *acc, double *jerk);
 
 // Wait for the work queued on s by polling an event (spinning) instead of
 // hipStreamSynchronize, for the short waits inside an evaluation where the

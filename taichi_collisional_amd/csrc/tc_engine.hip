@@ -89,7 +89,8 @@ static int build_tree(tc_context *ctx, int n) {
   const int ncrit = ctx->cfg.ncrit;
   // no wait for the sort queued before: the cell arrays are not in its use,
   // and a reallocation's hipFree synchronizes by itself
-  if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64), 0)) return e;
+  if (int e = ensure_cells(ctx, std::max(1024, 4 * n / std::max(ncrit, 1) + 64 + ctx->ncl), 0))
+    return e;
   TC_CHECK(ctx->tree_state.ensure(sizeof(TreeState)));
   TC_CHECK(ctx->h_tree.ensure(sizeof(TreeState) / sizeof(int)));
   TreeState *hst = reinterpret_cast<TreeState *>(ctx->h_tree.p);
@@ -104,9 +105,10 @@ static int build_tree(tc_context *ctx, int n) {
     TC_CHECK(ctx->split_scan.ensure(sizeof(int) * 2 * (size_t)cap));
     std::memset(hst, 0, sizeof(TreeState));
     hst->ls = 0;
-    hst->le = 1;
+    hst->le = ctx->ncl;  // one root, or one per cluster of a forest
     TC_CHECK(hipMemcpyAsync(ctx->tree_state.p, hst, sizeof(TreeState), hipMemcpyHostToDevice, s));
-    launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s);
+    launch_root(cells_of(ctx), n, ctx->tp.as<TreeParams>(), s,
+                ctx->ncl > 1 ? ctx->cl_start.as<int>() : nullptr, ctx->ncl);
     int queued = 0, batch = std::max(4, ctx->tree_levels + 1);
     for (;;) {
       for (int i = 0; i < batch; i++, queued++)
@@ -230,6 +232,11 @@ static int upward_min(tc_context *ctx) {
 // whose output would not fit its buffer ends the walk with a flag and what it
 // needed: the buffers then grow and the walk starts again (in practice only
 // on the first evaluation of a workload).
+__global__ void k_root_pairs(int2 *__restrict__ f, int ncl) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < ncl) f[k] = make_int2(k, k);
+}
+
 static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
   hipStream_t s = ctx->stream;
   TravState *hst = reinterpret_cast<TravState *>(ctx->h_pin + 40);
@@ -254,11 +261,10 @@ static int traverse(tc_context *ctx, int n, const MacArgs &mac) {
                     cf = std::min({cap(ctx->front_a), cap(ctx->front_b),
                                    (long long)std::min<size_t>(ctx->tcls.bytes, 0x7fffffff)});
     std::memset(hst, 0, sizeof(TravState));
-    hst->nin[0] = 1;
-    ctx->h_pin[32] = ctx->h_pin[33] = 0;  // the root pair (0, 0)
+    hst->nin[0] = ctx->ncl;  // the root pair (0, 0), or (k, k) for every cluster
     TC_CHECK(hipMemcpyAsync(ctx->tstate.p, hst, sizeof(TravState), hipMemcpyHostToDevice, s));
-    TC_CHECK(hipMemcpyAsync(ctx->front_a.p, ctx->h_pin + 32, sizeof(int2), hipMemcpyHostToDevice,
-                            s));
+    hipLaunchKernelGGL(k_root_pairs, dim3((ctx->ncl + 255) / 256), dim3(256), 0, s,
+                       ctx->front_a.as<int2>(), ctx->ncl);
     int k = 0, batch = std::max(8, known + 1);
     for (;;) {
       for (int i = 0; i < batch; i++, k++) {
@@ -491,13 +497,25 @@ static int copy_acc_old(tc_context *ctx, int n, double *out) {
 static int sort_and_build(tc_context *ctx, int n, const double *x, const double *v,
                           const double *m, const unsigned char *flags) {
   hipStream_t s = ctx->stream;
-  launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
-                ctx->dshift.as<double>(), s);
+  const bool forest = ctx->ncl > 1;
+  if (forest) {
+    launch_bounds_clusters(x, ctx->ncl, ctx->cl_start.as<int>(), ctx->tp.as<TreeParams>(),
+                           ctx->dshift.as<double>(), s);
+    launch_cluster_ids(ctx->cl_start.as<int>(), ctx->ncl, n, ctx->cl_id.as<int>(), s);
+  } else {
+    launch_bounds(x, n, ctx->part.as<double>(), 1024, ctx->tp.as<TreeParams>(),
+                  ctx->dshift.as<double>(), s);
+  }
   launch_morton(x, n, ctx->tp.as<TreeParams>(), ctx->key_hi.as<uint64_t>(),
-                ctx->key_a.as<uint64_t>(), ctx->val_a.as<int>(), s);
+                ctx->key_a.as<uint64_t>(), ctx->val_a.as<int>(), s,
+                forest ? ctx->cl_id.as<int>() : nullptr);
   if (launch_sort(ctx->key_a.as<uint64_t>(), ctx->key_b.as<uint64_t>(), ctx->val_a.as<int>(),
                   ctx->val_b.as<int>(), ctx->key_hi.as<uint64_t>(), n, ctx->sort_tmp.p,
-                  ctx->sort_tmp.bytes, s)) {
+                  ctx->sort_tmp.bytes, s) ||
+      (forest && launch_sort_clusters(ctx->key_a.as<uint64_t>(), ctx->key_b.as<uint64_t>(),
+                                      ctx->val_a.as<int>(), ctx->val_b.as<int>(),
+                                      ctx->cl_id.as<int>(), ctx->ncl, n, ctx->sort_tmp.p,
+                                      ctx->sort_tmp.bytes, s))) {
     ctx->err = "radix sort failed";
     return -4;
   }
@@ -505,11 +523,136 @@ static int sort_and_build(tc_context *ctx, int n, const double *x, const double 
   return build_tree(ctx, n);
 }
 
+// Per-cell orders and angles (tc_learn.hip): a probe walk at probe_theta
+// gives each leaf's M2L and P2P list lengths; with the moments done, the
+// leaves' features are formed and every leaf takes its (q, theta) -- the
+// learned choice, or the uniform one -- and the internal cells take the
+// largest order and smallest angle below them.
+__global__ void k_mass_sum(const double *__restrict__ m, int n, double *__restrict__ out) {
+  double s = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += m[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+static int choose_cell_orders(tc_context *ctx, int n, const double *m) {
+  hipStream_t s = ctx->stream;
+  const int nc = ctx->st.ncells, nl = ctx->nleaves;
+  TC_CHECK(ctx->cq.ensure(sizeof(int) * (size_t)nc));
+  TC_CHECK(ctx->ctheta.ensure(sizeof(double) * (size_t)nc));
+  TC_CHECK(ctx->probe_m2l.ensure(sizeof(int) * (size_t)nc));
+  TC_CHECK(ctx->probe_p2p.ensure(sizeof(int) * (size_t)nc));
+  TC_CHECK(ctx->leaf_feat.ensure(sizeof(float) * kLeafFeatures * (size_t)std::max(nl, 1)));
+  TC_CHECK(ctx->leaf_pred.ensure(sizeof(float) * (size_t)std::max(nl, 1)));
+  MacArgs pa = mac_args(ctx, kMacTheta);
+  pa.theta = ctx->probe_theta;
+  if (int e = traverse(ctx, n, pa)) return e;
+  TC_CHECK(hipMemsetAsync(ctx->probe_m2l.p, 0, sizeof(int) * (size_t)nc, s));
+  TC_CHECK(hipMemsetAsync(ctx->probe_p2p.p, 0, sizeof(int) * (size_t)nc, s));
+  launch_count_targets(ctx->m2l.as<int2>(), (int)ctx->st.n_m2l, ctx->probe_m2l.as<int>(), s);
+  launch_count_targets(ctx->p2p.as<int2>(), (int)ctx->st.n_p2p_cells, ctx->probe_p2p.as<int>(),
+                       s);
+  // mean body mass (sum over the gathered bodies)
+  double *msum = reinterpret_cast<double *>(ctx->h_pin + 120);
+  TC_CHECK(ctx->accout.ensure(sizeof(double)));
+  TC_CHECK(hipMemsetAsync(ctx->accout.p, 0, sizeof(double), s));
+  hipLaunchKernelGGL(k_mass_sum, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, s,
+                     bodies_of(ctx).q, n, ctx->accout.as<double>());
+  TC_CHECK(hipMemcpyAsync(msum, ctx->accout.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  TC_CHECK(hipStreamWaitEvent(s, ctx->ev[13], 0));  // the moments
+  TC_CHECK(spin_wait(ctx, s));
+  const double mean_mass = *msum / n;
+  (void)m;
+  launch_leaf_features(cells_of(ctx), ctx->leaf_ids.as<int>(), nl, ctx->cfg.p,
+                       ctx->probe_m2l.as<int>(), ctx->probe_p2p.as<int>(), mean_mass,
+                       ctx->leaf_feat.as<float>(), s);
+  if (ctx->adaptive == 2) {
+    launch_leaf_select(ctx->mlp_params.as<float>(), ctx->leaf_feat.as<float>(),
+                       ctx->leaf_ids.as<int>(), nl, ctx->grid, ctx->log_target,
+                       ctx->cq.as<int>(), ctx->ctheta.as<double>(), ctx->leaf_pred.as<float>(),
+                       s);
+  } else {
+    launch_leaf_uniform(ctx->leaf_ids.as<int>(), nl, ctx->ad_q, ctx->ad_theta, ctx->cq.as<int>(),
+                        ctx->ctheta.as<double>(), s);
+  }
+  const int L = (int)ctx->level_start.size() - 1;
+  for (int l = L - 1; l >= 0; l--)
+    launch_propagate(cells_of(ctx), ctx->level_start[l], ctx->level_start[l + 1],
+                     ctx->cq.as<int>(), ctx->ctheta.as<double>(), s);
+  return 0;
+}
+
+// M2L chunks (grouped by target) partitioned by their target's order, and
+// one launch per order present; M2L pair counts per order for the flop count.
+__global__ void k_chunk_hist(const int4 *__restrict__ ch, int nch, const int *__restrict__ q,
+                             int *__restrict__ hist) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nch) return;
+  const int4 c = ch[k];
+  atomicAdd(&hist[q[c.x]], 1);
+  atomicAdd(&hist[kMaxP + 1 + q[c.x]], c.z);
+}
+
+__global__ void k_chunk_scatter(const int4 *__restrict__ ch, int nch, const int *__restrict__ q,
+                                int *__restrict__ cur, int4 *__restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nch) return;
+  const int4 c = ch[k];
+  out[atomicAdd(&cur[q[c.x]], 1)] = c;
+}
+
+static int m2l_by_order(tc_context *ctx, M2LArgs ma, int nchunks) {
+  hipStream_t s = ctx->stream;
+  const int H = kMaxP + 1;
+  TC_CHECK(ctx->chunk_hist.ensure(sizeof(int) * 3 * H));
+  TC_CHECK(ctx->chunk_part.ensure(sizeof(int4) * (size_t)std::max(nchunks, 1)));
+  int *hist = ctx->chunk_hist.as<int>();
+  TC_CHECK(hipMemsetAsync(hist, 0, sizeof(int) * 3 * H, s));
+  if (nchunks > 0)
+    hipLaunchKernelGGL(k_chunk_hist, dim3((nchunks + 255) / 256), dim3(256), 0, s, ma.chunks,
+                       nchunks, ctx->cq.as<int>(), hist);
+  int *h = ctx->h_pin + 64;  // [64, 64 + 2H): chunk and pair counts per order
+  TC_CHECK(hipMemcpyAsync(h, hist, sizeof(int) * 2 * H, hipMemcpyDeviceToHost, s));
+  TC_CHECK(spin_wait(ctx, s));
+  int off[kMaxP + 2];
+  off[0] = 0;
+  for (int q = 0; q < H; q++) off[q + 1] = off[q] + h[q];
+  for (int q = 0; q < H; q++) ctx->m2l_by_order[q] = h[H + q];
+  // cursors start at each order's offset
+  TC_CHECK(hipMemcpyAsync(hist + 2 * H, off, sizeof(int) * H, hipMemcpyHostToDevice, s));
+  if (nchunks > 0)
+    hipLaunchKernelGGL(k_chunk_scatter, dim3((nchunks + 255) / 256), dim3(256), 0, s, ma.chunks,
+                       nchunks, ctx->cq.as<int>(), hist + 2 * H, ctx->chunk_part.as<int4>());
+  const int P = ctx->cfg.p;
+  for (int q = 0; q <= P; q++) {
+    if (off[q + 1] == off[q]) continue;
+    if (q < 2) {
+      ctx->err = "per-cell order below 2";
+      return -2;
+    }
+    M2LArgs mq = ma;
+    mq.chunks = ctx->chunk_part.as<int4>() + off[q];
+    mq.nchunks = off[q + 1] - off[q];
+    mq.stride = nterm(P);
+    mq.quirk = ma.quirk && q == P;
+    if (launch_m2l(q, mq, s)) {
+      ctx->err = "expansion order not compiled";
+      return -2;
+    }
+  }
+  if (off[H] != off[P + 1]) {
+    ctx->err = "per-cell order above the context's order";
+    return -2;
+  }
+  return 0;
+}
+
 static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                        const unsigned char *flags, double *pot, double *acc, double *jerk) {
   hipStream_t s = ctx->stream;
   const tc_config &cfg = ctx->cfg;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
+  std::memset(ctx->m2l_by_order, 0, sizeof(ctx->m2l_by_order));
   ctx->has_acc_old = false;
   ctx->last_n = n;
   if (n <= 0) {
@@ -518,19 +661,31 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   }
   if (int e = ensure_bodies(ctx, n)) return e;
   // root-shift values: three drand48() per evaluation, as getBounds draws them
+  // (a forest: three per cluster, in cluster order)
   double u[3];
-  if (ctx->fixed_shift) {
-    std::memcpy(u, ctx->shift, sizeof(u));
+  if (ctx->ncl > 1) {
+    std::vector<double> us(3 * (size_t)ctx->ncl);
+    for (int k = 0; k < ctx->ncl; k++) {
+      for (int d = 0; d < 3; d++)
+        u[d] = ctx->fixed_shift ? ctx->shift[d] : ctx->rng.next() - 0.5;
+      std::memcpy(&us[3 * (size_t)k], u, sizeof(u));
+    }
+    TC_CHECK(hipMemcpy(ctx->dshift.p, us.data(), sizeof(double) * us.size(),
+                       hipMemcpyHostToDevice));
   } else {
-    for (int d = 0; d < 3; d++) u[d] = ctx->rng.next() - 0.5;
+    if (ctx->fixed_shift) {
+      std::memcpy(u, ctx->shift, sizeof(u));
+    } else {
+      for (int d = 0; d < 3; d++) u[d] = ctx->rng.next() - 0.5;
+    }
+    // through pinned memory: a copy from pageable memory is staged and holds the host
+    std::memcpy(ctx->h_pin + 24, u, sizeof(u));
+    TC_CHECK(hipMemcpyAsync(ctx->dshift.p, ctx->h_pin + 24, sizeof(u), hipMemcpyHostToDevice, s));
   }
   std::memcpy(ctx->shift, u, sizeof(u));
-  // through pinned memory: a copy from pageable memory is staged and holds the host
-  std::memcpy(ctx->h_pin + 24, u, sizeof(u));
-  TC_CHECK(hipMemcpyAsync(ctx->dshift.p, ctx->h_pin + 24, sizeof(u), hipMemcpyHostToDevice, s));
   TC_CHECK(hipEventRecord(ctx->ev[0], s));
 
-  if (cfg.direct_below > 0 && n < cfg.direct_below) {
+  if (cfg.direct_below > 0 && n < cfg.direct_below && ctx->ncl == 1) {
     // all-pairs sum in the caller's order (direct(), traverse_eager.h:295-304)
     std::vector<int> idx(n);
     for (int i = 0; i < n; i++) idx[i] = i;
@@ -586,7 +741,15 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   }
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
   // ---- traversal and lists
-  if (int e = traverse(ctx, n, mac_args(ctx, err_mac ? kMacError : kMacTheta))) return e;
+  const bool per_cell = !err_mac && ctx->adaptive != 0;
+  if (per_cell) {
+    if (int e = choose_cell_orders(ctx, n, m)) return e;
+    MacArgs ma = mac_args(ctx, kMacTheta);
+    ma.theta_cell = ctx->ctheta.as<double>();
+    if (int e = traverse(ctx, n, ma)) return e;
+  } else {
+    if (int e = traverse(ctx, n, mac_args(ctx, err_mac ? kMacError : kMacTheta))) return e;
+  }
   TC_CHECK(hipEventRecord(ctx->ev[3], s));
   int nchunks = 0;
   if (int e = build_lists(ctx, nchunks)) return e;
@@ -604,9 +767,14 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   ma.L = ctx->cL.as<double>();
   ma.Ld = ctx->cLd.as<double>();
   ma.quirk = cfg.quirk_factorial;
-  if (launch_m2l(cfg.p, ma, s)) {
-    ctx->err = "expansion order not compiled";
-    return -2;
+  if (per_cell) {
+    if (int e = m2l_by_order(ctx, ma, nchunks)) return e;
+  } else {
+    if (launch_m2l(cfg.p, ma, s)) {
+      ctx->err = "expansion order not compiled";
+      return -2;
+    }
+    ctx->m2l_by_order[cfg.p] = ctx->st.n_m2l;
   }
   // the GPU runs M2L while the host builds the P2P work items
   if (int e = build_p2p_items(ctx)) return e;
@@ -824,6 +992,7 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        double *pot, double *acc, double *tstep, double *acc_old) {
   hipStream_t s = ctx->stream;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
+  std::memset(ctx->m2l_by_order, 0, sizeof(ctx->m2l_by_order));
   ctx->has_acc_old = false;
   ctx->last_n = n;
   const bool steps = mode & TC_MASTER_STEPS, pair_mass = mode & TC_MASTER_PAIR_MASS;
@@ -890,6 +1059,11 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
 }
 
 // ------------------------------------------------------------------ C ABI ---
+int eval_soa_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
+                    const unsigned char *flags, double *pot, double *acc, double *jerk) {
+  return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+}
+
 extern "C" {
 
 int tc_supported_orders(int *orders, int maxn) {
@@ -934,7 +1108,7 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     tc_destroy(ctx);
     return -3;
   }
-  if (hipHostMalloc((void **)&ctx->h_pin, 64 * sizeof(int)) != hipSuccess) {
+  if (hipHostMalloc((void **)&ctx->h_pin, 256 * sizeof(int)) != hipSuccess) {
     tc_destroy(ctx);
     return -3;
   }
@@ -970,7 +1144,9 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->lowF, &ctx->cPn, &ctx->accout, &ctx->lowP, &ctx->ev_all, &ctx->ev_ux,
                  &ctx->ev_uv, &ctx->ev_um, &ctx->ev_ufl, &ctx->ev_upot, &ctx->ev_uacc,
                  &ctx->ev_uts, &ctx->ev_uao, &ctx->ev_tmp, &ctx->ev_flags, &ctx->ev_offs,
-                 &ctx->ev_scan_tmp};
+                 &ctx->ev_scan_tmp, &ctx->mlp_params, &ctx->cq, &ctx->ctheta, &ctx->probe_m2l,
+                 &ctx->probe_p2p, &ctx->leaf_feat, &ctx->leaf_pred, &ctx->chunk_part,
+                 &ctx->chunk_hist, &ctx->mlp_Y, &ctx->mlp_H, &ctx->mlp_D, &ctx->mlp_dY};
   for (DBuf *b : all) b->release();
   for (DBuf &b : ctx->ev_lev) b.release();
   ctx->h_leaves.release();
@@ -1023,6 +1199,135 @@ int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius) {
   return 0;
 }
 
+int tc_set_adaptive(tc_context *ctx, int mode, int q, double theta, double probe_theta) {
+  if (!ctx) return -1;
+  if (mode < 0 || mode > 2 || !(probe_theta > 0 && probe_theta < 2)) {
+    ctx->err = "tc_set_adaptive: mode 0, 1 or 2 and 0 < probe_theta < 2";
+    return -1;
+  }
+  if (mode == 1 && (q < 2 || q > ctx->cfg.p || !(theta > 0 && theta < 2))) {
+    ctx->err = "tc_set_adaptive: uniform order must be in [2, p] and 0 < theta < 2";
+    return -1;
+  }
+  if (mode == 2 && !ctx->have_model) {
+    ctx->err = "tc_set_adaptive: no order model (tc_set_order_model)";
+    return -1;
+  }
+  if (mode != 0 && ctx->cfg.mac != TC_MAC_THETA) {
+    ctx->err = "tc_set_adaptive: per-cell orders need the theta MAC";
+    return -1;
+  }
+  ctx->adaptive = mode;
+  ctx->ad_q = q;
+  ctx->ad_theta = theta;
+  ctx->probe_theta = probe_theta;
+  return 0;
+}
+
+int tc_set_order_model(tc_context *ctx, const float *params, int nparams, const int *q,
+                       const float *theta, const float *cost, int ncand, double target_rms) {
+  if (!ctx) return -1;
+  if (!params || nparams != kMlpParams || !q || !theta || !cost || ncand < 1 ||
+      ncand > kMaxCand || !(target_rms > 0)) {
+    ctx->err = "tc_set_order_model: bad arguments";
+    return -1;
+  }
+  OrderGrid g{};
+  g.ncand = ncand;
+  for (int c = 0; c < ncand; c++) {
+    if (q[c] < 2 || q[c] > ctx->cfg.p || !(theta[c] > 0 && theta[c] < 2)) {
+      ctx->err = "tc_set_order_model: candidate order outside [2, p] or angle outside (0, 2)";
+      return -1;
+    }
+    g.q[c] = q[c];
+    g.theta[c] = theta[c];
+    g.cost[c] = cost[c];
+  }
+  TC_CHECK(ctx->mlp_params.ensure(sizeof(float) * kMlpParams));
+  TC_CHECK(hipMemcpy(ctx->mlp_params.p, params, sizeof(float) * kMlpParams,
+                     hipMemcpyHostToDevice));
+  ctx->grid = g;
+  ctx->log_target = (float)std::log10(target_rms);
+  ctx->have_model = true;
+  return 0;
+}
+
+int tc_leaf_count(const tc_context *ctx) { return ctx ? ctx->nleaves : -1; }
+
+int tc_leaf_data(tc_context *ctx, float *features, int *leaf_of_body, float *pred) {
+  if (!ctx) return -1;
+  if (ctx->adaptive == 0 || ctx->last_n <= 0) {
+    ctx->err = "tc_leaf_data: the last evaluation did not choose per-cell orders";
+    return -2;
+  }
+  hipStream_t s = ctx->stream;
+  const int nl = ctx->nleaves;
+  if (features)
+    TC_CHECK(hipMemcpyAsync(features, ctx->leaf_feat.p, sizeof(float) * kLeafFeatures * nl,
+                            hipMemcpyDeviceToDevice, s));
+  if (pred && ctx->adaptive == 2)
+    TC_CHECK(hipMemcpyAsync(pred, ctx->leaf_pred.p, sizeof(float) * nl, hipMemcpyDeviceToDevice,
+                            s));
+  if (leaf_of_body)
+    launch_leaf_of_body(cells_of(ctx), bodies_of(ctx), ctx->leaf_ids.as<int>(), nl, leaf_of_body,
+                        s);
+  TC_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int tc_leaf_errors(tc_context *ctx, const double *acc, const double *acc_ref, float *out) {
+  if (!ctx || !acc || !acc_ref || !out) return -1;
+  if (ctx->last_n <= 0 || ctx->nleaves <= 0) {
+    ctx->err = "tc_leaf_errors: no tree (evaluate first)";
+    return -2;
+  }
+  launch_leaf_errors(cells_of(ctx), bodies_of(ctx), ctx->leaf_ids.as<int>(), ctx->nleaves, acc,
+                     acc_ref, out, ctx->stream);
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tc_m2l_order_counts(const tc_context *ctx, long long *counts, int maxp) {
+  if (!ctx || !counts) return -1;
+  for (int q = 0; q <= maxp; q++) counts[q] = q <= kMaxP ? ctx->m2l_by_order[q] : 0;
+  return kMaxP;
+}
+
+int tc_mlp_param_count(void) { return kMlpParams; }
+int tc_mlp_input_count(void) { return kMlpIn; }
+
+int tc_mlp_forward(tc_context *ctx, const float *params, const float *X, int B, float *Y) {
+  if (!ctx || !params || (B > 0 && (!X || !Y)) || B < 0) return -1;
+  launch_mlp_fwd(params, X, B, Y, nullptr, ctx->stream);
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tc_mlp_grad(tc_context *ctx, const float *params, const float *X, const float *T, int B,
+                float *G, float *loss) {
+  if (!ctx || !params || !G || !loss || B < 0 || (B > 0 && (!X || !T))) return -1;
+  if (B > ctx->mlp_cap) {
+    const size_t b = (size_t)B;
+    TC_CHECK(ctx->mlp_Y.ensure(sizeof(float) * b));
+    TC_CHECK(ctx->mlp_dY.ensure(sizeof(float) * b));
+    TC_CHECK(ctx->mlp_H.ensure(sizeof(float) * 3 * kMlpH * b));
+    TC_CHECK(ctx->mlp_D.ensure(sizeof(float) * 3 * kMlpH * b));
+    ctx->mlp_cap = B;
+  }
+  launch_mlp_grad(params, X, T, B, ctx->mlp_Y.as<float>(), ctx->mlp_H.as<float>(),
+                  ctx->mlp_D.as<float>(), ctx->mlp_dY.as<float>(), G, loss, ctx->stream);
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tc_mlp_adam(tc_context *ctx, float *params, const float *G, float *m, float *v, float lr,
+                int step) {
+  if (!ctx || !params || !G || !m || !v || step < 1 || !(lr > 0)) return -1;
+  launch_adam(params, G, m, v, lr, step, ctx->stream);
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int tc_last_shift(const tc_context *ctx, double frac[3]) {
   if (!ctx || !frac) return -1;
   for (int d = 0; d < 3; d++) frac[d] = ctx->shift[d];
@@ -1037,6 +1342,37 @@ int tc_eval_soa(tc_context *ctx, int n, const double *x, const double *v, const 
     return -1;
   }
   return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+}
+
+int tc_eval_clusters(tc_context *ctx, int ncl, const int *starts, const double *x,
+                     const double *v, const double *m, const unsigned char *flags, double *pot,
+                     double *acc, double *jerk) {
+  if (!ctx) return -1;
+  if (ncl < 1 || !starts || !x || !v || !m || starts[0] != 0) {
+    ctx->err = "tc_eval_clusters: ncl >= 1, starts[0] == 0, x, v and m required";
+    return -1;
+  }
+  for (int k = 0; k < ncl; k++)
+    if (starts[k + 1] <= starts[k]) {
+      ctx->err = "tc_eval_clusters: every cluster needs at least one body (starts increasing)";
+      return -1;
+    }
+  if (ctx->adaptive != 0) {
+    ctx->err = "tc_eval_clusters: per-cell orders are not available for clusters";
+    return -1;
+  }
+  const int n = starts[ncl];
+  if (ncl == 1) return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+  TC_CHECK(ctx->cl_start.ensure(sizeof(int) * (size_t)(ncl + 1)));
+  TC_CHECK(ctx->cl_id.ensure(sizeof(int) * (size_t)n));
+  TC_CHECK(ctx->tp.ensure(sizeof(TreeParams) * (size_t)ncl));
+  TC_CHECK(ctx->dshift.ensure(sizeof(double) * 3 * (size_t)ncl));
+  TC_CHECK(hipMemcpyAsync(ctx->cl_start.p, starts, sizeof(int) * (ncl + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+  ctx->ncl = ncl;
+  const int e = eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+  ctx->ncl = 1;
+  return e;
 }
 
 int tc_eval_bodies(tc_context *ctx, tc_body *bodies, int n) {

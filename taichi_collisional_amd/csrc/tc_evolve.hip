@@ -136,6 +136,12 @@ __global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *_
   out[pos] = is[k];
 }
 
+// time-step sums of zero: kick_naive's 1/sqrt(sqrt(0)) gives every body +inf
+__global__ void k_no_step_sums(double *__restrict__ out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = 0.0;
+}
+
 __global__ void k_iota(int *__restrict__ out, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = k;
@@ -156,7 +162,8 @@ struct Sub {
 class Evolver {
  public:
   Evolver(tc_context *ctx, const tc_particles &p, int mode, double dt_param)
-      : ctx_(ctx), p_(p), mode_(mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS)), dtp_(dt_param),
+      : ctx_(ctx), p_(p), mode_(mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS)),
+        jerk_fmm_((mode & TC_EVOLVE_JERK_FMM) != 0), dtp_(dt_param),
         lev_(ctx->ev_lev), all_(ctx->ev_all), ux_(ctx->ev_ux), uv_(ctx->ev_uv), um_(ctx->ev_um),
         ufl_(ctx->ev_ufl), upot_(ctx->ev_upot), uacc_(ctx->ev_uacc), uts_(ctx->ev_uts),
         uao_(ctx->ev_uao), tmp_(ctx->ev_tmp), flags_(ctx->ev_flags), offs_(ctx->ev_offs),
@@ -168,6 +175,7 @@ class Evolver {
   tc_context *ctx_;
   tc_particles p_;
   int mode_;
+  bool jerk_fmm_;  // forces from the jerk build's evaluation (no time steps)
   double dtp_;
   double simtime_ = 0;
   // index buffers: one per recursion level holds that level's subsystem
@@ -226,6 +234,16 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
                      sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass, p_.acc,
                      shift_fac, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
                      ufl_.as<unsigned char>());
+  if (jerk_fmm_ && !exact) {
+    // the jerk build's get_force_and_potential on the union (no time step:
+    // +inf, so that split never makes a fast level)
+    if (steps) {
+      hipLaunchKernelGGL(k_no_step_sums, grid_for(nu), dim3(256), 0, s, uts_.as<double>(), nu);
+    }
+    return eval_soa_device(ctx, nu, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
+                           ufl_.as<unsigned char>(), upot_.as<double>(), uacc_.as<double>(),
+                           nullptr);
+  }
   const int mode = exact ? (mode_ & ~TC_MASTER_FMM) : mode_;
   double *ao = nullptr;
   if (with_acc_old && (mode & TC_MASTER_FMM)) {
@@ -409,8 +427,13 @@ extern "C" int tc_master_evolve(tc_context *ctx, const tc_particles *p, double *
     ctx->err = "tc_master_evolve: dt_param > 0 and a finite dt are required";
     return -1;
   }
-  if (mode & ~(TC_MASTER_FMM | TC_MASTER_PAIR_MASS)) {
-    ctx->err = "tc_master_evolve: unknown mode bits (TC_MASTER_FMM, TC_MASTER_PAIR_MASS)";
+  if (mode & ~(TC_MASTER_FMM | TC_MASTER_PAIR_MASS | TC_EVOLVE_JERK_FMM)) {
+    ctx->err = "tc_master_evolve: unknown mode bits (TC_MASTER_FMM, TC_MASTER_PAIR_MASS, "
+               "TC_EVOLVE_JERK_FMM)";
+    return -1;
+  }
+  if ((mode & TC_EVOLVE_JERK_FMM) && (mode & (TC_MASTER_FMM | TC_MASTER_PAIR_MASS))) {
+    ctx->err = "tc_master_evolve: TC_EVOLVE_JERK_FMM excludes the master force modes";
     return -1;
   }
   tcfmm::Evolver ev(ctx, *p, mode, dt_param);

@@ -70,15 +70,24 @@ struct TreeParams {
 // ---- tree (tc_tree.hip)
 int launch_bounds(const double *x3, int n, double *partial, int nblocks_max, TreeParams *d_tp,
                   const double *shift_frac_dev, hipStream_t s);
+// cid: cluster of every body (d_tp indexed by it), or null for one tree
 int launch_morton(const double *x3, int n, const TreeParams *d_tp, uint64_t *key_hi,
-                  uint64_t *key_lo, int *idx, hipStream_t s);
+                  uint64_t *key_lo, int *idx, hipStream_t s, const int *cid = nullptr);
+// forest of clusters (bodies grouped by cluster, ncl + 1 starts)
+int launch_bounds_clusters(const double *x3, int ncl, const int *start, TreeParams *d_tp,
+                           const double *shift_frac_dev, hipStream_t s);
+int launch_cluster_ids(const int *start, int ncl, int n, int *cid, hipStream_t s);
+int launch_sort_clusters(uint64_t *key_a, uint64_t *key_b, int *val_a, int *val_b,
+                         const int *cid, int ncl, int n, void *temp, size_t temp_bytes,
+                         hipStream_t s);
 size_t sort_temp_bytes(int n);
 int launch_sort(uint64_t *key_a, uint64_t *key_b, int *val_a, int *val_b, const uint64_t *key_hi,
                 int n, void *temp, size_t temp_bytes, hipStream_t s);
 int launch_gather(const double *x3, const double *v3, const double *q,
                   const unsigned char *flags_in, const int *perm, int n, BodiesDev b,
                   hipStream_t s);
-int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s);
+int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s,
+                const int *start = nullptr, int ncl = 1);
 // Device-side state of a tree build (ints): the level being split [ls, le),
 // leaves and levels so far, the level being placed (cls, cle, its first leaf
 // slot cbase, go), done, overflow (1: the cell arrays are too small, need
@@ -111,6 +120,7 @@ struct MacArgs {
   int steps;                // kMacMaster: the get_steps walk
   double theta, facc;
   const double *Pn;         // [cell][p+1], kMacError
+  const double *theta_cell; // kMacTheta: the target cell's own angle (null: theta)
   double comb[kMaxP + 1];   // C(p,k), kMacError
 };
 struct FactTable {
@@ -216,6 +226,38 @@ int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids
 // master (non-jerk) variant, tc_master.hip
 void launch_iota(int *p, int n, hipStream_t s);
 void launch_zero_u8(unsigned char *p, int n, hipStream_t s);
+// ---- learned per-cell order and opening angle (tc_learn.hip)
+constexpr int kLeafFeatures = 8, kMlpIn = kLeafFeatures + 2, kMlpH = 64;
+constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + 2 * (kMlpH * kMlpH + kMlpH) + kMlpH + 1;
+constexpr int kMaxCand = 64;
+// Candidate (order, angle) pairs of the choice and their relative costs.
+struct OrderGrid {
+  int ncand;
+  int q[kMaxCand];
+  float theta[kMaxCand];
+  float cost[kMaxCand];
+};
+void launch_mlp_fwd(const float *P, const float *X, int B, float *Y, float *H, hipStream_t s);
+// loss = mean (Y - T)^2 and its gradient G (zeroed first); Y [B], H and D
+// [3][B][64], dY [B] are scratch
+void launch_mlp_grad(const float *P, const float *X, const float *T, int B, float *Y, float *H,
+                     float *D, float *dY, float *G, float *loss, hipStream_t s);
+void launch_adam(float *P, const float *G, float *m, float *v, float lr, int t, hipStream_t s);
+void launch_count_targets(const int2 *pairs, int n, int *cnt, hipStream_t s);
+void launch_leaf_features(CellsDev c, const int *leaf_ids, int nleaves, int p,
+                          const int *m2l_cnt, const int *p2p_cnt, double mean_mass, float *F,
+                          hipStream_t s);
+void launch_leaf_select(const float *P, const float *F, const int *leaf_ids, int nleaves,
+                        const OrderGrid &g, float log_target, int *q, double *theta, float *pred,
+                        hipStream_t s);
+void launch_leaf_uniform(const int *leaf_ids, int nleaves, int qv, double tv, int *q,
+                         double *theta, hipStream_t s);
+void launch_propagate(CellsDev c, int ls, int le, int *q, double *theta, hipStream_t s);
+void launch_leaf_errors(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves,
+                        const double *acc, const double *ref, float *out, hipStream_t s);
+void launch_leaf_of_body(CellsDev c, BodiesDev b, const int *leaf_ids, int nleaves, int *out,
+                         hipStream_t s);
+
 // ---- fixed scratch of tc_direct_soa_noalloc (tc_noalloc.hip)
 hipError_t noalloc_scratch(unsigned char **out);
 int noalloc_max_bodies();

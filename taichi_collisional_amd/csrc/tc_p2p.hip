@@ -60,8 +60,8 @@ static_assert(kTile >= 64 && 2 * kTile * 32 >= 7 * kWave * 8, "tile holds the re
 // kHiLo: each offset is held as a float pair (hi + lo) and the difference is
 // (hi_j - hi_i) + (lo_j - lo_i), accurate to FP32 rounding of the separation
 // itself; otherwise the offsets are rounded to one float each.  Pairs with
-// r^2 below the close radius squared (including r = 0) contribute nothing
-// here and set `close`; the caller evaluates them in FP64.  The jerk's second
+// r^2 <= the close radius squared (so r = 0 always) contribute nothing here
+// and set `close`; the caller evaluates them in FP64.  The jerk's second
 // term is accumulated without the factor -3 (applied once at the end).
 struct Acc10f {
   float p, ax, ay, az, jx, jy, jz, kx, ky, kz;
@@ -80,7 +80,7 @@ __device__ __forceinline__ void pair_fp32(float xh, float yh, float zh, float xl
   }
   const float dvx = V.x - vx, dvy = V.y - vy, dvz = V.z - vz;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const bool c = r2 < rc2;
+  const bool c = r2 <= rc2;  // r = 0 included, also when rc2 == 0
   close |= c;
   const float ri = c ? 0.0f : __builtin_amdgcn_rsqf(r2);
   const float ri2 = ri * ri;

@@ -112,6 +112,86 @@ int launch_bounds(const double *x3, int n, double *partial, int nblocks_max, Tre
   return 0;
 }
 
+// Forest of independent clusters (bodies grouped by cluster, cluster k =
+// [start[k], start[k+1])): one block per cluster reduces its bounds and forms
+// its TreeParams as k_bounds_final does for a single root, with the cluster's
+// own shift values shift_frac[3k..3k+2].
+__global__ __launch_bounds__(256) void k_bounds_clusters(const double *__restrict__ x3,
+                                                         const int *__restrict__ start,
+                                                         const double *__restrict__ shift_frac,
+                                                         TreeParams *tp) {
+#pragma clang fp contract(off)
+  __shared__ double smn[3][256], smx[3][256];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int b0 = start[k], b1 = start[k + 1];
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[3 * (size_t)b0 + d];
+  for (int i = b0 + t; i < b1; i += 256)
+    for (int d = 0; d < 3; d++) {
+      const double v = x3[3 * (size_t)i + d];
+      mn[d] = fmin(v, mn[d]);
+      mx[d] = fmax(v, mx[d]);
+    }
+  for (int d = 0; d < 3; d++) smn[d][t] = mn[d], smx[d][t] = mx[d];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int d = 0; d < 3; d++) {
+        smn[d][t] = fmin(smn[d][t], smn[d][t + o]);
+        smx[d][t] = fmax(smx[d][t], smx[d][t + o]);
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  double X0[3], box = -1, R0 = 0;
+  for (int d = 0; d < 3; d++) {
+    mn[d] = smn[d][0];
+    mx[d] = smx[d][0];
+    X0[d] = (mx[d] + mn[d]) / 2;
+    if (mx[d] - mn[d] > box) box = mx[d] - mn[d];
+  }
+  for (int d = 0; d < 3; d++) X0[d] += shift_frac[3 * k + d] * box / 25;
+  for (int d = 0; d < 3; d++) {
+    R0 = fmax(X0[d] - mn[d], R0);
+    R0 = fmax(mx[d] - X0[d], R0);
+  }
+  R0 *= 1.00001;
+  for (int d = 0; d < 3; d++) {
+    tp[k].X0[d] = X0[d];
+    tp[k].mn[d] = mn[d];
+    tp[k].mx[d] = mx[d];
+  }
+  tp[k].R0 = R0;
+}
+
+int launch_bounds_clusters(const double *x3, int ncl, const int *start, TreeParams *d_tp,
+                           const double *shift_frac_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_clusters, dim3(ncl), dim3(256), 0, s, x3, start, shift_frac_dev,
+                     d_tp);
+  return 0;
+}
+
+// Cluster of every body (input order) from the cluster starts.
+__global__ void k_cluster_ids(const int *__restrict__ start, int ncl, int n,
+                              int *__restrict__ cid) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = ncl - 1;  // last cluster with start <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  cid[i] = lo;
+}
+
+int launch_cluster_ids(const int *start, int ncl, int n, int *cid, hipStream_t s) {
+  hipLaunchKernelGGL(k_cluster_ids, dim3((n + 255) / 256), dim3(256), 0, s, start, ncl, n, cid);
+  return 0;
+}
+
 // ---------------------------------------------------------------- keys ------
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
   v &= 0x1fffffULL;
@@ -124,13 +204,15 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3
 }
 
 __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ x3, int n,
-                                                const TreeParams *__restrict__ tp,
+                                                const TreeParams *__restrict__ tps,
+                                                const int *__restrict__ cid,
                                                 uint64_t *__restrict__ key_hi,
                                                 uint64_t *__restrict__ key_lo,
                                                 int *__restrict__ idx) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  const TreeParams *tp = tps + (cid ? cid[i] : 0);
   const double maxint = (double)(1u << 28);
   unsigned int M[3];
   for (int d = 0; d < 3; d++) {
@@ -146,8 +228,8 @@ __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ x3, i
 }
 
 int launch_morton(const double *x3, int n, const TreeParams *d_tp, uint64_t *key_hi,
-                  uint64_t *key_lo, int *idx, hipStream_t s) {
-  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, s, x3, n, d_tp, key_hi,
+                  uint64_t *key_lo, int *idx, hipStream_t s, const int *cid) {
+  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, s, x3, n, d_tp, cid, key_hi,
                      key_lo, idx);
   return 0;
 }
@@ -163,6 +245,30 @@ __global__ void k_gather_keys(const uint64_t *__restrict__ key_hi, const int *__
                               int n, uint64_t *__restrict__ out) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) out[i] = key_hi[perm[i]];
+}
+
+// Forest: a last stable pass on the cluster id of the permuted bodies, so the
+// order is (cluster, key); each cluster keeps its input range.
+__global__ void k_gather_cid(const int *__restrict__ cid, const int *__restrict__ perm, int n,
+                             uint64_t *__restrict__ out) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = (uint64_t)cid[perm[i]];
+}
+
+int launch_sort_clusters(uint64_t *key_a, uint64_t *key_b, int *val_a, int *val_b,
+                         const int *cid, int ncl, int n, void *temp, size_t temp_bytes,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_cid, dim3((n + 255) / 256), dim3(256), 0, s, cid, val_a, n, key_a);
+  int bits = 1;
+  while ((1 << bits) < ncl) bits++;
+  size_t tb = temp_bytes;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, key_a, key_b, val_a, val_b, n, 0, bits, s) !=
+      hipSuccess)
+    return -1;
+  return hipMemcpyAsync(val_a, val_b, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s) ==
+                 hipSuccess
+             ? 0
+             : -1;
 }
 
 // key_a holds the low keys on entry; on return val_a holds the permutation.
@@ -210,19 +316,24 @@ int launch_gather(const double *x3, const double *v3, const double *q,
 }
 
 // ---------------------------------------------------------------- cells -----
-__global__ void k_root(CellsDev c, int n, const TreeParams *__restrict__ tp) {
-  if (threadIdx.x != 0) return;
-  c.level[0] = 0;
-  c.nchild[0] = 0;
-  c.child[0] = -1;
-  c.body[0] = 0;
-  c.nbody[0] = n;
-  for (int d = 0; d < 3; d++) c.Xg[d] = tp->X0[d];
-  c.Rg[0] = tp->R0;
+// Root cells: one (all n bodies), or one per cluster of a forest (cluster k
+// = [start[k], start[k+1]) with its own TreeParams).
+__global__ void k_root(CellsDev c, int n, const TreeParams *__restrict__ tp,
+                       const int *__restrict__ start, int ncl) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= ncl) return;
+  c.level[k] = 0;
+  c.nchild[k] = 0;
+  c.child[k] = -1;
+  c.body[k] = start ? start[k] : 0;
+  c.nbody[k] = start ? start[k + 1] - start[k] : n;
+  for (int d = 0; d < 3; d++) c.Xg[3 * k + d] = tp[k].X0[d];
+  c.Rg[k] = tp[k].R0;
 }
 
-int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s) {
-  hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, s, c, n, d_tp);
+int launch_root(CellsDev c, int n, const TreeParams *d_tp, hipStream_t s, const int *start,
+                int ncl) {
+  hipLaunchKernelGGL(k_root, dim3((ncl + 63) / 64), dim3(64), 0, s, c, n, d_tp, start, ncl);
   return 0;
 }
 
@@ -401,7 +512,10 @@ __global__ __launch_bounds__(256) void k_children(CellsDev c, const TreeState *_
     c.child[ci] = first;
     const int level = c.level[ci];
     const int end = c.body[ci] + c.nbody[ci];
-    const double r = ldexp(tp->R0, -(level + 1));
+    // R0 2^-(level+1), from the parent's radius (exact: a power-of-two scaling),
+    // so that the roots of a forest may each have their own R0
+    const double r = ldexp(c.Rg[ci], -1);
+    (void)tp;
     int jc = 0;
     for (int i = 0; i < 8; i++) {
       const int o = offs[8 * k + i];
@@ -486,7 +600,8 @@ __device__ __forceinline__ int trav_classify(const CellsDev &c, const MacArgs &a
     Ri = c.R[i];
     Rj = c.R[j];
     if (a.mode == kMacTheta) {
-      const double R2 = mac_norm(dx, dy, dz) * a.theta * a.theta;
+      const double th = a.theta_cell ? a.theta_cell[i] : a.theta;
+      const double R2 = mac_norm(dx, dy, dz) * th * th;
       accept = R2 > (Ri + Rj) * (Ri + Rj);
     } else if (a.mode == kMacMaster) {
       const double R2 = mac_norm(dx, dy, dz);

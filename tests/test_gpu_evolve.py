@@ -129,3 +129,69 @@ def test_gpu_step_loop_edges(torch_cuda):
     assert np.array_equal(st["pos"].cpu().numpy(), x)
     with pytest.raises(RuntimeError, match="dt_param"):
         f.master_evolve(st, 0.01, dt_param=0.0)
+
+
+def test_gpu_step_loop_jerk_fmm_equals_master_loop_when_exact(torch_cuda):
+    """TC_EVOLVE_JERK_FMM drives the same fourth-order loop with the jerk
+    build's evaluation.  With all-pairs forces (direct_below > n) and a dt
+    below every body's time step (no split), it must reproduce the master
+    loop with its default (all-pairs) forces to round-off."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    n, dt = 2000, 2.0 ** -20
+    x, v, m = ics.plummer(n, seed=17)
+    fm = tca.Fmm(p=8, ncrit=64, direct_below=0)
+    sm = _state(torch_cuda, x, v, m)
+    tm = fm.master_evolve(sm, dt, mode=0)
+    assert float(sm["timestep"].min()) >= dt  # single level in the master loop too
+    fj = tca.Fmm(p=8, ncrit=64, direct_below=n + 1)
+    sj = _state(torch_cuda, x, v, m)
+    tj = fj.master_evolve(sj, dt, mode=tca.TC_EVOLVE_JERK_FMM)
+    torch_cuda.cuda.synchronize()
+    assert tm == tj == dt
+    assert bool(torch_cuda.isinf(sj["timestep"]).all())
+    dp = float((sj["pos"] - sm["pos"]).abs().max())
+    dv = float((sj["vel"] - sm["vel"]).abs().max())
+    print("jerk-FMM loop vs master loop: dpos %.1e dvel %.1e" % (dp, dv))
+    assert dp < 1e-14 and dv < 1e-12
+    fm.close()
+    fj.close()
+
+
+def test_gpu_c5_fourth_order_steps_dehnen(torch_cuda):
+    """The learned-order workload's integration as a bounded test: Dehnen
+    gamma = 1.5, ten fourth-order steps of dt = 1e-4 with every force the
+    jerk build's theta-MAC FMM at the fallback p = 12 / theta = 0.4
+    (N = 2^18 here; the benchmark runs N = 2^23).  Energy is conserved to the
+    force error and the final forces meet the 1e-7 gate."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    from conftest import relerr_vec
+    torch = torch_cuda
+    n, dt = 1 << 18, 1e-4
+    x, v, m = ics.dehnen(n, gamma=1.5, seed=23)
+    f = tca.Fmm(p=12, ncrit=64, theta=0.4, direct_below=0)
+    f.set_seed(23)
+    st = _state(torch, x, v, m)
+
+    def energy():
+        kin = 0.5 * float(torch.sum(st["mass"][:, None] * st["vel"] ** 2))
+        _, _, _ = f.eval(st["pos"], st["vel"], st["mass"], pot=st["pot"])
+        return kin - 0.5 * float(torch.sum(st["mass"] * st["pot"]))
+
+    e0 = energy()
+    t = 0.0
+    for _ in range(10):
+        t = f.master_evolve(st, dt, simtime=t, mode=tca.TC_EVOLVE_JERK_FMM)
+    e1 = energy()
+    rng = np.random.default_rng(3)
+    tg = torch.from_numpy(np.sort(rng.choice(n, 2048, replace=False)).astype(np.int32)).cuda()
+    _, acc, _ = f.eval(st["pos"], st["vel"], st["mass"])
+    _, da, _ = f.direct(st["pos"], st["vel"], st["mass"], tg)
+    err = relerr_vec(acc[tg.long()].cpu().numpy(), da.cpu().numpy())
+    print("C5-style: t=%.1e dE/E %.2e, final force rms %.2e max %.2e" % (
+        t, abs(e1 - e0) / abs(e0), err[0], err[1]))
+    assert abs(t - 10 * dt) < 1e-15
+    assert abs(e1 - e0) < 1e-6 * abs(e0)
+    assert err[0] <= 1e-7
+    f.close()

@@ -1,0 +1,175 @@
+"""The learned per-cell order and opening angle on the GPU.
+
+* The MLP kernels (forward, gradient, Adam) against the CPU oracle
+  (oracle/mlp_oracle.py, itself pinned by finite differences).
+* The per-cell path at a uniform (q, theta) equals the fixed-order method:
+  same pair counts, same results to round-off (q = p, and q < p against a
+  context of order q).
+* Training on measured per-leaf errors fits them (small run).
+* The committed model (order_model.npz, trained by
+  `python -m taichi_collisional_amd.learn train`): on snapshots it was not
+  trained on, the learned choice meets the rms target against the FP64
+  direct sum and does less M2L + P2P work than the fixed p = 12 / theta =
+  0.4 fallback.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr_vec
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_mlp_kernels_match_oracle(torch_cuda):
+    import mlp_oracle as mo
+    import taichi_collisional_amd as tca
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    P = (mo.init_params(1) + rng.normal(scale=0.05, size=mo.NPARAM)).astype(np.float32)
+    B = 3001
+    X = rng.normal(size=(B, mo.NIN)).astype(np.float32)
+    T = rng.normal(size=B).astype(np.float32)
+    f = tca.Fmm(p=4)
+    Pd, Xd, Td = (torch.from_numpy(a).cuda() for a in (P, X, T))
+    y = f.mlp_forward(Pd, Xd).cpu().numpy()
+    yo = mo.forward(P.astype(np.float64), X.astype(np.float64))
+    assert np.max(np.abs(y - yo)) < 1e-4 * max(1.0, np.max(np.abs(yo)))
+    G, loss = f.mlp_grad(Pd, Xd, Td)
+    lo, go = mo.loss_and_grad(P.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    G = G.cpu().numpy().astype(np.float64)
+    print("loss gpu %.6f oracle %.6f; grad rel %.2e" % (
+        loss, lo, np.linalg.norm(G - go) / np.linalg.norm(go)))
+    assert abs(loss - lo) < 1e-4 * lo
+    assert np.linalg.norm(G - go) < 1e-4 * np.linalg.norm(go)
+    o = 0
+    for name, sh in mo.SIZES:  # every block, not just the norm
+        n = int(np.prod(sh))
+        blk, ob = G[o:o + n], go[o:o + n]
+        assert np.max(np.abs(blk - ob)) <= 1e-4 * np.max(np.abs(go)) + 1e-7, name
+        o += n
+    # one Adam step: p -= lr * m_hat / (sqrt(v_hat) + eps) with m = 0.1 g, v = 0.001 g^2
+    m = torch.zeros_like(Pd)
+    v = torch.zeros_like(Pd)
+    Gd = torch.from_numpy(G.astype(np.float32)).cuda()
+    f.mlp_adam(Pd, Gd, m, v, 1e-3, 1)
+    g32 = G.astype(np.float32).astype(np.float64)
+    expect = P - 1e-3 * g32 / (np.abs(g32) + 1e-8)
+    assert np.max(np.abs(Pd.cpu().numpy() - expect)) < 1e-6
+    f.close()
+
+
+def _eval(torch, f, x, v, m):
+    dx, dv, dm = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, v, m))
+    pot, acc, jerk = f.eval(dx, dv, dm)
+    torch.cuda.synchronize()
+    return pot.cpu().numpy(), acc.cpu().numpy(), jerk.cpu().numpy(), f.stats()
+
+
+@pytest.mark.parametrize("p_ctx,q,theta", [(12, 12, 0.4), (12, 8, 0.5), (20, 10, 0.45)])
+def test_uniform_per_cell_equals_fixed(torch_cuda, p_ctx, q, theta):
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import ics
+    x, v, m = ics.plummer(30000, seed=p_ctx + q)
+    a = tca.Fmm(p=p_ctx, ncrit=64, theta=0.4, direct_below=0, quirk_factorial=False)
+    a.set_shift([0.1, -0.2, 0.3])
+    a.set_adaptive(1, q, theta)
+    b = tca.Fmm(p=q, ncrit=64, theta=theta, direct_below=0, quirk_factorial=False)
+    b.set_shift([0.1, -0.2, 0.3])
+    pa, aa, ja, sa = _eval(torch_cuda, a, x, v, m)
+    pb, ab, jb, sb = _eval(torch_cuda, b, x, v, m)
+    assert sa["n_m2l"] == sb["n_m2l"] and sa["n_p2p_pairs"] == sb["n_p2p_pairs"]
+    counts = a.m2l_order_counts()
+    assert counts[q] == sa["n_m2l"] and sum(counts) == sa["n_m2l"]
+    ea, ej = relerr_vec(aa, ab)[1], relerr_vec(ja, jb)[1]
+    ep = float(np.max(np.abs(pa - pb) / np.abs(pb)))
+    print("p=%d q=%d theta=%.2f: acc %.1e jerk %.1e pot %.1e" % (p_ctx, q, theta, ea, ej, ep))
+    assert ea < 1e-11 and ej < 1e-10 and ep < 1e-12
+    # features and leaf map are there
+    feat, lob, _ = a.leaf_data()
+    assert feat.shape == (sa["nleaves"], 8) and bool(torch_cuda.isfinite(feat).all())
+    lob = lob.cpu().numpy()
+    assert lob.min() == 0 and lob.max() == sa["nleaves"] - 1
+    a.close()
+    b.close()
+
+
+def test_training_fits_measured_errors(torch_cuda):
+    """A short run of the pipeline: 4 snapshots measured, 6 epochs of Adam with
+    the library's kernels; the fit explains most of the variance of the
+    per-leaf log errors on a held-out snapshot."""
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import learn
+    torch = torch_cuda
+    f = tca.Fmm(p=learn.P_MAX, ncrit=64, theta=0.4, direct_below=0)
+    f.set_seed(1)
+    Xs, Ts = [], []
+    for kind, n, seed in [("plummer", 1 << 15, 101), ("dehnen15", 1 << 15, 102),
+                          ("dehnen10", 1 << 15, 103), ("plummer", 1 << 15, 104),
+                          ("dehnen15", 1 << 15, 105)]:
+        x, v, m = learn.make_bodies(kind, n, seed)
+        X, T, w = learn.measure_snapshot(torch, tca, f, x, v, m, seed)
+        assert w.shape == (45, 2) and np.all(w > 0)
+        Xs.append(X)
+        Ts.append(T)
+    Xtr, Ttr = torch.cat(Xs[:4]), torch.cat(Ts[:4])
+    P, hist = learn.fit(torch, tca, f, Xtr, Ttr, epochs=6, log=print)
+    assert hist[-1] < hist[0]
+    y = f.mlp_forward(torch.from_numpy(P).cuda(), Xs[4].contiguous())
+    mse = float(torch.mean((y - Ts[4]) ** 2))
+    var = float(torch.var(Ts[4]))
+    print("held-out mse %.3f, variance %.3f, R^2 %.3f" % (mse, var, 1 - mse / var))
+    assert mse < 0.3 * var
+    f.close()
+
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     "taichi_collisional_amd", "order_model.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order_model.npz in the tree")
+@pytest.mark.parametrize("kind,n,seed", [("plummer", 1 << 18, 9001), ("dehnen15", 1 << 18, 9002)])
+def test_learned_choice_meets_target_with_less_work(torch_cuda, kind, n, seed):
+    import taichi_collisional_amd as tca
+    from taichi_collisional_amd import flops, learn
+    torch = torch_cuda
+    model = learn.load_model(MODEL)
+    x, v, m = learn.make_bodies(kind, n, seed)
+    dx, dv, dm = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, v, m))
+    rng = np.random.default_rng(seed)
+    tg = torch.from_numpy(np.sort(rng.choice(n, 4096, replace=False)).astype(np.int32)).cuda()
+
+    def work(f):
+        st = f.stats()
+        c = f.m2l_order_counts()
+        return sum(c[q] * flops.c_m2l(q) for q in range(len(c))) + 44 * st["n_p2p_pairs"]
+
+    fl = tca.Fmm(p=learn.P_MAX, ncrit=64, theta=0.4, direct_below=0)
+    fl.set_seed(seed)
+    learn.apply_model(fl, model, target_rms=1e-7)
+    _, acc_l, _ = fl.eval(dx, dv, dm)
+    _, da, _ = fl.direct(dx, dv, dm, tg)
+    el = relerr_vec(acc_l[tg.long()].cpu().numpy(), da.cpu().numpy())
+    wl = work(fl)
+    ff = tca.Fmm(p=12, ncrit=64, theta=0.4, direct_below=0)
+    ff.set_seed(seed)
+    _, acc_f, _ = ff.eval(dx, dv, dm)
+    ef = relerr_vec(acc_f[tg.long()].cpu().numpy(), da.cpu().numpy())
+    wf = work(ff)
+    print("%s n=%d: learned rms %.2e max %.2e work %.3e | fixed p=12/0.4 rms %.2e work %.3e "
+          "(%.2fx less work); orders used %s" % (
+              kind, n, el[0], el[1], wl, ef[0], wf, wf / wl,
+              {q: c for q, c in enumerate(fl.m2l_order_counts()) if c}))
+    assert el[0] <= 1e-7
+    assert wl < wf
+    fl.close()
+    ff.close()
