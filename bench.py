@@ -211,6 +211,107 @@ def timed_row(fmm, torch, dx, dv, dm, out, idx, da, dj, reps=5, warm=2):
             "m2l_pairs": sts[-1]["n_m2l"], "p2p_body_pairs": sts[-1]["n_p2p_pairs"]}
 
 
+def clusters_row(tca, ics, torch, dev, seed=7, k=256, reps=3):
+    """Many independent Plummer clusters (N log-uniform in 2^10..2^16, equal
+    masses within a cluster, p = 8, theta = 0.5): one forest evaluation of all
+    of them (tc_eval_clusters) against one evaluation per cluster."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(10 * np.log(2), 16 * np.log(2), size=k)).astype(int)
+    xs, vs, ms = [], [], []
+    for i, nn in enumerate(sizes):
+        x, v, m = ics.plummer(int(nn), seed=seed * 1000 + i)
+        xs.append(x + rng.normal(scale=20.0, size=3))
+        vs.append(v)
+        ms.append(m)
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(starts[-1])
+    dx, dv, dm = (torch.from_numpy(np.concatenate(a)).to(dev) for a in (xs, vs, ms))
+    out = (torch.empty(n, dtype=torch.float64, device=dev),
+           torch.empty(n, 3, dtype=torch.float64, device=dev),
+           torch.empty(n, 3, dtype=torch.float64, device=dev))
+    f = tca.Fmm(p=8, ncrit=64, theta=0.5, direct_below=0)
+    f.set_seed(seed)
+    f.eval_clusters(starts, dx, dv, dm, pot=out[0], acc=out[1], jerk=out[2])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        f.eval_clusters(starts, dx, dv, dm, pot=out[0], acc=out[1], jerk=out[2])
+    torch.cuda.synchronize()
+    forest_ms = (time.perf_counter() - t) * 1e3 / reps
+    st = f.stats()
+    # one evaluation per cluster on the same context (after a warm-up pass)
+    views = [(dx[a:b], dv[a:b], dm[a:b], out[0][a:b], out[1][a:b], out[2][a:b])
+             for a, b in zip(starts[:-1], starts[1:])]
+    for vw in views:
+        f.eval(*vw[:3], pot=vw[3], acc=vw[4], jerk=vw[5])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for vw in views:
+        f.eval(*vw[:3], pot=vw[3], acc=vw[4], jerk=vw[5])
+    torch.cuda.synchronize()
+    each_ms = (time.perf_counter() - t) * 1e3
+    f.close()
+    return {"what": "%d independent Plummer clusters (N log-uniform 2^10..2^16, %d bodies), "
+                    "p=8, theta=0.5: all clusters in one evaluation (each phase one launch "
+                    "for all clusters) vs one evaluation per cluster" % (k, n),
+            "bodies": n, "forest_ms": forest_ms, "per_cluster_ms": each_ms,
+            "speedup": each_ms / forest_ms, "particles_per_s": n / (forest_ms * 1e-3),
+            "m2l_pairs": st["n_m2l"], "p2p_body_pairs": st["n_p2p_pairs"]}
+
+
+def cusp_steps_row(tca, ics, torch, dev, n, learned=False, steps=10, dt=1e-4, seed=5):
+    """Dehnen gamma = 1.5 (N bodies): ten fourth-order steps of dt = 1e-4
+    (the master GRADIENT loop with the jerk build's theta-MAC FMM forces,
+    TC_EVOLVE_JERK_FMM) at the fallback p = 12 / theta = 0.4, or with the
+    learned per-leaf choice; energy change and the final force error."""
+    import numpy as np
+    from taichi_collisional_amd import flops, learn
+    x, v, m = ics.dehnen(n, gamma=1.5, seed=seed)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    st = dict(pos=t(x), pos_e=torch.zeros(n, 3, dtype=torch.float64, device=dev), vel=t(v),
+              mass=t(m), timestep=torch.zeros(n, dtype=torch.float64, device=dev),
+              acc=torch.zeros(n, 3, dtype=torch.float64, device=dev),
+              pot=torch.zeros(n, dtype=torch.float64, device=dev),
+              acc_old=torch.zeros(n, dtype=torch.float64, device=dev))
+    if learned:
+        # candidates up to the fallback's order (the context's order)
+        f = tca.Fmm(p=12, ncrit=64, theta=0.4, direct_below=0)
+        learn.apply_model(f, learn.load_model(), target_rms=1e-7)
+    else:
+        f = tca.Fmm(p=12, ncrit=64, theta=0.4, direct_below=0)
+    f.set_seed(seed)
+
+    def energy():
+        f.eval(st["pos"], st["vel"], st["mass"], pot=st["pot"])
+        kin = 0.5 * float(torch.sum(st["mass"][:, None] * st["vel"] ** 2))
+        return kin - 0.5 * float(torch.sum(st["mass"] * st["pot"]))
+
+    e0 = energy()
+    ev_ms = f.stats()["ms_total"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim = 0.0
+    for _ in range(steps):
+        sim = f.master_evolve(st, dt, simtime=sim, mode=tca.TC_EVOLVE_JERK_FMM)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / steps
+    e1 = energy()
+    rng = np.random.default_rng(seed)
+    tg = torch.from_numpy(np.sort(rng.choice(n, 4096, replace=False)).astype(np.int32)).to(dev)
+    _, acc, _ = f.eval(st["pos"], st["vel"], st["mass"])
+    c = f.m2l_order_counts()
+    work = sum(c[q] * flops.c_m2l(q) for q in range(len(c))) + 44.0 * f.stats()["n_p2p_pairs"]
+    _, da, _ = f.direct(st["pos"], st["vel"], st["mass"], tg)
+    ea = torch.linalg.norm(acc[tg.long()] - da, dim=1) / torch.linalg.norm(da, dim=1)
+    f.close()
+    return {"ms_per_step": ms_step, "evals_per_step": 5, "ms_per_eval_first": ev_ms,
+            "energy_rel_change": abs(e1 - e0) / abs(e0),
+            "final_force_rms": float(torch.sqrt(torch.mean(ea ** 2))),
+            "final_force_max": float(ea.max()), "m2l_plus_p2p_flops_final_eval": work,
+            "m2l_pairs_by_order": {q: cc for q, cc in enumerate(c) if cc}}
+
+
 def pareto(args):
     """Cost and accuracy of the ways to trade one for the other on the workload
     (Plummer N, seed): fixed (p, theta) on a grid, the error-controlled MAC at
@@ -257,12 +358,13 @@ def pareto(args):
             f.close()
     if os.path.exists(learn.MODEL_PATH):
         model = learn.load_model()
-        for target in (3e-7, 2e-7, 1e-7, 5e-8):
-            f = tca.Fmm(p=learn.P_MAX, ncrit=args.ncrit, direct_below=0)
-            f.set_seed(args.seed)
-            learn.apply_model(f, model, target_rms=target)
-            emit("learned", f, target_rms=target)
-            f.close()
+        for pc in (12, 14):
+            for target in (2e-7, 1e-7, 5e-8):
+                f = tca.Fmm(p=pc, ncrit=args.ncrit, direct_below=0)
+                f.set_seed(args.seed)
+                learn.apply_model(f, model, target_rms=target)
+                emit("learned", f, target_rms=target, p_max=pc)
+                f.close()
 
 
 def main():
@@ -287,6 +389,11 @@ def main():
                          "of known size for scaling HBM byte counters)")
     ap.add_argument("--only-master-evolve", action="store_true",
                     help="time only the master step loop row (for profiling it) and print it")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the rows of the other workloads (many clusters, the "
+                         "fourth-order steps on the cusp, the learned choice)")
+    ap.add_argument("--c5-n", type=int, default=1 << 23,
+                    help="bodies of the fourth-order-steps row (Dehnen gamma=1.5)")
     ap.add_argument("--pareto", action="store_true",
                     help="instead of the benchmark line: time and error of a grid of fixed "
                          "(p, theta), of the error-controlled MAC and of the learned choice "
@@ -450,6 +557,30 @@ def main():
             if row.get("cpu_reference_seconds"):
                 row["gpu_over_cpu"] = row["cpu_reference_seconds"] / (row["ms_per_step"] * 1e-3)
         extra["master_evolve"] = row
+
+    if not args.no_extra and not args.no_workloads:
+        from taichi_collisional_amd import learn
+        extra["clusters_c4"] = clusters_row(tca, ics, torch, dev)
+        row = {"what": "Dehnen gamma=1.5 N=%d, 10 fourth-order steps dt=1e-4 (GRADIENT loop, "
+                       "jerk-build theta-MAC FMM forces, 5 evaluations per step)" % args.c5_n,
+               "fallback_p12_theta04": cusp_steps_row(tca, ics, torch, dev, args.c5_n)}
+        if os.path.exists(learn.MODEL_PATH):
+            row["learned"] = cusp_steps_row(tca, ics, torch, dev, args.c5_n, learned=True)
+            row["learned_speedup"] = (row["fallback_p12_theta04"]["ms_per_step"] /
+                                      row["learned"]["ms_per_step"])
+        extra["cusp_steps_c5"] = row
+        if os.path.exists(learn.MODEL_PATH):
+            flr = tca.Fmm(p=args.p, ncrit=args.ncrit, direct_below=0)
+            flr.set_seed(args.seed)
+            learn.apply_model(flr, learn.load_model(), target_rms=1e-7)
+            r = timed_row(flr, torch, dx, dv, dm, (pot, acc, jerk), idx, da, dj)
+            c = flr.m2l_order_counts()
+            r["m2l_flops"] = float(sum(c[q] * flops.c_m2l(q) for q in range(len(c))))
+            r["speedup_vs_fixed_p12"] = ms / r["ms_per_eval"]
+            r["what"] = ("same workload, learned per-leaf opening angle at order %d, target rms "
+                         "1e-7 (FP64)" % args.p)
+            extra["learned_c2"] = r
+            flr.close()
 
     if args.calibrate_copy_mib > 0:
         src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)

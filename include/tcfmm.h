@@ -117,18 +117,21 @@ int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius);
  *   mode 1: every leaf at (q, theta) through the per-cell path;
  *   mode 2: the learned choice (needs tc_set_order_model). */
 int tc_set_adaptive(tc_context *ctx, int mode, int q, double theta, double probe_theta);
-/* The MLP (tc_mlp_param_count() floats: W1[64][10] b1[64] W2[64][64] b2[64]
- * W3[64][64] b3[64] W4[64] b4, tanh hidden units) maps a leaf's
+/* The MLP (tc_mlp_param_count() floats: W1[64][11] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[64] b4, tanh hidden units) maps a leaf's 9
  * features and a candidate (q / 20, theta) to log10 of the leaf's rms
  * relative force error; each leaf takes the cheapest of the ncand candidates
  * (relative costs cost[c]) predicted at or below target_rms, else the most
- * accurate.  params: host memory. */
+ * accurate.  Each feature is clamped to [feat_lo, feat_hi] (9 values each,
+ * the training range) before the network sees it.  Host memory. */
 int tc_set_order_model(tc_context *ctx, const float *params, int nparams, const int *q,
-                       const float *theta, const float *cost, int ncand, double target_rms);
+                       const float *theta, const float *cost, int ncand, const float *feat_lo,
+                       const float *feat_hi, double target_rms);
 /* After an evaluation with per-cell orders: the number of leaves; per leaf
- * the 8 features (level/16, log10 R/R_root, log2(bodies)/6, log10 mass/mean
+ * the 9 features (level/16, log10 R/R_root, log2(bodies)/6, log10 mass/mean
  * body mass, log10 of the order-2 and order-4 multipole norms over M_0 R^n,
- * log10(1 + M2L / P2P list length in the probe walk)), the leaf index of
+ * log10(1 + M2L / P2P list length in the probe walk), log10 of the distance
+ * from the root centre over R_root), the leaf index of
  * every body (caller order) and, in mode 2, each leaf's predicted log10
  * error.  Device pointers; any may be NULL. */
 int tc_leaf_count(const tc_context *ctx);
@@ -139,15 +142,17 @@ int tc_leaf_data(tc_context *ctx, float *features, int *leaf_of_body, float *pre
 int tc_leaf_errors(tc_context *ctx, const double *acc, const double *acc_ref, float *out);
 /* M2L cell pairs of the last evaluation per truncation order, counts[0..maxp]. */
 int tc_m2l_order_counts(const tc_context *ctx, long long *counts, int maxp);
-/* Training kernels, FP32, device pointers.  X: [B][10] inputs, T: [B]
- * targets; tc_mlp_grad writes the gradient of mean((Y - T)^2) into G
- * (nparams floats) and the loss into *loss; tc_mlp_adam applies one Adam
- * step (beta 0.9 / 0.999, eps 1e-8, step >= 1) to params with moments m, v. */
+/* Training kernels, FP32, device pointers.  X: [B][11] inputs, T: [B]
+ * targets; tc_mlp_grad writes the gradient of the expectile loss
+ * mean(w (Y - T)^2), w = tau where Y < T and 1 - tau elsewhere (0 < tau < 1),
+ * into G (nparams floats) and the loss into *loss; tc_mlp_adam applies one
+ * Adam step (beta 0.9 / 0.999, eps 1e-8, step >= 1) to params with moments
+ * m, v. */
 int tc_mlp_param_count(void);
 int tc_mlp_input_count(void);
 int tc_mlp_forward(tc_context *ctx, const float *params, const float *X, int B, float *Y);
 int tc_mlp_grad(tc_context *ctx, const float *params, const float *X, const float *T, int B,
-                float *G, float *loss);
+                float tau, float *G, float *loss);
 int tc_mlp_adam(tc_context *ctx, float *params, const float *G, float *m, float *v, float lr,
                 int step);
 

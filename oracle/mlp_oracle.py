@@ -8,12 +8,12 @@ this restatement is mathematics: its analytic gradient is checked against
 central finite differences (tests/test_mlp_oracle.py), and the forward pass
 against a plain layer-by-layer evaluation.
 
-Parameter layout (the kernels'): W1[64][10] b1[64] W2[64][64] b2[64]
+Parameter layout (the kernels'): W1[64][11] b1[64] W2[64][64] b2[64]
 W3[64][64] b3[64] W4[64] b4[1]; tanh hidden units, linear output.
 """
 import numpy as np
 
-NIN, NH = 10, 64
+NIN, NH = 11, 64
 SIZES = [("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
          ("W3", (NH, NH)), ("b3", (NH,)), ("W4", (NH,)), ("b4", (1,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SIZES)
@@ -57,16 +57,18 @@ def forward(P, X, keep=False):
     return (y, (h1, h2, h3)) if keep else y
 
 
-def loss_and_grad(P, X, T):
-    """mean((y - T)^2) and its gradient with respect to every parameter."""
+def loss_and_grad(P, X, T, tau=0.5):
+    """The expectile loss mean(w (y - T)^2), w = tau where y < T and 1 - tau
+    elsewhere, and its gradient with respect to every parameter."""
     w = unpack(P)
     X = np.asarray(X, dtype=np.float64)
     T = np.asarray(T, dtype=np.float64)
     B = X.shape[0]
     y, (h1, h2, h3) = forward(P, X, keep=True)
     r = y - T
-    loss = float(np.mean(r * r))
-    dy = 2.0 * r / B
+    wt = np.where(r < 0, tau, 1.0 - tau)
+    loss = float(np.mean(wt * r * r))
+    dy = 2.0 * wt * r / B
     g = {"W4": h3.T @ dy, "b4": np.array([dy.sum()])}
     d3 = dy[:, None] * w["W4"][None, :] * (1 - h3 * h3)
     g["W3"], g["b3"] = d3.T @ h2, d3.sum(0)

@@ -109,13 +109,14 @@ def lib():
         L.tc_set_precision.argtypes = [vp, C.c_int, C.c_double]
         L.tc_set_adaptive.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
         L.tc_eval_clusters.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
-        L.tc_set_order_model.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_double]
+        L.tc_set_order_model.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp,
+                                         C.c_double]
         L.tc_leaf_count.argtypes = [vp]
         L.tc_leaf_data.argtypes = [vp, vp, vp, vp]
         L.tc_leaf_errors.argtypes = [vp, vp, vp, vp]
         L.tc_m2l_order_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
         L.tc_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
-        L.tc_mlp_grad.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp]
+        L.tc_mlp_grad.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, vp, vp]
         L.tc_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_int]
         L.tc_eval_bodies.argtypes = [vp, vp, C.c_int]
         L.tc_eval_soa.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
@@ -575,27 +576,32 @@ class Fmm:
         self._check(lib().tc_set_adaptive(self.h, int(mode), int(q), float(theta),
                                           float(probe_theta)), "tc_set_adaptive")
 
-    def set_order_model(self, params, q, theta, cost, target_rms):
-        """The MLP parameters (float32, mlp_param_count() values) and the candidate
-        grid: orders q, angles theta, relative costs (one each per candidate)."""
+    def set_order_model(self, params, q, theta, cost, feat_lo, feat_hi, target_rms):
+        """The MLP parameters (float32, mlp_param_count() values), the candidate
+        grid (orders q, angles theta, relative costs: one each per candidate)
+        and the features' training range [feat_lo, feat_hi] (9 values each)."""
         import numpy as np
         P = np.ascontiguousarray(params, dtype=np.float32)
         qa = np.ascontiguousarray(q, dtype=np.int32)
         ta = np.ascontiguousarray(theta, dtype=np.float32)
         ca = np.ascontiguousarray(cost, dtype=np.float32)
-        if not (qa.size == ta.size == ca.size):
-            raise ValueError("q, theta and cost need one entry per candidate")
+        lo = np.ascontiguousarray(feat_lo, dtype=np.float32)
+        hi = np.ascontiguousarray(feat_hi, dtype=np.float32)
+        if not (qa.size == ta.size == ca.size) or lo.size != 9 or hi.size != 9:
+            raise ValueError("q, theta and cost need one entry per candidate, the feature "
+                             "range 9 values each")
         self._check(lib().tc_set_order_model(self.h, P.ctypes.data, P.size, qa.ctypes.data,
                                              ta.ctypes.data, ca.ctypes.data, qa.size,
-                                             float(target_rms)), "tc_set_order_model")
+                                             lo.ctypes.data, hi.ctypes.data, float(target_rms)),
+                    "tc_set_order_model")
 
     def leaf_data(self):
-        """After an evaluation with per-cell orders: (features [nleaves, 8],
+        """After an evaluation with per-cell orders: (features [nleaves, 9],
         leaf index of every body [n], predicted log10 error [nleaves] or None)."""
         import torch
         nl = lib().tc_leaf_count(self.h)
         dev = torch.device("cuda", torch.cuda.current_device())
-        f = torch.empty(nl, 8, dtype=torch.float32, device=dev)
+        f = torch.empty(nl, 9, dtype=torch.float32, device=dev)
         lob = torch.empty(self._last_n, dtype=torch.int32, device=dev)
         pred = torch.empty(nl, dtype=torch.float32, device=dev)
         self._check(lib().tc_leaf_data(self.h, _ptr(f), _ptr(lob), _ptr(pred)), "tc_leaf_data")
@@ -625,8 +631,9 @@ class Fmm:
                     "tc_mlp_forward")
         return Y
 
-    def mlp_grad(self, params, X, T):
-        """(gradient of mean((Y - T)^2) w.r.t. params, loss)."""
+    def mlp_grad(self, params, X, T, tau=0.5):
+        """(gradient of the expectile loss mean(w (Y - T)^2) w.r.t. params, loss);
+        w = tau where Y < T, 1 - tau elsewhere."""
         import torch
         _mlp_check(params, X)
         if T.dtype != torch.float32 or T.shape != (X.shape[0],) or not T.is_contiguous():
@@ -634,7 +641,7 @@ class Fmm:
         G = torch.empty_like(params)
         loss = torch.empty(1, dtype=torch.float32, device=X.device)
         self._check(lib().tc_mlp_grad(self.h, _ptr(params), _ptr(X), _ptr(T), X.shape[0],
-                                      _ptr(G), _ptr(loss)), "tc_mlp_grad")
+                                      float(tau), _ptr(G), _ptr(loss)), "tc_mlp_grad")
         return G, float(loss.item())
 
     def mlp_adam(self, params, G, m, v, lr, step):

@@ -169,7 +169,7 @@ struct tc_context {
   OrderGrid grid{};
   bool have_model = false;
   DBuf mlp_params, cq, ctheta, probe_m2l, probe_p2p, leaf_feat, leaf_pred;
-  DBuf chunk_part, chunk_hist;
+  DBuf chunk_part, chunk_hist, chunk_sort[5];  // chunk_sort: keys, keys, idx, idx, temp
   long long m2l_by_order[kMaxP + 1] = {};  // M2L pairs per truncation order
   DBuf mlp_Y, mlp_H, mlp_D, mlp_dY;         // tc_mlp_grad scratch
   int mlp_cap = 0;

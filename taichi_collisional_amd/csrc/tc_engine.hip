@@ -584,21 +584,25 @@ static int choose_cell_orders(tc_context *ctx, int n, const double *m) {
 
 // M2L chunks (grouped by target) partitioned by their target's order, and
 // one launch per order present; M2L pair counts per order for the flop count.
-__global__ void k_chunk_hist(const int4 *__restrict__ ch, int nch, const int *__restrict__ q,
-                             int *__restrict__ hist) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nch) return;
-  const int4 c = ch[k];
-  atomicAdd(&hist[q[c.x]], 1);
-  atomicAdd(&hist[kMaxP + 1 + q[c.x]], c.z);
-}
-
-__global__ void k_chunk_scatter(const int4 *__restrict__ ch, int nch, const int *__restrict__ q,
-                                int *__restrict__ cur, int4 *__restrict__ out) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nch) return;
-  const int4 c = ch[k];
-  out[atomicAdd(&cur[q[c.x]], 1)] = c;
+// Per-order chunk and pair counts: a block histogram in LDS, then one global
+// atomic per order per block (most chunks share a few orders, so direct
+// global atomics would all hit the same two words).
+__global__ __launch_bounds__(256) void k_chunk_hist(const int4 *__restrict__ ch, int nch,
+                                                    const int *__restrict__ q,
+                                                    int *__restrict__ hist) {
+  constexpr int H = kMaxP + 1;
+  __shared__ int sh[2 * H];
+  for (int t = threadIdx.x; t < 2 * H; t += 256) sh[t] = 0;
+  __syncthreads();
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < nch; k += gridDim.x * 256) {
+    const int4 c = ch[k];
+    const int o = q[c.x];
+    atomicAdd(&sh[o], 1);
+    atomicAdd(&sh[H + o], c.z);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 2 * H; t += 256)
+    if (sh[t]) atomicAdd(&hist[t], sh[t]);
 }
 
 static int m2l_by_order(tc_context *ctx, M2LArgs ma, int nchunks) {
@@ -609,8 +613,8 @@ static int m2l_by_order(tc_context *ctx, M2LArgs ma, int nchunks) {
   int *hist = ctx->chunk_hist.as<int>();
   TC_CHECK(hipMemsetAsync(hist, 0, sizeof(int) * 3 * H, s));
   if (nchunks > 0)
-    hipLaunchKernelGGL(k_chunk_hist, dim3((nchunks + 255) / 256), dim3(256), 0, s, ma.chunks,
-                       nchunks, ctx->cq.as<int>(), hist);
+    hipLaunchKernelGGL(k_chunk_hist, dim3(std::min(1024, (nchunks + 255) / 256)), dim3(256), 0,
+                       s, ma.chunks, nchunks, ctx->cq.as<int>(), hist);
   int *h = ctx->h_pin + 64;  // [64, 64 + 2H): chunk and pair counts per order
   TC_CHECK(hipMemcpyAsync(h, hist, sizeof(int) * 2 * H, hipMemcpyDeviceToHost, s));
   TC_CHECK(spin_wait(ctx, s));
@@ -618,11 +622,18 @@ static int m2l_by_order(tc_context *ctx, M2LArgs ma, int nchunks) {
   off[0] = 0;
   for (int q = 0; q < H; q++) off[q + 1] = off[q] + h[q];
   for (int q = 0; q < H; q++) ctx->m2l_by_order[q] = h[H + q];
-  // cursors start at each order's offset
-  TC_CHECK(hipMemcpyAsync(hist + 2 * H, off, sizeof(int) * H, hipMemcpyHostToDevice, s));
-  if (nchunks > 0)
-    hipLaunchKernelGGL(k_chunk_scatter, dim3((nchunks + 255) / 256), dim3(256), 0, s, ma.chunks,
-                       nchunks, ctx->cq.as<int>(), hist + 2 * H, ctx->chunk_part.as<int4>());
+  // a stable sort by order: each order's chunks keep the list's target order
+  const size_t nb = sizeof(int) * (size_t)std::max(nchunks, 1);
+  for (int k = 0; k < 4; k++) TC_CHECK(ctx->chunk_sort[k].ensure(nb));
+  TC_CHECK(ctx->chunk_sort[4].ensure(pair_sort_temp_bytes(std::max(nchunks, 1)) + 256));
+  if (launch_sort_chunks_by_order(ma.chunks, nchunks, ctx->cq.as<int>(),
+                                  ctx->chunk_sort[0].as<int>(), ctx->chunk_sort[1].as<int>(),
+                                  ctx->chunk_sort[2].as<int>(), ctx->chunk_sort[3].as<int>(),
+                                  ctx->chunk_part.as<int4>(), ctx->chunk_sort[4].p,
+                                  ctx->chunk_sort[4].bytes, s)) {
+    ctx->err = "chunk sort failed";
+    return -4;
+  }
   const int P = ctx->cfg.p;
   for (int q = 0; q <= P; q++) {
     if (off[q + 1] == off[q]) continue;
@@ -1146,7 +1157,9 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->ev_uts, &ctx->ev_uao, &ctx->ev_tmp, &ctx->ev_flags, &ctx->ev_offs,
                  &ctx->ev_scan_tmp, &ctx->mlp_params, &ctx->cq, &ctx->ctheta, &ctx->probe_m2l,
                  &ctx->probe_p2p, &ctx->leaf_feat, &ctx->leaf_pred, &ctx->chunk_part,
-                 &ctx->chunk_hist, &ctx->mlp_Y, &ctx->mlp_H, &ctx->mlp_D, &ctx->mlp_dY};
+                 &ctx->chunk_hist, &ctx->mlp_Y, &ctx->mlp_H, &ctx->mlp_D, &ctx->mlp_dY,
+                 &ctx->chunk_sort[0], &ctx->chunk_sort[1], &ctx->chunk_sort[2],
+                 &ctx->chunk_sort[3], &ctx->chunk_sort[4]};
   for (DBuf *b : all) b->release();
   for (DBuf &b : ctx->ev_lev) b.release();
   ctx->h_leaves.release();
@@ -1225,15 +1238,24 @@ int tc_set_adaptive(tc_context *ctx, int mode, int q, double theta, double probe
 }
 
 int tc_set_order_model(tc_context *ctx, const float *params, int nparams, const int *q,
-                       const float *theta, const float *cost, int ncand, double target_rms) {
+                       const float *theta, const float *cost, int ncand, const float *feat_lo,
+                       const float *feat_hi, double target_rms) {
   if (!ctx) return -1;
   if (!params || nparams != kMlpParams || !q || !theta || !cost || ncand < 1 ||
-      ncand > kMaxCand || !(target_rms > 0)) {
+      ncand > kMaxCand || !(target_rms > 0) || !feat_lo || !feat_hi) {
     ctx->err = "tc_set_order_model: bad arguments";
     return -1;
   }
   OrderGrid g{};
   g.ncand = ncand;
+  for (int i = 0; i < kLeafFeatures; i++) {
+    if (!(feat_lo[i] <= feat_hi[i])) {
+      ctx->err = "tc_set_order_model: feature range with lo > hi";
+      return -1;
+    }
+    g.lo[i] = feat_lo[i];
+    g.hi[i] = feat_hi[i];
+  }
   for (int c = 0; c < ncand; c++) {
     if (q[c] < 2 || q[c] > ctx->cfg.p || !(theta[c] > 0 && theta[c] < 2)) {
       ctx->err = "tc_set_order_model: candidate order outside [2, p] or angle outside (0, 2)";
@@ -1304,8 +1326,10 @@ int tc_mlp_forward(tc_context *ctx, const float *params, const float *X, int B, 
 }
 
 int tc_mlp_grad(tc_context *ctx, const float *params, const float *X, const float *T, int B,
-                float *G, float *loss) {
-  if (!ctx || !params || !G || !loss || B < 0 || (B > 0 && (!X || !T))) return -1;
+                float tau, float *G, float *loss) {
+  if (!ctx || !params || !G || !loss || B < 0 || (B > 0 && (!X || !T)) || !(tau > 0) ||
+      !(tau < 1))
+    return -1;
   if (B > ctx->mlp_cap) {
     const size_t b = (size_t)B;
     TC_CHECK(ctx->mlp_Y.ensure(sizeof(float) * b));
@@ -1314,7 +1338,7 @@ int tc_mlp_grad(tc_context *ctx, const float *params, const float *X, const floa
     TC_CHECK(ctx->mlp_D.ensure(sizeof(float) * 3 * kMlpH * b));
     ctx->mlp_cap = B;
   }
-  launch_mlp_grad(params, X, T, B, ctx->mlp_Y.as<float>(), ctx->mlp_H.as<float>(),
+  launch_mlp_grad(params, X, T, B, tau, ctx->mlp_Y.as<float>(), ctx->mlp_H.as<float>(),
                   ctx->mlp_D.as<float>(), ctx->mlp_dY.as<float>(), G, loss, ctx->stream);
   TC_CHECK(hipStreamSynchronize(ctx->stream));
   return 0;

@@ -153,6 +153,10 @@ int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *
                        size_t temp_bytes, hipStream_t s);
 int launch_make_chunks(const int *offs, const int *count, const int *chunk_offs, int ncells,
                        int4 *chunks, hipStream_t s);
+// chunks stably sorted by q[target] (orders < 32); scratch key/idx arrays of n ints
+int launch_sort_chunks_by_order(const int4 *chunks, int n, const int *q, int *key_a, int *key_b,
+                                int *idx_a, int *idx_b, int4 *out, void *temp, size_t temp_bytes,
+                                hipStream_t s);
 
 // ---- upward (tc_upward.hip)
 // what: kUpGeom (ball X, R, has_sink), kUpMin (min_acc from acc_old),
@@ -227,7 +231,7 @@ int launch_l2p(int p, CellsDev c, BodiesDev b, ResultsDev r, const int *leaf_ids
 void launch_iota(int *p, int n, hipStream_t s);
 void launch_zero_u8(unsigned char *p, int n, hipStream_t s);
 // ---- learned per-cell order and opening angle (tc_learn.hip)
-constexpr int kLeafFeatures = 8, kMlpIn = kLeafFeatures + 2, kMlpH = 64;
+constexpr int kLeafFeatures = 9, kMlpIn = kLeafFeatures + 2, kMlpH = 64;
 constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + 2 * (kMlpH * kMlpH + kMlpH) + kMlpH + 1;
 constexpr int kMaxCand = 64;
 // Candidate (order, angle) pairs of the choice and their relative costs.
@@ -236,12 +240,15 @@ struct OrderGrid {
   int q[kMaxCand];
   float theta[kMaxCand];
   float cost[kMaxCand];
+  // each leaf feature clamped to the range seen in training before the MLP
+  float lo[kLeafFeatures], hi[kLeafFeatures];
 };
 void launch_mlp_fwd(const float *P, const float *X, int B, float *Y, float *H, hipStream_t s);
-// loss = mean (Y - T)^2 and its gradient G (zeroed first); Y [B], H and D
-// [3][B][64], dY [B] are scratch
-void launch_mlp_grad(const float *P, const float *X, const float *T, int B, float *Y, float *H,
-                     float *D, float *dY, float *G, float *loss, hipStream_t s);
+// loss = mean w (Y - T)^2 with w = tau where Y < T and 1 - tau elsewhere
+// (tau = 0.5: half the squared error) and its gradient G (zeroed first);
+// Y [B], H and D [3][B][64], dY [B] are scratch
+void launch_mlp_grad(const float *P, const float *X, const float *T, int B, float tau, float *Y,
+                     float *H, float *D, float *dY, float *G, float *loss, hipStream_t s);
 void launch_adam(float *P, const float *G, float *m, float *v, float lr, int t, hipStream_t s);
 void launch_count_targets(const int2 *pairs, int n, int *cnt, hipStream_t s);
 void launch_leaf_features(CellsDev c, const int *leaf_ids, int nleaves, int p,

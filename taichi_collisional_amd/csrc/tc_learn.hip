@@ -1,8 +1,10 @@
 // Learned per-cell expansion order and opening angle.
 //
-// A small MLP (10 inputs, three hidden layers of 64 tanh units, one linear
+// A small MLP (11 inputs, three hidden layers of 64 tanh units, one linear
 // output, FP32) predicts log10 of a leaf's rms relative force error from the
-// leaf's features and a candidate (order q, opening angle theta).  Before the
+// leaf's features and a candidate (order q, opening angle theta).  It is
+// fitted with an asymmetric (expectile) squared loss that weighs
+// under-predicted errors more, so that its predictions lean to the safe side.  Before the
 // walk, every leaf takes the cheapest candidate whose predicted error is below
 // the target (or, if none is, the most accurate one); internal cells take the
 // largest order and the smallest angle below them, so a leaf's ancestors are
@@ -16,8 +18,8 @@
 // the multipole norms it uses (integrator.h:387-399) are the analogues: the
 // error estimate there is a formula, here it is fitted.
 //
-// Parameter layout (floats): W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64]
-// b3[64] W4[64] b4[1]; 9,089 values.  Weights are read at addresses that are
+// Parameter layout (floats): W1[64][11] b1[64] W2[64][64] b2[64] W3[64][64]
+// b3[64] W4[64] b4[1]; 9,153 values.  Weights are read at addresses that are
 // the same on every lane, so they load through the scalar cache and enter
 // the FMAs as scalar operands.
 #include <hip/hip_runtime.h>
@@ -86,21 +88,23 @@ __global__ __launch_bounds__(64) void k_mlp_fwd(const float *__restrict__ P,
   Y[b] = y;
 }
 
-// Backward of the mean squared error (1/B) sum_b (Y[b] - T[b])^2 through the
-// output and the three tanh layers: the layer deltas D[l][b][64]
+// Backward of the expectile loss (1/B) sum_b w_b (Y[b] - T[b])^2, w_b = tau
+// where Y[b] < T[b] (an under-predicted error) and 1 - tau elsewhere, through
+// the output and the three tanh layers: the layer deltas D[l][b][64]
 // (dLoss/dpre-activation), dY[b], and the loss (added into *loss).
 __global__ __launch_bounds__(64) void k_mlp_bwd(const float *__restrict__ P,
                                                 const float *__restrict__ H,
                                                 const float *__restrict__ Y,
-                                                const float *__restrict__ T, int B,
+                                                const float *__restrict__ T, int B, float tau,
                                                 float *__restrict__ D, float *__restrict__ dY,
                                                 float *__restrict__ loss) {
   const int b = blockIdx.x * 64 + threadIdx.x;
   float l = 0.0f;
   if (b < B) {
     const float r = Y[b] - T[b];
-    l = r * r / (float)B;
-    const float dy = 2.0f * r / (float)B;
+    const float w = r < 0.0f ? tau : 1.0f - tau;
+    l = w * r * r / (float)B;
+    const float dy = 2.0f * w * r / (float)B;
     dY[b] = dy;
     float d3[kMlpH], d2[kMlpH];
 #pragma unroll
@@ -205,7 +209,9 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
 // Per-leaf features (kLeafFeatures floats): level / 16; log10(R / R_root);
 // log2(bodies) / 6; log10(mass / mean body mass); log10 of the order-2 and
 // order-4 multipole norms relative to M_0 R^n; log10(1 + M2L and P2P list
-// lengths of the leaf in the probe walk).
+// lengths of the leaf in the probe walk); log10 of the leaf's distance from
+// the root's centre over R_root (a leaf far out in a halo feels the whole
+// system as a few large far-field terms).
 __global__ void k_leaf_features(CellsDev c, const int *__restrict__ leaf_ids, int nleaves,
                                 int NT, int P, const int *__restrict__ m2l_cnt,
                                 const int *__restrict__ p2p_cnt, double mean_mass,
@@ -233,6 +239,9 @@ __global__ void k_leaf_features(CellsDev c, const int *__restrict__ leaf_ids, in
   f[5] = (float)log10(q4 + 1e-6);
   f[6] = (float)log10(1.0 + (double)m2l_cnt[ci]);
   f[7] = (float)log10(1.0 + (double)p2p_cnt[ci]);
+  const double ex = c.X[3 * ci] - c.X[0], ey = c.X[3 * ci + 1] - c.X[1],
+               ez = c.X[3 * ci + 2] - c.X[2];
+  f[8] = (float)log10(sqrt(ex * ex + ey * ey + ez * ez) / R0 + 1e-3);
 }
 
 // Pairs per target cell of a walk's list (counts zeroed by the caller).
@@ -243,7 +252,8 @@ __global__ void k_count_targets(const int2 *__restrict__ pairs, int n, int *__re
 
 // ---------------------------------------------------------------- choice
 // One block per leaf, one lane per candidate (q, theta): predicted log10
-// error, then the cheapest candidate at or below log_target (else the most
+// error (the features clamped to their training range: no extrapolation),
+// then the cheapest candidate at or below log_target (else the most
 // accurate); the leaf's choice goes to q[leaf cell], theta[leaf cell].
 __global__ __launch_bounds__(64) void k_leaf_select(const float *__restrict__ P,
                                                     const float *__restrict__ F,
@@ -256,7 +266,8 @@ __global__ __launch_bounds__(64) void k_leaf_select(const float *__restrict__ P,
   const int cc = act ? c : 0;
   float x[kMlpIn];
 #pragma unroll
-  for (int i = 0; i < kLeafFeatures; i++) x[i] = F[(size_t)l * kLeafFeatures + i];
+  for (int i = 0; i < kLeafFeatures; i++)
+    x[i] = fminf(fmaxf(F[(size_t)l * kLeafFeatures + i], g.lo[i]), g.hi[i]);
   x[kLeafFeatures] = (float)g.q[cc] / 20.0f;
   x[kLeafFeatures + 1] = (float)g.theta[cc];
   const float y = mlp_eval(P, x);
@@ -346,13 +357,14 @@ void launch_mlp_fwd(const float *P, const float *X, int B, float *Y, float *H, h
   if (B > 0) hipLaunchKernelGGL(k_mlp_fwd, dim3((B + 63) / 64), dim3(64), 0, s, P, X, B, Y, H);
 }
 
-void launch_mlp_grad(const float *P, const float *X, const float *T, int B, float *Y, float *H,
-                     float *D, float *dY, float *G, float *loss, hipStream_t s) {
+void launch_mlp_grad(const float *P, const float *X, const float *T, int B, float tau, float *Y,
+                     float *H, float *D, float *dY, float *G, float *loss, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, g256(kMlpParams), dim3(256), 0, s, G, kMlpParams, 0.0f);
   hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, s, loss, 1, 0.0f);
   if (B <= 0) return;
   hipLaunchKernelGGL(k_mlp_fwd, dim3((B + 63) / 64), dim3(64), 0, s, P, X, B, Y, H);
-  hipLaunchKernelGGL(k_mlp_bwd, dim3((B + 63) / 64), dim3(64), 0, s, P, H, Y, T, B, D, dY, loss);
+  hipLaunchKernelGGL(k_mlp_bwd, dim3((B + 63) / 64), dim3(64), 0, s, P, H, Y, T, B, tau, D, dY,
+                     loss);
   const int per = 256;
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((B + per - 1) / per), dim3(64), 0, s, X, H, D, dY, B, per,
                      G);

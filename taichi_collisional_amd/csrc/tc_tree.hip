@@ -876,6 +876,38 @@ int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *
   return 0;
 }
 
+// M2L chunks in a stable order by their target's order (per-cell orders):
+// each order's chunks keep the target order of the list, which keeps
+// neighbouring targets (sharing sources in L2) on one XCD.
+__global__ void k_chunk_keys(const int4 *__restrict__ ch, int n, const int *__restrict__ q,
+                             int *__restrict__ key, int *__restrict__ idx) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  key[k] = q[ch[k].x];
+  idx[k] = k;
+}
+
+__global__ void k_chunk_gather(const int4 *__restrict__ ch, const int *__restrict__ idx, int n,
+                               int4 *__restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) out[k] = ch[idx[k]];
+}
+
+int launch_sort_chunks_by_order(const int4 *chunks, int n, const int *q, int *key_a, int *key_b,
+                                int *idx_a, int *idx_b, int4 *out, void *temp, size_t temp_bytes,
+                                hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_chunk_keys, dim3((n + 255) / 256), dim3(256), 0, s, chunks, n, q, key_a,
+                     idx_a);
+  size_t tb = temp_bytes;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, key_a, key_b, idx_a, idx_b, n, 0, 5, s) !=
+      hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_chunk_gather, dim3((n + 255) / 256), dim3(256), 0, s, chunks, idx_b, n,
+                     out);
+  return 0;
+}
+
 // chunk_offs = exclusive scan of ceil(count/64) over cells
 __global__ void k_make_chunks(const int *__restrict__ offs, const int *__restrict__ count,
                               const int *__restrict__ chunk_offs, int ncells,

@@ -8,7 +8,7 @@ import mlp_oracle as mo
 
 def test_param_count_matches_library():
     import taichi_collisional_amd as tca
-    assert mo.NPARAM == tca.mlp_param_count() == 9089
+    assert mo.NPARAM == tca.mlp_param_count() == 9153
 
 
 def test_forward_equals_per_sample_loop():
@@ -31,7 +31,8 @@ def test_gradient_equals_finite_differences():
     P = mo.init_params(4) + rng.normal(scale=0.05, size=mo.NPARAM)
     X = rng.normal(size=(33, mo.NIN))
     T = rng.normal(size=33)
-    _, g = mo.loss_and_grad(P, X, T)
+    tau = 0.8  # the expectile weights: both branches of w are crossed
+    _, g = mo.loss_and_grad(P, X, T, tau)
     # every parameter block, a few entries each (biases and weights alike)
     idx, o = [], 0
     for _, sh in mo.SIZES:
@@ -42,5 +43,6 @@ def test_gradient_equals_finite_differences():
     for k in idx:
         e = np.zeros(mo.NPARAM)
         e[k] = h
-        fd = (mo.loss_and_grad(P + e, X, T)[0] - mo.loss_and_grad(P - e, X, T)[0]) / (2 * h)
+        fd = (mo.loss_and_grad(P + e, X, T, tau)[0] -
+              mo.loss_and_grad(P - e, X, T, tau)[0]) / (2 * h)
         assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(g[k]), (k, fd, g[k])
