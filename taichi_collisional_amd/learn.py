@@ -2,6 +2,7 @@
 training and the model file.
 
     python -m taichi_collisional_amd.learn train [--snapshots 200] [--out PATH]
+    python -m taichi_collisional_amd.learn calibrate [--out PATH]
 
 1. Measurement.  For each seeded snapshot (Plummer and Dehnen gamma = 1.5 /
    1.0 spheres of 2^15 .. 2^17 bodies, some at 2^18 and 2^19) the FP64 direct sum
@@ -251,9 +252,57 @@ def apply_model(fmm, model, target_rms=1e-7, margin=1.0, min_order=MIN_ORDER, or
     fmm.set_adaptive(2, probe_theta=PROBE_THETA)
 
 
+def calibrate(torch, tca, model, cases=(("plummer", 1 << 18, 9001), ("dehnen15", 1 << 18, 9002),
+                                       ("plummer", 1 << 20, 1), ("dehnen15", 1 << 20, 2),
+                                       ("dehnen10", 1 << 19, 77)),
+              thetas=(0.4, 0.35, 0.3), margins=(1.0, 0.5), target=1e-7, p=12, log=print):
+    """The learned per-leaf angle against fixed angles on snapshots outside the
+    training set: rms relative force error (FP64 direct sum, seeded 4096-body
+    subset), M2L + P2P work (flop) and ms of one evaluation per setting."""
+    import time
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rows = []
+    for kind, n, seed in cases:
+        x, v, m = make_bodies(kind, n, seed)
+        dx, dv, dm = (torch.from_numpy(a).to(dev) for a in (x, v, m))
+        rng = np.random.default_rng(seed)
+        tg = torch.from_numpy(np.sort(rng.choice(n, 4096, replace=False)).astype(np.int32)).to(dev)
+        f0 = tca.Fmm(p=4, direct_below=0)
+        _, da, _ = f0.direct(dx, dv, dm, tg)
+        f0.close()
+
+        def run(f):
+            f.set_seed(seed)
+            f.eval(dx, dv, dm)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _, acc, _ = f.eval(dx, dv, dm)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) * 1e3
+            st = f.stats()
+            c = f.m2l_order_counts()
+            w = sum(c[q] * flops.c_m2l(q) for q in range(len(c))) + 44.0 * st["n_p2p_pairs"]
+            e = torch.linalg.norm(acc[tg.long()] - da, dim=1) / torch.linalg.norm(da, dim=1)
+            return {"rms": float(torch.sqrt((e ** 2).mean())), "max": float(e.max()),
+                    "work": float(w), "ms": ms}
+        row = {"kind": kind, "n": n, "seed": seed}
+        for th in thetas:
+            f = tca.Fmm(p=p, ncrit=64, theta=th, direct_below=0)
+            row["fixed_%.2f" % th] = run(f)
+            f.close()
+        for mg in margins:
+            f = tca.Fmm(p=p, ncrit=64, theta=0.4, direct_below=0)
+            apply_model(f, model, target_rms=target, margin=mg)
+            row["learned_margin_%.1f" % mg] = run(f)
+            f.close()
+        log(json.dumps(row))
+        rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "calibrate"])
     ap.add_argument("--snapshots", type=int, default=200)
     ap.add_argument("--validation", type=int, default=12)
     ap.add_argument("--epochs", type=int, default=30)
@@ -262,6 +311,9 @@ def main():
     args = ap.parse_args()
     import torch
     import taichi_collisional_amd as tca
+    if args.cmd == "calibrate":
+        calibrate(torch, tca, load_model(args.out), log=lambda r: print(r, flush=True))
+        return
     t0 = time.time()
     fmm = tca.Fmm(p=P_MAX, ncrit=64, theta=0.4, direct_below=0)
     fmm.set_seed(1)
