@@ -312,6 +312,35 @@ def cusp_steps_row(tca, ics, torch, dev, n, learned=False, steps=10, dt=1e-4, se
             "m2l_pairs_by_order": {q: cc for q, cc in enumerate(c) if cc}}
 
 
+def kroupa_mixed_row(tca, ics, torch, dev, n=1 << 22, seed=3, p=12, theta=0.4):
+    """Plummer N with Kroupa IMF masses and 10 % of the bodies in binaries
+    (a in [1e-5, 1e-3]): the FP64 near field against the mixed forms (FP64
+    pairs inside r = 1e-3, which holds the binaries); error against the FP64
+    direct sum on a seeded 4096-body subset."""
+    import numpy as np
+    x, v, m = ics.kroupa_binaries(n, seed=seed)
+    dx, dv, dm = (torch.from_numpy(a).to(dev) for a in (x, v, m))
+    out = (torch.empty(n, dtype=torch.float64, device=dev),
+           torch.empty(n, 3, dtype=torch.float64, device=dev),
+           torch.empty(n, 3, dtype=torch.float64, device=dev))
+    rng = np.random.default_rng(seed)
+    tg = torch.from_numpy(np.sort(rng.choice(n, 4096, replace=False)).astype(np.int32)).to(dev)
+    f0 = tca.Fmm(p=4, direct_below=0)
+    _, da, dj = f0.direct(dx, dv, dm, tg)
+    f0.close()
+    row = {"what": "Plummer N=%d, Kroupa IMF, 10%% of the bodies in binaries (a 1e-5..1e-3), "
+                   "p=%d, theta=%g: FP64 near field vs the mixed forms (FP64 pairs r < 1e-3)"
+                   % (n, p, theta)}
+    for name, mode in (("fp64", tca.TC_P2P_FP64), ("fp32_offsets", tca.TC_P2P_MIXED),
+                       ("fp32_hilo_offsets", tca.TC_P2P_MIXED_HILO)):
+        f = tca.Fmm(p=p, ncrit=64, theta=theta, direct_below=0)
+        f.set_seed(seed)
+        f.set_precision(mode, close_radius=1e-3)
+        row[name] = timed_row(f, torch, dx, dv, dm, out, tg.long(), da, dj, reps=2, warm=1)
+        f.close()
+    return row
+
+
 def pareto(args):
     """Cost and accuracy of the ways to trade one for the other on the workload
     (Plummer N, seed): fixed (p, theta) on a grid, the error-controlled MAC at
@@ -569,6 +598,7 @@ def main():
             row["learned_speedup"] = (row["fallback_p12_theta04"]["ms_per_step"] /
                                       row["learned"]["ms_per_step"])
         extra["cusp_steps_c5"] = row
+        extra["kroupa_binaries_c3"] = kroupa_mixed_row(tca, ics, torch, dev)
         if os.path.exists(learn.MODEL_PATH):
             flr = tca.Fmm(p=args.p, ncrit=args.ncrit, direct_below=0)
             flr.set_seed(args.seed)

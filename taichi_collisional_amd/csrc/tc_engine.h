@@ -190,6 +190,14 @@ struct tc_context {
 int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        const double *m, const unsigned char *flags, int mode, double dt,
                        double *pot, double *acc, double *tstep, double *acc_old);
+// The master default build's all-pairs force without time steps, queued on
+// the context's stream with no wait: the bodies must already be in the
+// context's body arrays (order = identity) with the leaf-end flags zeroed.
+// master_direct_prepare sizes those arrays for n bodies first.
+int master_direct_prepare(tc_context *ctx, int n);
+void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
+                         double *acc);
+
 // The jerk build's get_force_and_potential on device arrays (tc_eval_soa);
 // used by the step loop's TC_EVOLVE_JERK_FMM mode.
 int eval_soa_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,

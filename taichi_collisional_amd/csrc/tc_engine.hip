@@ -658,6 +658,22 @@ static int m2l_by_order(tc_context *ctx, M2LArgs ma, int nchunks) {
   return 0;
 }
 
+// The all-pairs branch's sums (caller order) into the outputs; bodies that
+// are not sinks get zeros (any output may be null).
+__global__ void k_direct_out(const unsigned char *__restrict__ flags, int n,
+                             const double *__restrict__ p, const double *__restrict__ a,
+                             const double *__restrict__ j, double *__restrict__ pot,
+                             double *__restrict__ acc, double *__restrict__ jerk) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const bool sink = !flags || (flags[i] & kSink);
+  if (pot) pot[i] = sink ? p[i] : 0.0;
+  for (int d = 0; d < 3; d++) {
+    if (acc) acc[3 * (size_t)i + d] = sink ? a[3 * (size_t)i + d] : 0.0;
+    if (jerk) jerk[3 * (size_t)i + d] = sink ? j[3 * (size_t)i + d] : 0.0;
+  }
+}
+
 static int eval_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                        const unsigned char *flags, double *pot, double *acc, double *jerk) {
   hipStream_t s = ctx->stream;
@@ -698,34 +714,21 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
 
   if (cfg.direct_below > 0 && n < cfg.direct_below && ctx->ncl == 1) {
     // all-pairs sum in the caller's order (direct(), traverse_eager.h:295-304)
-    std::vector<int> idx(n);
-    for (int i = 0; i < n; i++) idx[i] = i;
     BodiesDev b = bodies_of(ctx);
     TC_CHECK(ctx->dtg.ensure(sizeof(int) * n));
-    TC_CHECK(hipMemcpyAsync(ctx->dtg.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    launch_iota(ctx->dtg.as<int>(), n, s);
     launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, b, s);
     TC_CHECK(ctx->dpot.ensure(sizeof(double) * n));
     TC_CHECK(ctx->dacc.ensure(sizeof(double) * 3 * n));
     TC_CHECK(ctx->djerk.ensure(sizeof(double) * 3 * n));
     launch_direct(b, n, ctx->dtg.as<int>(), n, ctx->dpot.as<double>(), ctx->dacc.as<double>(),
                   ctx->djerk.as<double>(), s);
-    // non-sinks receive nothing
-    std::vector<unsigned char> fl(n, 3);
-    if (flags) TC_CHECK(hipMemcpyAsync(fl.data(), flags, n, hipMemcpyDeviceToHost, s));
-    std::vector<double> hp(n), ha(3 * n), hj(3 * n);
-    TC_CHECK(hipMemcpyAsync(hp.data(), ctx->dpot.p, 8 * n, hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipMemcpyAsync(ha.data(), ctx->dacc.p, 24 * n, hipMemcpyDeviceToHost, s));
-    TC_CHECK(hipMemcpyAsync(hj.data(), ctx->djerk.p, 24 * n, hipMemcpyDeviceToHost, s));
+    // to the outputs; non-sinks receive nothing
+    hipLaunchKernelGGL(k_direct_out, dim3((n + 255) / 256), dim3(256), 0, s, flags, n,
+                       ctx->dpot.as<double>(), ctx->dacc.as<double>(), ctx->djerk.as<double>(),
+                       pot, acc, jerk);
     TC_CHECK(hipStreamSynchronize(s));
-    for (int i = 0; i < n; i++)
-      if (!(fl[i] & kSink)) {
-        hp[i] = 0;
-        for (int d = 0; d < 3; d++) ha[3 * i + d] = hj[3 * i + d] = 0;
-      }
-    if (pot) TC_CHECK(hipMemcpyAsync(pot, hp.data(), 8 * n, hipMemcpyHostToDevice, s));
-    if (acc) TC_CHECK(hipMemcpyAsync(acc, ha.data(), 24 * n, hipMemcpyHostToDevice, s));
-    if (jerk) TC_CHECK(hipMemcpyAsync(jerk, hj.data(), 24 * n, hipMemcpyHostToDevice, s));
-    TC_CHECK(hipStreamSynchronize(s));
+    TC_CHECK(hipGetLastError());
     return 0;
   }
 
@@ -1073,6 +1076,21 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
 int eval_soa_device(tc_context *ctx, int n, const double *x, const double *v, const double *m,
                     const unsigned char *flags, double *pot, double *acc, double *jerk) {
   return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
+}
+
+int master_direct_prepare(tc_context *ctx, int n) {
+  if (int e = ensure_bodies(ctx, n)) return e;
+  TC_CHECK(ctx->mlend.ensure((size_t)n));
+  const int nslab = master_direct_slabs(n);
+  if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 5 * (size_t)nslab * n));
+  return 0;
+}
+
+void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
+                         double *acc) {
+  launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, false, pair_mass, dt,
+                       pot, acc, nullptr, master_direct_slabs(n), ctx->mdpart.as<double>(),
+                       ctx->stream);
 }
 
 extern "C" {
@@ -1553,10 +1571,8 @@ int tc_direct_soa(tc_context *ctx, int n, const double *x, const double *v, cons
   }
   if (int e = ensure_bodies(ctx, n)) return e;
   hipStream_t s = ctx->stream;
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; i++) idx[i] = i;
   TC_CHECK(ctx->dtg.ensure(sizeof(int) * (size_t)n));
-  TC_CHECK(hipMemcpyAsync(ctx->dtg.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  launch_iota(ctx->dtg.as<int>(), n, s);
   launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, bodies_of(ctx), s);
   launch_direct(bodies_of(ctx), n, targets, nt, pot, acc, jerk, s);
   TC_CHECK(hipStreamSynchronize(s));

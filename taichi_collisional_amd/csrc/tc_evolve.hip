@@ -48,6 +48,36 @@ __global__ void k_gather_union(const int *__restrict__ is, int ns, const int *__
   fl[k] = sink ? (kSink | (sinks_are_sources ? kSource : 0)) : kSource;
 }
 
+// The same union straight into the context's body arrays (structure of
+// arrays, order = identity, no leaf ends), for the all-pairs force without a
+// tree: one launch instead of a gather into staging and a second gather.
+__global__ void k_gather_union_bodies(const int *__restrict__ is, int ns,
+                                      const int *__restrict__ js, int nj, bool sinks_are_sources,
+                                      const double *__restrict__ pos,
+                                      const double *__restrict__ vel,
+                                      const double *__restrict__ mass,
+                                      const double *__restrict__ acc, double shift_fac,
+                                      BodiesDev b, unsigned char *__restrict__ lend) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns + nj) return;
+  const bool sink = k < ns;
+  const int i = sink ? is[k] : js[k - ns];
+  double x[3];
+  for (int d = 0; d < 3; d++)
+    x[d] = shift_fac != 0.0 ? pos[3 * i + d] + acc[3 * i + d] * shift_fac : pos[3 * i + d];
+  b.x[k] = x[0];
+  b.y[k] = x[1];
+  b.z[k] = x[2];
+  b.vx[k] = vel[3 * i];
+  b.vy[k] = vel[3 * i + 1];
+  b.vz[k] = vel[3 * i + 2];
+  b.q[k] = mass[i];
+  b.flags[k] = sink ? (kSink | (sinks_are_sources ? kSource : 0)) : kSource;
+  b.order[k] = k;
+  b.acc_old[k] = 0.0;
+  lend[k] = 0;
+}
+
 // vel[sinks] += F * dt * fac (fac: 1 for kicks, 2/3 for gradient kicks, applied
 // as ((F dt) 2) / 3 like integrator.h:350-353); pot[sinks] = p when set_pot.
 __global__ void k_apply_kick(const int *__restrict__ is, int ns, const double *__restrict__ F,
@@ -230,6 +260,18 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
     TC_CHECK(hipMemsetAsync(uacc_.p, 0, 24 * (size_t)sinks.n, s));
     return 0;
   }
+  const int mode = exact ? (mode_ & ~TC_MASTER_FMM) : mode_;
+  if (!steps && !(mode & TC_MASTER_FMM) && !(jerk_fmm_ && !exact)) {
+    // the default build's all-pairs force: gathered straight into the body
+    // arrays and queued with no wait (the loop reads nothing back from it)
+    if (int e = master_direct_prepare(ctx, nu)) return e;
+    hipLaunchKernelGGL(k_gather_union_bodies, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
+                       sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass,
+                       p_.acc, shift_fac, bodies_of(ctx), ctx->mlend.as<unsigned char>());
+    master_direct_queue(ctx, nu, (mode & TC_MASTER_PAIR_MASS) != 0, dtp_, upot_.as<double>(),
+                        uacc_.as<double>());
+    return 0;
+  }
   hipLaunchKernelGGL(k_gather_union, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
                      sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass, p_.acc,
                      shift_fac, ux_.as<double>(), uv_.as<double>(), um_.as<double>(),
@@ -244,7 +286,6 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
                            ufl_.as<unsigned char>(), upot_.as<double>(), uacc_.as<double>(),
                            nullptr);
   }
-  const int mode = exact ? (mode_ & ~TC_MASTER_FMM) : mode_;
   double *ao = nullptr;
   if (with_acc_old && (mode & TC_MASTER_FMM)) {
     // kick_naive builds fresh bodies: the pre-pass starts from 0
