@@ -418,6 +418,10 @@ def main():
                          "of known size for scaling HBM byte counters)")
     ap.add_argument("--only-master-evolve", action="store_true",
                     help="time only the master step loop row (for profiling it) and print it")
+    ap.add_argument("--only", choices=["clusters_c4", "cusp_steps_c5", "kroupa_binaries_c3",
+                                       "learned_c2"],
+                    help="time only this row of the other workloads and print it (for "
+                         "profiling it)")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the rows of the other workloads (many clusters, the "
                          "fourth-order steps on the cusp, the learned choice)")
@@ -444,6 +448,32 @@ def main():
 
     if args.pareto:
         pareto(args)
+        return
+    if args.only:
+        import torch
+        import taichi_collisional_amd as tca
+        from taichi_collisional_amd import ics
+        dev = torch.device("cuda:0")
+        if args.only == "clusters_c4":
+            row = clusters_row(tca, ics, torch, dev)
+        elif args.only == "kroupa_binaries_c3":
+            row = kroupa_mixed_row(tca, ics, torch, dev)
+        elif args.only == "learned_c2":
+            from taichi_collisional_amd import learn
+            x, v, m = ics.plummer(args.n, seed=args.seed)
+            dx, dv, dm = (torch.from_numpy(a).to(dev) for a in (x, v, m))
+            f = tca.Fmm(p=args.p, ncrit=args.ncrit, direct_below=0)
+            f.set_seed(args.seed)
+            learn.apply_model(f, learn.load_model(), target_rms=1e-7)
+            for _ in range(4):
+                f.eval(dx, dv, dm)
+            torch.cuda.synchronize()
+            st = f.stats()
+            row = {k: v for k, v in st.items() if k.startswith("ms_") or k.startswith("n_")}
+            f.close()
+        else:
+            row = cusp_steps_row(tca, ics, torch, dev, args.c5_n, steps=2)
+        print(json.dumps({args.only: row}), flush=True)
         return
 
     cpu = None
