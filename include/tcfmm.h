@@ -104,6 +104,33 @@ int tc_last_shift(const tc_context *ctx, double frac[3]);
 enum { TC_P2P_FP64 = 0, TC_P2P_MIXED = 1, TC_P2P_MIXED_HILO = 2 };
 int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius);
 
+/* ---- Launch shape of the near field (no reference counterpart: the
+ * reference's P2P, kernel_fast_lean.h:499-699, is a CPU loop over leaf
+ * pairs) ----------------------------------------------------------------
+ * tile: source bodies per LDS tile (64, 128 or 192); list_slice: P2P-list
+ * entries per work item (a leaf's list is split into slices that run as
+ * separate waves; 1..65535); stage_rounds: the cost of staging one tile, in
+ * rounds of pair evaluations, that the cut of each leaf into target groups
+ * trades against idle lanes (0 favours groups that fill the wave, larger
+ * values fewer groups); targets_per_lane: 1 or 2 (one LDS read of a source
+ * body serves that many pairs; the FP64 form only, the mixed forms always
+ * run 1).  Defaults 128, 128, 8, 1.  Results do not depend on the shape
+ * beyond summation order (FP64; the mixed forms' FP32 tile sums also follow
+ * the tile's contents).
+ * tc_tune_p2p: after an evaluation, time that evaluation's near field under
+ * every shape of the grid {1,2} x {64,128,192} x {64,128,256} x {0,2,4,8}
+ * (targets per lane x tile x list slice x staging cost), reps launches each,
+ * and keep the fastest for the following evaluations; best_ms / default_ms
+ * (may be NULL) receive its time and that of the shape set before the call;
+ * grid_ms (NULL or 72 doubles) every shape's time in that nesting order.
+ * Returns -1 without a completed tree evaluation. */
+int tc_set_p2p_shape(tc_context *ctx, int tile, int list_slice, double stage_rounds,
+                     int targets_per_lane);
+int tc_get_p2p_shape(tc_context *ctx, int *tile, int *list_slice, double *stage_rounds,
+                     int *targets_per_lane);
+int tc_tune_p2p(tc_context *ctx, int reps, double *best_ms, double *default_ms,
+                double *grid_ms);
+
 /* ---- Per-cell expansion order and opening angle (no reference counterpart;
  * the reference's analogue is its error-controlled walk,
  * traverse_eager.h:59-122, which sets the MAC from a formula) ------------

@@ -107,6 +107,11 @@ def lib():
         L.tc_set_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_last_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_set_precision.argtypes = [vp, C.c_int, C.c_double]
+        L.tc_set_p2p_shape.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.tc_get_p2p_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.tc_tune_p2p.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
         L.tc_set_adaptive.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
         L.tc_eval_clusters.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
         L.tc_set_order_model.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp,
@@ -153,7 +158,8 @@ def lib():
 
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
-                    "tc_set_precision", "tc_eval_clusters", "tc_set_adaptive", "tc_set_order_model", "tc_leaf_count",
+                    "tc_set_precision", "tc_set_p2p_shape", "tc_get_p2p_shape", "tc_tune_p2p",
+                    "tc_eval_clusters", "tc_set_adaptive", "tc_set_order_model", "tc_leaf_count",
                     "tc_leaf_data", "tc_leaf_errors", "tc_m2l_order_counts", "tc_mlp_param_count",
                     "tc_mlp_input_count", "tc_mlp_forward", "tc_mlp_grad", "tc_mlp_adam",
                     "tc_eval_bodies", "tc_eval_soa", "tc_direct_soa", "tc_direct_soa_noalloc",
@@ -293,6 +299,40 @@ class Fmm:
         close_radius."""
         self._check(lib().tc_set_precision(self.h, int(p2p_mode), float(close_radius)),
                     "tc_set_precision")
+
+    def set_p2p_shape(self, tile=128, list_slice=128, stage_rounds=8.0, targets_per_lane=1):
+        """Launch shape of the near field (tc_set_p2p_shape): source bodies per
+        LDS tile (64, 128, 192), list entries per work item, the staging cost
+        that sets how leaves are cut into target groups, and targets per lane
+        (1 or 2)."""
+        self._check(lib().tc_set_p2p_shape(self.h, int(tile), int(list_slice),
+                                           float(stage_rounds), int(targets_per_lane)),
+                    "tc_set_p2p_shape")
+
+    def p2p_shape(self):
+        """(tile, list_slice, stage_rounds, targets_per_lane)"""
+        t, l, g, r = C.c_int(), C.c_int(), C.c_double(), C.c_int()
+        self._check(lib().tc_get_p2p_shape(self.h, C.byref(t), C.byref(l), C.byref(g),
+                                           C.byref(r)), "tc_get_p2p_shape")
+        return t.value, l.value, g.value, r.value
+
+    P2P_TUNE_GRID = [(t, l, g, r) for r in (1, 2) for t in (64, 128, 192) for l in (64, 128, 256)
+                     for g in (0.0, 2.0, 4.0, 8.0)]
+
+    def tune_p2p(self, reps=2, grid=False):
+        """After an evaluation: time its near field under a grid of launch
+        shapes (P2P_TUNE_GRID, as (tile, list_slice, stage_rounds,
+        targets_per_lane)) and keep the fastest (tc_tune_p2p).  Returns
+        (shape, best ms, ms of the shape set before), plus {shape: ms} for the
+        whole grid when grid=True."""
+        b, d = C.c_double(), C.c_double()
+        g = (C.c_double * len(self.P2P_TUNE_GRID))()
+        self._check(lib().tc_tune_p2p(self.h, int(reps), C.byref(b), C.byref(d), g),
+                    "tc_tune_p2p")
+        out = (self.p2p_shape(), b.value, d.value)
+        if grid:
+            out += ({sh: g[k] for k, sh in enumerate(self.P2P_TUNE_GRID)},)
+        return out
 
     def set_seed(self, seed):
         self._check(lib().tc_set_seed(self.h, int(seed)), "tc_set_seed")

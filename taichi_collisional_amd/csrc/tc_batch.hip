@@ -624,17 +624,23 @@ int tc_P2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncell
   group_pairs(ncells, targets, sources, npairs, off, cnt, src);
   std::vector<int2> ranges(src.size());
   for (size_t k = 0; k < src.size(); k++) ranges[k] = make_int2(B.body[src[k]], B.nbody[src[k]]);
-  // work items: target groups of <= 32 bodies x list slices of <= 128 entries
+  // work items: target groups (the context's split of each cell's bodies) x
+  // list slices of <= p2p_list entries
   std::vector<int4> items;
+  P2PSplit &sp = ctx->p2p_split;
+  const int per_lane = p2p_targets_per_lane(TC_P2P_FP64, ctx->p2p_per_lane);
+  sp.configure(ctx->p2p_stage, (ctx->p2p_tile * 7) / 8, per_lane);
+  const int kList = ctx->p2p_list;
   for (int i = 0; i < ncells; i++) {
     if (!cnt[i] || B.nbody[i] == 0) continue;
-    const int nb = B.nbody[i], ng = (nb + 31) / 32;
+    const int nb = B.nbody[i];
+    sp.grow(nb);
+    const int *gs = sp.sizes(nb);
     int o = 0;
-    for (int g = 0; g < ng; g++) {
-      const int k = (nb - o) / (ng - g);
-      for (int e0 = 0; e0 < cnt[i]; e0 += 128)
-        items.push_back(make_int4(i, o, e0, std::min(128, cnt[i] - e0) | (k << 16)));
-      o += k;
+    for (int g = 0; g < sp.count(nb); g++) {
+      for (int e0 = 0; e0 < cnt[i]; e0 += kList)
+        items.push_back(make_int4(i, o, e0, std::min(kList, cnt[i] - e0) | (gs[g] << 16)));
+      o += gs[g];
     }
   }
   if (!items.empty()) {
@@ -653,7 +659,8 @@ int tc_P2P(tc_context *ctx, tc_body *bodies, int nbodies, void *cells, int ncell
     TC_CHECK(hipMemsetAsync(B.npairs.p, 0, 8, s));
     launch_p2p(B.cells(), B.bodies(), B.results(), B.aux1.as<int4>(), (int)items.size(),
                B.aux2.as<int>(), B.aux2.as<int>() + ncells, B.aux3.as<int2>(),
-               B.npairs.as<unsigned long long>(), s);
+               B.npairs.as<unsigned long long>(), s, TC_P2P_FP64, 0.0, ctx->p2p_tile,
+               per_lane);
     TC_CHECK(hipStreamSynchronize(s));
     TC_CHECK(hipGetLastError());
   }

@@ -96,6 +96,15 @@ struct tc_context {
   bool fixed_shift = false;
   int p2p_mode = 0;           // TC_P2P_* (tc_set_precision)
   double close_radius = 0.0;  // mixed P2P: pairs closer than this in FP64
+  // P2P launch shape (tc_set_p2p_shape / tc_tune_p2p): source bodies per LDS
+  // tile, list entries per work item, the staging cost (in pair rounds) of
+  // the target-group split, and targets per lane (FP64 form)
+  int p2p_tile = 128;
+  int p2p_list = 128;
+  double p2p_stage = 8.0;
+  int p2p_per_lane = 1;
+  P2PSplit p2p_split;
+  bool p2p_ready = false;  // the last evaluation's near-field inputs are in place
   double shift[3] = {0, 0, 0};
   // bodies
   int cap_n = 0;

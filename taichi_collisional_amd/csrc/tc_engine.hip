@@ -369,32 +369,40 @@ static int build_lists(tc_context *ctx, int &nchunks) {
   return 0;
 }
 
-// P2P work items: every target group x slices of <= kItemList list entries.
-// The per-leaf list lengths were copied to pinned memory before the M2L
-// launch (and the stream synchronized after that copy), so this host loop runs
-// while the GPU is busy with M2L; the upload is queued behind it.
+// P2P work items: every target group x slices of <= p2p_list list entries.
+// Each leaf is cut into target groups of <= 64 bodies by the lane model of
+// the kernel (P2PSplit).  The per-leaf list lengths were copied to pinned
+// memory before the M2L launch (and the stream synchronized after that
+// copy), so this host loop runs while the GPU is busy with M2L; the upload is
+// queued behind it.
 static int build_p2p_items(tc_context *ctx) {
-  constexpr int kItemList = 128;
+  const int kList = ctx->p2p_list;
   TC_CHECK(spin_event(ctx->ev[11]));  // leaf ids and body counts (build_tree)
   TC_CHECK(spin_event(ctx->ev[15]));  // list lengths (build_lists)
   const int *pcnt = ctx->h_pcnt.p, *leaves = ctx->h_leaves.p, *nbd = ctx->h_nbd.p;
+  P2PSplit &sp = ctx->p2p_split;
+  sp.configure(ctx->p2p_stage, (ctx->p2p_tile * 7) / 8,
+               p2p_targets_per_lane(ctx->p2p_mode, ctx->p2p_per_lane));
+  int nbmax = 1;
+  for (int l = 0; l < ctx->nleaves; l++) nbmax = std::max(nbmax, nbd[leaves[l]]);
+  sp.grow(nbmax);
   size_t nitem = 0;
   for (int l = 0; l < ctx->nleaves; l++) {
     const int i = leaves[l];
-    nitem += (size_t)((nbd[i] + 31) / 32) * ((pcnt[i] + kItemList - 1) / kItemList);
+    nitem += (size_t)sp.count(nbd[i]) * ((pcnt[i] + kList - 1) / kList);
   }
   TC_CHECK(ctx->h_items.ensure(std::max<size_t>(nitem, 1)));
   int4 *items = ctx->h_items.p;
   size_t k = 0;
   for (int l = 0; l < ctx->nleaves; l++) {
-    // near-field target groups of <= 32 bodies, balanced within the leaf
-    const int i = leaves[l], nb = nbd[i], ng = (nb + 31) / 32, cnt = pcnt[i];
+    const int i = leaves[l], nb = nbd[i], cnt = pcnt[i];
+    const int ng = sp.count(nb);
+    const int *gs = sp.sizes(nb);
     int o = 0;
     for (int g = 0; g < ng; g++) {
-      const int gs = (nb - o) / (ng - g);
-      for (int e0 = 0; e0 < cnt; e0 += kItemList)
-        items[k++] = make_int4(i, o, e0, std::min(kItemList, cnt - e0) | (gs << 16));
-      o += gs;
+      for (int e0 = 0; e0 < cnt; e0 += kList)
+        items[k++] = make_int4(i, o, e0, std::min(kList, cnt - e0) | (gs[g] << 16));
+      o += gs[g];
     }
   }
   ctx->n_items = nitem;
@@ -681,6 +689,7 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   std::memset(ctx->m2l_by_order, 0, sizeof(ctx->m2l_by_order));
   ctx->has_acc_old = false;
+  ctx->p2p_ready = false;
   ctx->last_n = n;
   if (n <= 0) {
     ctx->err = "n must be positive";
@@ -810,8 +819,10 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
              (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
              ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2,
-             ctx->p2p_mode, ctx->close_radius);
+             ctx->p2p_mode, ctx->close_radius, ctx->p2p_tile,
+             p2p_targets_per_lane(ctx->p2p_mode, ctx->p2p_per_lane));
   TC_CHECK(hipEventRecord(ctx->ev[10], s2));
+  ctx->p2p_ready = true;
   // ---- downward into the far-field result set
   ResultsDev rf = {ctx->fpot.as<double>(), ctx->fax.as<double>(), ctx->fay.as<double>(),
                    ctx->faz.as<double>(), ctx->fjx.as<double>(), ctx->fjy.as<double>(),
@@ -1227,6 +1238,125 @@ int tc_set_precision(tc_context *ctx, int p2p_mode, double close_radius) {
   }
   ctx->p2p_mode = p2p_mode;
   ctx->close_radius = p2p_mode == TC_P2P_FP64 ? 0.0 : close_radius;
+  return 0;
+}
+
+int tc_set_p2p_shape(tc_context *ctx, int tile, int list_slice, double stage_rounds,
+                     int targets_per_lane) {
+  if (!ctx) return -1;
+  if (!p2p_tile_supported(tile)) {
+    ctx->err = "P2P tile must be 64, 128 or 192 bodies";
+    return -1;
+  }
+  if (list_slice < 1 || list_slice > 0xffff) {
+    ctx->err = "P2P list slice must be in 1..65535";
+    return -1;
+  }
+  if (!(stage_rounds >= 0.0 && stage_rounds <= 1e6)) {
+    ctx->err = "P2P staging cost must be in [0, 1e6]";
+    return -1;
+  }
+  if (targets_per_lane != 1 && targets_per_lane != 2) {
+    ctx->err = "P2P targets per lane must be 1 or 2";
+    return -1;
+  }
+  ctx->p2p_tile = tile;
+  ctx->p2p_list = list_slice;
+  ctx->p2p_stage = stage_rounds;
+  ctx->p2p_per_lane = targets_per_lane;
+  return 0;
+}
+
+int tc_get_p2p_shape(tc_context *ctx, int *tile, int *list_slice, double *stage_rounds,
+                     int *targets_per_lane) {
+  if (!ctx) return -1;
+  if (tile) *tile = ctx->p2p_tile;
+  if (list_slice) *list_slice = ctx->p2p_list;
+  if (stage_rounds) *stage_rounds = ctx->p2p_stage;
+  if (targets_per_lane) *targets_per_lane = ctx->p2p_per_lane;
+  return 0;
+}
+
+// Re-run the last evaluation's near field under every launch shape of a small
+// grid (targets per lane x tile x list slice x staging cost, in that nesting
+// order), each `reps` times, timed with events on the near-field stream; keep
+// the fastest.  The lists, ranges and host copies of the list lengths are
+// those of the last evaluation; the launches add into the context's own
+// near-field result set (zeroed by the next evaluation), never into the
+// caller's arrays.  The mixed-precision forms run one target per lane, so
+// their grid's two halves time the same kernels.
+int tc_tune_p2p(tc_context *ctx, int reps, double *best_ms, double *default_ms,
+                double *grid_ms) {
+  if (!ctx) return -1;
+  if (!ctx->p2p_ready) {
+    ctx->err = "tc_tune_p2p needs a completed tree evaluation on this context";
+    return -1;
+  }
+  if (reps < 1 || reps > 100) {
+    ctx->err = "reps must be in 1..100";
+    return -1;
+  }
+  TC_CHECK(hipStreamSynchronize(ctx->stream));
+  TC_CHECK(hipStreamSynchronize(ctx->near_stream));
+  const int per_lanes[] = {1, 2};
+  const int tiles[] = {64, 128, 192};
+  const int lists[] = {64, 128, 256};
+  const double stages[] = {0.0, 2.0, 4.0, 8.0};
+  const int t0 = ctx->p2p_tile, l0 = ctx->p2p_list, r0 = ctx->p2p_per_lane;
+  const double g0 = ctx->p2p_stage;
+  hipStream_t s2 = ctx->near_stream;
+  auto run = [&](int per_lane, int tile, int list, double stage, float *ms) -> int {
+    ctx->p2p_per_lane = per_lane;
+    ctx->p2p_tile = tile;
+    ctx->p2p_list = list;
+    ctx->p2p_stage = stage;
+    if (int e = build_p2p_items(ctx)) return e;
+    TC_CHECK(hipStreamSynchronize(ctx->stream));  // the item upload
+    float best = 1e30f;
+    for (int r = 0; r < reps; r++) {
+      TC_CHECK(hipEventRecord(ctx->ev[9], s2));
+      launch_p2p(cells_of(ctx), bodies_of(ctx), results_of(ctx), ctx->p2p_items.as<int4>(),
+                 (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
+                 ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2,
+                 ctx->p2p_mode, ctx->close_radius, tile,
+                 p2p_targets_per_lane(ctx->p2p_mode, per_lane));
+      TC_CHECK(hipEventRecord(ctx->ev[10], s2));
+      TC_CHECK(hipEventSynchronize(ctx->ev[10]));
+      float t = 0;
+      TC_CHECK(hipEventElapsedTime(&t, ctx->ev[9], ctx->ev[10]));
+      best = std::min(best, t);
+    }
+    *ms = best;
+    return 0;
+  };
+  float tdef = 0;
+  if (int e = run(r0, t0, l0, g0, &tdef)) return e;
+  float tbest = tdef;
+  int br = r0, bt = t0, bl = l0;
+  double bg = g0;
+  int k = 0;
+  for (int per_lane : per_lanes)
+    for (int tile : tiles)
+      for (int list : lists)
+        for (double stage : stages) {
+          float t = 0;
+          if (int e = run(per_lane, tile, list, stage, &t)) return e;
+          if (grid_ms) grid_ms[k] = t;
+          k++;
+          if (t < tbest) {
+            tbest = t;
+            br = per_lane;
+            bt = tile;
+            bl = list;
+            bg = stage;
+          }
+        }
+  ctx->p2p_per_lane = br;
+  ctx->p2p_tile = bt;
+  ctx->p2p_list = bl;
+  ctx->p2p_stage = bg;
+  if (best_ms) *best_ms = tbest;
+  if (default_ms) *default_ms = tdef;
   return 0;
 }
 

@@ -188,10 +188,33 @@ int launch_m2l(int p, const M2LArgs &a, hipStream_t s);
 // mode: 0 FP64 (the reference's DOUBLE_P2P form); 1 mixed: FP32 pairs on
 // offsets from the target group's first body, FP64 for pairs closer than
 // close_radius; 2 mixed with the offsets held as float hi + lo pairs
+// tile: source bodies per LDS tile, one of 64, 128, 192 (p2p_tile_supported);
+// per_lane: targets per lane, 1 or 2 (2 for the FP64 form only, see
+// p2p_targets_per_lane); the groups' sizes must follow the same value
 int launch_p2p(CellsDev c, BodiesDev b, ResultsDev r, const int4 *groups, int ngroups,
                const int *offs, const int *count, const int2 *ranges,
                unsigned long long *npairs, hipStream_t s, int mode = 0,
-               double close_radius = 0.0);
+               double close_radius = 0.0, int tile = 128, int per_lane = 1);
+bool p2p_tile_supported(int tile);
+// targets per lane the P2P kernel of this precision mode runs for a request
+int p2p_targets_per_lane(int mode, int requested);
+
+// How the P2P kernel's target groups cut a leaf of nb bodies (host side):
+// sizes(nb) lists the group sizes (<= 64 per_lane each, summing to nb) of
+// least cost under the kernel's lane model; the table grows on demand.
+class P2PSplit {
+ public:
+  void configure(double stage_rounds, int fill, int per_lane);
+  void grow(int nmax);
+  int count(int nb) const { return first_[nb + 1] - first_[nb]; }
+  const int *sizes(int nb) const { return sizes_.data() + first_[nb]; }
+
+ private:
+  double stage_ = -1.0;
+  int fill_ = -1, per_lane_ = 1;
+  std::vector<int> first_{0}, sizes_, choice_{0};
+  std::vector<double> cost_{0.0};
+};
 int launch_p2p_ranges(CellsDev c, const int *src, int n, int2 *ranges, hipStream_t s);
 int launch_direct(BodiesDev b, int n, const int *targets, int nt, double *pot, double *acc,
                   double *jerk, hipStream_t s);
