@@ -726,30 +726,32 @@ __global__ __launch_bounds__(256) void k_trav_count(CellsDev c, const int2 *__re
 
 // the block offsets (exclusive scans of pass 1's totals, to blk[3 G ..]) and
 // the state for pass 2 and the next step; a step whose output would not fit
-// sets the overflow flag, records what it needed and ends the walk
-__global__ __launch_bounds__(1024) void k_trav_scan(int *__restrict__ blk, TravState *st, int par,
-                                                   long long cap_m2l, long long cap_p2p,
-                                                   long long cap_front) {
-  constexpr int G = kTravBlocks, PER = G / 1024;
-  static_assert(G % 1024 == 0, "whole entries per thread");
+// sets the overflow flag, records what it needed and ends the walk.  One
+// wave: it must find room on a CU while the upward moments (second stream)
+// fill the machine with small blocks, which a 1024-thread block waits out.
+__global__ __launch_bounds__(64) void k_trav_scan(int *__restrict__ blk, TravState *st, int par,
+                                                  long long cap_m2l, long long cap_p2p,
+                                                  long long cap_front) {
+  constexpr int G = kTravBlocks, PER = G / kWave;
+  static_assert(G % kWave == 0, "whole entries per lane");
   const int t = threadIdx.x;
-  int3 a[PER];
   int3 sum = make_int3(0, 0, 0);
-#pragma unroll
   for (int k = 0; k < PER; k++) {
     const int i = t * PER + k;
-    a[k] = make_int3(blk[i], blk[G + i], blk[2 * G + i]);
-    sum = make_int3(sum.x + a[k].x, sum.y + a[k].y, sum.z + a[k].z);
+    sum = make_int3(sum.x + blk[i], sum.y + blk[G + i], sum.z + blk[2 * G + i]);
   }
   int3 tot;
-  int3 run = block_excl_scan3_n<16>(sum, &tot);
-#pragma unroll
+  int3 run;
+  run.x = wave_excl_scan(sum.x, &tot.x);
+  run.y = wave_excl_scan(sum.y, &tot.y);
+  run.z = wave_excl_scan(sum.z, &tot.z);
   for (int k = 0; k < PER; k++) {
     const int i = t * PER + k;
+    const int3 a = make_int3(blk[i], blk[G + i], blk[2 * G + i]);
     blk[3 * G + i] = run.x;
     blk[4 * G + i] = run.y;
     blk[5 * G + i] = run.z;
-    run = make_int3(run.x + a[k].x, run.y + a[k].y, run.z + a[k].z);
+    run = make_int3(run.x + a.x, run.y + a.y, run.z + a.z);
   }
   if (t == 0) {
     const int nin = st->nin[par];
@@ -818,7 +820,7 @@ int launch_trav_step(CellsDev c, const TravArgs &a, TravState *st, int par, int 
                      long long cap_m2l, long long cap_p2p, long long cap_front, hipStream_t s) {
   hipLaunchKernelGGL(k_trav_count, dim3(kTravBlocks), dim3(256), 0, s, c, a.fin,
                      (const TravState *)st, par, a.mac, blk, a.cls);
-  hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(1024), 0, s, blk, st, par, cap_m2l, cap_p2p,
+  hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(kWave), 0, s, blk, st, par, cap_m2l, cap_p2p,
                      cap_front);
   hipLaunchKernelGGL(k_trav_write, dim3(kTravBlocks), dim3(256), 0, s, c, a,
                      (const TravState *)st, par, (const int *)blk);
