@@ -513,10 +513,12 @@ def main():
         fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
     torch.cuda.synchronize()
     stats = []
+    shifts = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
         stats.append(fmm.stats())
+        shifts.append(fmm.last_shift())  # host copy of the three values drawn (no GPU work)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
@@ -531,16 +533,33 @@ def main():
               for k in ("ms_tree", "ms_upward", "ms_traverse", "ms_lists", "ms_m2l", "ms_p2p",
                         "ms_downward", "ms_total")}
 
-    # accuracy vs the FP64 direct sum on a seeded 4096-body subset (not timed)
+    # accuracy vs the FP64 direct sum on a seeded 4096-body subset (not timed),
+    # for every timed evaluation: each drew its own root shift (the reference's
+    # three drand48() values), which moves the cell boundaries and so the
+    # error; the evaluations are repeated with those shifts
     rng = np.random.default_rng(12345)
     tg = torch.from_numpy(rng.choice(args.n, min(4096, args.n), replace=False).astype(np.int32))
     dp, da, dj = fmm.direct(dx, dv, dm, tg.to(dev))
     idx = tg.long().to(dev)
-    ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
-    ej = torch.linalg.norm(jerk[idx] - dj, dim=1) / torch.linalg.norm(dj, dim=1)
-    err = {"force_rms": float(torch.sqrt(torch.mean(ea ** 2))), "force_max": float(ea.max()),
-           "jerk_rms": float(torch.sqrt(torch.mean(ej ** 2))), "jerk_max": float(ej.max()),
-           "subset": int(tg.numel()), "gate_force_rms_1e-7": bool(torch.sqrt(torch.mean(ea ** 2)) <= 1e-7)}
+    per_eval = []
+    for sh in shifts:
+        fmm.set_shift(sh)
+        fmm.eval(dx, dv, dm, pot=pot, acc=acc, jerk=jerk)
+        ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
+        ej = torch.linalg.norm(jerk[idx] - dj, dim=1) / torch.linalg.norm(dj, dim=1)
+        per_eval.append((float(torch.sqrt(torch.mean(ea ** 2))), float(ea.max()),
+                         float(torch.sqrt(torch.mean(ej ** 2))), float(ej.max())))
+    pe = np.array(per_eval)
+    err = {"force_rms": float(np.median(pe[:, 0])), "force_rms_max": float(pe[:, 0].max()),
+           "force_max": float(pe[:, 1].max()), "jerk_rms": float(np.median(pe[:, 2])),
+           "jerk_max": float(pe[:, 3].max()), "subset": int(tg.numel()),
+           "force_rms_per_eval": [float(q) for q in pe[:, 0]],
+           "root_shifts": [[float(c) for c in sh] for sh in shifts],
+           "gate_force_rms_1e-7": "%d of %d timed evaluations" % (int(np.sum(pe[:, 0] <= 1e-7)),
+                                                                   len(pe)),
+           "note": "force_rms and jerk_rms are medians over the timed evaluations, the maxima "
+                   "over them; the error moves with the root shift (at this p and theta about "
+                   "one shift in twelve gives rms above 1e-7)"}
 
     extra = {}
     if not args.no_extra:
