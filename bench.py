@@ -211,6 +211,52 @@ def timed_row(fmm, torch, dx, dv, dm, out, idx, da, dj, reps=5, warm=2):
             "m2l_pairs": sts[-1]["n_m2l"], "p2p_body_pairs": sts[-1]["n_p2p_pairs"]}
 
 
+def gate_row(tca, torch, dx, dv, dm, idx, da, p=12, theta=0.4, ncrit=64, nshift=8, seed=2024):
+    """The error gate across root shifts: each method evaluated at the same
+    nshift root shifts (fixed, seeded), with its time per evaluation (the
+    library's own event timing of the whole evaluation, median) and the force
+    rms against the FP64 direct sum of the bodies idx at every shift; how many
+    shifts meet rms <= 1e-7.  Methods: the fixed p / theta of the benchmark, a
+    smaller fixed angle (0.35, also with the mixed-precision near field: hi +
+    lo offsets, FP64 pairs inside r = 1e-3), the order p + 2 at the same angle,
+    and the learned per-leaf angle (target 1e-7), in FP64 and mixed."""
+    import numpy as np
+    from taichi_collisional_amd import learn
+    rng = np.random.default_rng(seed)
+    shifts = rng.uniform(-0.5, 0.5, size=(nshift, 3))
+    model = learn.load_model() if os.path.exists(learn.MODEL_PATH) else None
+    methods = [("fixed_p%d_theta%.2f" % (p, theta), p, theta, False, False),
+               ("fixed_p%d_theta0.35" % p, p, 0.35, False, False),
+               ("fixed_p%d_theta0.35_mixed" % p, p, 0.35, False, True),
+               ("fixed_p%d_theta%.2f" % (p + 2, theta), p + 2, theta, False, False)]
+    if model is not None:
+        methods += [("learned_target1e-7", p, theta, True, False),
+                    ("learned_target1e-7_mixed", p, theta, True, True)]
+    out = {"what": "force rms vs the FP64 direct sum at %d fixed root shifts per method; "
+                   "ms_per_eval is the median over them" % nshift}
+    for name, pm, th, learned, mixed in methods:
+        f = tca.Fmm(p=pm, ncrit=ncrit, theta=th, direct_below=0)
+        if learned:
+            learn.apply_model(f, model, target_rms=1e-7)
+        if mixed:
+            f.set_precision(tca.TC_P2P_MIXED_HILO, close_radius=1e-3)
+        f.set_shift(shifts[0])
+        f.eval(dx, dv, dm)  # warm-up
+        ms, rms = [], []
+        for sh in shifts:
+            f.set_shift(sh)
+            _, acc, _ = f.eval(dx, dv, dm)
+            torch.cuda.synchronize()
+            ms.append(f.stats()["ms_total"])
+            ea = torch.linalg.norm(acc[idx] - da, dim=1) / torch.linalg.norm(da, dim=1)
+            rms.append(float(torch.sqrt(torch.mean(ea ** 2))))
+        out[name] = {"ms_per_eval": float(np.median(ms)), "force_rms_median": float(np.median(rms)),
+                     "force_rms_max": float(max(rms)), "force_rms": rms,
+                     "shifts_meeting_1e-7": "%d of %d" % (sum(r <= 1e-7 for r in rms), nshift)}
+        f.close()
+    return out
+
+
 def clusters_row(tca, ics, torch, dev, seed=7, k=256, reps=3):
     """Many independent Plummer clusters (N log-uniform in 2^10..2^16, equal
     masses within a cluster, p = 8, theta = 0.5): one forest evaluation of all
@@ -422,7 +468,7 @@ def main():
     ap.add_argument("--only-master-evolve", action="store_true",
                     help="time only the master step loop row (for profiling it) and print it")
     ap.add_argument("--only", choices=["clusters_c4", "cusp_steps_c5", "kroupa_binaries_c3",
-                                       "learned_c2"],
+                                       "learned_c2", "gate_c2"],
                     help="time only this row of the other workloads and print it (for "
                          "profiling it)")
     ap.add_argument("--no-workloads", action="store_true",
@@ -453,6 +499,7 @@ def main():
         pareto(args)
         return
     if args.only:
+        import numpy as np
         import torch
         import taichi_collisional_amd as tca
         from taichi_collisional_amd import ics
@@ -461,6 +508,16 @@ def main():
             row = clusters_row(tca, ics, torch, dev)
         elif args.only == "kroupa_binaries_c3":
             row = kroupa_mixed_row(tca, ics, torch, dev)
+        elif args.only == "gate_c2":
+            x, v, m = ics.plummer(args.n, seed=args.seed)
+            dx, dv, dm = (torch.from_numpy(a).to(dev) for a in (x, v, m))
+            rng = np.random.default_rng(12345)
+            tg = torch.from_numpy(rng.choice(args.n, min(4096, args.n),
+                                             replace=False).astype(np.int32)).to(dev)
+            f0 = tca.Fmm(p=4, direct_below=0)
+            _, da, _ = f0.direct(dx, dv, dm, tg)
+            f0.close()
+            row = gate_row(tca, torch, dx, dv, dm, tg.long(), da, args.p, args.theta, args.ncrit)
         elif args.only == "learned_c2":
             from taichi_collisional_amd import learn
             x, v, m = ics.plummer(args.n, seed=args.seed)
@@ -672,6 +729,8 @@ def main():
                          "1e-7 (FP64)" % args.p)
             extra["learned_c2"] = r
             flr.close()
+        extra["gate_c2"] = gate_row(tca, torch, dx, dv, dm, idx, da, args.p, args.theta,
+                                    args.ncrit)
 
     if args.calibrate_copy_mib > 0:
         src = torch.ones(args.calibrate_copy_mib << 17, dtype=torch.float64, device=dev)
