@@ -337,6 +337,13 @@ int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v,
                        const unsigned char *flags, int mode, double dt_param, double *pot,
                        double *acc, double *timestep, double *acc_old);
 
+/* With TC_MASTER_STEPS and without TC_MASTER_FMM, systems of at most
+ * max_bodies bodies (capped at 4096; the default) build their tree in a single
+ * launch instead of the level-by-level build: the same tree, the same body
+ * order and leaves, so the same results; 0 selects the level-by-level build
+ * at every size.  No reference counterpart (a launch-count choice). */
+int tc_set_small_tree(tc_context *ctx, int max_bodies);
+
 /* Per-kernel entry points over the master Cell (exafmm.h:50-71: M[NTERM],
  * L[NTERM], Pn[P+1] first, then NCHILD ... has_sink; no Mdot / Ldot) and
  * tc_master_body records, with the list-order semantics of the tc_P2M ...

@@ -108,6 +108,7 @@ def lib():
         L.tc_last_shift.argtypes = [vp, C.POINTER(C.c_double)]
         L.tc_set_precision.argtypes = [vp, C.c_int, C.c_double]
         L.tc_set_p2p_shape.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.tc_set_small_tree.argtypes = [vp, C.c_int]
         L.tc_get_p2p_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.tc_tune_p2p.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -159,6 +160,7 @@ def lib():
 
 EXPORTED_SYMBOLS = ["tc_create", "tc_destroy", "tc_set_seed", "tc_set_shift", "tc_last_shift",
                     "tc_set_precision", "tc_set_p2p_shape", "tc_get_p2p_shape", "tc_tune_p2p",
+                    "tc_set_small_tree",
                     "tc_eval_clusters", "tc_set_adaptive", "tc_set_order_model", "tc_leaf_count",
                     "tc_leaf_data", "tc_leaf_errors", "tc_m2l_order_counts", "tc_mlp_param_count",
                     "tc_mlp_input_count", "tc_mlp_forward", "tc_mlp_grad", "tc_mlp_adam",
@@ -308,6 +310,11 @@ class Fmm:
         self._check(lib().tc_set_p2p_shape(self.h, int(tile), int(list_slice),
                                            float(stage_rounds), int(targets_per_lane)),
                     "tc_set_p2p_shape")
+
+    def set_small_tree(self, max_bodies=4096):
+        """Master time-step evaluations of at most max_bodies bodies build their
+        tree in one launch (tc_set_small_tree); 0: always level by level."""
+        self._check(lib().tc_set_small_tree(self.h, int(max_bodies)), "tc_set_small_tree")
 
     def p2p_shape(self):
         """(tile, list_slice, stage_rounds, targets_per_lane)"""

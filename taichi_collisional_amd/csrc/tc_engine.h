@@ -144,6 +144,10 @@ struct tc_context {
   // master variant: last-body-of-leaf flags, time-step sums (caller order,
   // tree order), symmetric M2L list and its scan
   DBuf mlend, mts, mts_sorted, msym_cnt, msym_off, msym_tmp, m2l_sym;
+  // master time-step sums up to small_tree_max bodies build their tree in one
+  // launch (launch_small_tree; 0: always the multi-kernel build)
+  int small_tree_max = kSmallTree;
+  DBuf small_ci, small_cg, small_out;
   DBuf mdpart;  // partial sums of the source slabs of the master direct sum
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
   // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],

@@ -1046,6 +1046,7 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                             s));
   }
   TC_CHECK(ctx->mlend.ensure((size_t)n));
+  bool small = false;
   if (!steps && !fmm) {
     // direct() without time steps: the sums over leaf pairs are the all-pairs
     // sums, and the leaves do not enter them, so no tree is built
@@ -1053,6 +1054,15 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     launch_iota(ctx->dtg.as<int>(), n, s);
     launch_gather(x, v, m, flags, ctx->dtg.as<int>(), n, bodies_of(ctx), s);
     launch_zero_u8(ctx->mlend.as<unsigned char>(), n, s);
+  } else if (!fmm && n <= std::min(ctx->small_tree_max, kSmallTree)) {
+    // the time-step sums of a small system: its whole tree in one launch
+    TC_CHECK(ctx->small_ci.ensure(sizeof(int) * 4 * (size_t)n));
+    TC_CHECK(ctx->small_cg.ensure(sizeof(double) * 8 * (size_t)n));
+    TC_CHECK(ctx->small_out.ensure(sizeof(int) * 2));
+    launch_small_tree(x, v, m, flags, n, ctx->cfg.ncrit, bodies_of(ctx),
+                      ctx->mlend.as<unsigned char>(), ctx->small_ci.as<int>(),
+                      ctx->small_cg.as<double>(), ctx->small_out.as<int>(), s);
+    small = true;
   } else {
     if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
     if (fmm) return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
@@ -1065,7 +1075,17 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
   launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, steps, pair_mass, dt,
                        pot, acc, tstep, nslab, ctx->mdpart.as<double>(), s);
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
+  if (small)
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 136, ctx->small_out.p, sizeof(int) * 2,
+                            hipMemcpyDeviceToHost, s));
   TC_CHECK(spin_wait(ctx, s));
+  if (small) {
+    if (ctx->h_pin[137] == 2) {
+      ctx->err = "tree deeper than 64 levels (more than ncrit coincident bodies?)";
+      return -1;
+    }
+    ctx->nleaves = ctx->h_pin[136];
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
@@ -1264,6 +1284,16 @@ int tc_set_p2p_shape(tc_context *ctx, int tile, int list_slice, double stage_rou
   ctx->p2p_list = list_slice;
   ctx->p2p_stage = stage_rounds;
   ctx->p2p_per_lane = targets_per_lane;
+  return 0;
+}
+
+int tc_set_small_tree(tc_context *ctx, int max_bodies) {
+  if (!ctx) return -1;
+  if (max_bodies < 0) {
+    ctx->err = "max_bodies must be >= 0";
+    return -1;
+  }
+  ctx->small_tree_max = std::min(max_bodies, kSmallTree);
   return 0;
 }
 

@@ -103,6 +103,14 @@ struct TreeState {
 // per-level cells hold cap entries (nch and scan: 2 x cap).
 int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap, int *offs,
                       int *nch, int *scan, const TreeParams *d_tp, int *leaf_ids, hipStream_t s);
+// The whole tree of at most kSmallTree bodies in one launch (one workgroup),
+// for the master time-step sums: bodies gathered into tree order, lend set on
+// the last body of every leaf; out[0] = leaves, out[1] = 0 or 2 (too deep).
+// ci: 4 n ints, cg: 8 n doubles of scratch.
+constexpr int kSmallTree = 4096;
+int launch_small_tree(const double *x3, const double *v3, const double *q,
+                      const unsigned char *flags, int n, int ncrit, BodiesDev b,
+                      unsigned char *lend, int *ci, double *cg, int *out, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 

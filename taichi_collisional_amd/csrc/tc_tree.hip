@@ -548,6 +548,284 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
   return 0;
 }
 
+// ---------------------------------------------------------- small trees ----
+// The master direct sum with the time-step estimate needs the leaves of the
+// tree of its bodies (the minimum of the estimate runs per source leaf), and
+// the step loop asks for it dozens of times per step on a few thousand
+// bodies, where the multi-kernel build above is some forty launches and a
+// host wait.  For n <= kSmallTree one workgroup builds the same tree in one
+// launch, with the same arithmetic:
+//  * bounds without a root shift (k_bounds_final with zero shift values);
+//  * 28-bit-per-axis keys (k_morton); the order of the stable radix sort on
+//    (high, low) key is that of (high key, low key, input index): a bitonic
+//    sort in LDS on (high key, index), then each run of equal high keys
+//    (bodies closer than 2^-21 of the box; rare) reordered by (low key,
+//    index);
+//  * the gather (k_gather);
+//  * levels split as k_split does (eight lanes per cell: a count per octant up
+//    to 32 bodies, else the search for the start of octants >= j that never
+//    tests the cell's first body), children placed as k_children does.
+// It marks the last body of every leaf in lend (cleared first) and counts
+// the leaves: out[0] = leaves, out[1] = 0, or 2 when the tree would need more
+// than kMaxTreeLevels levels (the multi-kernel build's error).  Cells live in
+// global scratch: two level buffers of n cells (a level never holds more
+// cells than bodies), begin / count in ci, centre and radius in cg.
+__device__ __forceinline__ void small_key(const double *x3, int i, const double *X0, double R0,
+                                          uint64_t &hi, uint64_t &lo) {
+#pragma clang fp contract(off)
+  const double maxint = (double)(1u << 28);
+  unsigned int M[3];
+  for (int d = 0; d < 3; d++) {
+    const double f = (x3[3 * (size_t)i + d] - (X0[d] - R0)) / (2.0 * R0);
+    M[d] = (unsigned int)floor(f * maxint);
+  }
+  hi = (spread3(M[2] >> 7) << 2) | (spread3(M[1] >> 7) << 1) | spread3(M[0] >> 7);
+  lo = (spread3(M[2] & 0x7f) << 2) | (spread3(M[1] & 0x7f) << 1) | spread3(M[0] & 0x7f);
+}
+
+__global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ x3,
+                                                     const double *__restrict__ v3,
+                                                     const double *__restrict__ q,
+                                                     const unsigned char *__restrict__ flags_in,
+                                                     int n, int ncrit, BodiesDev b,
+                                                     unsigned char *__restrict__ lend,
+                                                     int *__restrict__ ci, double *__restrict__ cg,
+                                                     int *__restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ uint64_t skey[kSmallTree];
+  __shared__ unsigned short sidx[kSmallTree];
+  __shared__ double wred[16][6];
+  __shared__ double sX0[3], sR0;
+  __shared__ int ncur, nnext, nleaf, status;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  // ---- bounds (min / max do not depend on the order)
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[d];
+  for (int i = t; i < n; i += 1024)
+    for (int d = 0; d < 3; d++) {
+      const double v = x3[3 * (size_t)i + d];
+      mn[d] = fmin(v, mn[d]);
+      mx[d] = fmax(v, mx[d]);
+    }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = fmin(mn[d], __shfl_xor(mn[d], o, kWave));
+      mx[d] = fmax(mx[d], __shfl_xor(mx[d], o, kWave));
+    }
+  if (lane == 0)
+    for (int d = 0; d < 3; d++) {
+      wred[w][d] = mn[d];
+      wred[w][3 + d] = mx[d];
+    }
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 1; k < 16; k++)
+      for (int d = 0; d < 3; d++) {
+        mn[d] = fmin(wred[k][d], mn[d]);
+        mx[d] = fmax(wred[k][3 + d], mx[d]);
+      }
+    double X0[3], box = -1;
+    for (int d = 0; d < 3; d++) X0[d] = (mx[d] + mn[d]) / 2;
+    for (int d = 0; d < 3; d++)
+      if (mx[d] - mn[d] > box) box = mx[d] - mn[d];
+    for (int d = 0; d < 3; d++) X0[d] += 0.0 * box / 25;  // the master tree: no shift
+    double R0 = 0;
+    for (int d = 0; d < 3; d++) {
+      R0 = fmax(X0[d] - mn[d], R0);
+      R0 = fmax(mx[d] - X0[d], R0);
+    }
+    R0 *= 1.00001;
+    for (int d = 0; d < 3; d++) sX0[d] = X0[d];
+    sR0 = R0;
+    ncur = 1;
+    nnext = 0;
+    nleaf = 0;
+    status = 0;
+  }
+  __syncthreads();
+  const double X0[3] = {sX0[0], sX0[1], sX0[2]};
+  const double R0 = sR0;
+  // ---- keys and the bitonic sort on (high key, index)
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  for (int i = t; i < n2; i += 1024) {
+    if (i < n) {
+      uint64_t hi, lo;
+      small_key(x3, i, X0, R0, hi, lo);
+      skey[i] = hi;
+      sidx[i] = (unsigned short)i;
+    } else {
+      skey[i] = ~0ull;
+      sidx[i] = 0xffff;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int u = t; u < n2 / 2; u += 1024) {
+        const int a = 2 * u - (u & (stride - 1)), c = a + stride;
+        const bool up = (a & size) == 0;
+        const uint64_t ka = skey[a], kc = skey[c];
+        const unsigned short ia = sidx[a], ic = sidx[c];
+        const bool gt = ka > kc || (ka == kc && ia > ic);
+        if (gt == up) {
+          skey[a] = kc;
+          skey[c] = ka;
+          sidx[a] = ic;
+          sidx[c] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  // runs of equal high keys: by (low key, index), insertion sort per run
+  for (int k = t; k < n - 1; k += 1024) {
+    if (skey[k] != skey[k + 1] || (k > 0 && skey[k - 1] == skey[k])) continue;
+    int e = k + 1;
+    while (e + 1 < n && skey[e + 1] == skey[k]) e++;
+    for (int a = k + 1; a <= e; a++) {
+      const unsigned short ia = sidx[a];
+      uint64_t hd, la;
+      small_key(x3, ia, X0, R0, hd, la);
+      int c = a - 1;
+      while (c >= k) {
+        uint64_t lc;
+        small_key(x3, sidx[c], X0, R0, hd, lc);
+        if (lc < la || (lc == la && sidx[c] < ia)) break;
+        sidx[c + 1] = sidx[c];
+        c--;
+      }
+      sidx[c + 1] = ia;
+    }
+  }
+  __syncthreads();
+  // ---- the gather into tree order
+  for (int k = t; k < n; k += 1024) {
+    const int i = sidx[k];
+    b.x[k] = x3[3 * (size_t)i];
+    b.y[k] = x3[3 * (size_t)i + 1];
+    b.z[k] = x3[3 * (size_t)i + 2];
+    b.vx[k] = v3[3 * (size_t)i];
+    b.vy[k] = v3[3 * (size_t)i + 1];
+    b.vz[k] = v3[3 * (size_t)i + 2];
+    b.q[k] = q[i];
+    b.flags[k] = flags_in ? flags_in[i] : (unsigned char)(kSink | kSource);
+    b.order[k] = i;
+    b.acc_old[k] = 0.0;
+    lend[k] = 0;
+  }
+  // root
+  if (t == 0) {
+    ci[0] = 0;
+    ci[1] = n;
+    for (int d = 0; d < 3; d++) cg[d] = X0[d];
+    cg[3] = R0;
+  }
+  __syncthreads();
+  // ---- levels
+  int *cur_i = ci, *nxt_i = ci + 2 * (size_t)n;
+  double *cur_g = cg, *nxt_g = cg + 4 * (size_t)n;
+  const int j = t & 7;
+  for (int level = 0;; level++) {
+    const int nc = ncur;
+    if (nc == 0) break;
+    if (level >= kMaxTreeLevels) {
+      if (t == 0) status = 2;
+      break;
+    }
+    for (int base = 0; base < nc; base += 128) {
+      const int k = base + (t >> 3);
+      const bool on = k < nc;
+      int begin = 0, nb = 0;
+      double c0 = 0, c1 = 0, c2 = 0, r = 0;
+      if (on) {
+        begin = cur_i[2 * k];
+        nb = cur_i[2 * k + 1];
+        c0 = cur_g[4 * k];
+        c1 = cur_g[4 * k + 1];
+        c2 = cur_g[4 * k + 2];
+        r = cur_g[4 * k + 3];
+      }
+      const bool split = on && nb > ncrit;
+      if (on && !split && j == 0) {
+        lend[begin + nb - 1] = 1;
+        atomicAdd(&nleaf, 1);
+      }
+      const int end = begin + nb;
+      const bool small = nb <= 32;
+      int o = begin, sz = 0;
+      if (split) {
+        if (small) {
+          for (int i = begin; i < end; i++) {
+            const int oc = (b.x[i] >= c0) + ((b.y[i] >= c1) << 1) + ((b.z[i] >= c2) << 2);
+            sz += oc == j;
+          }
+        } else if (j > 0) {
+          int l = begin, rr = end;
+          while (l < rr - 1) {
+            const int mid = (l + rr) / 2;
+            const int oc = (b.x[mid] >= c0) + ((b.y[mid] >= c1) << 1) + ((b.z[mid] >= c2) << 2);
+            if (oc >= j)
+              rr = mid;
+            else
+              l = mid;
+          }
+          o = l + 1;
+        }
+      }
+      // within the group of eight (every lane of a group shares `split`)
+      int incl = sz;
+#pragma unroll
+      for (int d = 1; d < 8; d <<= 1) {
+        const int tt = __shfl(incl, lane - d, kWave);
+        if (j >= d) incl += tt;
+      }
+      const int onext = __shfl(o, lane + 1, kWave);
+      if (split) {
+        if (small)
+          o = begin + incl - sz;
+        else
+          sz = (j < 7 ? onext : end) - o;
+        if (sz > 0) {
+          const int slot = atomicAdd(&nnext, 1);
+          const double rc = ldexp(r, -1);
+          nxt_i[2 * slot] = o;
+          nxt_i[2 * slot + 1] = sz;
+          nxt_g[4 * slot] = c0 + rc * (((j >> 0) & 1) * 2 - 1);
+          nxt_g[4 * slot + 1] = c1 + rc * (((j >> 1) & 1) * 2 - 1);
+          nxt_g[4 * slot + 2] = c2 + rc * (((j >> 2) & 1) * 2 - 1);
+          nxt_g[4 * slot + 3] = rc;
+        }
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      ncur = nnext;
+      nnext = 0;
+    }
+    int *ti = cur_i;
+    cur_i = nxt_i;
+    nxt_i = ti;
+    double *tg = cur_g;
+    cur_g = nxt_g;
+    nxt_g = tg;
+    __syncthreads();
+  }
+  __syncthreads();
+  if (t == 0) {
+    out[0] = nleaf;
+    out[1] = status;
+  }
+}
+
+int launch_small_tree(const double *x3, const double *v3, const double *q,
+                      const unsigned char *flags, int n, int ncrit, BodiesDev b,
+                      unsigned char *lend, int *ci, double *cg, int *out, hipStream_t s) {
+  if (n < 1 || n > kSmallTree) return -1;
+  hipLaunchKernelGGL(k_small_tree, dim3(1), dim3(1024), 0, s, x3, v3, q, flags, n, ncrit, b,
+                     lend, ci, cg, out);
+  return 0;
+}
+
 size_t scan_temp_bytes(int n) {
   size_t b = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int *)nullptr, (int *)nullptr, n);

@@ -1,0 +1,88 @@
+"""The one-launch tree of the master time-step sums (tc_set_small_tree):
+for systems up to 4096 bodies it must give the tree of the level-by-level
+build -- the same body order and the same leaves -- so the potentials,
+accelerations and time-step sums are bit for bit those of the level-by-level
+build.  Cases cover one body, ncrit 1..300, sink/source flags, the scalar
+(pair-mass) form, coincident bodies and bodies closer than 2^-21 of the box
+(equal high keys, ordered by the low key and the input index), and a system
+whose tree would need more than 64 levels (the same error from both).  The
+level-by-level build itself is pinned against the oracle and the reference
+(tests/test_gpu_master.py)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _eval(torch, tca, x, v, m, flags, ncrit, small, mode=0):
+    f = tca.Fmm(p=8, ncrit=ncrit, direct_below=0)
+    f.set_small_tree(4096 if small else 0)
+    dx, dv, dm = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, v, m))
+    fl = None if flags is None else torch.from_numpy(flags).cuda()
+    pot, acc, ts = f.master_eval(dx, dv, dm, flags=fl, get_steps=True, dt_param=0.025,
+                                 mode=mode)
+    torch.cuda.synchronize()
+    out = (pot.cpu().numpy(), acc.cpu().numpy(), ts.cpu().numpy(), f.stats()["n_p2p_cells"])
+    f.close()
+    return out
+
+
+def _case(kind, n, seed):
+    from taichi_collisional_amd import ics
+    x, v, m = ics.plummer(n, seed=seed)
+    rng = np.random.default_rng(seed)
+    if kind == "coincident":  # pairs of bodies at one point
+        k = n // 8
+        x[1:2 * k:2] = x[0:2 * k:2]
+    elif kind == "close":  # partners 1e-9 apart: equal high keys, different low keys
+        k = n // 8
+        x[1:2 * k:2] = x[0:2 * k:2] + rng.normal(scale=1e-9, size=(k, 3))
+    return x, v, m
+
+
+@pytest.mark.parametrize("kind,n,ncrit,flags,pair_mass", [
+    ("plummer", 1, 64, False, False),
+    ("plummer", 2, 1, False, False),
+    ("plummer", 777, 16, True, False),
+    ("plummer", 2000, 64, False, True),
+    ("plummer", 4096, 300, False, False),
+    ("plummer", 4096, 1, True, False),
+    ("coincident", 3000, 8, False, False),
+    ("close", 4000, 4, True, True),
+])
+def test_small_tree_equals_level_build(torch_cuda, kind, n, ncrit, flags, pair_mass):
+    import taichi_collisional_amd as tca
+    x, v, m = _case(kind, n, seed=n + ncrit)
+    fl = None
+    if flags:
+        rng = np.random.default_rng(n)
+        fl = rng.choice(np.array([3, 3, 3, 1, 2], np.uint8), size=n)
+    mode = tca.TC_MASTER_PAIR_MASS if pair_mass else 0
+    a = _eval(torch_cuda, tca, x, v, m, fl, ncrit, True, mode)
+    b = _eval(torch_cuda, tca, x, v, m, fl, ncrit, False, mode)
+    print("%s n=%d ncrit=%d: leaves^2 %d / %d" % (kind, n, ncrit, a[3], b[3]))
+    assert a[3] == b[3]  # the same number of leaves
+    for u, w in zip(a[:3], b[:3]):
+        assert np.array_equal(u, w)
+
+
+def test_small_tree_too_deep(torch_cuda):
+    """Nine coincident bodies with ncrit 4 never split below ncrit: both builds
+    stop at 64 levels with the same error."""
+    import taichi_collisional_amd as tca
+    x, v, m = _case("plummer", 200, seed=3)
+    x[:9] = x[0]
+    msgs = []
+    for small in (True, False):
+        with pytest.raises(RuntimeError) as e:
+            _eval(torch_cuda, tca, x, v, m, None, 4, small)
+        msgs.append(str(e.value).split(": ", 1)[-1])
+    assert msgs[0] == msgs[1] and "64 levels" in msgs[0]
