@@ -208,6 +208,11 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
 // context's body arrays (order = identity) with the leaf-end flags zeroed.
 // master_direct_prepare sizes those arrays for n bodies first.
 int master_direct_prepare(tc_context *ctx, int n);
+// The same with the step loop's kick applied in the sum (false: nothing
+// launched, the caller queues master_direct_queue and its own kick).
+bool master_direct_queue_kick(tc_context *ctx, int n, bool pair_mass, double dtp, const int *is,
+                              int ns, double dt, bool gradient, bool set_pot, double *vel,
+                              double *pot);
 void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
                          double *acc);
 

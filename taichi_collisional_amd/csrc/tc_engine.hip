@@ -1117,6 +1117,14 @@ int master_direct_prepare(tc_context *ctx, int n) {
   return 0;
 }
 
+bool master_direct_queue_kick(tc_context *ctx, int n, bool pair_mass, double dtp, const int *is,
+                              int ns, double dt, bool gradient, bool set_pot, double *vel,
+                              double *pot) {
+  return launch_master_direct_kick(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, pair_mass,
+                                   dtp, master_direct_slabs(n), ctx->mdpart.as<double>(), is, ns,
+                                   dt, gradient, set_pot, vel, pot, ctx->stream) == 0;
+}
+
 void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
                          double *acc) {
   launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, false, pair_mass, dt,

@@ -166,6 +166,45 @@ __global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *_
   out[pos] = is[k];
 }
 
+// split in one launch for a subsystem of at most 4096 bodies (one
+// workgroup): the flags, their exclusive scan, the count of fast bodies
+// (*nfast) and the same stable partition as k_split_flags + scan +
+// k_split_scatter.
+constexpr int kSplitSmall = 4096;
+__global__ __launch_bounds__(1024) void k_split_small(const int *__restrict__ is, int ns,
+                                                      const double *__restrict__ ts, double dt,
+                                                      int *__restrict__ out,
+                                                      int *__restrict__ nfast) {
+  __shared__ int wsum[16];
+  constexpr int PER = kSplitSmall / 1024;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  int f[PER], idx[PER], cnt = 0;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int k = t * PER + q;
+    idx[q] = k < ns ? is[k] : 0;
+    f[q] = (k < ns && ts[idx[q]] < dt) ? 1 : 0;
+    cnt += f[q];
+  }
+  int wt;
+  const int ex = wave_excl_scan(cnt, &wt);
+  if (lane == 0) wsum[w] = wt;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int k = 0; k < 16; k++) {
+    if (k < w) before += wsum[k];
+    total += wsum[k];
+  }
+  int run = before + ex;  // fast bodies before this thread's first entry
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int k = t * PER + q;
+    if (k < ns) out[f[q] ? run : total + (k - run)] = idx[q];
+    run += f[q];
+  }
+  if (t == 0) *nfast = total;
+}
+
 // time-step sums of zero: kick_naive's 1/sqrt(sqrt(0)) gives every body +inf
 __global__ void k_no_step_sums(double *__restrict__ out, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,6 +224,15 @@ inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 struct Sub {
   const int *idx = nullptr;
   int n = 0;
+};
+
+// A kick to apply to the sinks of a force call where the force kernels can
+// (the all-pairs sum in slabs): vel[idx[k]] += F_k dt (2/3 of it for a
+// gradient kick), pot[idx[k]] = p_k with set_pot; done tells the caller.
+struct KickSpec {
+  double dt;
+  bool gradient, set_pot;
+  bool done = false;
 };
 
 // The scratch buffers live in the context (grown on demand, released by
@@ -217,7 +265,7 @@ class Evolver {
   int ensure_union(int nu);
   // exact: drop TC_MASTER_FMM, so that the sum is the all-pairs one
   int force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
-            bool with_acc_old, bool exact = false);
+            bool with_acc_old, bool exact = false, KickSpec *kick = nullptr);
   int kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
   int kick_cpu(Sub s1, Sub s2, double dt);
   int kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
@@ -247,7 +295,7 @@ int Evolver::ensure_union(int nu) {
 // 383-424).  Results for the sinks are the first sinks.n entries of upot_ /
 // uacc_ (and uts_ with steps).
 int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
-                   bool with_acc_old, bool exact) {
+                   bool with_acc_old, bool exact, KickSpec *kick) {
   tc_context *ctx = ctx_;
   const int nu = sinks.n + sources.n;
   if (nu == 0) return 0;
@@ -268,8 +316,13 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
     hipLaunchKernelGGL(k_gather_union_bodies, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
                        sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass,
                        p_.acc, shift_fac, bodies_of(ctx), ctx->mlend.as<unsigned char>());
-    master_direct_queue(ctx, nu, (mode & TC_MASTER_PAIR_MASS) != 0, dtp_, upot_.as<double>(),
-                        uacc_.as<double>());
+    const bool pm = (mode & TC_MASTER_PAIR_MASS) != 0;
+    if (kick && master_direct_queue_kick(ctx, nu, pm, dtp_, sinks.idx, sinks.n, kick->dt,
+                                         kick->gradient, kick->set_pot, p_.vel, p_.pot)) {
+      kick->done = true;
+      return 0;
+    }
+    master_direct_queue(ctx, nu, pm, dtp_, upot_.as<double>(), uacc_.as<double>());
     return 0;
   }
   hipLaunchKernelGGL(k_gather_union, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
@@ -301,7 +354,9 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
 // kick (integrator.h:378-457): vel[sinks] += F dt, pot[sinks] = p
 int Evolver::kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast) {
   if (sinks.n == 0) return 0;
-  if (int e = force(sinks, sources, !sinks_are_fast, 0.0, false, false)) return e;
+  KickSpec ks{dt, false, true};
+  if (int e = force(sinks, sources, !sinks_are_fast, 0.0, false, false, false, &ks)) return e;
+  if (ks.done) return 0;
   hipLaunchKernelGGL(k_apply_kick, grid_for(sinks.n), dim3(256), 0, ctx_->stream, sinks.idx,
                      sinks.n, uacc_.as<double>(), upot_.as<double>(), dt, false, true, p_.vel,
                      p_.pot);
@@ -316,7 +371,9 @@ int Evolver::kick_cpu(Sub s1, Sub s2, double dt) {
   if (s1.n == 0) return 0;
   // s2 may contain s1 (join(slow, fast)); r = 0 pairs are skipped either way,
   // so s1 as sinks that are not sources plus s2 as sources is the same sum
-  if (int e = force(s1, s2, false, 0.0, false, false, true)) return e;
+  KickSpec ks{dt, false, false};
+  if (int e = force(s1, s2, false, 0.0, false, false, true, &ks)) return e;
+  if (ks.done) return 0;
   hipLaunchKernelGGL(k_apply_kick, grid_for(s1.n), dim3(256), 0, ctx_->stream, s1.idx, s1.n,
                      uacc_.as<double>(), upot_.as<double>(), dt, false, false, p_.vel, p_.pot);
   return 0;
@@ -328,7 +385,9 @@ int Evolver::kick_cpu(Sub s1, Sub s2, double dt) {
 int Evolver::kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast) {
   if (sinks.n == 0) return 0;
   const double fac = dt * dt / 24.0;
-  if (int e = force(sinks, sources, !sinks_are_fast, fac, false, false)) return e;
+  KickSpec ks{dt, true, false};
+  if (int e = force(sinks, sources, !sinks_are_fast, fac, false, false, false, &ks)) return e;
+  if (ks.done) return 0;
   hipLaunchKernelGGL(k_apply_kick, grid_for(sinks.n), dim3(256), 0, ctx_->stream, sinks.idx,
                      sinks.n, uacc_.as<double>(), upot_.as<double>(), dt, true, false, p_.vel,
                      p_.pot);
@@ -371,18 +430,29 @@ int Evolver::split(int level, Sub s, double dt, Sub &slow, Sub &fast) {
   TC_CHECK(flags_.ensure(sizeof(int) * ((size_t)s.n + 1)));
   TC_CHECK(offs_.ensure(sizeof(int) * ((size_t)s.n + 1)));
   TC_CHECK(scan_tmp_.ensure(scan_temp_bytes(s.n + 1) + 256));
-  hipLaunchKernelGGL(k_split_flags, grid_for(s.n + 1), dim3(256), 0, st, s.idx, s.n, p_.timestep,
-                     std::fabs(dt), flags_.as<int>());
-  if (launch_scan(flags_.as<int>(), offs_.as<int>(), s.n + 1, scan_tmp_.p, scan_tmp_.bytes, st)) {
-    ctx->err = "scan failed";
-    return -4;
+  int nfast = 0;
+  if (s.n <= kSplitSmall) {
+    hipLaunchKernelGGL(k_split_small, dim3(1), dim3(1024), 0, st, s.idx, s.n, p_.timestep,
+                       std::fabs(dt), out.as<int>(), flags_.as<int>());
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, flags_.as<int>(), sizeof(int), hipMemcpyDeviceToHost,
+                            st));
+    TC_CHECK(spin_wait(ctx, st));
+    nfast = ctx->h_pin[12];
+  } else {
+    hipLaunchKernelGGL(k_split_flags, grid_for(s.n + 1), dim3(256), 0, st, s.idx, s.n,
+                       p_.timestep, std::fabs(dt), flags_.as<int>());
+    if (launch_scan(flags_.as<int>(), offs_.as<int>(), s.n + 1, scan_tmp_.p, scan_tmp_.bytes,
+                    st)) {
+      ctx->err = "scan failed";
+      return -4;
+    }
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, offs_.as<int>() + s.n, sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+    TC_CHECK(spin_wait(ctx, st));
+    nfast = ctx->h_pin[12];
+    hipLaunchKernelGGL(k_split_scatter, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
+                       flags_.as<int>(), offs_.as<int>(), nfast, out.as<int>());
   }
-  TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, offs_.as<int>() + s.n, sizeof(int),
-                          hipMemcpyDeviceToHost, st));
-  TC_CHECK(spin_wait(ctx, st));
-  int nfast = ctx->h_pin[12];
-  hipLaunchKernelGGL(k_split_scatter, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
-                     flags_.as<int>(), offs_.as<int>(), nfast, out.as<int>());
   if (nfast == 1) nfast = 0;
   fast = Sub{out.as<int>(), nfast};
   slow = Sub{out.as<int>() + nfast, s.n - nfast};

@@ -301,6 +301,14 @@ hipError_t noalloc_scratch(unsigned char **out);
 int noalloc_max_bodies();
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
                       int n, unsigned char *lend, hipStream_t s);
+// The force-only direct sum of the step loop's kicks with the kick applied in
+// the slab-sum kernel: vel[is[k]] += F_k dt (2/3 of it for gradient kicks),
+// pot[is[k]] = p_k with set_pot, for the sinks k < ns (the first ns bodies in
+// the caller's order).  Returns -1 (nothing launched) when nslab < 2.
+int launch_master_direct_kick(const BodiesDev &b, const unsigned char *lend, int n, bool pair_mass,
+                              double dtp, int nslab, double *part, const int *is, int ns,
+                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
+                              hipStream_t s);
 // Source slabs for launch_master_direct at n bodies (1: no partial sums).
 int master_direct_slabs(int n);
 // part: 5 * nslab * n doubles of partial sums when nslab > 1

@@ -372,6 +372,51 @@ __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double 
   if (kSteps) tstep[o] = sink ? v[nc - 1] : 0.0;
 }
 
+// The slabs' partial sums of the force-only direct sum, added in slab order,
+// and the kick applied at once (the step loop's kicks): for every sink k of
+// the union (bodies [0, ns) in the caller's order, body is[k] of the
+// particle arrays) vel += F dt (gradient kicks: F dt 2/3, as ((F dt) 2) / 3)
+// and, with set_pot, pot = p -- the arithmetic of a separate kick kernel on
+// the summed results, without writing them out first.
+__global__ void k_master_direct_sum_kick(BodiesDev b, int n, int nslab,
+                                         const double *__restrict__ part,
+                                         const int *__restrict__ is, int ns, double dt,
+                                         bool gradient, bool set_pot, double *__restrict__ vel,
+                                         double *__restrict__ pot) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int o = b.order[i];
+  if (o >= ns) return;
+  double v[4] = {0, 0, 0, 0};
+  for (int q = 0; q < nslab; q++) {
+    const double *r = part + (size_t)q * 4 * n + i;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] += r[(size_t)k * n];
+  }
+  const int j = is[o];
+  for (int d = 0; d < 3; d++) {
+    const double F = v[1 + d];
+    vel[3 * j + d] += gradient ? F * dt * 2.0 / 3.0 : F * dt;
+  }
+  if (set_pot) pot[j] = v[0];
+}
+
+int launch_master_direct_kick(const BodiesDev &b, const unsigned char *lend, int n, bool pair_mass,
+                              double dtp, int nslab, double *part, const int *is, int ns,
+                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
+                              hipStream_t s) {
+  if (nslab < 2 || !part) return -1;  // the caller kicks from the written results
+  const int nb = (n + kMT - 1) / kMT;
+  const int slab = (n + nslab - 1) / nslab;
+  const dim3 grid(nb, nslab);
+  (void)pair_mass;  // the forces alone do not depend on the step estimate's form
+  k_master_direct<false, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dtp, nullptr, nullptr,
+                                                      nullptr, part);
+  k_master_direct_sum_kick<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, is, ns, dt, gradient,
+                                                           set_pot, vel, pot);
+  return 0;
+}
+
 int master_direct_slabs(int n) {
   // about 1024 blocks (four per CU) when n is small, at least 64 sources a
   // slab, at most 32 slabs, so that the partial sums (32 x 5 x n doubles
