@@ -147,7 +147,7 @@ struct tc_context {
   // master time-step sums up to small_tree_max bodies build their tree in one
   // launch (launch_small_tree; 0: always the multi-kernel build)
   int small_tree_max = kSmallTree;
-  DBuf small_ci, small_cg, small_out;
+  DBuf small_ci, small_cg, small_out, small_lidx, small_pmx;
   DBuf mdpart;  // partial sums of the source slabs of the master direct sum
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
   // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],
@@ -206,15 +206,15 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
 // The master default build's all-pairs force without time steps, queued on
 // the context's stream with no wait: the bodies must already be in the
 // context's body arrays (order = identity) with the leaf-end flags zeroed.
-// master_direct_prepare sizes those arrays for n bodies first.
-int master_direct_prepare(tc_context *ctx, int n);
-// The same with the step loop's kick applied in the sum (false: nothing
-// launched, the caller queues master_direct_queue and its own kick).
-bool master_direct_queue_kick(tc_context *ctx, int n, bool pair_mass, double dtp, const int *is,
-                              int ns, double dt, bool gradient, bool set_pot, double *vel,
-                              double *pot);
-void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
-                         double *acc);
+// master_direct_prepare sizes those arrays for n bodies, of which the first
+// ns are the sinks, first.
+int master_direct_prepare(tc_context *ctx, int n, int ns);
+// The same with the step loop's kick applied in the sum, summed for the
+// first ns bodies only (false: nothing launched, the caller queues
+// master_direct_queue and its own kick).
+bool master_direct_queue_kick(tc_context *ctx, int n, const int *is, int ns, double dt,
+                              bool gradient, bool set_pot, double *vel, double *pot);
+void master_direct_queue(tc_context *ctx, int n, double *pot, double *acc);
 
 // The jerk build's get_force_and_potential on device arrays (tc_eval_soa);
 // used by the step loop's TC_EVOLVE_JERK_FMM mode.

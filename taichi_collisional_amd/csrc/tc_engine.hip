@@ -1058,9 +1058,12 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     // the time-step sums of a small system: its whole tree in one launch
     TC_CHECK(ctx->small_ci.ensure(sizeof(int) * 4 * (size_t)n));
     TC_CHECK(ctx->small_cg.ensure(sizeof(double) * 8 * (size_t)n));
-    TC_CHECK(ctx->small_out.ensure(sizeof(int) * 2));
+    TC_CHECK(ctx->small_out.ensure(sizeof(int) * 4));
+    TC_CHECK(ctx->small_lidx.ensure(sizeof(int) * (size_t)n));
+    TC_CHECK(ctx->small_pmx.ensure(sizeof(double) * (size_t)n));
     launch_small_tree(x, v, m, flags, n, ctx->cfg.ncrit, bodies_of(ctx),
-                      ctx->mlend.as<unsigned char>(), ctx->small_ci.as<int>(),
+                      ctx->mlend.as<unsigned char>(), ctx->small_lidx.as<int>(),
+                      ctx->small_pmx.as<double>(), ctx->small_ci.as<int>(),
                       ctx->small_cg.as<double>(), ctx->small_out.as<int>(), s);
     small = true;
   } else {
@@ -1069,11 +1072,22 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     launch_leaf_ends(ctx->leaf_ids.as<int>(), ctx->nleaves, ctx->cbody.as<int>(),
                      ctx->cnbody.as<int>(), n, ctx->mlend.as<unsigned char>(), s);
   }
-  const int nslab = master_direct_slabs(n);
-  if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 5 * (size_t)nslab * n));
-  TC_CHECK(hipEventRecord(ctx->ev[1], s));
-  launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, steps, pair_mass, dt,
-                       pot, acc, tstep, nslab, ctx->mdpart.as<double>(), s);
+  if (steps) {
+    // leaf-cutting slabs where the one-launch tree gives the leaf data
+    const int nslab = master_direct_steps_slabs(n, small);
+    TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 9 * (size_t)nslab * n));
+    TC_CHECK(hipEventRecord(ctx->ev[1], s));
+    launch_master_direct_steps(bodies_of(ctx), ctx->mlend.as<unsigned char>(),
+                               small ? ctx->small_lidx.as<int>() : nullptr,
+                               small ? ctx->small_pmx.as<double>() : nullptr,
+                               small ? ctx->small_out.as<int>() + 2 : nullptr, n, pair_mass, dt,
+                               pot, acc, tstep, nslab, ctx->mdpart.as<double>(), s);
+  } else {
+    const int nslab = master_direct_slabs(n, n);
+    if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 4 * (size_t)nslab * n));
+    TC_CHECK(hipEventRecord(ctx->ev[1], s));
+    launch_master_direct(bodies_of(ctx), n, pot, acc, nslab, ctx->mdpart.as<double>(), s);
+  }
   TC_CHECK(hipEventRecord(ctx->ev[2], s));
   if (small)
     TC_CHECK(hipMemcpyAsync(ctx->h_pin + 136, ctx->small_out.p, sizeof(int) * 2,
@@ -1109,27 +1123,24 @@ int eval_soa_device(tc_context *ctx, int n, const double *x, const double *v, co
   return eval_device(ctx, n, x, v, m, flags, pot, acc, jerk);
 }
 
-int master_direct_prepare(tc_context *ctx, int n) {
+int master_direct_prepare(tc_context *ctx, int n, int ns) {
   if (int e = ensure_bodies(ctx, n)) return e;
   TC_CHECK(ctx->mlend.ensure((size_t)n));
-  const int nslab = master_direct_slabs(n);
-  if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 5 * (size_t)nslab * n));
+  const int nslab = std::max(master_direct_slabs(n, n), master_direct_slabs(ns, n));
+  if (nslab > 1) TC_CHECK(ctx->mdpart.ensure(sizeof(double) * 4 * (size_t)nslab * n));
   return 0;
 }
 
-bool master_direct_queue_kick(tc_context *ctx, int n, bool pair_mass, double dtp, const int *is,
-                              int ns, double dt, bool gradient, bool set_pot, double *vel,
-                              double *pot) {
-  return launch_master_direct_kick(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, pair_mass,
-                                   dtp, master_direct_slabs(n), ctx->mdpart.as<double>(), is, ns,
-                                   dt, gradient, set_pot, vel, pot, ctx->stream) == 0;
+bool master_direct_queue_kick(tc_context *ctx, int n, const int *is, int ns, double dt,
+                              bool gradient, bool set_pot, double *vel, double *pot) {
+  return launch_master_direct_kick(bodies_of(ctx), n, master_direct_slabs(ns, n),
+                                   ctx->mdpart.as<double>(), is, ns, dt, gradient, set_pot, vel,
+                                   pot, ctx->stream) == 0;
 }
 
-void master_direct_queue(tc_context *ctx, int n, bool pair_mass, double dt, double *pot,
-                         double *acc) {
-  launch_master_direct(bodies_of(ctx), ctx->mlend.as<unsigned char>(), n, false, pair_mass, dt,
-                       pot, acc, nullptr, master_direct_slabs(n), ctx->mdpart.as<double>(),
-                       ctx->stream);
+void master_direct_queue(tc_context *ctx, int n, double *pot, double *acc) {
+  launch_master_direct(bodies_of(ctx), n, pot, acc, master_direct_slabs(n, n),
+                       ctx->mdpart.as<double>(), ctx->stream);
 }
 
 extern "C" {

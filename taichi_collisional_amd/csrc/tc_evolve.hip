@@ -80,6 +80,8 @@ __global__ void k_gather_union_bodies(const int *__restrict__ is, int ns,
 
 // vel[sinks] += F * dt * fac (fac: 1 for kicks, 2/3 for gradient kicks, applied
 // as ((F dt) 2) / 3 like integrator.h:350-353); pot[sinks] = p when set_pot.
+// F = p = null: a force and potential of zero (no source), the same
+// arithmetic on 0.0.
 __global__ void k_apply_kick(const int *__restrict__ is, int ns, const double *__restrict__ F,
                              const double *__restrict__ p, double dt, bool gradient,
                              bool set_pot, double *__restrict__ vel, double *__restrict__ pot) {
@@ -87,10 +89,11 @@ __global__ void k_apply_kick(const int *__restrict__ is, int ns, const double *_
   if (k >= ns) return;
   const int i = is[k];
   for (int d = 0; d < 3; d++) {
-    const double dv = gradient ? F[3 * k + d] * dt * 2.0 / 3.0 : F[3 * k + d] * dt;
+    const double f = F ? F[3 * k + d] : 0.0;
+    const double dv = gradient ? f * dt * 2.0 / 3.0 : f * dt;
     vel[3 * i + d] += dv;
   }
-  if (set_pot) pot[i] = p[k];
+  if (set_pot) pot[i] = p ? p[k] : 0.0;
 }
 
 // kick_naive's bookkeeping (integrator.h:250-300) for the subsystem's bodies:
@@ -303,7 +306,15 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
   hipStream_t s = ctx->stream;
   if (!sinks_are_sources && sources.n == 0 && !steps) {
     // no body exerts a force (kick(fast, slow) with an empty slow set): the
-    // reference's evaluation returns zeros; skip it
+    // reference's evaluation returns zeros; skip it (a kick of zero in one
+    // launch)
+    if (kick) {
+      hipLaunchKernelGGL(k_apply_kick, grid_for(sinks.n), dim3(256), 0, s, sinks.idx, sinks.n,
+                         nullptr, nullptr, kick->dt, kick->gradient, kick->set_pot, p_.vel,
+                         p_.pot);
+      kick->done = true;
+      return 0;
+    }
     TC_CHECK(hipMemsetAsync(upot_.p, 0, 8 * (size_t)sinks.n, s));
     TC_CHECK(hipMemsetAsync(uacc_.p, 0, 24 * (size_t)sinks.n, s));
     return 0;
@@ -312,17 +323,16 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
   if (!steps && !(mode & TC_MASTER_FMM) && !(jerk_fmm_ && !exact)) {
     // the default build's all-pairs force: gathered straight into the body
     // arrays and queued with no wait (the loop reads nothing back from it)
-    if (int e = master_direct_prepare(ctx, nu)) return e;
+    if (int e = master_direct_prepare(ctx, nu, sinks.n)) return e;
     hipLaunchKernelGGL(k_gather_union_bodies, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
                        sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass,
                        p_.acc, shift_fac, bodies_of(ctx), ctx->mlend.as<unsigned char>());
-    const bool pm = (mode & TC_MASTER_PAIR_MASS) != 0;
-    if (kick && master_direct_queue_kick(ctx, nu, pm, dtp_, sinks.idx, sinks.n, kick->dt,
-                                         kick->gradient, kick->set_pot, p_.vel, p_.pot)) {
+    if (kick && master_direct_queue_kick(ctx, nu, sinks.idx, sinks.n, kick->dt, kick->gradient,
+                                         kick->set_pot, p_.vel, p_.pot)) {
       kick->done = true;
       return 0;
     }
-    master_direct_queue(ctx, nu, pm, dtp_, upot_.as<double>(), uacc_.as<double>());
+    master_direct_queue(ctx, nu, upot_.as<double>(), uacc_.as<double>());
     return 0;
   }
   hipLaunchKernelGGL(k_gather_union, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,

@@ -105,12 +105,15 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
                       int *nch, int *scan, const TreeParams *d_tp, int *leaf_ids, hipStream_t s);
 // The whole tree of at most kSmallTree bodies in one launch (one workgroup),
 // for the master time-step sums: bodies gathered into tree order, lend set on
-// the last body of every leaf; out[0] = leaves, out[1] = 0 or 2 (too deep).
+// the last body of every leaf; out[0] = leaves, out[1] = 0 or 2 (too deep),
+// out[2] = 1 when two bodies are (next to) coincident; lidx / pmx (or null):
+// each body's leaf's last index and the mass before it in its leaf.
 // ci: 4 n ints, cg: 8 n doubles of scratch.
 constexpr int kSmallTree = 4096;
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
-                      unsigned char *lend, int *ci, double *cg, int *out, hipStream_t s);
+                      unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
+                      hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 
@@ -303,18 +306,30 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
                       int n, unsigned char *lend, hipStream_t s);
 // The force-only direct sum of the step loop's kicks with the kick applied in
 // the slab-sum kernel: vel[is[k]] += F_k dt (2/3 of it for gradient kicks),
-// pot[is[k]] = p_k with set_pot, for the sinks k < ns (the first ns bodies in
-// the caller's order).  Returns -1 (nothing launched) when nslab < 2.
-int launch_master_direct_kick(const BodiesDev &b, const unsigned char *lend, int n, bool pair_mass,
-                              double dtp, int nslab, double *part, const int *is, int ns,
-                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
-                              hipStream_t s);
-// Source slabs for launch_master_direct at n bodies (1: no partial sums).
-int master_direct_slabs(int n);
-// part: 5 * nslab * n doubles of partial sums when nslab > 1
-void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
-                          bool pair_mass, double dt, double *pot, double *acc, double *tstep,
-                          int nslab, double *part, hipStream_t s);
+// pot[is[k]] = p_k with set_pot, for the sinks k < ns, which must be the
+// first ns bodies (stored in the caller's order); only they are summed for.
+// part: 4 * nslab * n doubles.  Returns -1 (nothing launched) when nslab < 2.
+int launch_master_direct_kick(const BodiesDev &b, int n, int nslab, double *part, const int *is,
+                              int ns, double dt, bool gradient, bool set_pot, double *vel,
+                              double *pot, hipStream_t s);
+// The direct sum with the time-step estimate, in slabs that may cut source
+// leaves when the one-launch small tree gave lidx (each body's leaf's last
+// index) and pmx (the mass of the leaf's bodies before it), and (vector form)
+// flag[0] == 0 (no zero distance between distinct bodies); otherwise in
+// leaf-aligned slabs.  part: 9 * nslab * n doubles.  Writes pot, acc, tstep
+// in the caller's order (zero for non-sinks).
+int master_direct_steps_slabs(int n, bool uniform);
+void launch_master_direct_steps(const BodiesDev &b, const unsigned char *lend, const int *lidx,
+                                const double *pmx, const int *flag, int n, bool pair_mass,
+                                double dt, double *pot, double *acc, double *tstep, int nslab,
+                                double *part, hipStream_t s);
+// Source slabs of the force-only direct sum for nt targets among n bodies
+// (1: no partial sums).
+int master_direct_slabs(int nt, int n);
+// The force-only direct sum (P2P_simple over every pair); part: 4 * nslab * n
+// doubles of partial sums when nslab > 1.
+void launch_master_direct(const BodiesDev &b, int n, double *pot, double *acc, int nslab,
+                          double *part, hipStream_t s);
 // symmetric M2L list: for every accepted (i, j) with i < j and a sink in
 // either cell, (i, j) and (j, i); cnt/off hold npairs + 1 ints
 void launch_sym_count(const int2 *pairs, int npairs, const unsigned char *has_sink, int *cnt,

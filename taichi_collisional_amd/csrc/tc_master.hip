@@ -127,41 +127,125 @@ __device__ int next_leaf_start(const unsigned char *__restrict__ lend, int n, in
   }
 }
 
-// Sources may be split into gridDim.y slabs of about `slab` bodies each, so
-// that a small system still fills the GPU.  A slab starts and ends where a
-// source leaf does (lend marks each leaf's last body): the running minimum
-// of the time-step estimate belongs to one leaf, so a leaf is never cut.
-// Each slab writes its partial sums, in sorted order, to part[slab][nc][n]
-// (potential, acceleration and, with steps, the time-step sum: nc = 4 or 5);
-// k_master_direct_sum adds the slabs in order.  With one slab the results
-// are written directly.
-template <bool kSteps, bool kPairMass>
-__global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsigned char *lend,
-                                                       int n, int slab, double dt, double *pot,
-                                                       double *acc, double *tstep,
+// The force alone (no time steps): one thread per target i < nt, sources
+// [0, n).  Sources may be split into gridDim.y uniform slabs of `slab` bodies,
+// so that a small system still fills the GPU; each slab then writes its
+// partial sums, in sorted order, to part[slab][4][n] (potential,
+// acceleration) and k_master_direct_sum / k_master_direct_sum_kick add the
+// slabs in order.  With one slab the results are written directly.  The
+// leaves do not matter here: P2P_simple sums every pair.
+__global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, int n, int nt, int slab,
+                                                       double *pot, double *acc,
                                                        double *__restrict__ part) {
-  __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT], svx[kMT], svy[kMT], svz[kMT];
-  __shared__ unsigned char sf[kMT];
+  __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT];
   const int t = threadIdx.x;
   int j0 = 0, j1 = n;
   if (gridDim.y > 1) {
-    // both ends move forward to the next leaf start (or n), by the same rule,
-    // so neighbouring slabs meet
     j0 = (int)min((long long)n, (long long)blockIdx.y * slab);
     j1 = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
-    if (kSteps) {
-      j0 = next_leaf_start(lend, n, j0);
-      j1 = next_leaf_start(lend, n, j1);
+  }
+  const int i = blockIdx.x * kMT + t;
+  const bool live = i < nt;
+  // lanes past nt run the loop on a harmless stand-in body and write nothing
+  const double xi = live ? b.x[i] : 0.0, yi = live ? b.y[i] : 0.0, zi = live ? b.z[i] : 0.0;
+  double ax = 0, ay = 0, az = 0, pt = 0;
+  for (int base = j0; base < j1; base += kMT) {
+    __syncthreads();
+    const int j = base + t;
+    if (j < j1) {
+      sx[t] = b.x[j];
+      sy[t] = b.y[j];
+      sz[t] = b.z[j];
+      // P2P_simple: sources only (a non-source weighs nothing)
+      sq[t] = (b.flags[j] & kSource) ? b.q[j] : 0.0;
     }
+    __syncthreads();
+    const int cnt = min(kMT, j1 - base);
+    for (int k = 0; k < cnt; k++) {
+      const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      // r2 = 0 masked
+      const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
+      const double w = r2 > 0 ? sq[k] * ir : 0.0;
+      pt += w;
+      const double w3 = w * ir * ir;
+      ax += dx * w3;
+      ay += dy * w3;
+      az += dz * w3;
+    }
+  }
+  if (!live) return;
+  if (gridDim.y > 1) {
+    double *q = part + (size_t)blockIdx.y * 4 * n + i;
+    q[0] = pt;
+    q[(size_t)n] = ax;
+    q[2 * (size_t)n] = ay;
+    q[3 * (size_t)n] = az;
+    return;
+  }
+  const bool sink = b.flags[i] & kSink;
+  const int o = b.order[i];
+  pot[o] = sink ? pt : 0.0;
+  acc[3 * o] = sink ? ax : 0.0;
+  acc[3 * o + 1] = sink ? ay : 0.0;
+  acc[3 * o + 2] = sink ? az : 0.0;
+}
+
+// ---- the direct sum with time steps, in slabs that may cut leaves -----------
+// With the time-step estimate the sum runs per source leaf: a running
+// minimum, and (vector form) a target mass mi that grows along the leaf.  A
+// slab that must not cut leaves leaves most of the GPU idle on a few thousand
+// bodies with ncrit = 300.  Here, when the tree is a one-launch small tree
+// (launch_small_tree gives every body its leaf's last index lidx and the mass
+// of the leaf's bodies before it, pmx) and no two bodies are at zero distance
+// (flag[0] == 0: no r^2 = 0 pair other than a body with itself), slabs are
+// uniform: a slab starting inside a leaf starts mi where the leaf's running
+// sum would be there -- q_i + pmx[j0], less q_i itself when the target sits
+// in that leaf before j0 -- and each slab reports, per target, its leaf
+// pieces: the first and the last (leaf id = last index, running minimum) and
+// the sum over the leaves it holds whole; k_master_direct_merge joins the
+// pieces of a leaf across slabs (minimum), then adds min^-4 per leaf, slab by
+// slab.  Otherwise (the level-built tree, or coincident bodies) the slabs are
+// leaf-aligned as in k_master_direct and the same format and merge serve.
+// part: [slab][9][n] doubles: potential, acceleration (3), the whole leaves'
+// sum, first piece minimum, last piece minimum, first and last piece ids.
+constexpr int kStepCols = 9;
+
+template <bool kPairMass>
+__global__ __launch_bounds__(kMT) void k_master_direct_steps(BodiesDev b,
+                                                             const unsigned char *lend,
+                                                             const int *__restrict__ lidx,
+                                                             const double *__restrict__ pmx,
+                                                             const int *__restrict__ flag, int n,
+                                                             int slab, double dt,
+                                                             double *__restrict__ part) {
+  __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT], svx[kMT], svy[kMT], svz[kMT];
+  __shared__ unsigned char sf[kMT];
+  const int t = threadIdx.x;
+  // uniform slabs: the small tree's leaf data, and (vector form) no zero
+  // distances between distinct bodies
+  const bool uniform = lidx && pmx && (kPairMass || (flag && flag[0] == 0));
+  int j0 = (int)min((long long)n, (long long)blockIdx.y * slab);
+  int j1 = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
+  if (!uniform) {
+    j0 = next_leaf_start(lend, n, j0);
+    j1 = next_leaf_start(lend, n, j1);
   }
   const int i = blockIdx.x * kMT + t;
   const bool live = i < n;
-  // lanes past n run the loop on a harmless stand-in body and write nothing
   const double xi = live ? b.x[i] : 0.0, yi = live ? b.y[i] : 0.0, zi = live ? b.z[i] : 0.0;
   const double vxi = live ? b.vx[i] : 0.0, vyi = live ? b.vy[i] : 0.0,
                vzi = live ? b.vz[i] : 0.0;
   const double qi = live ? b.q[i] : 1.0;
-  double ax = 0, ay = 0, az = 0, pt = 0, tsum = 0, mi = qi, ts = 1e38;
+  double ax = 0, ay = 0, az = 0, pt = 0, ts = 1e38, mi = qi;
+  if (uniform && !kPairMass && j0 < j1 && j0 > 0 && !lend[j0 - 1]) {
+    // the slab starts inside a leaf: its running mass up to j0
+    const bool own = live && lidx[i] == lidx[j0] && i < j0;
+    mi = own ? pmx[j0] : qi + pmx[j0];
+  }
+  // pieces: first (fid, fmin), whole leaves (whole), the one open (cid)
+  double fmin_ = 0, whole = 0, lmin = 0;
+  int fid = -1, lid = -1;
   for (int base = j0; base < j1; base += kMT) {
     __syncthreads();
     const int j = base + t;
@@ -170,11 +254,9 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
       sy[t] = b.y[j];
       sz[t] = b.z[j];
       sq[t] = b.q[j];
-      if (kSteps) {
-        svx[t] = b.vx[j];
-        svy[t] = b.vy[j];
-        svz[t] = b.vz[j];
-      }
+      svx[t] = b.vx[j];
+      svy[t] = b.vy[j];
+      svz[t] = b.vz[j];
       sf[t] = ((b.flags[j] & kSource) ? 1 : 0) | (lend[j] ? 2 : 0);
     }
     __syncthreads();
@@ -184,10 +266,8 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
       const double r2 = dx * dx + dy * dy + dz * dz;
       const unsigned char f = sf[k];
       const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
-      if (kSteps)
-        pair_tau<kPairMass>(dx, dy, dz, svx[k] - vxi, svy[k] - vyi, svz[k] - vzi, r2, ir, qi,
-                            sq[k], dt, mi, ts);
-      // P2P_simple / P2P force: sources only, r2 = 0 masked
+      pair_tau<kPairMass>(dx, dy, dz, svx[k] - vxi, svy[k] - vyi, svz[k] - vzi, r2, ir, qi,
+                          sq[k], dt, mi, ts);
       const bool use = (f & 1) && r2 > 0;
       const double w = use ? sq[k] * ir : 0.0;
       pt += w;
@@ -195,33 +275,173 @@ __global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, const unsign
       ax += dx * w3;
       ay += dy * w3;
       az += dz * w3;
-      if (kSteps && (f & 2)) {
-        double t4 = ts * ts;
-        t4 *= t4;
-        tsum += 1.0 / t4;
-        ts = 1e38;
-        mi = qi;
+      const bool end = (f & 2) != 0;
+      if (end || base + k == j1 - 1) {
+        // a piece closes: at the end of its leaf, or at the end of the slab
+        const int id = end ? base + k : lidx[base + k];
+        if (fid < 0) {
+          fid = id;
+          fmin_ = ts;
+        } else {
+          if (lid >= 0) {
+            double t4 = lmin * lmin;
+            t4 *= t4;
+            whole += 1.0 / t4;
+          }
+          lid = id;
+          lmin = ts;
+        }
+        if (end) {
+          ts = 1e38;
+          mi = qi;
+        }
       }
     }
   }
   if (!live) return;
-  if (gridDim.y > 1) {
-    constexpr int nc = kSteps ? 5 : 4;
-    double *q = part + (size_t)blockIdx.y * nc * n + i;
-    q[0] = pt;
-    q[(size_t)n] = ax;
-    q[2 * (size_t)n] = ay;
-    q[3 * (size_t)n] = az;
-    if (kSteps) q[4 * (size_t)n] = tsum;
-    return;
+  if (lid < 0) {  // one piece (or none, for an empty slab)
+    lid = fid;
+    lmin = fmin_;
+  }
+  double *o = part + (size_t)blockIdx.y * kStepCols * n + i;
+  o[0] = pt;
+  o[(size_t)n] = ax;
+  o[2 * (size_t)n] = ay;
+  o[3 * (size_t)n] = az;
+  o[4 * (size_t)n] = whole;
+  o[5 * (size_t)n] = fmin_;
+  o[6 * (size_t)n] = lmin;
+  o[7 * (size_t)n] = (double)fid;
+  o[8 * (size_t)n] = (double)lid;
+}
+
+// A run of consecutive slabs, per target: the force sums, and its source
+// leaf pieces -- the first (id fid = the leaf's last body, running minimum
+// fmin), the sum of min^-4 over the leaves strictly between (whole), and the
+// last (lid, lmin); fid == lid for a single piece, fid = -1 for no source.
+struct StepRun {
+  double v[4], whole, fmin, lmin;
+  int fid, lid;
+};
+
+__device__ __forceinline__ double inv4(double m) {
+  double t4 = m * m;
+  t4 *= t4;
+  return 1.0 / t4;
+}
+
+// a then b: a leaf cut between the two (a's last piece and b's first have
+// one id) joins by the minimum; a piece that ends up between the new first
+// and last adds its min^-4 to whole
+__device__ __forceinline__ StepRun join_runs(const StepRun &a, const StepRun &b) {
+  StepRun r;
+#pragma unroll
+  for (int k = 0; k < 4; k++) r.v[k] = a.v[k] + b.v[k];
+  if (a.fid < 0 || b.fid < 0) {
+    const StepRun &c = a.fid < 0 ? b : a;
+    r.whole = c.whole;
+    r.fmin = c.fmin;
+    r.lmin = c.lmin;
+    r.fid = c.fid;
+    r.lid = c.lid;
+    return r;
+  }
+  const bool sa = a.fid == a.lid, sb = b.fid == b.lid;
+  r.fid = a.fid;
+  r.fmin = a.fmin;
+  r.lid = b.lid;
+  r.lmin = b.lmin;
+  r.whole = a.whole + b.whole;
+  if (a.lid == b.fid) {
+    const double m = fmin(a.lmin, b.fmin);
+    if (sa) r.fmin = m;
+    if (sb) r.lmin = m;
+    if (!sa && !sb) r.whole += inv4(m);
+  } else {
+    if (!sa) r.whole += inv4(a.lmin);
+    if (!sb) r.whole += inv4(b.fmin);
+  }
+  return r;
+}
+
+// Per target, the slabs' runs joined in slab order: kMergeGroups groups of
+// threads take consecutive ranges of slabs for the same 64 targets, then the
+// first group joins the groups' runs; the time-step sum is the first piece's
+// min^-4, whole, and the last piece's (for more than one piece).  Results go
+// to the caller's order for sinks.
+constexpr int kMergeGroups = 8;
+__global__ __launch_bounds__(64 * kMergeGroups) void k_master_direct_merge(
+    BodiesDev b, int n, int nslab, const double *__restrict__ part, double *pot, double *acc,
+    double *tstep) {
+  __shared__ StepRun runs[kMergeGroups][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int per = (nslab + kMergeGroups - 1) / kMergeGroups;
+  const int q0 = min(nslab, g * per), q1 = min(nslab, q0 + per);
+  StepRun acc_run;
+#pragma unroll
+  for (int k = 0; k < 4; k++) acc_run.v[k] = 0.0;
+  acc_run.whole = acc_run.fmin = acc_run.lmin = 0.0;
+  acc_run.fid = acc_run.lid = -1;
+  if (i < n)
+    for (int q = q0; q < q1; q++) {
+      const double *r = part + (size_t)q * kStepCols * n + i;
+      StepRun c;
+#pragma unroll
+      for (int k = 0; k < 4; k++) c.v[k] = r[(size_t)k * n];
+      c.whole = r[4 * (size_t)n];
+      c.fmin = r[5 * (size_t)n];
+      c.lmin = r[6 * (size_t)n];
+      c.fid = (int)r[7 * (size_t)n];
+      c.lid = (int)r[8 * (size_t)n];
+      acc_run = join_runs(acc_run, c);
+    }
+  runs[g][lane] = acc_run;
+  __syncthreads();
+  if (g != 0 || i >= n) return;
+  for (int u = 1; u < kMergeGroups; u++) acc_run = join_runs(acc_run, runs[u][lane]);
+  double tsum = 0.0;
+  if (acc_run.fid >= 0) {
+    tsum = inv4(acc_run.fmin) + acc_run.whole;
+    if (acc_run.lid != acc_run.fid) tsum += inv4(acc_run.lmin);
   }
   const bool sink = b.flags[i] & kSink;
   const int o = b.order[i];
-  pot[o] = sink ? pt : 0.0;
-  acc[3 * o] = sink ? ax : 0.0;
-  acc[3 * o + 1] = sink ? ay : 0.0;
-  acc[3 * o + 2] = sink ? az : 0.0;
-  if (kSteps) tstep[o] = sink ? tsum : 0.0;
+  pot[o] = sink ? acc_run.v[0] : 0.0;
+  acc[3 * o] = sink ? acc_run.v[1] : 0.0;
+  acc[3 * o + 1] = sink ? acc_run.v[2] : 0.0;
+  acc[3 * o + 2] = sink ? acc_run.v[3] : 0.0;
+  tstep[o] = sink ? tsum : 0.0;
+}
+
+int master_direct_steps_slabs(int n, bool uniform) {
+  if (!uniform) {
+    // slabs aligned to leaves: at least ncrit-sized, so fewer of them
+    const int nb = (n + kMT - 1) / kMT;
+    return std::max(1, std::min({(1024 + nb - 1) / nb, (n + 63) / 64, 32}));
+  }
+  // uniform slabs of >= 64 sources, about 1024 blocks
+  const int nb = (n + kMT - 1) / kMT;
+  const int want = (1024 + nb - 1) / nb;
+  return std::max(1, std::min({want, (n + 63) / 64, 64}));
+}
+
+void launch_master_direct_steps(const BodiesDev &b, const unsigned char *lend, const int *lidx,
+                                const double *pmx, const int *flag, int n, bool pair_mass,
+                                double dt, double *pot, double *acc, double *tstep, int nslab,
+                                double *part, hipStream_t s) {
+  const int nb = (n + kMT - 1) / kMT;
+  if (nslab < 1) nslab = 1;
+  const int slab = (n + nslab - 1) / nslab;
+  const dim3 grid(nb, nslab);
+  if (!pair_mass)
+    k_master_direct_steps<false><<<grid, kMT, 0, s>>>(b, lend, lidx, pmx, flag, n, slab, dt,
+                                                       part);
+  else
+    k_master_direct_steps<true><<<grid, kMT, 0, s>>>(b, lend, lidx, pmx, flag, n, slab, dt,
+                                                      part);
+  k_master_direct_merge<<<(n + 63) / 64, 64 * kMergeGroups, 0, s>>>(b, n, nslab, part, pot, acc,
+                                                                     tstep);
 }
 
 // ---- the build with -DFMM ---------------------------------------------------
@@ -349,19 +569,15 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
 
 // The slabs' partial sums of k_master_direct, added in slab order, written
 // to caller order for sinks (zero for the others).
-template <bool kSteps>
 __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double *__restrict__ part,
-                                    double *pot, double *acc, double *tstep) {
-  constexpr int nc = kSteps ? 5 : 4;
+                                    double *pot, double *acc) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double v[nc];
-#pragma unroll
-  for (int k = 0; k < nc; k++) v[k] = 0;
+  double v[4] = {0, 0, 0, 0};
   for (int q = 0; q < nslab; q++) {
-    const double *r = part + (size_t)q * nc * n + i;
+    const double *r = part + (size_t)q * 4 * n + i;
 #pragma unroll
-    for (int k = 0; k < nc; k++) v[k] += r[(size_t)k * n];
+    for (int k = 0; k < 4; k++) v[k] += r[(size_t)k * n];
   }
   const bool sink = b.flags[i] & kSink;
   const int o = b.order[i];
@@ -369,31 +585,27 @@ __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double 
   acc[3 * o] = sink ? v[1] : 0.0;
   acc[3 * o + 1] = sink ? v[2] : 0.0;
   acc[3 * o + 2] = sink ? v[3] : 0.0;
-  if (kSteps) tstep[o] = sink ? v[nc - 1] : 0.0;
 }
 
 // The slabs' partial sums of the force-only direct sum, added in slab order,
 // and the kick applied at once (the step loop's kicks): for every sink k of
-// the union (bodies [0, ns) in the caller's order, body is[k] of the
+// the union (bodies [0, ns), stored in the caller's order, body is[k] of the
 // particle arrays) vel += F dt (gradient kicks: F dt 2/3, as ((F dt) 2) / 3)
 // and, with set_pot, pot = p -- the arithmetic of a separate kick kernel on
 // the summed results, without writing them out first.
-__global__ void k_master_direct_sum_kick(BodiesDev b, int n, int nslab,
-                                         const double *__restrict__ part,
+__global__ void k_master_direct_sum_kick(int n, int nslab, const double *__restrict__ part,
                                          const int *__restrict__ is, int ns, double dt,
                                          bool gradient, bool set_pot, double *__restrict__ vel,
                                          double *__restrict__ pot) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int o = b.order[i];
-  if (o >= ns) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= ns) return;
   double v[4] = {0, 0, 0, 0};
   for (int q = 0; q < nslab; q++) {
-    const double *r = part + (size_t)q * 4 * n + i;
+    const double *r = part + (size_t)q * 4 * n + k;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += r[(size_t)k * n];
+    for (int c = 0; c < 4; c++) v[c] += r[(size_t)c * n];
   }
-  const int j = is[o];
+  const int j = is[k];
   for (int d = 0; d < 3; d++) {
     const double F = v[1 + d];
     vel[3 * j + d] += gradient ? F * dt * 2.0 / 3.0 : F * dt;
@@ -401,51 +613,38 @@ __global__ void k_master_direct_sum_kick(BodiesDev b, int n, int nslab,
   if (set_pot) pot[j] = v[0];
 }
 
-int launch_master_direct_kick(const BodiesDev &b, const unsigned char *lend, int n, bool pair_mass,
-                              double dtp, int nslab, double *part, const int *is, int ns,
-                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
-                              hipStream_t s) {
-  if (nslab < 2 || !part) return -1;  // the caller kicks from the written results
-  const int nb = (n + kMT - 1) / kMT;
-  const int slab = (n + nslab - 1) / nslab;
-  const dim3 grid(nb, nslab);
-  (void)pair_mass;  // the forces alone do not depend on the step estimate's form
-  k_master_direct<false, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dtp, nullptr, nullptr,
-                                                      nullptr, part);
-  k_master_direct_sum_kick<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, is, ns, dt, gradient,
-                                                           set_pot, vel, pot);
-  return 0;
-}
-
-int master_direct_slabs(int n) {
-  // about 1024 blocks (four per CU) when n is small, at least 64 sources a
-  // slab, at most 32 slabs, so that the partial sums (32 x 5 x n doubles
+int master_direct_slabs(int nt, int n) {
+  // about 1024 blocks (four per CU) over the nt targets, at least 64 sources
+  // a slab, at most 32 slabs, so that the partial sums (32 x 4 x n doubles
   // written and read back) stay well below the force kernel's own time; one
-  // slab (no partial sums) from 256 K bodies on
-  const int nb = (n + kMT - 1) / kMT;
+  // slab (no partial sums) from 256 K targets on
+  const int nb = (std::max(nt, 1) + kMT - 1) / kMT;
   const int want = (1024 + nb - 1) / nb;
   return std::max(1, std::min({want, (n + 63) / 64, 32}));
 }
 
-void launch_master_direct(const BodiesDev &b, const unsigned char *lend, int n, bool steps,
-                          bool pair_mass, double dt, double *pot, double *acc, double *tstep,
-                          int nslab, double *part, hipStream_t s) {
+int launch_master_direct_kick(const BodiesDev &b, int n, int nslab, double *part, const int *is,
+                              int ns, double dt, bool gradient, bool set_pot, double *vel,
+                              double *pot, hipStream_t s) {
+  if (nslab < 2 || !part) return -1;  // the caller kicks from the written results
+  if (ns < 1) return 0;
+  const int slab = (n + nslab - 1) / nslab;
+  const dim3 grid((ns + kMT - 1) / kMT, nslab);
+  k_master_direct<<<grid, kMT, 0, s>>>(b, n, ns, slab, nullptr, nullptr, part);
+  k_master_direct_sum_kick<<<(ns + 255) / 256, 256, 0, s>>>(n, nslab, part, is, ns, dt, gradient,
+                                                            set_pot, vel, pot);
+  return 0;
+}
+
+void launch_master_direct(const BodiesDev &b, int n, double *pot, double *acc, int nslab,
+                          double *part, hipStream_t s) {
   const int nb = (n + kMT - 1) / kMT;
   if (nslab < 1 || !part) nslab = 1;
   const int slab = (n + nslab - 1) / nslab;
   const dim3 grid(nb, nslab);
-  if (!steps)
-    k_master_direct<false, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
-  else if (!pair_mass)
-    k_master_direct<true, false><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
-  else
-    k_master_direct<true, true><<<grid, kMT, 0, s>>>(b, lend, n, slab, dt, pot, acc, tstep, part);
-  if (nslab > 1) {
-    if (steps)
-      k_master_direct_sum<true><<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc, tstep);
-    else
-      k_master_direct_sum<false><<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc, tstep);
-  }
+  k_master_direct<<<grid, kMT, 0, s>>>(b, n, n, slab, pot, acc, part);
+  if (nslab > 1)
+    k_master_direct_sum<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc);
 }
 
 }  // namespace tcfmm

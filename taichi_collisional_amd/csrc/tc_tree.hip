@@ -567,7 +567,14 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
 //    tests the cell's first body), children placed as k_children does.
 // It marks the last body of every leaf in lend (cleared first) and counts
 // the leaves: out[0] = leaves, out[1] = 0, or 2 when the tree would need more
-// than kMaxTreeLevels levels (the multi-kernel build's error).  Cells live in
+// than kMaxTreeLevels levels (the multi-kernel build's error).  With lidx /
+// pmx it also gives every body its leaf's last index and the mass of the
+// leaf's bodies before it, and out[2] = 1 when two bodies sit closer than
+// 1e-100 in every coordinate (checked for every pair with equal high keys
+// and for neighbours in key order): then a pair other than a body with
+// itself may have r^2 = 0, which the time-step sums skip.  Equal positions
+// always share a key; distinct ones give r^2 = 0 only when every coordinate
+// difference is below 1e-162, which needs coordinates below about 1e-146.  Cells live in
 // global scratch: two level buffers of n cells (a level never holds more
 // cells than bodies), begin / count in ci, centre and radius in cg.
 __device__ __forceinline__ void small_key(const double *x3, int i, const double *X0, double R0,
@@ -589,14 +596,18 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
                                                      const unsigned char *__restrict__ flags_in,
                                                      int n, int ncrit, BodiesDev b,
                                                      unsigned char *__restrict__ lend,
+                                                     int *__restrict__ lidx,
+                                                     double *__restrict__ pmx,
                                                      int *__restrict__ ci, double *__restrict__ cg,
                                                      int *__restrict__ out) {
 #pragma clang fp contract(off)
   __shared__ uint64_t skey[kSmallTree];
   __shared__ unsigned short sidx[kSmallTree];
+  // positions in tree order for the level splits (136 KB of LDS in all)
+  __shared__ double spx[kSmallTree], spy[kSmallTree], spz[kSmallTree];
   __shared__ double wred[16][6];
   __shared__ double sX0[3], sR0;
-  __shared__ int ncur, nnext, nleaf, status;
+  __shared__ int ncur, nnext, nleaf, status, near;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   // ---- bounds (min / max do not depend on the order)
   double mn[3], mx[3];
@@ -641,6 +652,7 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     nnext = 0;
     nleaf = 0;
     status = 0;
+    near = 0;
   }
   __syncthreads();
   const double X0[3] = {sX0[0], sX0[1], sX0[2]};
@@ -677,11 +689,25 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
       }
       __syncthreads();
     }
+  // bodies at (next to) zero distance: every pair of a run of equal high
+  // keys, and neighbours in key order (see near_zero)
+  auto near_zero = [&](int a, int c) {
+    const double dx = x3[3 * (size_t)a] - x3[3 * (size_t)c];
+    const double dy = x3[3 * (size_t)a + 1] - x3[3 * (size_t)c + 1];
+    const double dz = x3[3 * (size_t)a + 2] - x3[3 * (size_t)c + 2];
+    return fabs(dx) < 1e-100 && fabs(dy) < 1e-100 && fabs(dz) < 1e-100;
+  };
+  for (int k = t; k < n - 1; k += 1024)
+    if (near_zero(sidx[k], sidx[k + 1])) near = 1;
+  __syncthreads();  // before the runs below move sidx entries
   // runs of equal high keys: by (low key, index), insertion sort per run
   for (int k = t; k < n - 1; k += 1024) {
     if (skey[k] != skey[k + 1] || (k > 0 && skey[k - 1] == skey[k])) continue;
     int e = k + 1;
     while (e + 1 < n && skey[e + 1] == skey[k]) e++;
+    for (int a = k; a <= e; a++)
+      for (int c = a + 1; c <= e; c++)
+        if (near_zero(sidx[a], sidx[c])) near = 1;
     for (int a = k + 1; a <= e; a++) {
       const unsigned short ia = sidx[a];
       uint64_t hd, la;
@@ -701,9 +727,14 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   // ---- the gather into tree order
   for (int k = t; k < n; k += 1024) {
     const int i = sidx[k];
-    b.x[k] = x3[3 * (size_t)i];
-    b.y[k] = x3[3 * (size_t)i + 1];
-    b.z[k] = x3[3 * (size_t)i + 2];
+    const double px = x3[3 * (size_t)i], py = x3[3 * (size_t)i + 1],
+                 pz = x3[3 * (size_t)i + 2];
+    spx[k] = px;
+    spy[k] = py;
+    spz[k] = pz;
+    b.x[k] = px;
+    b.y[k] = py;
+    b.z[k] = pz;
     b.vx[k] = v3[3 * (size_t)i];
     b.vy[k] = v3[3 * (size_t)i + 1];
     b.vz[k] = v3[3 * (size_t)i + 2];
@@ -746,9 +777,13 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
         r = cur_g[4 * k + 3];
       }
       const bool split = on && nb > ncrit;
-      if (on && !split && j == 0) {
-        lend[begin + nb - 1] = 1;
-        atomicAdd(&nleaf, 1);
+      if (on && !split) {
+        if (j == 0) {
+          lend[begin + nb - 1] = 1;
+          atomicAdd(&nleaf, 1);
+        }
+        if (lidx)
+          for (int k2 = begin + j; k2 < begin + nb; k2 += 8) lidx[k2] = begin + nb - 1;
       }
       const int end = begin + nb;
       const bool small = nb <= 32;
@@ -756,14 +791,14 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
       if (split) {
         if (small) {
           for (int i = begin; i < end; i++) {
-            const int oc = (b.x[i] >= c0) + ((b.y[i] >= c1) << 1) + ((b.z[i] >= c2) << 2);
+            const int oc = (spx[i] >= c0) + ((spy[i] >= c1) << 1) + ((spz[i] >= c2) << 2);
             sz += oc == j;
           }
         } else if (j > 0) {
           int l = begin, rr = end;
           while (l < rr - 1) {
             const int mid = (l + rr) / 2;
-            const int oc = (b.x[mid] >= c0) + ((b.y[mid] >= c1) << 1) + ((b.z[mid] >= c2) << 2);
+            const int oc = (spx[mid] >= c0) + ((spy[mid] >= c1) << 1) + ((spz[mid] >= c2) << 2);
             if (oc >= j)
               rr = mid;
             else
@@ -811,18 +846,61 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     __syncthreads();
   }
   __syncthreads();
+  if (pmx) {
+    // the mass before each body in its leaf: a scan over tree order that
+    // restarts after every leaf end; thread t holds bodies 4t .. 4t+3, its
+    // carry (the mass after the chunk's last leaf end, or the whole chunk's
+    // mass with a flag for "no end") joins the others' by a wave scan, then
+    // the sixteen wave totals
+    double qv[4], cs = 0.0;
+    bool ev[4], cf = false;
+    for (int e = 0; e < 4; e++) {
+      const int k = 4 * t + e;
+      qv[e] = k < n ? b.q[k] : 0.0;
+      ev[e] = k < n && lend[k];
+      cs = ev[e] ? 0.0 : cs + qv[e];
+      cf = cf || ev[e];
+    }
+    // inclusive scan of (cf, cs): (fa, sa) then (fb, sb) -> (fa|fb, fb ? sb : sa+sb)
+    for (int d = 1; d < kWave; d <<= 1) {
+      const double us = __shfl_up(cs, d, kWave);
+      const int uf = __shfl_up((int)cf, d, kWave);
+      if (lane >= d) {
+        cs = cf ? cs : us + cs;
+        cf = cf || uf;
+      }
+    }
+    if (lane == kWave - 1) {
+      wred[w][4] = cs;
+      wred[w][5] = cf ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // exclusive carry into this thread: earlier waves, then earlier lanes
+    double carry = 0.0;
+    for (int u = 0; u < w; u++) carry = wred[u][5] != 0.0 ? wred[u][4] : carry + wred[u][4];
+    const double ls = __shfl_up(cs, 1, kWave);
+    const int lf = __shfl_up((int)cf, 1, kWave);
+    if (lane > 0) carry = lf ? ls : carry + ls;
+    for (int e = 0; e < 4; e++) {
+      const int k = 4 * t + e;
+      if (k < n) pmx[k] = carry;
+      carry = ev[e] ? 0.0 : carry + qv[e];
+    }
+  }
   if (t == 0) {
     out[0] = nleaf;
     out[1] = status;
+    out[2] = near;
   }
 }
 
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
-                      unsigned char *lend, int *ci, double *cg, int *out, hipStream_t s) {
+                      unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
+                      hipStream_t s) {
   if (n < 1 || n > kSmallTree) return -1;
   hipLaunchKernelGGL(k_small_tree, dim3(1), dim3(1024), 0, s, x3, v3, q, flags, n, ncrit, b,
-                     lend, ci, cg, out);
+                     lend, lidx, pmx, ci, cg, out);
   return 0;
 }
 

@@ -1,8 +1,11 @@
 """The one-launch tree of the master time-step sums (tc_set_small_tree):
 for systems up to 4096 bodies it must give the tree of the level-by-level
-build -- the same body order and the same leaves -- so the potentials,
-accelerations and time-step sums are bit for bit those of the level-by-level
-build.  Cases cover one body, ncrit 1..300, sink/source flags, the scalar
+build -- the same body order and the same leaves.  With it the time-step
+sum runs in uniform slabs that cut leaves (running minima and leaf masses
+joined across slabs), the level-built tree in slabs aligned to leaves; the
+two differ only in the order of the sums, so the potentials, accelerations
+and time-step sums agree to 1e-12 relative (a different leaf partition moves
+a time-step sum by far more).  Cases cover one body, ncrit 1..300, sink/source flags, the scalar
 (pair-mass) form, coincident bodies and bodies closer than 2^-21 of the box
 (equal high keys, ordered by the low key and the input index), and a system
 whose tree would need more than 64 levels (the same error from both).  The
@@ -70,8 +73,11 @@ def test_small_tree_equals_level_build(torch_cuda, kind, n, ncrit, flags, pair_m
     b = _eval(torch_cuda, tca, x, v, m, fl, ncrit, False, mode)
     print("%s n=%d ncrit=%d: leaves^2 %d / %d" % (kind, n, ncrit, a[3], b[3]))
     assert a[3] == b[3]  # the same number of leaves
-    for u, w in zip(a[:3], b[:3]):
-        assert np.array_equal(u, w)
+    assert np.allclose(a[0], b[0], rtol=1e-12, atol=0)  # potentials: positive terms
+    da = np.linalg.norm(a[1] - b[1], axis=1)
+    na = np.linalg.norm(b[1], axis=1)
+    assert np.all(da <= 1e-12 * na + 1e-13 * na.max())
+    assert np.allclose(a[2], b[2], rtol=1e-12, atol=0)  # time-step sums
 
 
 def test_small_tree_too_deep(torch_cuda):
