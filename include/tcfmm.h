@@ -392,7 +392,10 @@ int tc_master_P2P(tc_context *ctx, tc_master_body *bodies, int nbodies, void *ce
  * particle (exafmm.h:73-86) that the loop reads or writes; the arrays keep
  * their order (the reference's split swaps records; identity is the index).
  * timestep, acc, pot, acc_old are outputs (as kick_naive leaves them); pos_e
- * holds the compensated-summation error terms and must start at 0. */
+ * holds the compensated-summation error terms and must start at 0.
+ * On an error return the particle arrays may be partly updated: the time-step
+ * trees of subsystems of up to 4096 bodies report "tree deeper than 64
+ * levels" at the loop's next split, not before the kick that follows. */
 typedef struct tc_particles {
   int n;
   double *pos;       /* [n][3] */

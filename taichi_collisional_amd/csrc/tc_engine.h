@@ -148,6 +148,11 @@ struct tc_context {
   // launch (launch_small_tree; 0: always the multi-kernel build)
   int small_tree_max = kSmallTree;
   DBuf small_ci, small_cg, small_out, small_lidx, small_pmx;
+  // the step loop's calls do not wait for the small tree's status: a tree
+  // too deep sets small_sticky (one int on the device), which the loop reads
+  // with its next split
+  bool defer_small_check = false;
+  DBuf small_sticky;
   DBuf mdpart;  // partial sums of the source slabs of the master direct sum
   // [0..8] phase boundaries; [9], [10] start and end of P2P on near_stream;
   // [11] the tree's leaf ids and body counts are in h_leaves, h_nbd; [12],

@@ -113,6 +113,36 @@ __global__ void k_naive_results(const int *__restrict__ is, int ns, const double
   tmp[k] = 1.0 / sqrt(sqrt(tsum[k]));
 }
 
+// k_naive_results, k_min and k_set_timestep in one workgroup, for
+// subsystems of at most kNaiveSmall bodies: the same writes and the same
+// minimum (fmin is exact and ignores NaN in any order).
+constexpr int kNaiveSmall = 16384;
+__global__ __launch_bounds__(1024) void k_naive_small(const int *__restrict__ is, int ns,
+                                                      const double *__restrict__ F,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ tsum,
+                                                      const double *__restrict__ ao,
+                                                      double *__restrict__ acc,
+                                                      double *__restrict__ pot,
+                                                      double *__restrict__ acc_old,
+                                                      double *__restrict__ ts) {
+  __shared__ double red[16];
+  double v = INFINITY;
+  for (int k = threadIdx.x; k < ns; k += 1024) {
+    const int i = is[k];
+    pot[i] = p[k];
+    for (int d = 0; d < 3; d++) acc[3 * i + d] = F[3 * k + d];
+    acc_old[i] = ao ? ao[k] : 0.0;
+    v = fmin(v, 1.0 / sqrt(sqrt(tsum[k])));
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < 16; w++) r = fmin(r, red[w]);
+  for (int k = threadIdx.x; k < ns; k += 1024) ts[is[k]] = r;
+}
+
 __global__ void k_set_timestep(const int *__restrict__ is, int ns, const double *__restrict__ mn,
                                double *__restrict__ ts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -174,10 +204,11 @@ __global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *_
 // (*nfast) and the same stable partition as k_split_flags + scan +
 // k_split_scatter.
 constexpr int kSplitSmall = 4096;
+// nfast[1] = *sticky (the step loop's small-tree status, 0 without one).
 __global__ __launch_bounds__(1024) void k_split_small(const int *__restrict__ is, int ns,
                                                       const double *__restrict__ ts, double dt,
-                                                      int *__restrict__ out,
-                                                      int *__restrict__ nfast) {
+                                                      int *__restrict__ out, int *__restrict__ nfast,
+                                                      const int *sticky) {
   __shared__ int wsum[16];
   constexpr int PER = kSplitSmall / 1024;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
@@ -205,7 +236,10 @@ __global__ __launch_bounds__(1024) void k_split_small(const int *__restrict__ is
     if (k < ns) out[f[q] ? run : total + (k - run)] = idx[q];
     run += f[q];
   }
-  if (t == 0) *nfast = total;
+  if (t == 0) {
+    nfast[0] = total;
+    nfast[1] = sticky ? *sticky : 0;
+  }
 }
 
 // time-step sums of zero: kick_naive's 1/sqrt(sqrt(0)) gives every body +inf
@@ -266,6 +300,10 @@ class Evolver {
       &scan_tmp_;
 
   int ensure_union(int nu);
+  int small_tree_error() {
+    ctx_->err = "tree deeper than 64 levels (more than ncrit coincident bodies?)";
+    return -1;
+  }
   // exact: drop TC_MASTER_FMM, so that the sum is the all-pairs one
   int force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_fac, bool steps,
             bool with_acc_old, bool exact = false, KickSpec *kick = nullptr);
@@ -413,6 +451,12 @@ int Evolver::kick_naive(Sub s) {
   hipStream_t st = ctx->stream;
   double *tmp = tmp_.as<double>();
   const double *ao = (mode_ & TC_MASTER_FMM) ? uao_.as<double>() : nullptr;
+  if (s.n <= kNaiveSmall) {
+    hipLaunchKernelGGL(k_naive_small, dim3(1), dim3(1024), 0, st, s.idx, s.n, uacc_.as<double>(),
+                       upot_.as<double>(), uts_.as<double>(), ao, p_.acc, p_.pot, p_.acc_old,
+                       p_.timestep);
+    return 0;
+  }
   hipLaunchKernelGGL(k_naive_results, grid_for(s.n), dim3(256), 0, st, s.idx, s.n,
                      uacc_.as<double>(), upot_.as<double>(), uts_.as<double>(), ao, p_.acc,
                      p_.pot, p_.acc_old, tmp);
@@ -443,11 +487,13 @@ int Evolver::split(int level, Sub s, double dt, Sub &slow, Sub &fast) {
   int nfast = 0;
   if (s.n <= kSplitSmall) {
     hipLaunchKernelGGL(k_split_small, dim3(1), dim3(1024), 0, st, s.idx, s.n, p_.timestep,
-                       std::fabs(dt), out.as<int>(), flags_.as<int>());
-    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, flags_.as<int>(), sizeof(int), hipMemcpyDeviceToHost,
-                            st));
+                       std::fabs(dt), out.as<int>(), flags_.as<int>(),
+                       ctx->small_sticky.as<int>());
+    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, flags_.as<int>(), 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, st));
     TC_CHECK(spin_wait(ctx, st));
     nfast = ctx->h_pin[12];
+    if (ctx->h_pin[13] == 2) return small_tree_error();
   } else {
     hipLaunchKernelGGL(k_split_flags, grid_for(s.n + 1), dim3(256), 0, st, s.idx, s.n,
                        p_.timestep, std::fabs(dt), flags_.as<int>());
@@ -526,10 +572,23 @@ int Evolver::run(double *simtime, double dt) {
     ctx->ev_all_n = n;
   }
   const Sub all{all_.as<int>(), n};
+  // the time-step sums' one-launch trees report a tree too deep through
+  // small_sticky, read with every split of up to kSplitSmall bodies and at
+  // the end, instead of a wait after each tree
+  TC_CHECK(ctx->small_sticky.ensure(sizeof(int)));
+  TC_CHECK(hipMemsetAsync(ctx->small_sticky.p, 0, sizeof(int), ctx->stream));
+  struct Defer {
+    tc_context *c;
+    ~Defer() { c->defer_small_check = false; }
+  } defer{ctx};
+  ctx->defer_small_check = true;
   if (int e = kick_naive(all)) return e;
   if (int e = omelyan(0, all, simtime_, simtime_ + dt, dt, true)) return e;
+  TC_CHECK(hipMemcpyAsync(ctx->h_pin + 13, ctx->small_sticky.p, sizeof(int),
+                          hipMemcpyDeviceToHost, ctx->stream));
   TC_CHECK(hipStreamSynchronize(ctx->stream));
   TC_CHECK(hipGetLastError());
+  if (ctx->h_pin[13] == 2) return small_tree_error();
   *simtime = simtime_;
   return 0;
 }

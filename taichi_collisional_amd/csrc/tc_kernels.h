@@ -107,13 +107,14 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
 // for the master time-step sums: bodies gathered into tree order, lend set on
 // the last body of every leaf; out[0] = leaves, out[1] = 0 or 2 (too deep),
 // out[2] = 1 when two bodies are (next to) coincident; lidx / pmx (or null):
-// each body's leaf's last index and the mass before it in its leaf.
+// each body's leaf's last index and the mass before it in its leaf; sticky
+// (or null) is set to 2 with out[1] = 2 and otherwise left as it is.
 // ci: 4 n ints, cg: 8 n doubles of scratch.
 constexpr int kSmallTree = 4096;
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      hipStream_t s);
+                      int *sticky, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 

@@ -558,36 +558,65 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
 //  * bounds without a root shift (k_bounds_final with zero shift values);
 //  * 28-bit-per-axis keys (k_morton); the order of the stable radix sort on
 //    (high, low) key is that of (high key, low key, input index): a bitonic
-//    sort in LDS on (high key, index), then each run of equal high keys
-//    (bodies closer than 2^-21 of the box; rare) reordered by (low key,
-//    index);
+//    sort in registers of one 64-bit value per body, the high key's top 52
+//    bits (its cell at level 17) above the input index, then each run of
+//    equal 52-bit prefixes (bodies closer than 2^-17 of the box; rare)
+//    reordered by (high key, low key, index);
 //  * the gather (k_gather);
 //  * levels split as k_split does (eight lanes per cell: a count per octant up
 //    to 32 bodies, else the search for the start of octants >= j that never
-//    tests the cell's first body), children placed as k_children does.
+//    tests the cell's first body), children placed as k_children does, with
+//    the positions read from LDS.
 // It marks the last body of every leaf in lend (cleared first) and counts
 // the leaves: out[0] = leaves, out[1] = 0, or 2 when the tree would need more
 // than kMaxTreeLevels levels (the multi-kernel build's error).  With lidx /
 // pmx it also gives every body its leaf's last index and the mass of the
 // leaf's bodies before it, and out[2] = 1 when two bodies sit closer than
-// 1e-100 in every coordinate (checked for every pair with equal high keys
-// and for neighbours in key order): then a pair other than a body with
-// itself may have r^2 = 0, which the time-step sums skip.  Equal positions
-// always share a key; distinct ones give r^2 = 0 only when every coordinate
-// difference is below 1e-162, which needs coordinates below about 1e-146.  Cells live in
-// global scratch: two level buffers of n cells (a level never holds more
-// cells than bodies), begin / count in ci, centre and radius in cg.
-__device__ __forceinline__ void small_key(const double *x3, int i, const double *X0, double R0,
-                                          uint64_t &hi, uint64_t &lo) {
+// 1e-100 in every coordinate (checked for every pair within a run of equal
+// prefixes and for neighbours in tree order): then a pair other than a body
+// with itself may have r^2 = 0, which the time-step sums skip.  Equal
+// positions always share a key; distinct ones give r^2 = 0 only when every
+// coordinate difference is below 1e-162, which needs coordinates below about
+// 1e-146.  Cells live in global scratch: two level buffers of n cells (a
+// level never holds more cells than bodies), begin / count in ci, centre and
+// radius in cg.
+__device__ __forceinline__ void small_key_of(const double *p, const double *X0, double R0,
+                                             uint64_t &hi, uint64_t &lo) {
 #pragma clang fp contract(off)
   const double maxint = (double)(1u << 28);
   unsigned int M[3];
   for (int d = 0; d < 3; d++) {
-    const double f = (x3[3 * (size_t)i + d] - (X0[d] - R0)) / (2.0 * R0);
+    const double f = (p[d] - (X0[d] - R0)) / (2.0 * R0);
     M[d] = (unsigned int)floor(f * maxint);
   }
   hi = (spread3(M[2] >> 7) << 2) | (spread3(M[1] >> 7) << 1) | spread3(M[0] >> 7);
   lo = (spread3(M[2] & 0x7f) << 2) | (spread3(M[1] & 0x7f) << 1) | spread3(M[0] & 0x7f);
+}
+
+__device__ __forceinline__ void small_key(const double *x3, int i, const double *X0, double R0,
+                                          uint64_t &hi, uint64_t &lo) {
+  const double p[3] = {x3[3 * (size_t)i], x3[3 * (size_t)i + 1], x3[3 * (size_t)i + 2]};
+  small_key_of(p, X0, R0, hi, lo);
+}
+
+// One compare-exchange step of the bitonic network for element e holding k
+// against its partner e ^ stride holding pk: in blocks of `size` that sort
+// ascending ((e & size) == 0) the lower index keeps the smaller value,
+// descending blocks the reverse.
+__device__ __forceinline__ void bitonic_take(int e, int size, int stride, uint64_t pk,
+                                             uint64_t &k) {
+  const bool up = (e & size) == 0, lower = (e & stride) == 0;
+  if ((pk < k) == (lower == up)) k = pk;
+}
+
+// the same for two elements a < c of one thread
+__device__ __forceinline__ void bitonic_pair(int ea, int size, uint64_t &ka, uint64_t &kc) {
+  const bool up = (ea & size) == 0;
+  if ((ka > kc) == up) {
+    const uint64_t tk = ka;
+    ka = kc;
+    kc = tk;
+  }
 }
 
 __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ x3,
@@ -599,7 +628,7 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
                                                      int *__restrict__ lidx,
                                                      double *__restrict__ pmx,
                                                      int *__restrict__ ci, double *__restrict__ cg,
-                                                     int *__restrict__ out) {
+                                                     int *__restrict__ out, int *sticky) {
 #pragma clang fp contract(off)
   __shared__ uint64_t skey[kSmallTree];
   __shared__ unsigned short sidx[kSmallTree];
@@ -609,15 +638,21 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   __shared__ double sX0[3], sR0;
   __shared__ int ncur, nnext, nleaf, status, near;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  // ---- bounds (min / max do not depend on the order)
+  // ---- bounds (min / max do not depend on the order); thread t holds
+  // bodies 4t .. 4t+3 from here to the end of the sort
+  double px[4][3];
   double mn[3], mx[3];
   for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[d];
-  for (int i = t; i < n; i += 1024)
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int i = 4 * t + j;
+#pragma unroll
     for (int d = 0; d < 3; d++) {
-      const double v = x3[3 * (size_t)i + d];
-      mn[d] = fmin(v, mn[d]);
-      mx[d] = fmax(v, mx[d]);
+      px[j][d] = i < n ? x3[3 * (size_t)i + d] : x3[d];
+      mn[d] = fmin(px[j][d], mn[d]);
+      mx[d] = fmax(px[j][d], mx[d]);
     }
+  }
   for (int o = 32; o > 0; o >>= 1)
     for (int d = 0; d < 3; d++) {
       mn[d] = fmin(mn[d], __shfl_xor(mn[d], o, kWave));
@@ -657,66 +692,69 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   __syncthreads();
   const double X0[3] = {sX0[0], sX0[1], sX0[2]};
   const double R0 = sR0;
-  // ---- keys and the bitonic sort on (high key, index)
-  int n2 = 1;
+  // ---- keys and the bitonic sort, in registers, on one 64-bit value per
+  // body: the high key's top 52 bits (the cell at level 17) above the
+  // 12-bit input index.  The partner e ^ stride sits in the same thread
+  // (stride < 4), in another lane of the wave (stride < 256: a lane
+  // shuffle) or in another wave (through LDS); padding (~0) sorts last.
+  int n2 = 4;
   while (n2 < n) n2 <<= 1;
-  for (int i = t; i < n2; i += 1024) {
-    if (i < n) {
-      uint64_t hi, lo;
-      small_key(x3, i, X0, R0, hi, lo);
-      skey[i] = hi;
-      sidx[i] = (unsigned short)i;
-    } else {
-      skey[i] = ~0ull;
-      sidx[i] = 0xffff;
-    }
+  uint64_t kk[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int i = 4 * t + j;
+    uint64_t hi = 0, lo;
+    if (i < n) small_key_of(px[j], X0, R0, hi, lo);
+    kk[j] = i < n ? ((hi >> 11) << 12) | (uint64_t)i : ~0ull;
   }
-  __syncthreads();
   for (int size = 2; size <= n2; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int u = t; u < n2 / 2; u += 1024) {
-        const int a = 2 * u - (u & (stride - 1)), c = a + stride;
-        const bool up = (a & size) == 0;
-        const uint64_t ka = skey[a], kc = skey[c];
-        const unsigned short ia = sidx[a], ic = sidx[c];
-        const bool gt = ka > kc || (ka == kc && ia > ic);
-        if (gt == up) {
-          skey[a] = kc;
-          skey[c] = ka;
-          sidx[a] = ic;
-          sidx[c] = ia;
+      if (stride >= 4 * kWave) {
+        __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int j = 0; j < 4; j++) skey[4 * t + j] = kk[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int e = 4 * t + j;
+          bitonic_take(e, size, stride, skey[e ^ stride], kk[j]);
         }
+      } else if (stride >= 4) {
+        const int m = stride >> 2;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          bitonic_take(4 * t + j, size, stride, __shfl_xor(kk[j], m, kWave), kk[j]);
+      } else if (stride == 2) {
+        bitonic_pair(4 * t, size, kk[0], kk[2]);
+        bitonic_pair(4 * t + 1, size, kk[1], kk[3]);
+      } else {
+        bitonic_pair(4 * t, size, kk[0], kk[1]);
+        bitonic_pair(4 * t + 2, size, kk[2], kk[3]);
       }
-      __syncthreads();
     }
-  // bodies at (next to) zero distance: every pair of a run of equal high
-  // keys, and neighbours in key order (see near_zero)
-  auto near_zero = [&](int a, int c) {
-    const double dx = x3[3 * (size_t)a] - x3[3 * (size_t)c];
-    const double dy = x3[3 * (size_t)a + 1] - x3[3 * (size_t)c + 1];
-    const double dz = x3[3 * (size_t)a + 2] - x3[3 * (size_t)c + 2];
-    return fabs(dx) < 1e-100 && fabs(dy) < 1e-100 && fabs(dz) < 1e-100;
-  };
-  for (int k = t; k < n - 1; k += 1024)
-    if (near_zero(sidx[k], sidx[k + 1])) near = 1;
-  __syncthreads();  // before the runs below move sidx entries
-  // runs of equal high keys: by (low key, index), insertion sort per run
+  __syncthreads();
+  // skey: the 52-bit prefix, sidx: the input index
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    skey[4 * t + j] = kk[j] >> 12;
+    sidx[4 * t + j] = (unsigned short)(kk[j] & 0xfff);
+  }
+  __syncthreads();
+  // runs of equal prefixes (bodies in one cell of 2^-17 of the box; rare):
+  // by (high key, low key, index), insertion sort per run
   for (int k = t; k < n - 1; k += 1024) {
     if (skey[k] != skey[k + 1] || (k > 0 && skey[k - 1] == skey[k])) continue;
     int e = k + 1;
     while (e + 1 < n && skey[e + 1] == skey[k]) e++;
-    for (int a = k; a <= e; a++)
-      for (int c = a + 1; c <= e; c++)
-        if (near_zero(sidx[a], sidx[c])) near = 1;
     for (int a = k + 1; a <= e; a++) {
       const unsigned short ia = sidx[a];
-      uint64_t hd, la;
-      small_key(x3, ia, X0, R0, hd, la);
+      uint64_t ha, la;
+      small_key(x3, ia, X0, R0, ha, la);
       int c = a - 1;
       while (c >= k) {
-        uint64_t lc;
-        small_key(x3, sidx[c], X0, R0, hd, lc);
-        if (lc < la || (lc == la && sidx[c] < ia)) break;
+        uint64_t hc, lc;
+        small_key(x3, sidx[c], X0, R0, hc, lc);
+        if (hc < ha || (hc == ha && (lc < la || (lc == la && sidx[c] < ia)))) break;
         sidx[c + 1] = sidx[c];
         c--;
       }
@@ -724,25 +762,66 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     }
   }
   __syncthreads();
-  // ---- the gather into tree order
-  for (int k = t; k < n; k += 1024) {
-    const int i = sidx[k];
-    const double px = x3[3 * (size_t)i], py = x3[3 * (size_t)i + 1],
-                 pz = x3[3 * (size_t)i + 2];
-    spx[k] = px;
-    spy[k] = py;
-    spz[k] = pz;
-    b.x[k] = px;
-    b.y[k] = py;
-    b.z[k] = pz;
-    b.vx[k] = v3[3 * (size_t)i];
-    b.vy[k] = v3[3 * (size_t)i + 1];
-    b.vz[k] = v3[3 * (size_t)i + 2];
-    b.q[k] = q[i];
-    b.flags[k] = flags_in ? flags_in[i] : (unsigned char)(kSink | kSource);
-    b.order[k] = i;
-    b.acc_old[k] = 0.0;
-    lend[k] = 0;
+  // ---- the gather into tree order (every load of a thread's four bodies
+  // issued before its stores)
+  {
+    int src[4];
+    double gx[4][3], gv[4][3], gq[4];
+    unsigned char gf[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int k = t + 1024 * r;
+      src[r] = k < n ? sidx[k] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = src[r];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        gx[r][d] = x3[3 * (size_t)i + d];
+        gv[r][d] = v3[3 * (size_t)i + d];
+      }
+      gq[r] = q[i];
+      gf[r] = flags_in ? flags_in[i] : (unsigned char)(kSink | kSource);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int k = t + 1024 * r;
+      if (k >= n) continue;
+      spx[k] = gx[r][0];
+      spy[k] = gx[r][1];
+      spz[k] = gx[r][2];
+      b.x[k] = gx[r][0];
+      b.y[k] = gx[r][1];
+      b.z[k] = gx[r][2];
+      b.vx[k] = gv[r][0];
+      b.vy[k] = gv[r][1];
+      b.vz[k] = gv[r][2];
+      b.q[k] = gq[r];
+      b.flags[k] = gf[r];
+      b.order[k] = src[r];
+      b.acc_old[k] = 0.0;
+      lend[k] = 0;
+    }
+  }
+  __syncthreads();
+  // bodies at (next to) zero distance: neighbours in tree order, and every
+  // pair within a run of equal prefixes
+  {
+    auto near_zero = [&](int a, int c) {
+      return fabs(spx[a] - spx[c]) < 1e-100 && fabs(spy[a] - spy[c]) < 1e-100 &&
+             fabs(spz[a] - spz[c]) < 1e-100;
+    };
+    bool nz = false;
+    for (int k = t; k < n - 1; k += 1024) {
+      nz = nz || near_zero(k, k + 1);
+      if (skey[k] != skey[k + 1] || (k > 0 && skey[k - 1] == skey[k])) continue;
+      int e = k + 1;
+      while (e + 1 < n && skey[e + 1] == skey[k]) e++;
+      for (int a = k; a <= e; a++)
+        for (int c = a + 1; c <= e; c++) nz = nz || near_zero(a, c);
+    }
+    if (nz) near = 1;
   }
   // root
   if (t == 0) {
@@ -891,16 +970,17 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     out[0] = nleaf;
     out[1] = status;
     out[2] = near;
+    if (sticky && status == 2) *sticky = 2;
   }
 }
 
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      hipStream_t s) {
+                      int *sticky, hipStream_t s) {
   if (n < 1 || n > kSmallTree) return -1;
   hipLaunchKernelGGL(k_small_tree, dim3(1), dim3(1024), 0, s, x3, v3, q, flags, n, ncrit, b,
-                     lend, lidx, pmx, ci, cg, out);
+                     lend, lidx, pmx, ci, cg, out, sticky);
   return 0;
 }
 

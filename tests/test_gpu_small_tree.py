@@ -92,3 +92,27 @@ def test_small_tree_too_deep(torch_cuda):
             _eval(torch_cuda, tca, x, v, m, None, 4, small)
         msgs.append(str(e.value).split(": ", 1)[-1])
     assert msgs[0] == msgs[1] and "64 levels" in msgs[0]
+
+
+def test_small_tree_too_deep_in_step_loop(torch_cuda):
+    """In the step loop the one-launch tree's status is read at the next
+    split instead of after each tree: nine coincident bodies with ncrit 4
+    still end the step with the too-deep error, as with the level-by-level
+    build (small trees off)."""
+    import taichi_collisional_amd as tca
+    torch = torch_cuda
+    x, v, m = _case("plummer", 200, seed=5)
+    x[:9] = x[0]
+    msgs = []
+    for small in (True, False):
+        st = {k: torch.zeros(200, 3, dtype=torch.float64, device="cuda") for k in ("pos_e", "acc")}
+        st.update({k: torch.zeros(200, dtype=torch.float64, device="cuda")
+                   for k in ("timestep", "pot", "acc_old")})
+        st["pos"], st["vel"], st["mass"] = (torch.from_numpy(a).cuda() for a in (x, v, m))
+        f = tca.Fmm(p=8, ncrit=4, direct_below=0)
+        f.set_small_tree(4096 if small else 0)
+        with pytest.raises(RuntimeError) as e:
+            f.master_evolve(st, 1.0 / 1024)
+        f.close()
+        msgs.append(str(e.value).split(": ", 1)[-1])
+    assert msgs[0] == msgs[1] and "64 levels" in msgs[0]
