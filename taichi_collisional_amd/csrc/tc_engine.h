@@ -130,6 +130,7 @@ struct tc_context {
   int trav_steps[4] = {0, 0, 0, 0};  // steps of the last walk per MAC mode (batch size)
   DBuf gsort1[4], gsort2[4];  // list grouping scratch per stream: keys in, out, sources, temp
   int *h_pin = nullptr;  // small pinned host buffer for per-step counts (256 ints)
+  int *h_pin_dev = nullptr;  // the same memory as kernels address it
   DBuf m2l_cnt, m2l_off, m2l_cur, m2l_src, m2l_coff, m2l_chunks;
   DBuf p2p_cnt, p2p_off, p2p_cur, p2p_src;
   // error-controlled MAC: p = 2 pre-pass, multipole norms, acc_old output
@@ -205,20 +206,26 @@ struct tc_context {
 // The master variant's get_force_and_potential on device arrays (x, v [n][3],
 // m, flags): pot, acc [n][3], tstep (TC_MASTER_STEPS), acc_old (in/out, the
 // TC_MASTER_FMM pre-pass).  tc_engine.hip; used by the step loop too.
+// With ix (the step loop's time-step sums), body i is record ix[i] of x, v,
+// m, flags; only for the one-launch small tree (error otherwise).
 int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        const double *m, const unsigned char *flags, int mode, double dt,
-                       double *pot, double *acc, double *tstep, double *acc_old);
+                       double *pot, double *acc, double *tstep, double *acc_old,
+                       const int *ix = nullptr);
 // The master default build's all-pairs force without time steps, queued on
 // the context's stream with no wait: the bodies must already be in the
 // context's body arrays (order = identity) with the leaf-end flags zeroed.
 // master_direct_prepare sizes those arrays for n bodies, of which the first
 // ns are the sinks, first.
 int master_direct_prepare(tc_context *ctx, int n, int ns);
-// The same with the step loop's kick applied in the sum, summed for the
-// first ns bodies only (false: nothing launched, the caller queues
-// master_direct_queue and its own kick).
-bool master_direct_queue_kick(tc_context *ctx, int n, const int *is, int ns, double dt,
-                              bool gradient, bool set_pot, double *vel, double *pot);
+// The same sum for the step loop's kicks, read straight from the particle
+// arrays (the union of sinks is and sources js, no gather into the body
+// arrays), with the kick applied in the sum (false: nothing launched, the
+// caller gathers, queues master_direct_queue and its own kick).
+bool master_direct_queue_kick(tc_context *ctx, const int *is, int ns, const int *js, int nj,
+                              bool sinks_are_sources, const double *pos, const double *acc,
+                              double shift_fac, const double *mass, double dt, bool gradient,
+                              bool set_pot, double *vel, double *pot);
 void master_direct_queue(tc_context *ctx, int n, double *pot, double *acc);
 
 // The jerk build's get_force_and_potential on device arrays (tc_eval_soa);

@@ -1014,7 +1014,8 @@ static int master_fmm_device(tc_context *ctx, int n, bool steps, bool pair_mass,
 // root shift (master build_tree.h), so the shift is zero here.
 int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                        const double *m, const unsigned char *flags, int mode, double dt,
-                       double *pot, double *acc, double *tstep, double *acc_old) {
+                       double *pot, double *acc, double *tstep, double *acc_old,
+                       const int *ix) {
   hipStream_t s = ctx->stream;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   std::memset(ctx->m2l_by_order, 0, sizeof(ctx->m2l_by_order));
@@ -1038,6 +1039,10 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
   // the -DFMM build runs direct() below ncrit bodies (integrator.h:462-465)
   const bool fmm = (mode & TC_MASTER_FMM) && n >= ctx->cfg.ncrit;
   const bool small = steps && !fmm && n <= std::min(ctx->small_tree_max, kSmallTree);
+  if (ix && !small) {
+    ctx->err = "an index list needs the one-launch tree (time steps, n <= its limit)";
+    return -1;
+  }
   if (steps || fmm) {
     // the master tree has no root shift (the one-launch tree takes none)
     const double zero[3] = {0, 0, 0};
@@ -1067,7 +1072,7 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
                       ctx->mlend.as<unsigned char>(), ctx->small_lidx.as<int>(),
                       ctx->small_pmx.as<double>(), ctx->small_ci.as<int>(),
                       ctx->small_cg.as<double>(), ctx->small_out.as<int>(),
-                      ctx->defer_small_check ? ctx->small_sticky.as<int>() : nullptr, s);
+                      ctx->defer_small_check ? ctx->small_sticky.as<int>() : nullptr, ix, s);
   } else {
     if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
     if (fmm) return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
@@ -1139,11 +1144,14 @@ int master_direct_prepare(tc_context *ctx, int n, int ns) {
   return 0;
 }
 
-bool master_direct_queue_kick(tc_context *ctx, int n, const int *is, int ns, double dt,
-                              bool gradient, bool set_pot, double *vel, double *pot) {
-  return launch_master_direct_kick(bodies_of(ctx), n, master_direct_slabs(ns, n),
-                                   ctx->mdpart.as<double>(), is, ns, dt, gradient, set_pot, vel,
-                                   pot, ctx->stream) == 0;
+bool master_direct_queue_kick(tc_context *ctx, const int *is, int ns, const int *js, int nj,
+                              bool sinks_are_sources, const double *pos, const double *acc,
+                              double shift_fac, const double *mass, double dt, bool gradient,
+                              bool set_pot, double *vel, double *pot) {
+  const int n = ns + nj;
+  return launch_master_direct_kick(is, ns, js, nj, sinks_are_sources, pos, acc, shift_fac, mass,
+                                   master_direct_slabs(ns, n), ctx->mdpart.as<double>(), dt,
+                                   gradient, set_pot, vel, pot, ctx->stream) == 0;
 }
 
 void master_direct_queue(tc_context *ctx, int n, double *pot, double *acc) {
@@ -1195,7 +1203,8 @@ int tc_create(const tc_config *cfg, void *stream, tc_context **out) {
     tc_destroy(ctx);
     return -3;
   }
-  if (hipHostMalloc((void **)&ctx->h_pin, 256 * sizeof(int)) != hipSuccess) {
+  if (hipHostMalloc((void **)&ctx->h_pin, 256 * sizeof(int)) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&ctx->h_pin_dev, ctx->h_pin, 0) != hipSuccess) {
     tc_destroy(ctx);
     return -3;
   }

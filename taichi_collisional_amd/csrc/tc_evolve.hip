@@ -96,6 +96,26 @@ __global__ void k_apply_kick(const int *__restrict__ is, int ns, const double *_
   if (set_pot) pot[i] = p ? p[k] : 0.0;
 }
 
+// Two kicks of zero force in a row on the same sinks (their sources empty):
+// vel += 0 * dt0 (* 2/3 for a gradient kick), then the same with dt1, and
+// pot = 0 with set_pot -- what two k_apply_kick launches with F = p = null
+// do, in one launch.
+__global__ void k_zero_kicks(const int *__restrict__ is, int ns, double dt0, bool gradient0,
+                             double dt1, bool gradient1, bool set_pot, double *__restrict__ vel,
+                             double *__restrict__ pot) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const int i = is[k];
+  const double f = 0.0;
+  for (int d = 0; d < 3; d++) {
+    double v = vel[3 * i + d];
+    v += gradient0 ? f * dt0 * 2.0 / 3.0 : f * dt0;
+    v += gradient1 ? f * dt1 * 2.0 / 3.0 : f * dt1;
+    vel[3 * i + d] = v;
+  }
+  if (set_pot) pot[i] = 0.0;
+}
+
 // kick_naive's bookkeeping (integrator.h:250-300) for the subsystem's bodies:
 // pot, acc, acc_old copied; the time step 1/sqrt(sqrt(sum)) into tmp for the
 // minimum; vel += F * dt with dt = 0 is skipped (it changes nothing finite).
@@ -204,7 +224,9 @@ __global__ void k_split_scatter(const int *__restrict__ is, int ns, const int *_
 // (*nfast) and the same stable partition as k_split_flags + scan +
 // k_split_scatter.
 constexpr int kSplitSmall = 4096;
-// nfast[1] = *sticky (the step loop's small-tree status, 0 without one).
+// nfast[1] = *sticky (the step loop's small-tree status, 0 without one);
+// nfast is the context's pinned host buffer, written straight from the
+// kernel (the host reads it once the stream has passed the kernel).
 __global__ __launch_bounds__(1024) void k_split_small(const int *__restrict__ is, int ns,
                                                       const double *__restrict__ ts, double dt,
                                                       int *__restrict__ out, int *__restrict__ nfast,
@@ -362,16 +384,26 @@ int Evolver::force(Sub sinks, Sub sources, bool sinks_are_sources, double shift_
     // the default build's all-pairs force: gathered straight into the body
     // arrays and queued with no wait (the loop reads nothing back from it)
     if (int e = master_direct_prepare(ctx, nu, sinks.n)) return e;
-    hipLaunchKernelGGL(k_gather_union_bodies, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
-                       sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass,
-                       p_.acc, shift_fac, bodies_of(ctx), ctx->mlend.as<unsigned char>());
-    if (kick && master_direct_queue_kick(ctx, nu, sinks.idx, sinks.n, kick->dt, kick->gradient,
-                                         kick->set_pot, p_.vel, p_.pot)) {
+    if (kick && master_direct_queue_kick(ctx, sinks.idx, sinks.n, sources.idx, sources.n,
+                                         sinks_are_sources, p_.pos, p_.acc, shift_fac, p_.mass,
+                                         kick->dt, kick->gradient, kick->set_pot, p_.vel,
+                                         p_.pot)) {
       kick->done = true;
       return 0;
     }
+    hipLaunchKernelGGL(k_gather_union_bodies, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
+                       sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass,
+                       p_.acc, shift_fac, bodies_of(ctx), ctx->mlend.as<unsigned char>());
     master_direct_queue(ctx, nu, upot_.as<double>(), uacc_.as<double>());
     return 0;
+  }
+  if (steps && !(mode & TC_MASTER_FMM) && !jerk_fmm_ && sources.n == 0 && sinks_are_sources &&
+      shift_fac == 0.0 && nu <= std::min(ctx->small_tree_max, kSmallTree)) {
+    // kick_naive's time-step sums on a small subsystem: the one-launch tree
+    // reads the particles through the index list (no union gather)
+    return master_eval_device(ctx, nu, p_.pos, p_.vel, p_.mass, nullptr,
+                              mode | TC_MASTER_STEPS, dtp_, upot_.as<double>(),
+                              uacc_.as<double>(), uts_.as<double>(), nullptr, sinks.idx);
   }
   hipLaunchKernelGGL(k_gather_union, grid_for(nu), dim3(256), 0, s, sinks.idx, sinks.n,
                      sources.idx, sources.n, sinks_are_sources, p_.pos, p_.vel, p_.mass, p_.acc,
@@ -487,10 +519,8 @@ int Evolver::split(int level, Sub s, double dt, Sub &slow, Sub &fast) {
   int nfast = 0;
   if (s.n <= kSplitSmall) {
     hipLaunchKernelGGL(k_split_small, dim3(1), dim3(1024), 0, st, s.idx, s.n, p_.timestep,
-                       std::fabs(dt), out.as<int>(), flags_.as<int>(),
+                       std::fabs(dt), out.as<int>(), ctx->h_pin_dev + 12,
                        ctx->small_sticky.as<int>());
-    TC_CHECK(hipMemcpyAsync(ctx->h_pin + 12, flags_.as<int>(), 2 * sizeof(int),
-                            hipMemcpyDeviceToHost, st));
     TC_CHECK(spin_wait(ctx, st));
     nfast = ctx->h_pin[12];
     if (ctx->h_pin[13] == 2) return small_tree_error();
@@ -549,10 +579,20 @@ int Evolver::omelyan(int level, Sub s, double stime, double etime, double dt,
     const Sub none;
     if (int e = kick(slow, none, 0.0, false)) return e;
     if (int e = kick_gradient(slow, none, dt, false)) return e;
-    if (int e = kick(slow, fast, 0.0, true)) return e;
-    if (int e = kick(fast, slow, 0.0, true)) return e;
-    if (int e = kick_gradient(slow, fast, dt, true)) return e;
-    if (int e = kick_gradient(fast, slow, dt, true)) return e;
+    if (fast.n == 0 || slow.n == 0) {
+      // one of the two sets is empty: of the four kicks between them, the
+      // two with the other set as sinks do nothing, and the other two (the
+      // pot-setting kick with dt 0, then the gradient kick) have no source,
+      // so they add zero forces -- applied in one launch
+      const Sub z = fast.n == 0 ? slow : fast;
+      hipLaunchKernelGGL(k_zero_kicks, grid_for(z.n), dim3(256), 0, ctx_->stream, z.idx, z.n, 0.0,
+                         false, dt, true, true, p_.vel, p_.pot);
+    } else {
+      if (int e = kick(slow, fast, 0.0, true)) return e;
+      if (int e = kick(fast, slow, 0.0, true)) return e;
+      if (int e = kick_gradient(slow, fast, dt, true)) return e;
+      if (int e = kick_gradient(fast, slow, dt, true)) return e;
+    }
   }
   if (int e = drift(slow, dt / 2)) return e;
   if (fast.n > 0)

@@ -108,13 +108,14 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
 // the last body of every leaf; out[0] = leaves, out[1] = 0 or 2 (too deep),
 // out[2] = 1 when two bodies are (next to) coincident; lidx / pmx (or null):
 // each body's leaf's last index and the mass before it in its leaf; sticky
-// (or null) is set to 2 with out[1] = 2 and otherwise left as it is.
+// (or null) is set to 2 with out[1] = 2 and otherwise left as it is; with
+// ix, body i is record ix[i] of x3, v3, q, flags.
 // ci: 4 n ints, cg: 8 n doubles of scratch.
 constexpr int kSmallTree = 4096;
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      int *sticky, hipStream_t s);
+                      int *sticky, const int *ix, hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 
@@ -305,14 +306,18 @@ hipError_t noalloc_scratch(unsigned char **out);
 int noalloc_max_bodies();
 void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const int *cnbody,
                       int n, unsigned char *lend, hipStream_t s);
-// The force-only direct sum of the step loop's kicks with the kick applied in
-// the slab-sum kernel: vel[is[k]] += F_k dt (2/3 of it for gradient kicks),
-// pot[is[k]] = p_k with set_pot, for the sinks k < ns, which must be the
-// first ns bodies (stored in the caller's order); only they are summed for.
-// part: 4 * nslab * n doubles.  Returns -1 (nothing launched) when nslab < 2.
-int launch_master_direct_kick(const BodiesDev &b, int n, int nslab, double *part, const int *is,
-                              int ns, double dt, bool gradient, bool set_pot, double *vel,
-                              double *pot, hipStream_t s);
+// The force-only direct sum of the step loop's kicks, read straight from the
+// particle arrays, with the kick applied in the slab-sum kernel: the bodies
+// are the union of the sinks is[0..ns) (sources too with sinks_are_sources)
+// and the sources js[0..nj), positions pos + acc_in * shift_fac (pos alone
+// for shift_fac = 0); vel[is[k]] += F_k dt (2/3 of it for gradient kicks),
+// pot[is[k]] = p_k with set_pot.  part: 4 * nslab * (ns + nj) doubles.
+// Returns -1 (nothing launched) when nslab < 2.
+int launch_master_direct_kick(const int *is, int ns, const int *js, int nj,
+                              bool sinks_are_sources, const double *pos, const double *acc_in,
+                              double shift_fac, const double *mass, int nslab, double *part,
+                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
+                              hipStream_t s);
 // The direct sum with the time-step estimate, in slabs that may cut source
 // leaves when the one-launch small tree gave lidx (each body's leaf's last
 // index) and pmx (the mass of the leaf's bodies before it), and (vector form)

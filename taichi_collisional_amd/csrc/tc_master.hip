@@ -32,6 +32,19 @@ namespace tcfmm {
 
 constexpr int kMT = 256;
 
+#ifndef TC_MD_UNROLL
+#define TC_MD_UNROLL 4
+#endif
+#ifndef TC_MD_R
+#define TC_MD_R 1
+#endif
+#ifndef TC_MD_MAXSLAB
+#define TC_MD_MAXSLAB 32
+#endif
+#ifndef TC_MDS_UNROLL
+#define TC_MDS_UNROLL 1
+#endif
+
 __global__ void k_fill_u8(unsigned char *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -134,61 +147,126 @@ __device__ int next_leaf_start(const unsigned char *__restrict__ lend, int n, in
 // acceleration) and k_master_direct_sum / k_master_direct_sum_kick add the
 // slabs in order.  With one slab the results are written directly.  The
 // leaves do not matter here: P2P_simple sums every pair.
-__global__ __launch_bounds__(kMT) void k_master_direct(BodiesDev b, int n, int nt, int slab,
-                                                       double *pot, double *acc,
+// The bodies of the force-only sum, read from the context's body arrays.
+struct BodySource {
+  BodiesDev b;
+  __device__ void target(int i, double &x, double &y, double &z) const {
+    x = b.x[i];
+    y = b.y[i];
+    z = b.z[i];
+  }
+  // P2P_simple: sources only (a non-source weighs nothing)
+  __device__ double4 source(int j) const {
+    return make_double4(b.x[j], b.y[j], b.z[j], (b.flags[j] & kSource) ? b.q[j] : 0.0);
+  }
+};
+
+// The same bodies straight from the step loop's particle arrays: the union
+// sinks ++ sources of a kick (body k < ns is particle is[k], a sink and, with
+// sinks_are_sources, a source; body ns + k is particle js[k], a source), the
+// positions shifted by acc * shift_fac (kick_gradient; 0 for plain kicks) as
+// the union gather shifts them.
+struct UnionSource {
+  const int *is, *js;
+  int ns;
+  bool sinks_are_sources;
+  const double *pos, *acc, *mass;
+  double shift_fac;
+  __device__ void at(int i, double &x, double &y, double &z) const {
+    const double *p = pos + 3 * (size_t)i;
+    if (shift_fac != 0.0) {
+      const double *a = acc + 3 * (size_t)i;
+      x = p[0] + a[0] * shift_fac;
+      y = p[1] + a[1] * shift_fac;
+      z = p[2] + a[2] * shift_fac;
+    } else {
+      x = p[0];
+      y = p[1];
+      z = p[2];
+    }
+  }
+  __device__ void target(int k, double &x, double &y, double &z) const { at(is[k], x, y, z); }
+  __device__ double4 source(int j) const {
+    const bool sink = j < ns;
+    const int i = sink ? is[j] : js[j - ns];
+    double x, y, z;
+    at(i, x, y, z);
+    return make_double4(x, y, z, (!sink || sinks_are_sources) ? mass[i] : 0.0);
+  }
+};
+
+// One thread per target i < nt (kR targets a thread), sources [0, n).
+// Sources may be split into gridDim.y uniform slabs of `slab` bodies, so that
+// a small system still fills the GPU; each slab then writes its partial sums,
+// in target order, to part[slab][4][n] (potential, acceleration) and
+// k_master_direct_sum / k_master_direct_sum_kick add the slabs in order.
+// With one slab (BodySource only) the results are written directly.  The
+// leaves do not matter here: P2P_simple sums every pair.
+template <int kR, class Src>
+__global__ __launch_bounds__(kMT) void k_master_direct(Src src, BodiesDev b, int n, int nt,
+                                                       int slab, double *pot, double *acc,
                                                        double *__restrict__ part) {
-  __shared__ double sx[kMT], sy[kMT], sz[kMT], sq[kMT];
+  // a source as (x, y, z, q): two 16-byte LDS reads per source for kR targets
+  __shared__ double4 ss[kMT];
   const int t = threadIdx.x;
   int j0 = 0, j1 = n;
   if (gridDim.y > 1) {
     j0 = (int)min((long long)n, (long long)blockIdx.y * slab);
     j1 = (int)min((long long)n, (long long)blockIdx.y * slab + slab);
   }
-  const int i = blockIdx.x * kMT + t;
-  const bool live = i < nt;
-  // lanes past nt run the loop on a harmless stand-in body and write nothing
-  const double xi = live ? b.x[i] : 0.0, yi = live ? b.y[i] : 0.0, zi = live ? b.z[i] : 0.0;
-  double ax = 0, ay = 0, az = 0, pt = 0;
+  // targets blockIdx.x * kR * kMT + r * kMT + t; lanes past nt run the loop
+  // on a harmless stand-in body and write nothing
+  double xi[kR], yi[kR], zi[kR], ax[kR], ay[kR], az[kR], pt[kR];
+#pragma unroll
+  for (int r = 0; r < kR; r++) {
+    const int i = (blockIdx.x * kR + r) * kMT + t;
+    xi[r] = yi[r] = zi[r] = 0.0;
+    if (i < nt) src.target(i, xi[r], yi[r], zi[r]);
+    ax[r] = ay[r] = az[r] = pt[r] = 0.0;
+  }
   for (int base = j0; base < j1; base += kMT) {
     __syncthreads();
     const int j = base + t;
-    if (j < j1) {
-      sx[t] = b.x[j];
-      sy[t] = b.y[j];
-      sz[t] = b.z[j];
-      // P2P_simple: sources only (a non-source weighs nothing)
-      sq[t] = (b.flags[j] & kSource) ? b.q[j] : 0.0;
-    }
+    if (j < j1) ss[t] = src.source(j);
     __syncthreads();
     const int cnt = min(kMT, j1 - base);
+#pragma unroll TC_MD_UNROLL
     for (int k = 0; k < cnt; k++) {
-      const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      // r2 = 0 masked
-      const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
-      const double w = r2 > 0 ? sq[k] * ir : 0.0;
-      pt += w;
-      const double w3 = w * ir * ir;
-      ax += dx * w3;
-      ay += dy * w3;
-      az += dz * w3;
+      const double4 sk = ss[k];
+#pragma unroll
+      for (int r = 0; r < kR; r++) {
+        const double dx = sk.x - xi[r], dy = sk.y - yi[r], dz = sk.z - zi[r];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        // r2 = 0 masked
+        const double ir = rsqrt_fp64(r2 > 0 ? r2 : 1e38);
+        const double w = r2 > 0 ? sk.w * ir : 0.0;
+        pt[r] += w;
+        const double w3 = w * ir * ir;
+        ax[r] += dx * w3;
+        ay[r] += dy * w3;
+        az[r] += dz * w3;
+      }
     }
   }
-  if (!live) return;
-  if (gridDim.y > 1) {
-    double *q = part + (size_t)blockIdx.y * 4 * n + i;
-    q[0] = pt;
-    q[(size_t)n] = ax;
-    q[2 * (size_t)n] = ay;
-    q[3 * (size_t)n] = az;
-    return;
+#pragma unroll
+  for (int r = 0; r < kR; r++) {
+    const int i = (blockIdx.x * kR + r) * kMT + t;
+    if (i >= nt) continue;
+    if (gridDim.y > 1) {
+      double *q = part + (size_t)blockIdx.y * 4 * n + i;
+      q[0] = pt[r];
+      q[(size_t)n] = ax[r];
+      q[2 * (size_t)n] = ay[r];
+      q[3 * (size_t)n] = az[r];
+      continue;
+    }
+    const bool sink = b.flags[i] & kSink;
+    const int o = b.order[i];
+    pot[o] = sink ? pt[r] : 0.0;
+    acc[3 * o] = sink ? ax[r] : 0.0;
+    acc[3 * o + 1] = sink ? ay[r] : 0.0;
+    acc[3 * o + 2] = sink ? az[r] : 0.0;
   }
-  const bool sink = b.flags[i] & kSink;
-  const int o = b.order[i];
-  pot[o] = sink ? pt : 0.0;
-  acc[3 * o] = sink ? ax : 0.0;
-  acc[3 * o + 1] = sink ? ay : 0.0;
-  acc[3 * o + 2] = sink ? az : 0.0;
 }
 
 // ---- the direct sum with time steps, in slabs that may cut leaves -----------
@@ -261,6 +339,7 @@ __global__ __launch_bounds__(kMT) void k_master_direct_steps(BodiesDev b,
     }
     __syncthreads();
     const int cnt = min(kMT, j1 - base);
+#pragma unroll TC_MDS_UNROLL
     for (int k = 0; k < cnt; k++) {
       const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz;
@@ -567,18 +646,40 @@ void launch_leaf_ends(const int *leaf_ids, int nleaves, const int *cbody, const 
     k_leaf_ends<<<(nleaves + 255) / 256, 256, 0, s>>>(leaf_ids, nleaves, cbody, cnbody, lend);
 }
 
-// The slabs' partial sums of k_master_direct, added in slab order, written
-// to caller order for sinks (zero for the others).
-__global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double *__restrict__ part,
-                                    double *pot, double *acc) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  double v[4] = {0, 0, 0, 0};
-  for (int q = 0; q < nslab; q++) {
-    const double *r = part + (size_t)q * 4 * n + i;
+// The slabs' partial sums of k_master_direct for target i, added in slab
+// order by kMergeGroups groups of threads (consecutive ranges of slabs for
+// the same 64 targets), then the groups' sums in group order by the first
+// group; true for the first group's live lanes, which hold the totals.
+__device__ __forceinline__ bool sum_slabs(int i, int nlive, int n, int nslab,
+                                          const double *__restrict__ part, double v[4]) {
+  __shared__ double gs[kMergeGroups][4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int per = (nslab + kMergeGroups - 1) / kMergeGroups;
+  const int q0 = min(nslab, g * per), q1 = min(nslab, q0 + per);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += r[(size_t)k * n];
-  }
+  for (int c = 0; c < 4; c++) v[c] = 0.0;
+  if (i < nlive)
+    for (int q = q0; q < q1; q++) {
+      const double *r = part + (size_t)q * 4 * n + i;
+#pragma unroll
+      for (int c = 0; c < 4; c++) v[c] += r[(size_t)c * n];
+    }
+#pragma unroll
+  for (int c = 0; c < 4; c++) gs[g][c][lane] = v[c];
+  __syncthreads();
+  if (g != 0 || i >= nlive) return false;
+  for (int u = 1; u < kMergeGroups; u++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] += gs[u][c][lane];
+  return true;
+}
+
+// written to caller order for sinks (zero for the others)
+__global__ __launch_bounds__(64 * kMergeGroups) void k_master_direct_sum(
+    BodiesDev b, int n, int nslab, const double *__restrict__ part, double *pot, double *acc) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  double v[4];
+  if (!sum_slabs(i, n, n, nslab, part, v)) return;
   const bool sink = b.flags[i] & kSink;
   const int o = b.order[i];
   pot[o] = sink ? v[0] : 0.0;
@@ -587,24 +688,18 @@ __global__ void k_master_direct_sum(BodiesDev b, int n, int nslab, const double 
   acc[3 * o + 2] = sink ? v[3] : 0.0;
 }
 
-// The slabs' partial sums of the force-only direct sum, added in slab order,
-// and the kick applied at once (the step loop's kicks): for every sink k of
-// the union (bodies [0, ns), stored in the caller's order, body is[k] of the
-// particle arrays) vel += F dt (gradient kicks: F dt 2/3, as ((F dt) 2) / 3)
-// and, with set_pot, pot = p -- the arithmetic of a separate kick kernel on
-// the summed results, without writing them out first.
-__global__ void k_master_direct_sum_kick(int n, int nslab, const double *__restrict__ part,
-                                         const int *__restrict__ is, int ns, double dt,
-                                         bool gradient, bool set_pot, double *__restrict__ vel,
-                                         double *__restrict__ pot) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= ns) return;
-  double v[4] = {0, 0, 0, 0};
-  for (int q = 0; q < nslab; q++) {
-    const double *r = part + (size_t)q * 4 * n + k;
-#pragma unroll
-    for (int c = 0; c < 4; c++) v[c] += r[(size_t)c * n];
-  }
+// The same sums with the kick applied at once (the step loop's kicks): for
+// every sink k of the union (bodies [0, ns), stored in the caller's order,
+// body is[k] of the particle arrays) vel += F dt (gradient kicks: F dt 2/3,
+// as ((F dt) 2) / 3) and, with set_pot, pot = p -- the arithmetic of a
+// separate kick kernel on the summed results, without writing them out
+// first.
+__global__ __launch_bounds__(64 * kMergeGroups) void k_master_direct_sum_kick(
+    int n, int nslab, const double *__restrict__ part, const int *__restrict__ is, int ns,
+    double dt, bool gradient, bool set_pot, double *__restrict__ vel, double *__restrict__ pot) {
+  const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+  double v[4];
+  if (!sum_slabs(k, ns, n, nslab, part, v)) return;
   const int j = is[k];
   for (int d = 0; d < 3; d++) {
     const double F = v[1 + d];
@@ -620,31 +715,36 @@ int master_direct_slabs(int nt, int n) {
   // slab (no partial sums) from 256 K targets on
   const int nb = (std::max(nt, 1) + kMT - 1) / kMT;
   const int want = (1024 + nb - 1) / nb;
-  return std::max(1, std::min({want, (n + 63) / 64, 32}));
+  return std::max(1, std::min({want, (n + 63) / 64, TC_MD_MAXSLAB}));
 }
 
-int launch_master_direct_kick(const BodiesDev &b, int n, int nslab, double *part, const int *is,
-                              int ns, double dt, bool gradient, bool set_pot, double *vel,
-                              double *pot, hipStream_t s) {
+int launch_master_direct_kick(const int *is, int ns, const int *js, int nj,
+                              bool sinks_are_sources, const double *pos, const double *acc_in,
+                              double shift_fac, const double *mass, int nslab, double *part,
+                              double dt, bool gradient, bool set_pot, double *vel, double *pot,
+                              hipStream_t s) {
   if (nslab < 2 || !part) return -1;  // the caller kicks from the written results
   if (ns < 1) return 0;
+  const int n = ns + nj;
   const int slab = (n + nslab - 1) / nslab;
-  const dim3 grid((ns + kMT - 1) / kMT, nslab);
-  k_master_direct<<<grid, kMT, 0, s>>>(b, n, ns, slab, nullptr, nullptr, part);
-  k_master_direct_sum_kick<<<(ns + 255) / 256, 256, 0, s>>>(n, nslab, part, is, ns, dt, gradient,
-                                                            set_pot, vel, pot);
+  const dim3 grid((ns + kMT * TC_MD_R - 1) / (kMT * TC_MD_R), nslab);
+  const UnionSource src{is, js, ns, sinks_are_sources, pos, acc_in, mass, shift_fac};
+  k_master_direct<TC_MD_R><<<grid, kMT, 0, s>>>(src, BodiesDev{}, n, ns, slab, nullptr, nullptr,
+                                                part);
+  k_master_direct_sum_kick<<<(ns + 63) / 64, 64 * kMergeGroups, 0, s>>>(
+      n, nslab, part, is, ns, dt, gradient, set_pot, vel, pot);
   return 0;
 }
 
 void launch_master_direct(const BodiesDev &b, int n, double *pot, double *acc, int nslab,
                           double *part, hipStream_t s) {
-  const int nb = (n + kMT - 1) / kMT;
+  const int nb = (n + kMT * TC_MD_R - 1) / (kMT * TC_MD_R);
   if (nslab < 1 || !part) nslab = 1;
   const int slab = (n + nslab - 1) / nslab;
   const dim3 grid(nb, nslab);
-  k_master_direct<<<grid, kMT, 0, s>>>(b, n, n, slab, pot, acc, part);
+  k_master_direct<TC_MD_R><<<grid, kMT, 0, s>>>(BodySource{b}, b, n, n, slab, pot, acc, part);
   if (nslab > 1)
-    k_master_direct_sum<<<(n + 255) / 256, 256, 0, s>>>(b, n, nslab, part, pot, acc);
+    k_master_direct_sum<<<(n + 63) / 64, 64 * kMergeGroups, 0, s>>>(b, n, nslab, part, pot, acc);
 }
 
 }  // namespace tcfmm

@@ -628,7 +628,8 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
                                                      int *__restrict__ lidx,
                                                      double *__restrict__ pmx,
                                                      int *__restrict__ ci, double *__restrict__ cg,
-                                                     int *__restrict__ out, int *sticky) {
+                                                     int *__restrict__ out, int *sticky,
+                                                     const int *__restrict__ ix) {
 #pragma clang fp contract(off)
   __shared__ uint64_t skey[kSmallTree];
   __shared__ unsigned short sidx[kSmallTree];
@@ -638,17 +639,21 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   __shared__ double sX0[3], sR0;
   __shared__ int ncur, nnext, nleaf, status, near;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  // body i is record ix[i] of x3, v3, q, flags_in (record i without ix)
+  auto rec = [&](int i) { return ix ? ix[i] : i; };
   // ---- bounds (min / max do not depend on the order); thread t holds
   // bodies 4t .. 4t+3 from here to the end of the sort
   double px[4][3];
   double mn[3], mx[3];
-  for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[d];
+  const int r0 = rec(0);
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[3 * (size_t)r0 + d];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int i = 4 * t + j;
+    const int ri = i < n ? rec(i) : r0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      px[j][d] = i < n ? x3[3 * (size_t)i + d] : x3[d];
+      px[j][d] = x3[3 * (size_t)ri + d];
       mn[d] = fmin(px[j][d], mn[d]);
       mx[d] = fmax(px[j][d], mx[d]);
     }
@@ -749,11 +754,11 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     for (int a = k + 1; a <= e; a++) {
       const unsigned short ia = sidx[a];
       uint64_t ha, la;
-      small_key(x3, ia, X0, R0, ha, la);
+      small_key(x3, rec(ia), X0, R0, ha, la);
       int c = a - 1;
       while (c >= k) {
         uint64_t hc, lc;
-        small_key(x3, sidx[c], X0, R0, hc, lc);
+        small_key(x3, rec(sidx[c]), X0, R0, hc, lc);
         if (hc < ha || (hc == ha && (lc < la || (lc == la && sidx[c] < ia)))) break;
         sidx[c + 1] = sidx[c];
         c--;
@@ -775,7 +780,7 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int i = src[r];
+      const int i = rec(src[r]);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         gx[r][d] = x3[3 * (size_t)i + d];
@@ -977,10 +982,10 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      int *sticky, hipStream_t s) {
+                      int *sticky, const int *ix, hipStream_t s) {
   if (n < 1 || n > kSmallTree) return -1;
   hipLaunchKernelGGL(k_small_tree, dim3(1), dim3(1024), 0, s, x3, v3, q, flags, n, ncrit, b,
-                     lend, lidx, pmx, ci, cg, out, sticky);
+                     lend, lidx, pmx, ci, cg, out, sticky, ix);
   return 0;
 }
 
