@@ -1244,7 +1244,8 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->probe_p2p, &ctx->leaf_feat, &ctx->leaf_pred, &ctx->chunk_part,
                  &ctx->chunk_hist, &ctx->mlp_Y, &ctx->mlp_H, &ctx->mlp_D, &ctx->mlp_dY,
                  &ctx->chunk_sort[0], &ctx->chunk_sort[1], &ctx->chunk_sort[2],
-                 &ctx->chunk_sort[3], &ctx->chunk_sort[4]};
+                 &ctx->chunk_sort[3], &ctx->chunk_sort[4], &ctx->small_ci, &ctx->small_cg,
+                 &ctx->small_out, &ctx->small_lidx, &ctx->small_pmx, &ctx->small_sticky};
   for (DBuf *b : all) b->release();
   for (DBuf &b : ctx->ev_lev) b.release();
   ctx->h_leaves.release();
