@@ -1271,10 +1271,12 @@ int launch_trav_step(CellsDev c, const TravArgs &a, TravState *st, int par, int 
 // Group (target, source) pairs by target: stable radix sort on the target,
 // then the start and end of each target's run.
 size_t pair_sort_temp_bytes(int n) {
-  size_t b = 0;
+  size_t b = 0, bk = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, b, (int *)nullptr, (int *)nullptr, (int *)nullptr,
                                      (int *)nullptr, n, 0, 32);
-  return b;
+  hipcub::DeviceRadixSort::SortKeys(nullptr, bk, (unsigned *)nullptr, (unsigned *)nullptr, n, 0,
+                                    32);
+  return std::max(b, bk);
 }
 
 __global__ void k_split_pairs(const int2 *__restrict__ pairs, int n, int *__restrict__ tg,
@@ -1302,12 +1304,46 @@ __global__ void k_run_counts(const int *__restrict__ start, int *__restrict__ cn
   if (c < ncells) cnt[c] -= start[c];
 }
 
+// Cell ids below 2^16: one 32-bit word per pair, target above source, so the
+// sort moves half the bytes of a key + value sort (the same stable order).
+__global__ void k_pack_pairs(const int2 *__restrict__ pairs, int n, unsigned *__restrict__ w) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int2 p = pairs[k];
+  w[k] = ((unsigned)p.x << 16) | (unsigned)p.y;
+}
+
+// k_runs on the sorted words, writing the sources as it goes
+__global__ void k_runs_packed(const unsigned *__restrict__ w, int n, int *__restrict__ start,
+                              int *__restrict__ endp, int *__restrict__ src) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const unsigned v = w[k];
+  const int t = (int)(v >> 16);
+  src[k] = (int)(v & 0xffffu);
+  if (k == 0 || (int)(w[k - 1] >> 16) != t) start[t] = k;
+  if (k == n - 1 || (int)(w[k + 1] >> 16) != t) endp[t] = k + 1;
+}
+
 // After the call: src[] holds sources grouped by target (stable), start[] /
 // count[] (zeroed by the caller) the run of every target.
 int launch_group_pairs(const int2 *pairs, int n, int key_bits, int ncells, int *tg_a, int *tg_b,
                        int *src_a, int *src, int *start, int *count, void *temp,
                        size_t temp_bytes, hipStream_t s) {
   if (n <= 0) return 0;
+  if (ncells <= (1 << 16) && key_bits <= 16) {
+    unsigned *wa = reinterpret_cast<unsigned *>(tg_a), *wb = reinterpret_cast<unsigned *>(tg_b);
+    hipLaunchKernelGGL(k_pack_pairs, dim3((n + 255) / 256), dim3(256), 0, s, pairs, n, wa);
+    size_t tb = temp_bytes;
+    if (hipcub::DeviceRadixSort::SortKeys(temp, tb, wa, wb, n, 16, 16 + key_bits, s) !=
+        hipSuccess)
+      return -1;
+    hipLaunchKernelGGL(k_runs_packed, dim3((n + 255) / 256), dim3(256), 0, s, wb, n, start,
+                       count, src);
+    hipLaunchKernelGGL(k_run_counts, dim3((ncells + 255) / 256), dim3(256), 0, s, start, count,
+                       ncells);
+    return 0;
+  }
   hipLaunchKernelGGL(k_split_pairs, dim3((n + 255) / 256), dim3(256), 0, s, pairs, n, tg_a, src_a);
   size_t tb = temp_bytes;
   if (hipcub::DeviceRadixSort::SortPairs(temp, tb, tg_a, tg_b, src_a, src, n, 0, key_bits, s) !=
