@@ -149,6 +149,9 @@ struct tc_context {
   // launch (launch_small_tree; 0: always the multi-kernel build)
   int small_tree_max = kSmallTree;
   DBuf small_ci, small_cg, small_out, small_lidx, small_pmx;
+  // the last one-launch tree's sorted order and its n (-1: none)
+  DBuf small_order;
+  int small_order_n = -1;
   // the step loop's calls do not wait for the small tree's status: a tree
   // too deep sets small_sticky (one int on the device), which the loop reads
   // with its next split

@@ -1068,11 +1068,17 @@ int master_eval_device(tc_context *ctx, int n, const double *x, const double *v,
     TC_CHECK(ctx->small_out.ensure(sizeof(int) * 4));
     TC_CHECK(ctx->small_lidx.ensure(sizeof(int) * (size_t)n));
     TC_CHECK(ctx->small_pmx.ensure(sizeof(double) * (size_t)n));
+    if (ctx->small_order.bytes < sizeof(unsigned short) * kSmallTree) {
+      TC_CHECK(ctx->small_order.ensure(sizeof(unsigned short) * kSmallTree));
+      ctx->small_order_n = -1;
+    }
     launch_small_tree(x, v, m, flags, n, ctx->cfg.ncrit, bodies_of(ctx),
                       ctx->mlend.as<unsigned char>(), ctx->small_lidx.as<int>(),
                       ctx->small_pmx.as<double>(), ctx->small_ci.as<int>(),
                       ctx->small_cg.as<double>(), ctx->small_out.as<int>(),
-                      ctx->defer_small_check ? ctx->small_sticky.as<int>() : nullptr, ix, s);
+                      ctx->defer_small_check ? ctx->small_sticky.as<int>() : nullptr, ix,
+                      ctx->small_order.as<unsigned short>(), ctx->small_order_n == n, s);
+    ctx->small_order_n = n;
   } else {
     if (int e = sort_and_build(ctx, n, x, v, m, flags)) return e;
     if (fmm) return master_fmm_device(ctx, n, steps, pair_mass, dt, pot, acc, tstep, acc_old);
@@ -1245,7 +1251,8 @@ int tc_destroy(tc_context *ctx) {
                  &ctx->chunk_hist, &ctx->mlp_Y, &ctx->mlp_H, &ctx->mlp_D, &ctx->mlp_dY,
                  &ctx->chunk_sort[0], &ctx->chunk_sort[1], &ctx->chunk_sort[2],
                  &ctx->chunk_sort[3], &ctx->chunk_sort[4], &ctx->small_ci, &ctx->small_cg,
-                 &ctx->small_out, &ctx->small_lidx, &ctx->small_pmx, &ctx->small_sticky};
+                 &ctx->small_out, &ctx->small_lidx, &ctx->small_pmx, &ctx->small_sticky,
+                 &ctx->small_order};
   for (DBuf *b : all) b->release();
   for (DBuf &b : ctx->ev_lev) b.release();
   ctx->h_leaves.release();

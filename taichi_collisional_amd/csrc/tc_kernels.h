@@ -109,13 +109,17 @@ int launch_tree_level(CellsDev c, BodiesDev b, TreeState *st, int ncrit, int cap
 // out[2] = 1 when two bodies are (next to) coincident; lidx / pmx (or null):
 // each body's leaf's last index and the mass before it in its leaf; sticky
 // (or null) is set to 2 with out[1] = 2 and otherwise left as it is; with
-// ix, body i is record ix[i] of x3, v3, q, flags.
+// ix, body i is record ix[i] of x3, v3, q, flags.  order (n ushorts, or
+// null) receives the bodies' sorted order by key prefix and index; with
+// use_order it holds the previous call's (any permutation of 0..n-1), which
+// is kept when still sorted instead of running the sort.
 // ci: 4 n ints, cg: 8 n doubles of scratch.
 constexpr int kSmallTree = 4096;
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      int *sticky, const int *ix, hipStream_t s);
+                      int *sticky, const int *ix, unsigned short *order, bool use_order,
+                      hipStream_t s);
 size_t scan_temp_bytes(int n);
 int launch_scan(const int *in, int *out, int n, void *temp, size_t temp_bytes, hipStream_t s);
 

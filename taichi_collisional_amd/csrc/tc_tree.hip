@@ -629,7 +629,8 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
                                                      double *__restrict__ pmx,
                                                      int *__restrict__ ci, double *__restrict__ cg,
                                                      int *__restrict__ out, int *sticky,
-                                                     const int *__restrict__ ix) {
+                                                     const int *__restrict__ ix,
+                                                     unsigned short *order, int use_order) {
 #pragma clang fp contract(off)
   __shared__ uint64_t skey[kSmallTree];
   __shared__ unsigned short sidx[kSmallTree];
@@ -642,15 +643,20 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   // body i is record ix[i] of x3, v3, q, flags_in (record i without ix)
   auto rec = [&](int i) { return ix ? ix[i] : i; };
   // ---- bounds (min / max do not depend on the order); thread t holds
-  // bodies 4t .. 4t+3 from here to the end of the sort
+  // elements 4t .. 4t+3 from here to the end of the sort: bodies 4t .. 4t+3,
+  // or with `order` (the previous call's sorted order of the n bodies) the
+  // bodies at those places of it
   double px[4][3];
+  int bid[4];
   double mn[3], mx[3];
   const int r0 = rec(0);
   for (int d = 0; d < 3; d++) mn[d] = mx[d] = x3[3 * (size_t)r0 + d];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int i = 4 * t + j;
-    const int ri = i < n ? rec(i) : r0;
+    const int e = 4 * t + j;
+    bid[j] = e < n ? (use_order ? (int)order[e] : e) : 0;
+    const int i = bid[j];
+    const int ri = e < n ? rec(i) : r0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       px[j][d] = x3[3 * (size_t)ri + d];
@@ -707,12 +713,104 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
   uint64_t kk[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int i = 4 * t + j;
+    const int e = 4 * t + j;
     uint64_t hi = 0, lo;
-    if (i < n) small_key_of(px[j], X0, R0, hi, lo);
-    kk[j] = i < n ? ((hi >> 11) << 12) | (uint64_t)i : ~0ull;
+    if (e < n) small_key_of(px[j], X0, R0, hi, lo);
+    kk[j] = e < n ? ((hi >> 11) << 12) | (uint64_t)bid[j] : ~0ull;
   }
-  for (int size = 2; size <= n2; size <<= 1)
+  // The values are distinct, so if the previous order still has them
+  // strictly increasing it is the sorted order and the network is skipped
+  // (bodies move little between the step loop's calls).
+  bool sorted = false;
+  if (use_order) {
+    // every neighbouring pair (e, e + 1) with e + 1 < n in increasing order
+    const int e0 = 4 * t;
+    bool up = true;
+#pragma unroll
+    for (int j = 0; j < 3; j++) up = up && (e0 + j + 1 >= n || kk[j] < kk[j + 1]);
+    const uint64_t nxt = __shfl_down(kk[0], 1, kWave);
+    if (lane == 0) skey[w] = kk[0];  // each wave's first value
+    __syncthreads();
+    const uint64_t nk = lane < kWave - 1 ? nxt : (w < 15 ? skey[w + 1] : ~0ull);
+    up = up && (e0 + 4 >= n || kk[3] < nk);
+    sorted = __syncthreads_and(up ? 1 : 0) != 0;
+  }
+  if (use_order && !sorted) {
+    // Nearly sorted: an element larger than everything before it and
+    // smaller than everything after it is already at its sorted place; the
+    // others fill the gaps between such elements, each gap with exactly the
+    // values that belong there, so sorting every gap finishes the sort.
+    // Gaps longer than 64 fall back to the sorting network.
+    const int e0 = 4 * t;
+    uint64_t tmax = 0, tmin = ~0ull;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (e0 + j < n) tmax = kk[j] > tmax ? kk[j] : tmax;
+      if (e0 + j < n) tmin = kk[j] < tmin ? kk[j] : tmin;
+    }
+    // inclusive wave scans: max from the front, min from the back
+    uint64_t pmax = tmax, smin = tmin;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint64_t a = __shfl_up(pmax, d, kWave), b = __shfl_down(smin, d, kWave);
+      if (lane >= d) pmax = a > pmax ? a : pmax;
+      if (lane + d < kWave) smin = b < smin ? b : smin;
+    }
+    __syncthreads();  // the check's skey reads are done
+    if (lane == kWave - 1) skey[w] = pmax;   // wave totals
+    if (lane == 0) skey[16 + w] = smin;
+    __syncthreads();
+    uint64_t before = 0, after = ~0ull;  // over earlier / later waves
+    for (int u = 0; u < w; u++) before = skey[u] > before ? skey[u] : before;
+    for (int u = w + 1; u < 16; u++) after = skey[16 + u] < after ? skey[16 + u] : after;
+    // exclusive over this thread's elements' predecessors / successors
+    uint64_t pm = __shfl_up(pmax, 1, kWave), sm = __shfl_down(smin, 1, kWave);
+    pm = lane > 0 ? (pm > before ? pm : before) : before;
+    sm = lane < kWave - 1 ? (sm < after ? sm : after) : after;
+    bool clean[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      uint64_t sj = sm;
+      for (int q = j + 1; q < 4; q++)
+        if (e0 + q < n) sj = kk[q] < sj ? kk[q] : sj;
+      const int e = e0 + j;
+      clean[j] = e >= n || ((e == 0 || pm < kk[j]) && (e == n - 1 || kk[j] < sj));
+      if (e < n) pm = kk[j] > pm ? kk[j] : pm;
+    }
+    __syncthreads();  // the totals are read
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      skey[e0 + j] = kk[j];
+      sidx[e0 + j] = clean[j] ? 0 : 1;
+    }
+    __syncthreads();
+    bool big = false;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = e0 + j;
+      if (clean[j] || (a > 0 && sidx[a - 1])) continue;  // not the start of a gap
+      int b = a;
+      while (b + 1 < n && sidx[b + 1] && b - a < 64) b++;
+      if (b - a >= 64) {
+        big = true;
+        continue;
+      }
+      for (int u = a + 1; u <= b; u++) {  // insertion sort of the gap
+        const uint64_t v = skey[u];
+        int c = u - 1;
+        while (c >= a && skey[c] > v) {
+          skey[c + 1] = skey[c];
+          c--;
+        }
+        skey[c + 1] = v;
+      }
+    }
+    if (__syncthreads_or(big ? 1 : 0) == 0) {
+      sorted = true;
+#pragma unroll
+      for (int j = 0; j < 4; j++) kk[j] = skey[e0 + j];
+    }
+  }
+  for (int size = 2; size <= (sorted ? 1 : n2); size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       if (stride >= 4 * kWave) {
         __syncthreads();  // the previous round's reads are done
@@ -738,11 +836,13 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
       }
     }
   __syncthreads();
-  // skey: the 52-bit prefix, sidx: the input index
+  // skey: the 52-bit prefix, sidx: the input index; the order for the next
+  // call (before the runs below reorder within equal prefixes)
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     skey[4 * t + j] = kk[j] >> 12;
     sidx[4 * t + j] = (unsigned short)(kk[j] & 0xfff);
+    if (order && 4 * t + j < n) order[4 * t + j] = (unsigned short)(kk[j] & 0xfff);
   }
   __syncthreads();
   // runs of equal prefixes (bodies in one cell of 2^-17 of the box; rare):
@@ -982,10 +1082,11 @@ __global__ __launch_bounds__(1024) void k_small_tree(const double *__restrict__ 
 int launch_small_tree(const double *x3, const double *v3, const double *q,
                       const unsigned char *flags, int n, int ncrit, BodiesDev b,
                       unsigned char *lend, int *lidx, double *pmx, int *ci, double *cg, int *out,
-                      int *sticky, const int *ix, hipStream_t s) {
+                      int *sticky, const int *ix, unsigned short *order, bool use_order,
+                      hipStream_t s) {
   if (n < 1 || n > kSmallTree) return -1;
   hipLaunchKernelGGL(k_small_tree, dim3(1), dim3(1024), 0, s, x3, v3, q, flags, n, ncrit, b,
-                     lend, lidx, pmx, ci, cg, out, sticky, ix);
+                     lend, lidx, pmx, ci, cg, out, sticky, ix, order, use_order ? 1 : 0);
   return 0;
 }
 

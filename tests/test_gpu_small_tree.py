@@ -116,3 +116,29 @@ def test_small_tree_too_deep_in_step_loop(torch_cuda):
         f.close()
         msgs.append(str(e.value).split(": ", 1)[-1])
     assert msgs[0] == msgs[1] and "64 levels" in msgs[0]
+
+
+def test_small_tree_previous_order(torch_cuda):
+    """A context keeps the one-launch tree's sorted order and, on the next
+    call with the same number of bodies, keeps it instead of sorting when it
+    is still sorted.  The results must equal a fresh context's bit for bit:
+    the same bodies again, a small drift (the step loop's case), a large one
+    and a shuffled input (the old order no longer sorted), and ncrit 1."""
+    import taichi_collisional_amd as tca
+    torch = torch_cuda
+    rng = np.random.default_rng(11)
+    for ncrit in (300, 1):
+        x, v, m = _case("plummer", 3000, seed=21)
+        cases = [(x, v, m), (x, v, m),
+                 (x + 1e-5 * v, v, m),
+                 (x + 3e-2 * rng.normal(size=x.shape), v, m)]
+        perm = rng.permutation(3000)
+        cases.append((x[perm], v[perm], m[perm]))
+        f = tca.Fmm(p=8, ncrit=ncrit, direct_below=0)
+        for xc, vc, mc in cases:
+            dx, dv, dm = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (xc, vc, mc))
+            got = f.master_eval(dx, dv, dm, get_steps=True, dt_param=0.025)
+            ref = _eval(torch, tca, xc, vc, mc, None, ncrit, True)
+            for u, w in zip(got, ref[:3]):
+                assert np.array_equal(u.cpu().numpy(), w)
+        f.close()
