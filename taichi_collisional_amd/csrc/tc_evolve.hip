@@ -163,6 +163,41 @@ __global__ __launch_bounds__(1024) void k_naive_small(const int *__restrict__ is
   for (int k = threadIdx.x; k < ns; k += 1024) ts[is[k]] = r;
 }
 
+// k_naive_small followed by the split of the same subsystem (omelyan calls
+// split right after kick_naive): every body of s now has the minimum step r,
+// so the stable partition by r < dt is s itself, all fast or all slow;
+// nfast[0] = ns or 0 and nfast[1] = *sticky go to pinned host memory as
+// k_split_small writes them.
+__global__ __launch_bounds__(1024) void k_naive_split_small(
+    const int *__restrict__ is, int ns, const double *__restrict__ F, const double *__restrict__ p,
+    const double *__restrict__ tsum, const double *__restrict__ ao, double *__restrict__ acc,
+    double *__restrict__ pot, double *__restrict__ acc_old, double *__restrict__ ts, double dt,
+    int *__restrict__ out, int *__restrict__ nfast, const int *sticky) {
+  __shared__ double red[16];
+  double v = INFINITY;
+  for (int k = threadIdx.x; k < ns; k += 1024) {
+    const int i = is[k];
+    pot[i] = p[k];
+    for (int d = 0; d < 3; d++) acc[3 * i + d] = F[3 * k + d];
+    acc_old[i] = ao ? ao[k] : 0.0;
+    v = fmin(v, 1.0 / sqrt(sqrt(tsum[k])));
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < 16; w++) r = fmin(r, red[w]);
+  for (int k = threadIdx.x; k < ns; k += 1024) {
+    const int i = is[k];
+    ts[i] = r;
+    out[k] = i;
+  }
+  if (threadIdx.x == 0) {
+    nfast[0] = r < dt ? ns : 0;
+    nfast[1] = sticky ? *sticky : 0;
+  }
+}
+
 __global__ void k_set_timestep(const int *__restrict__ is, int ns, const double *__restrict__ mn,
                                double *__restrict__ ts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -332,7 +367,14 @@ class Evolver {
   int kick(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
   int kick_cpu(Sub s1, Sub s2, double dt);
   int kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fast);
-  int kick_naive(Sub s);
+  // kick_naive, and with sf the split of the same subsystem that follows it
+  struct SplitFuse {
+    int level;
+    double dt;
+    Sub slow, fast;
+    bool done = false;
+  };
+  int kick_naive(Sub s, SplitFuse *sf = nullptr);
   int drift(Sub s, double dt);
   int split(int level, Sub s, double dt, Sub &slow, Sub &fast);
   int omelyan(int level, Sub s, double stime, double etime, double dt, bool calc_timestep);
@@ -476,13 +518,31 @@ int Evolver::kick_gradient(Sub sinks, Sub sources, double dt, bool sinks_are_fas
 
 // kick_naive(clevel, s, zerosys, zerosys, 0, true) (integrator.h:182-306)
 // with GRADIENT: every body of s gets the minimum over s of the time step.
-int Evolver::kick_naive(Sub s) {
+int Evolver::kick_naive(Sub s, SplitFuse *sf) {
   tc_context *ctx = ctx_;
   if (s.n == 0) return 0;
   if (int e = force(s, Sub{}, true, 0.0, true, true)) return e;
   hipStream_t st = ctx->stream;
   double *tmp = tmp_.as<double>();
   const double *ao = (mode_ & TC_MASTER_FMM) ? uao_.as<double>() : nullptr;
+  if (sf && s.n <= kSplitSmall && s.n <= kNaiveSmall) {
+    // with the split of s that follows, in one launch
+    if ((int)lev_.size() <= sf->level) lev_.resize(sf->level + 1);
+    DBuf &out = lev_[sf->level];
+    TC_CHECK(out.ensure(sizeof(int) * (size_t)std::max(s.n, 1)));
+    hipLaunchKernelGGL(k_naive_split_small, dim3(1), dim3(1024), 0, st, s.idx, s.n,
+                       uacc_.as<double>(), upot_.as<double>(), uts_.as<double>(), ao, p_.acc,
+                       p_.pot, p_.acc_old, p_.timestep, std::fabs(sf->dt), out.as<int>(),
+                       ctx->h_pin_dev + 12, ctx->small_sticky.as<int>());
+    TC_CHECK(spin_wait(ctx, st));
+    int nfast = ctx->h_pin[12];
+    if (ctx->h_pin[13] == 2) return small_tree_error();
+    if (nfast == 1) nfast = 0;
+    sf->fast = Sub{out.as<int>(), nfast};
+    sf->slow = Sub{out.as<int>() + nfast, s.n - nfast};
+    sf->done = true;
+    return 0;
+  }
   if (s.n <= kNaiveSmall) {
     hipLaunchKernelGGL(k_naive_small, dim3(1), dim3(1024), 0, st, s.idx, s.n, uacc_.as<double>(),
                        upot_.as<double>(), uts_.as<double>(), ao, p_.acc, p_.pot, p_.acc_old,
@@ -553,10 +613,16 @@ int Evolver::omelyan(int level, Sub s, double stime, double etime, double dt,
     ctx->err = "time step too small: more than 64 levels";
     return -2;
   }
-  if (calc_timestep)
-    if (int e = kick_naive(s)) return e;
   Sub slow, fast;
-  if (int e = split(level, s, dt, slow, fast)) return e;
+  SplitFuse sf{level, dt};
+  if (calc_timestep)
+    if (int e = kick_naive(s, &sf)) return e;
+  if (sf.done) {
+    slow = sf.slow;
+    fast = sf.fast;
+  } else if (int e = split(level, s, dt, slow, fast)) {
+    return e;
+  }
   if (fast.n == 0) simtime_ += dt;
   auto outer_kicks = [&]() -> int {
     if (s.n > 128) {
