@@ -32,6 +32,11 @@ namespace tcfmm {
 
 constexpr int kMT = 256;
 
+// Build-time shape of the direct sums (defaults measured at N = 4096 on
+// MI355X): unroll of the force-only source loop (4: 19 us, 1: 22 us), targets
+// per thread (2 gave fewer blocks and 10-30 % more time), the most source
+// slabs of the force-only sum (64: slower with its larger partial sums), and
+// the unroll of the time-step loop (2 and 4 changed nothing).
 #ifndef TC_MD_UNROLL
 #define TC_MD_UNROLL 4
 #endif
