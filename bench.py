@@ -490,8 +490,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (the evaluation "
+              "is not sharded across GPUs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.only_master_evolve:
