@@ -340,8 +340,11 @@ int tc_master_eval_soa(tc_context *ctx, int n, const double *x, const double *v,
 /* With TC_MASTER_STEPS and without TC_MASTER_FMM, systems of at most
  * max_bodies bodies (capped at 4096; the default) build their tree in a single
  * launch instead of the level-by-level build: the same tree, the same body
- * order and leaves, so the same results; 0 selects the level-by-level build
- * at every size.  No reference counterpart (a launch-count choice). */
+ * order and leaves; the time-step sum then runs in slabs that cut leaves, so
+ * the results agree with the level-by-level build's to round-off (the order
+ * of the sums).  The context keeps the last such tree's sorted order and
+ * starts the next one of the same size from it.  0 selects the level-by-level
+ * build at every size.  No reference counterpart (a launch-count choice). */
 int tc_set_small_tree(tc_context *ctx, int max_bodies);
 
 /* Per-kernel entry points over the master Cell (exafmm.h:50-71: M[NTERM],
