@@ -68,6 +68,34 @@ def test_gpu_step_loop_matches_reference(torch_cuda, oracle_lib, k):
     assert np.abs(out["pot"] - o["pot"]).max() <= 1e-10 * max(1.0, np.abs(o["pot"]).max())
 
 
+def test_gpu_step_loop_above_small_tree_size(torch_cuda, oracle_lib):
+    """One step of N = 5000 (dt = 1/4096, smallest step 3.4e-6): above the 4096
+    bodies of the one-launch time-step tree, so kick_naive runs the
+    level-by-level tree, the leaf-aligned time-step sum and, with the split,
+    the multi-launch partition.  The time-step sum depends on the leaf
+    partition (the pair mass grows along each source leaf, kernel_fast_lean.h:
+    539), hence on ncrit: the compiled reference's step was taken at ncrit =
+    65536 (one leaf, the all-pairs branch of get_force_and_potential), and the
+    multi-leaf partition (ncrit = 1000) is checked against the oracle's loop."""
+    g = golden("master_evolve_big.npz")
+    c = mmg.EVOLVE_BIG
+    x, v, m = mmg.ics.plummer(c["n"], seed=c["seed"])
+    assert mmg.sha(x, v, m) == str(g["sha"])
+    out, t = _run(torch_cuda, x, v, m, c["nsteps"], c["dt"], ncrit=1 << 16)
+    dp, dv = np.abs(out["pos"] - g["pos"]).max(), np.abs(out["vel"] - g["vel"]).max()
+    print("N=5000, one leaf: dpos %.2e dvel %.2e" % (dp, dv))
+    assert t == float(g["simtime"])
+    np.testing.assert_allclose(out["timestep"], g["ts"], rtol=1e-11)
+    assert dp < 1e-11 and dv < 1e-10
+    out, t = _run(torch_cuda, x, v, m, c["nsteps"], c["dt"], ncrit=1000)
+    o = oracle_lib.master_evolve(m, x, v, c["nsteps"], c["dt"], ncrit=1000)
+    dp, dv = np.abs(out["pos"] - o["pos"]).max(), np.abs(out["vel"] - o["vel"]).max()
+    print("N=5000, ncrit 1000: dpos %.2e dvel %.2e" % (dp, dv))
+    assert t == o["simtime"]
+    np.testing.assert_allclose(out["timestep"], o["timestep"], rtol=1e-11)
+    assert dp < 1e-11 and dv < 1e-10
+
+
 def test_gpu_step_loop_particle_records(torch_cuda):
     """The host particle-record form equals the device-array form."""
     import taichi_collisional_amd as tca
