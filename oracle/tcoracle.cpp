@@ -9,8 +9,8 @@
 // OpenMP over target cells (the reference uses OpenMP tasks + atomics; the
 // results differ only in summation order).
 //
-// Pinning (what fixes each part beyond this restatement) is listed in the
-// oracle's README and exercised by tests/test_oracle_*.py:
+// Pinning (what fixes each part beyond this restatement), exercised by
+// tests/test_oracle_*.py and tests/test_master_oracle.py:
 //   harmonics      closed forms (Legendre) in tests
 //   swap matrices  exact rational derivation + reference file hashes
 //   M2L_rotate     == O(p^4) M2L to round-off; == reference M2L_rotate (KAT)
@@ -18,6 +18,10 @@
 //   P2P/direct     long-double brute force + reference KATs
 //   tree/miniball  reference tree (same shift) + minimum-ball optimality checks
 //   whole FMM      reference end-to-end goldens + FP64 direct sum
+//   master direct + time step, -DFMM passes, step loop (orc_master_*)
+//                  goldens from the compiled master reference (direct sums,
+//                  walk/M2L/P2P passes, do_evolve at N = 100, 400 and 5000)
+//                  + numpy brute force for the forces
 #include <algorithm>
 #include <cmath>
 #include <complex>

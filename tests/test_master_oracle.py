@@ -179,6 +179,25 @@ def test_master_step_loop_matches_reference(oracle_lib, k):
     assert np.max(np.abs(o["pos"] - x)) > 1e-3
 
 
+def test_master_step_loop_matches_reference_one_leaf_n5000(oracle_lib):
+    """One step of N = 5000 (dt = 1/4096) with ncrit = 65536: every kick is
+    the all-pairs branch and the time-step sum's pair mass runs over one leaf
+    of all the sources (kernel_fast_lean.h:539); the oracle's loop against the
+    compiled reference's step (about 20 s on 8 cores)."""
+    g = golden("master_evolve_big.npz")
+    c = mmg.EVOLVE_BIG
+    x, v, m = mmg.ics.plummer(c["n"], seed=c["seed"])
+    assert mmg.sha(x, v, m) == str(g["sha"])
+    o = oracle_lib.master_evolve(m, x, v, c["nsteps"], c["dt"], ncrit=1 << 16)
+    assert o["simtime"] == float(g["simtime"])
+    np.testing.assert_allclose(o["timestep"], g["ts"], rtol=1e-12)
+    assert np.max(np.abs(o["pos"] - g["pos"])) < 1e-13
+    assert np.max(np.abs(o["vel"] - g["vel"])) < 1e-12
+    assert np.max(np.abs(o["pos_e"] - g["pos_e"])) < 1e-15
+    # the system moved (by about v dt ~ 2e-4)
+    assert np.max(np.abs(o["pos"] - x)) > 1e-5
+
+
 def test_master_step_loop_conserves_energy(oracle_lib):
     """A fourth-order scheme over the level hierarchy: energy error after a
     step small against the energy (a wrong kick weight fails this)."""
