@@ -8,11 +8,13 @@
 //       two blocks of 256 threads per parent cell, four children each: 64
 //       threads per child, each thread a strided share of the child's (output
 //       term, L or Ldot) tasks; the four harmonic tables and the parent's
-//       expansion sit in LDS (16 KB at p = 12, so a block finds room on a CU
-//       that the P2P kernel running alongside has otherwise filled).  A
-//       task's sum runs over contiguous runs of both tables, two terms at a
-//       time into two partial sums: the top levels are a few blocks each, and
-//       their time is that of one thread's chain of LDS reads and adds.
+//       expansion sit in LDS (16 KB at p = 12).  The P2P kernel running
+//       alongside holds 12 one-wave blocks per CU (154 KB of LDS, all but 8
+//       VGPRs per SIMD lane), so a level launched while it runs waits for its
+//       blocks to drain.  A task's sum runs over contiguous runs of both
+//       tables, two terms at a time into two partial sums: the top levels are
+//       a few blocks each, and their time (35-40 us) is that of one thread's
+//       chain of LDS reads and adds.
 //   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
 //       dY/dt with velocity -v_b, streamed order by order (three harmonic
 //       rows in registers); only n <= 1 of the local field is formed:
