@@ -799,15 +799,9 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
     }
     ctx->m2l_by_order[cfg.p] = ctx->st.n_m2l;
   }
-  // the GPU runs M2L while the host builds the P2P work items
-  if (int e = build_p2p_items(ctx)) return e;
-  TC_CHECK(hipEventRecord(ctx->ev[5], s));
-  // ---- near field on the second stream, after M2L (and the work-item upload
-  // queued behind it), alongside the downward pass: the downward pass's top
-  // levels are a handful of blocks each, which P2P's blocks fill around
+  // the near field's set-up on the second stream, beside M2L: it reads only
+  // the tree and the P2P list (already queued on that stream)
   hipStream_t s2 = ctx->near_stream;
-  TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[5], 0));
-  TC_CHECK(hipEventRecord(ctx->ev[9], s2));
   ResultsDev r = results_of(ctx);
   for (double *q : {r.pot, r.ax, r.ay, r.az, r.jx, r.jy, r.jz})
     TC_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s2));
@@ -816,6 +810,13 @@ static int eval_device(tc_context *ctx, int n, const double *x, const double *v,
   TC_CHECK(hipMemsetAsync(ctx->npairs.p, 0, sizeof(unsigned long long), s2));
   launch_p2p_ranges(cells_of(ctx), ctx->p2p_src.as<int>(), (int)ctx->st.n_p2p_cells,
                     ctx->p2p_ranges.as<int2>(), s2);
+  // the GPU runs M2L while the host builds the P2P work items
+  if (int e = build_p2p_items(ctx)) return e;
+  TC_CHECK(hipEventRecord(ctx->ev[5], s));
+  // ---- near field on the second stream, after M2L (and the work-item upload
+  // queued behind it), alongside the downward pass
+  TC_CHECK(hipStreamWaitEvent(s2, ctx->ev[5], 0));
+  TC_CHECK(hipEventRecord(ctx->ev[9], s2));
   launch_p2p(cells_of(ctx), bodies_of(ctx), r, ctx->p2p_items.as<int4>(),
              (int)ctx->n_items, ctx->p2p_off.as<int>(), ctx->p2p_cnt.as<int>(),
              ctx->p2p_ranges.as<int2>(), ctx->npairs.as<unsigned long long>(), s2,
