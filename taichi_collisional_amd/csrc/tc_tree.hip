@@ -431,38 +431,54 @@ __global__ __launch_bounds__(1024) void k_level_scan(const int *__restrict__ nch
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   if (t < 2) carry[t] = 0;
   __syncthreads();
-  constexpr int PER = 8, TILE = 1024 * PER;
+  // both arrays in one pass over tiles of 16 per thread (a 24k-cell level is
+  // two tiles): each tile is a few global-load and barrier round trips
+  constexpr int PER = 16, TILE = 1024 * PER;
   for (int base = 0; base < nl; base += TILE) {
-    for (int a = 0; a < 2; a++) {
-      const int *in = nch + (size_t)a * cap;
-      int *out = scan + (size_t)a * cap;
-      int v[PER], sum = 0;
+    int v0[PER], v1[PER], s0 = 0, s1 = 0;
 #pragma unroll
-      for (int q = 0; q < PER; q++) {
-        const int i = base + t * PER + q;
-        v[q] = i < nl ? in[i] : 0;
-        sum += v[q];
-      }
-      int wt;
-      const int ex = wave_excl_scan(sum, &wt);
-      if (lane == 0) wsum[a][w] = wt;
-      __syncthreads();
-      int before = carry[a], all = 0;
-      for (int k = 0; k < 16; k++) {
-        if (k < w) before += wsum[a][k];
-        all += wsum[a][k];
-      }
-      int run = before + ex;
-#pragma unroll
-      for (int q = 0; q < PER; q++) {
-        const int i = base + t * PER + q;
-        if (i < nl) out[i] = run;
-        run += v[q];
-      }
-      __syncthreads();
-      if (t == 0) carry[a] += all;
-      __syncthreads();
+    for (int q = 0; q < PER; q++) {
+      const int i = base + t * PER + q;
+      v0[q] = i < nl ? nch[i] : 0;
+      v1[q] = i < nl ? nch[(size_t)cap + i] : 0;
+      s0 += v0[q];
+      s1 += v1[q];
     }
+    int wt0, wt1;
+    const int ex0 = wave_excl_scan(s0, &wt0);
+    const int ex1 = wave_excl_scan(s1, &wt1);
+    if (lane == 0) {
+      wsum[0][w] = wt0;
+      wsum[1][w] = wt1;
+    }
+    __syncthreads();
+    int r0 = carry[0], r1 = carry[1], all0 = 0, all1 = 0;
+    for (int k = 0; k < 16; k++) {
+      if (k < w) {
+        r0 += wsum[0][k];
+        r1 += wsum[1][k];
+      }
+      all0 += wsum[0][k];
+      all1 += wsum[1][k];
+    }
+    r0 += ex0;
+    r1 += ex1;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int i = base + t * PER + q;
+      if (i < nl) {
+        scan[i] = r0;
+        scan[(size_t)cap + i] = r1;
+      }
+      r0 += v0[q];
+      r1 += v1[q];
+    }
+    __syncthreads();
+    if (t == 0) {
+      carry[0] += all0;
+      carry[1] += all1;
+    }
+    __syncthreads();
   }
   if (t == 0) {
     const int total = carry[0], leaves = carry[1];
