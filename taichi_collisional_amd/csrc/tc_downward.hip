@@ -15,7 +15,9 @@
 //       tables, two terms at a time into two partial sums: the top levels are
 //       a few blocks each, and their time (35-40 us) is that of one thread's
 //       chain of LDS reads and adds.
-//   L2P (kernel_fast_lean.h:1060-1128): lane = body, Y = Y(X_c - x_b) and
+//   L2P (kernel_fast_lean.h:1060-1128): two leaves per wave, 32 lanes each
+//       (leaves average ~19 bodies: one leaf per wave left 70% of the lanes
+//       idle); lane = body, Y = Y(X_c - x_b) and
 //       dY/dt with velocity -v_b, streamed order by order (three harmonic
 //       rows in registers); only n <= 1 of the local field is formed:
 //       p += Re Phi_0^0, a -= (Re Phi_1^1, Im Phi_1^1, Re Phi_1^0),
@@ -169,34 +171,39 @@ __device__ __forceinline__ void l2p_orders(double dx, double dy, double dz, doub
 
 template <int P>
 __global__ __launch_bounds__(64) void k_l2p(CellsDev c, BodiesDev b, ResultsDev r,
-                                            const int *__restrict__ leaf_ids) {
+                                            const int *__restrict__ leaf_ids, int nleaves) {
   constexpr int NT = nterm(P);
-  __shared__ cplx cl[NT], cld[NT];
-  const int lane = threadIdx.x;
-  const int ci = leaf_ids[blockIdx.x];
-  if (!c.has_sink[ci]) return;
-  const int b0 = c.body[ci], nb = c.nbody[ci];
-  const double *Lp = c.L + (size_t)ci * NT;
-  const double *Ldp = c.Ld + (size_t)ci * NT;
-  for (int t = lane; t < NT; t += kWave) {
-    int n = 0;
-    while ((n + 1) * (n + 1) <= t) n++;
-    int m = t - n * n - n;
-    cl[t] = packed_to_c(Lp, n, m);
-    cld[t] = packed_to_c(Ldp, n, m);
+  constexpr int kHalf = kWave / 2;  // lanes per leaf: two leaves per wave
+  __shared__ cplx cl[2][NT], cld[2][NT];
+  const int lane = threadIdx.x, h = lane / kHalf, u = lane - h * kHalf;
+  const int li = 2 * (int)blockIdx.x + h;
+  const int ci = li < nleaves ? leaf_ids[li] : -1;
+  const bool on = ci >= 0 && c.has_sink[ci];
+  if (on) {
+    const double *Lp = c.L + (size_t)ci * NT;
+    const double *Ldp = c.Ld + (size_t)ci * NT;
+    for (int t = u; t < NT; t += kHalf) {
+      int n = 0;
+      while ((n + 1) * (n + 1) <= t) n++;
+      int m = t - n * n - n;
+      cl[h][t] = packed_to_c(Lp, n, m);
+      cld[h][t] = packed_to_c(Ldp, n, m);
+    }
   }
   __syncthreads();
+  if (!on) return;
+  const int b0 = c.body[ci], nb = c.nbody[ci];
   const double X0 = c.X[3 * ci], X1 = c.X[3 * ci + 1], X2 = c.X[3 * ci + 2];
-  for (int base = 0; base < nb; base += kWave) {
-    const int i = b0 + base + lane;
-    if (base + lane >= nb) continue;
+  for (int base = 0; base < nb; base += kHalf) {
+    const int i = b0 + base + u;
+    if (base + u >= nb) continue;
     if (!(b.flags[i] & kSink)) continue;
     const double vx = -b.vx[i], vy = -b.vy[i], vz = -b.vz[i];
     const double dx = X0 - b.x[i], dy = X1 - b.y[i], dz = X2 - b.z[i];
     const double r2 = dx * dx + dy * dy + dz * dz;
     cplx y1[P + 1], y2[P + 1];
     L2PAcc a = {0, 0, 0, 0, 0, 0, {0, 0}, {0, 0}};
-    l2p_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, cl, cld, y1, y2, a);
+    l2p_orders<P, 0>(dx, dy, dz, r2, vx, vy, vz, cl[h], cld[h], y1, y2, a);
     r.pot[i] += a.p00 + 2.0 * a.h00;
     r.ax[i] -= a.p11.re;
     r.ay[i] -= a.p11.im;
@@ -213,7 +220,7 @@ static void l2l_go(CellsDev c, const int *ids, int ls, int ncell, hipStream_t s)
 }
 template <int P>
 static void l2p_go(CellsDev c, BodiesDev b, ResultsDev r, const int *ids, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2p<P>, dim3(n), dim3(64), 0, s, c, b, r, ids);
+  hipLaunchKernelGGL(k_l2p<P>, dim3((n + 1) / 2), dim3(64), 0, s, c, b, r, ids, n);
 }
 
 int launch_l2l(int p, CellsDev c, int ls, int le, hipStream_t s) {
